@@ -1,0 +1,509 @@
+// Implementation of the C ABI declared in include/illum_hiprt.h.
+//
+// Every entry point validates its handles and sizes on the host before any
+// kernel sees them: a BVH is refused unless its child indices strictly
+// increase along every edge (so traversal terminates) and its depth fits the
+// chosen kernel's stack, and ray/hit pointers must be non-null for every
+// plane the kernel reads or writes.
+#include "../../include/illum_hiprt.h"
+
+#include "core/layout.h"
+#include "core/raygen.h"
+#include "device/kernels.h"
+#include "host/bvh.h"
+#include "host/flatten.h"
+#include "host/scene.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+int ok() { return HIPRT_OK; }
+
+#define HIP_TRY(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return fail(HIPRT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Owned device allocation.
+struct dbuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    dbuf() = default;
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    ~dbuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        bytes = n;
+        return n ? hipMalloc(&p, n) : hipSuccess;
+    }
+    hipError_t upload(const void *src, size_t n) {
+        hipError_t e = alloc(n);
+        if (e != hipSuccess || n == 0) return e;
+        return hipMemcpy(p, src, n, hipMemcpyHostToDevice);
+    }
+};
+
+} // namespace
+
+struct hiprt_host_scene_s {
+    rt::scene sc;
+};
+
+struct hiprt_bvh_s {
+    rt::bvh b;
+    rt::bvh_stats st;
+};
+
+struct hiprt_context_s {
+    int device = 0;
+    dbuf counter;   // work counter of the persistent kernels
+};
+
+struct hiprt_scene_s {
+    hiprt_context ctx = nullptr;
+    rt::flat_scene host;   // host copy of the uploaded layout
+    dbuf nodes, tris, slot_ids;
+    rt::scene_desc desc{};
+};
+
+extern "C" {
+
+const char *hiprt_last_error(void) { return g_err.c_str(); }
+const char *hiprt_version(void) { return "illum-hiprt 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------------- host rays
+
+int hiprt_camera_rays_host(const float *cam, float *const *planes) {
+    if (!cam || !planes) return fail(HIPRT_ERR_ARG, "null argument");
+    for (int k = 0; k < 8; ++k)
+        if (!planes[k]) return fail(HIPRT_ERR_ARG, "null ray plane");
+    int w = (int)cam[10], h = (int)cam[11];
+    if (w <= 0 || h <= 0) return fail(HIPRT_ERR_ARG, "image size must be positive");
+    rt::cam_frame f = rt::make_cam_frame(rt::vec3f{cam[0], cam[1], cam[2]}, rt::vec3f{cam[3], cam[4], cam[5]},
+                                         rt::vec3f{cam[6], cam[7], cam[8]}, cam[9], w, h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            size_t i = (size_t)y * w + x;
+            rt::vec3f d = rt::cam_dir(f, x, y);
+            planes[0][i] = f.pos.x;
+            planes[1][i] = f.pos.y;
+            planes[2][i] = f.pos.z;
+            planes[3][i] = rt::kRayEps;
+            planes[4][i] = d.x;
+            planes[5][i] = d.y;
+            planes[6][i] = d.z;
+            planes[7][i] = FLT_MAX;
+        }
+    return ok();
+}
+
+// ------------------------------------------------------------------ host scene
+
+int hiprt_scene_generate(int config, hiprt_host_scene *out, float *cam) {
+    if (!out) return fail(HIPRT_ERR_ARG, "out is null");
+    if (config < 1 || config > 3) return fail(HIPRT_ERR_ARG, "config must be 1, 2 or 3");
+    auto *h = new (std::nothrow) hiprt_host_scene_s;
+    if (!h) return fail(HIPRT_ERR_ARG, "out of host memory");
+    rt::workload w = rt::make_workload(config);
+    h->sc = std::move(w.sc);
+    if (cam) {
+        const rt::camera &c = w.cam;
+        float v[15] = {c.pos.x, c.pos.y, c.pos.z, c.dir.x, c.dir.y, c.dir.z, c.up.x, c.up.y, c.up.z,
+                       c.fovy, (float)c.w, (float)c.h, w.light.x, w.light.y, w.light.z};
+        std::memcpy(cam, v, sizeof v);
+    }
+    *out = h;
+    return ok();
+}
+
+int hiprt_scene_random(uint32_t n_tris, float sigma, uint64_t seed, hiprt_host_scene *out) {
+    if (!out) return fail(HIPRT_ERR_ARG, "out is null");
+    auto *h = new (std::nothrow) hiprt_host_scene_s;
+    if (!h) return fail(HIPRT_ERR_ARG, "out of host memory");
+    h->sc = rt::gen_random_tris(n_tris, sigma, seed);
+    *out = h;
+    return ok();
+}
+
+int hiprt_scene_from_arrays(const float *pos, uint32_t n_vert, const uint32_t *idx, uint32_t n_tris,
+                            const uint32_t *mat, hiprt_host_scene *out) {
+    if (!out || (n_vert && !pos) || (n_tris && !idx)) return fail(HIPRT_ERR_ARG, "null array");
+    for (uint64_t k = 0; k < 3ull * n_tris; ++k)
+        if (idx[k] >= n_vert) return fail(HIPRT_ERR_ARG, "triangle index out of range");
+    auto *h = new (std::nothrow) hiprt_host_scene_s;
+    if (!h) return fail(HIPRT_ERR_ARG, "out of host memory");
+    h->sc.pos.resize(n_vert);
+    for (uint32_t i = 0; i < n_vert; ++i) h->sc.pos[i] = rt::vec3f{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    h->sc.idx.assign(idx, idx + 3ull * n_tris);
+    uint32_t max_mat = 0;
+    if (mat) {
+        h->sc.mat.assign(mat, mat + n_tris);
+        for (uint32_t m : h->sc.mat) max_mat = m > max_mat ? m : max_mat;
+    } else {
+        h->sc.mat.assign(n_tris, 0);
+    }
+    h->sc.mats.assign(max_mat + 1, rt::material{rt::v3(0.8f), rt::v3(0.0f)});
+    *out = h;
+    return ok();
+}
+
+int hiprt_scene_load_obj(const char *path, hiprt_host_scene *out) {
+    if (!path || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    auto *h = new (std::nothrow) hiprt_host_scene_s;
+    if (!h) return fail(HIPRT_ERR_ARG, "out of host memory");
+    std::string err;
+    if (!rt::read_obj(path, h->sc, err)) {
+        delete h;
+        return fail(HIPRT_ERR_IO, err);
+    }
+    *out = h;
+    return ok();
+}
+
+int hiprt_scene_write_obj(hiprt_host_scene s, const char *path) {
+    if (!s || !path) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!rt::write_obj(s->sc, path)) return fail(HIPRT_ERR_IO, std::string("cannot write ") + path);
+    return ok();
+}
+
+int hiprt_scene_info(hiprt_host_scene s, uint32_t *n_vert, uint32_t *n_tris) {
+    if (!s) return fail(HIPRT_ERR_ARG, "null scene");
+    if (n_vert) *n_vert = (uint32_t)s->sc.pos.size();
+    if (n_tris) *n_tris = s->sc.n_tris();
+    return ok();
+}
+
+int hiprt_scene_copy(hiprt_host_scene s, float *pos, uint32_t *idx, uint32_t *mat) {
+    if (!s) return fail(HIPRT_ERR_ARG, "null scene");
+    if (pos)
+        for (size_t i = 0; i < s->sc.pos.size(); ++i) {
+            pos[3 * i] = s->sc.pos[i].x;
+            pos[3 * i + 1] = s->sc.pos[i].y;
+            pos[3 * i + 2] = s->sc.pos[i].z;
+        }
+    if (idx && !s->sc.idx.empty()) std::memcpy(idx, s->sc.idx.data(), s->sc.idx.size() * 4);
+    if (mat && !s->sc.mat.empty()) std::memcpy(mat, s->sc.mat.data(), s->sc.mat.size() * 4);
+    return ok();
+}
+
+void hiprt_scene_free(hiprt_host_scene s) { delete s; }
+
+// -------------------------------------------------------------------- host BVH
+
+int hiprt_bvh_build(hiprt_host_scene s, const hiprt_bvh_params *params, hiprt_bvh *out) {
+    if (!s || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    rt::bvh_params p = rt::gpu_default_params();
+    if (params) {
+        if (params->split < 0 || params->split > 2) return fail(HIPRT_ERR_ARG, "split must be 0 (om), 1 (sm) or 2 (sah)");
+        if (params->max_tris < 1 || params->planes < 1) return fail(HIPRT_ERR_ARG, "max_tris and planes must be >= 1");
+        p.split = params->split == 0 ? rt::split_kind::om : (params->split == 1 ? rt::split_kind::sm : rt::split_kind::sah);
+        p.max_tris = params->max_tris;
+        p.planes = params->planes;
+        p.esc = params->esc != 0;
+    }
+    auto *h = new (std::nothrow) hiprt_bvh_s;
+    if (!h) return fail(HIPRT_ERR_ARG, "out of host memory");
+    h->b = rt::build_bvh(s->sc, p);
+    h->st = rt::compute_stats(h->b);
+    *out = h;
+    return ok();
+}
+
+int hiprt_bvh_info(hiprt_bvh b, uint32_t *n_nodes, uint32_t *n_index, uint32_t *root, uint32_t *depth) {
+    if (!b) return fail(HIPRT_ERR_ARG, "null bvh");
+    if (n_nodes) *n_nodes = (uint32_t)b->b.nodes.size();
+    if (n_index) *n_index = (uint32_t)b->b.index.size();
+    if (root) *root = b->b.root;
+    if (depth) *depth = b->st.depth;
+    return ok();
+}
+
+int hiprt_bvh_copy(hiprt_bvh b, void *nodes, uint32_t *index) {
+    if (!b) return fail(HIPRT_ERR_ARG, "null bvh");
+    if (nodes && !b->b.nodes.empty()) std::memcpy(nodes, b->b.nodes.data(), b->b.nodes.size() * sizeof(rt::bbvh_node));
+    if (index && !b->b.index.empty()) std::memcpy(index, b->b.index.data(), b->b.index.size() * 4);
+    return ok();
+}
+
+int hiprt_bvh_validate(hiprt_host_scene s, hiprt_bvh b, int32_t max_tris) {
+    if (!s || !b) return fail(HIPRT_ERR_ARG, "null argument");
+    std::string e = rt::validate_bvh(s->sc, b->b, max_tris);
+    if (!e.empty()) return fail(HIPRT_ERR_ARG, e);
+    return ok();
+}
+
+void hiprt_bvh_free(hiprt_bvh b) { delete b; }
+
+// --------------------------------------------------------------------- context
+
+int hiprt_create(int32_t device, hiprt_context *out) {
+    if (!out) return fail(HIPRT_ERR_ARG, "out is null");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(HIPRT_ERR_ARG, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<hiprt_context_s> c(new (std::nothrow) hiprt_context_s);
+    if (!c) return fail(HIPRT_ERR_ARG, "out of host memory");
+    c->device = device;
+    HIP_TRY(c->counter.alloc(256));
+    HIP_TRY(hipMemset(c->counter.p, 0, 256));
+    *out = c.release();
+    return ok();
+}
+
+int hiprt_destroy(hiprt_context ctx) {
+    if (!ctx) return fail(HIPRT_ERR_ARG, "null context");
+    hipSetDevice(ctx->device);
+    delete ctx;
+    return ok();
+}
+
+// ---------------------------------------------------------------- device scene
+
+namespace {
+
+// Checks a flattened tree and computes its depth.  Children must have larger
+// indices than their parent, which rules out cycles; leaf slot ranges must lie
+// inside the triangle records.
+std::string check_layout(rt::flat_scene &f) {
+    uint32_t n_slots = (uint32_t)f.tris.size();
+    auto leaf_ok = [&](int32_t ref) {
+        uint64_t s = rt::leaf_slot(ref), c = rt::leaf_count(ref);
+        return s + c <= n_slots;
+    };
+    if (f.root_ref == rt::kInvalidRef) {
+        if (!f.nodes2.empty() || !f.nodes4.empty()) return "nodes given for an empty tree";
+        f.depth = 0;
+        return "";
+    }
+    if (rt::ref_is_leaf(f.root_ref)) {
+        if (!leaf_ok(f.root_ref)) return "root leaf slots out of range";
+        f.depth = 0;
+        return "";
+    }
+    size_t n = f.bvh_kind == rt::BVH2 ? f.nodes2.size() : f.nodes4.size();
+    if (f.root_ref < 0 || (size_t)f.root_ref >= n) return "root index out of range";
+    std::vector<uint32_t> depth(n, 1);
+    for (size_t i = n; i-- > 0;) {
+        int32_t refs[4];
+        int nref = 0;
+        if (f.bvh_kind == rt::BVH2) {
+            refs[0] = f.nodes2[i].ref_l;
+            refs[1] = f.nodes2[i].ref_r;
+            nref = 2;
+        } else {
+            for (int k = 0; k < 4; ++k) refs[k] = f.nodes4[i].ref[k];
+            nref = 4;
+        }
+        int valid = 0;
+        for (int k = 0; k < nref; ++k) {
+            int32_t r = refs[k];
+            if (r == rt::kInvalidRef) {
+                if (f.bvh_kind == rt::BVH2) return "binary node with an absent child";
+                continue;
+            }
+            ++valid;
+            if (rt::ref_is_leaf(r)) {
+                if (!leaf_ok(r)) return "leaf slots out of range";
+                continue;
+            }
+            if ((size_t)r <= i || (size_t)r >= n) return "child index does not increase along an edge";
+            depth[i] = std::max(depth[i], depth[r] + 1);
+        }
+        if (valid == 0) return "node without children";
+    }
+    f.depth = depth[f.root_ref];
+    return "";
+}
+
+int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
+    std::string e = check_layout(f);
+    if (!e.empty()) return fail(HIPRT_ERR_ARG, "invalid BVH layout: " + e);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<hiprt_scene_s> s(new (std::nothrow) hiprt_scene_s);
+    if (!s) return fail(HIPRT_ERR_ARG, "out of host memory");
+    s->ctx = ctx;
+    s->host = std::move(f);
+    const rt::flat_scene &h = s->host;
+    if (h.bvh_kind == rt::BVH2) {
+        HIP_TRY(s->nodes.upload(h.nodes2.data(), h.nodes2.size() * sizeof(rt::node2)));
+    } else {
+        HIP_TRY(s->nodes.upload(h.nodes4.data(), h.nodes4.size() * sizeof(rt::node4)));
+    }
+    HIP_TRY(s->tris.upload(h.tris.data(), h.tris.size() * sizeof(rt::tri_rec)));
+    HIP_TRY(s->slot_ids.upload(h.slot_ids.data(), h.slot_ids.size() * 4));
+    s->desc.nodes = s->nodes.p;
+    s->desc.tris = static_cast<const rt::tri_rec *>(s->tris.p);
+    s->desc.slot_ids = static_cast<const uint32_t *>(s->slot_ids.p);
+    s->desc.root_ref = h.root_ref;
+    s->desc.bvh = h.bvh_kind;
+    s->desc.tri = h.tri_kind;
+    s->desc.depth = (int32_t)h.depth;
+    *out = s.release();
+    return ok();
+}
+
+} // namespace
+
+int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32_t bvh_format, int32_t tri_format,
+                       hiprt_scene *out) {
+    if (!ctx || !s || !b || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (bvh_format != HIPRT_BVH2 && bvh_format != HIPRT_BVH4) return fail(HIPRT_ERR_ARG, "unknown bvh_format");
+    if (tri_format != HIPRT_TRI_EXACT && tri_format != HIPRT_TRI_WOOP) return fail(HIPRT_ERR_ARG, "unknown tri_format");
+    rt::flat_scene f;
+    std::string e = rt::flatten(s->sc, b->b, bvh_format, tri_format, f);
+    if (!e.empty()) return fail(HIPRT_ERR_UNSUPPORTED, e);
+    return upload_flat(ctx, std::move(f), out);
+}
+
+int hiprt_scene_upload(hiprt_context ctx, const void *nodes, uint32_t n_nodes, int32_t bvh_format,
+                       const void *tri_records, const uint32_t *slot_ids, uint32_t n_slots, int32_t tri_format,
+                       int32_t root_ref, hiprt_scene *out) {
+    if (!ctx || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if ((n_nodes && !nodes) || (n_slots && (!tri_records || !slot_ids))) return fail(HIPRT_ERR_ARG, "null array");
+    if (bvh_format != HIPRT_BVH2 && bvh_format != HIPRT_BVH4) return fail(HIPRT_ERR_ARG, "unknown bvh_format");
+    if (tri_format != HIPRT_TRI_EXACT && tri_format != HIPRT_TRI_WOOP) return fail(HIPRT_ERR_ARG, "unknown tri_format");
+    rt::flat_scene f;
+    f.bvh_kind = bvh_format;
+    f.tri_kind = tri_format;
+    f.root_ref = root_ref;
+    if (bvh_format == HIPRT_BVH2) {
+        const auto *p = static_cast<const rt::node2 *>(nodes);
+        f.nodes2.assign(p, p + n_nodes);
+    } else {
+        const auto *p = static_cast<const rt::node4 *>(nodes);
+        f.nodes4.assign(p, p + n_nodes);
+    }
+    const auto *t = static_cast<const rt::tri_rec *>(tri_records);
+    f.tris.assign(t, t + n_slots);
+    f.slot_ids.assign(slot_ids, slot_ids + n_slots);
+    return upload_flat(ctx, std::move(f), out);
+}
+
+int hiprt_scene_layout(hiprt_scene sc, uint32_t *n_nodes, uint32_t *n_slots, int32_t *root_ref, uint32_t *depth,
+                       int32_t *bvh_format, int32_t *tri_format) {
+    if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
+    const rt::flat_scene &h = sc->host;
+    if (n_nodes) *n_nodes = (uint32_t)(h.bvh_kind == rt::BVH2 ? h.nodes2.size() : h.nodes4.size());
+    if (n_slots) *n_slots = (uint32_t)h.tris.size();
+    if (root_ref) *root_ref = h.root_ref;
+    if (depth) *depth = h.depth;
+    if (bvh_format) *bvh_format = h.bvh_kind;
+    if (tri_format) *tri_format = h.tri_kind;
+    return ok();
+}
+
+int hiprt_scene_download(hiprt_scene sc, void *nodes, void *tri_records, uint32_t *slot_ids) {
+    if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
+    const rt::flat_scene &h = sc->host;
+    if (nodes) {
+        if (h.bvh_kind == rt::BVH2 && !h.nodes2.empty())
+            std::memcpy(nodes, h.nodes2.data(), h.nodes2.size() * sizeof(rt::node2));
+        if (h.bvh_kind == rt::BVH4 && !h.nodes4.empty())
+            std::memcpy(nodes, h.nodes4.data(), h.nodes4.size() * sizeof(rt::node4));
+    }
+    if (tri_records && !h.tris.empty()) std::memcpy(tri_records, h.tris.data(), h.tris.size() * sizeof(rt::tri_rec));
+    if (slot_ids && !h.slot_ids.empty()) std::memcpy(slot_ids, h.slot_ids.data(), h.slot_ids.size() * 4);
+    return ok();
+}
+
+int hiprt_scene_destroy(hiprt_scene sc) {
+    if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
+    hipSetDevice(sc->ctx->device);
+    delete sc;
+    return ok();
+}
+
+// ----------------------------------------------------------------------- trace
+
+static const char *const kTracerNames[HIPRT_TRACER_COUNT] = {
+    "while-while", "persistent-while-while", "persistent-speculative-while-while", "dynamic-while-while"};
+
+int hiprt_tracer_id(const char *name) {
+    if (!name) return -1;
+    if (std::strcmp(name, "default") == 0) return HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
+    for (int i = 0; i < HIPRT_TRACER_COUNT; ++i)
+        if (std::strcmp(name, kTracerNames[i]) == 0) return i;
+    return -1;
+}
+
+const char *hiprt_tracer_name(int tracer) {
+    if (tracer < 0 || tracer >= HIPRT_TRACER_COUNT) return "";
+    return kTracerNames[tracer];
+}
+
+namespace {
+
+int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *occluded, const hiprt_hits *hits,
+                 bool any, int32_t tracer, const hiprt_tracer_opts *opts, void *stream) {
+    if (!sc || !rays) return fail(HIPRT_ERR_ARG, "null argument");
+    if (tracer < 0 || tracer >= HIPRT_TRACER_COUNT) return fail(HIPRT_ERR_ARG, "unknown tracer id");
+    if (n > 0xF0000000u) return fail(HIPRT_ERR_ARG, "too many rays in one batch");
+    if (n == 0) return ok();
+    if (!rays->d_ox || !rays->d_oy || !rays->d_oz || !rays->d_tmin || !rays->d_dx || !rays->d_dy || !rays->d_dz ||
+        !rays->d_tmax)
+        return fail(HIPRT_ERR_ARG, "null ray plane");
+    if (!any && (!hits || !hits->d_t || !hits->d_u || !hits->d_v || !hits->d_id))
+        return fail(HIPRT_ERR_ARG, "null hit plane");
+    if (any && !occluded && !(hits && hits->d_t))
+        return fail(HIPRT_ERR_ARG, "any-hit needs an occlusion array or hit planes");
+    if (any && hits && hits->d_t && (!hits->d_u || !hits->d_v || !hits->d_id))
+        return fail(HIPRT_ERR_ARG, "null hit plane");
+    if (sc->desc.bvh != rt::BVH2) return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH traversal is not available for this tracer");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::ray_soa r{rays->d_ox, rays->d_oy, rays->d_oz, rays->d_tmin, rays->d_dx, rays->d_dy, rays->d_dz, rays->d_tmax};
+    rt::hit_soa h{nullptr, nullptr, nullptr, nullptr};
+    if (hits) h = rt::hit_soa{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int depth = sc->desc.depth;
+    bool persistent_fits = depth <= rt::dev::kPersistentStackCap;
+    if (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits) {
+        if (depth > rt::dev::kPlainStackCap - 1)
+            return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack (" + std::to_string(depth) + " levels)");
+        HIP_TRY(rt::dev::launch_trace_plain(sc->desc, r, h, occluded, n, any, s));
+        return ok();
+    }
+    rt::dev::persistent_cfg cfg{};
+    cfg.speculative = tracer >= HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE;
+    cfg.refill = tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
+    cfg.refill_thr = (opts && opts->refill_threshold > 0) ? opts->refill_threshold : 40;
+    if (cfg.refill_thr > 64) cfg.refill_thr = 64;
+    cfg.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
+    HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
+                                             static_cast<uint32_t *>(sc->ctx->counter.p), any, cfg, s));
+    return ok();
+}
+
+} // namespace
+
+int hiprt_trace_closest(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, const hiprt_hits *hits, int32_t tracer,
+                        const hiprt_tracer_opts *opts, void *stream) {
+    return trace_common(sc, rays, n, nullptr, hits, false, tracer, opts, stream);
+}
+
+int hiprt_trace_any(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *d_occluded, const hiprt_hits *hits,
+                    int32_t tracer, const hiprt_tracer_opts *opts, void *stream) {
+    return trace_common(sc, rays, n, d_occluded, hits, true, tracer, opts, stream);
+}
+
+} // extern "C"

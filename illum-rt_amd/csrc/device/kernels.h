@@ -1,0 +1,48 @@
+// Host-callable launchers of the gfx950 kernels (implemented in the .hip
+// files of this directory).  All launches are asynchronous on `stream`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../core/layout.h"
+
+#include <cstdint>
+
+namespace rt {
+namespace dev {
+
+// Private-memory stack of the plain kernels; the host refuses BVHs deeper
+// than kPlainStackCap - 1 inner levels for them.
+constexpr int kPlainStackCap = 64;
+
+// Persistent kernels: per-lane stack of kRegStack entries in VGPRs backed by
+// kLdsStack entries per lane in LDS.  Total capacity kRegStack + kLdsStack.
+constexpr int kRegStack = 4;
+constexpr int kLdsStack = 28;
+constexpr int kPersistentStackCap = kRegStack + kLdsStack;
+constexpr int kPersistentWavesPerBlock = 4;
+
+hipError_t launch_trace_plain(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
+                              uint32_t n, bool any, hipStream_t stream);
+
+// Tunables of the persistent kernel.
+struct persistent_cfg {
+    int speculative;      // leaf postponement
+    int refill;           // fetch new rays when active lanes drop below refill_thr
+    int refill_thr;       // active-lane threshold for refill (out of 64)
+    int blocks;           // grid size; 0 = occupancy-derived default
+};
+
+// `counter` is one device uint32 that must be zero at launch; `count` is the
+// number of rays, read from device memory when count_ptr is non-null (so a
+// captured graph can trace a batch whose size an earlier kernel produced).
+hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
+                                   uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,
+                                   bool any, const persistent_cfg &cfg, hipStream_t stream);
+
+// Fills `n` uint32 words with `value` (used instead of hipMemsetAsync so the
+// clearing can be captured into graphs).
+hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t stream);
+
+} // namespace dev
+} // namespace rt

@@ -1,0 +1,154 @@
+// Device-side ray/box and ray/triangle tests and the per-ray traversal step
+// shared by every trace kernel.
+//
+// Exactness: with -ffp-contract=off and IEEE division (hipcc's default
+// -fhip-fp32-correctly-rounded-divide-sqrt), box_exact and tri_exact evaluate
+// the same IEEE operations in the same order as the reference's CPU tests
+// (intersect4, libgi/intersect.h:230-263; intersect, :19-70), including the
+// select-based min/max whose NaN behaviour decides rays with zero direction
+// components.  Traversal in the reference's order (rt/cpu/bvh2.cpp:72-137)
+// therefore returns bit-identical hits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../core/layout.h"
+
+#include <cfloat>
+#include <cstdint>
+
+namespace rt {
+namespace dev {
+
+struct dray {
+    float ox, oy, oz, tmin, dx, dy, dz, tmax;
+    float ix, iy, iz;   // 1/d, computed as ray::ray does (libgi/rt.h:28)
+};
+
+__device__ __forceinline__ dray load_ray(const ray_soa &r, uint32_t i) {
+    dray q;
+    q.ox = r.ox[i];
+    q.oy = r.oy[i];
+    q.oz = r.oz[i];
+    q.tmin = r.tmin[i];
+    q.dx = r.dx[i];
+    q.dy = r.dy[i];
+    q.dz = r.dz[i];
+    q.tmax = r.tmax[i];
+    q.ix = 1.0f / q.dx;
+    q.iy = 1.0f / q.dy;
+    q.iz = 1.0f / q.dz;
+    return q;
+}
+
+// One slab: the two ternaries of intersect4 (t1 = min, t2 = max as selects).
+__device__ __forceinline__ void slab(float lo, float hi, float o, float id, float &a, float &b) {
+    float t1 = (lo - o) * id;
+    float t2 = (hi - o) * id;
+    a = (t1 < t2) ? t1 : t2;
+    b = (t2 < t1) ? t1 : t2;
+}
+
+// intersect4 plus, for closest hit, the tracer's "dist < closest.t" cull.
+template <bool ANY>
+__device__ __forceinline__ bool box_exact(float xlo, float xhi, float ylo, float yhi, float zlo, float zhi,
+                                          const dray &r, float best, float &dist) {
+    float t1x, t2x, t1y, t2y, t1z, t2z;
+    slab(xlo, xhi, r.ox, r.ix, t1x, t2x);
+    slab(ylo, yhi, r.oy, r.iy, t1y, t2y);
+    slab(zlo, zhi, r.oz, r.iz, t1z, t2z);
+    float t1 = (t1x < t1y) ? t1y : t1x;
+    t1 = (t1z < t1) ? t1 : t1z;
+    float t2 = (t2x < t2y) ? t2x : t2y;
+    t2 = (t2z < t2) ? t2z : t2;
+    dist = t1;
+    bool hit = !(t1 > t2) && !(t2 < r.tmin) && !(t1 > r.tmax);
+    if (!ANY) hit = hit && (t1 < best);
+    return hit;
+}
+
+// Shirley/Cramer test on an exact record (A, A-B, A-C).  Accepts iff
+// tmin < t < tmax, beta > 0, gamma > 0, beta + gamma <= 1, and t < best.
+__device__ __forceinline__ bool tri_exact(const float4 &r0, const float4 &r1, const float4 &r2, const dray &r,
+                                          float best, float &t_out, float &u_out, float &v_out) {
+    const float a = r1.x, b = r1.y, c = r1.z;
+    const float d = r2.x, e = r2.y, f = r2.z;
+    const float g = r.dx, h = r.dy, i = r.dz;
+    const float j = r0.x - r.ox, k = r0.y - r.oy, l = r0.z - r.oz;
+    float c1 = e * i - h * f;
+    float c2 = g * f - d * i;
+    float c3 = d * h - e * g;
+    const float M = a * c1 + b * c2 + c * c3;
+    float beta = j * c1 + k * c2 + l * c3;
+    c1 = a * k - j * b;
+    c2 = j * c - a * l;
+    c3 = b * l - k * c;
+    float gamma = i * c1 + h * c2 + g * c3;
+    float tt = -(f * c1 + e * c2 + d * c3);
+    tt /= M;
+    if (!(tt > r.tmin && tt < r.tmax && tt < best)) return false;
+    beta /= M;
+    gamma /= M;
+    if (!(beta > 0 && gamma > 0 && beta + gamma <= 1)) return false;
+    t_out = tt;
+    u_out = beta;
+    v_out = gamma;
+    return true;
+}
+
+// Woop test: t from the plane row, then the two barycentric rows; the same
+// acceptance rule as the exact test.  FMAs are explicit: this format is not
+// bit-compatible with the CPU test in any case.
+__device__ __forceinline__ bool tri_woop(const float4 &m0, const float4 &m1, const float4 &m2, const dray &r,
+                                         float best, float &t_out, float &u_out, float &v_out) {
+    float oz = fmaf(m2.x, r.ox, fmaf(m2.y, r.oy, fmaf(m2.z, r.oz, m2.w)));
+    float dz = fmaf(m2.x, r.dx, fmaf(m2.y, r.dy, m2.z * r.dz));
+    float tt = -oz / dz;
+    if (!(tt > r.tmin && tt < r.tmax && tt < best)) return false;
+    float ou = fmaf(m0.x, r.ox, fmaf(m0.y, r.oy, fmaf(m0.z, r.oz, m0.w)));
+    float du = fmaf(m0.x, r.dx, fmaf(m0.y, r.dy, m0.z * r.dz));
+    float u = fmaf(tt, du, ou);
+    if (!(u > 0)) return false;
+    float ov = fmaf(m1.x, r.ox, fmaf(m1.y, r.oy, fmaf(m1.z, r.oz, m1.w)));
+    float dv = fmaf(m1.x, r.dx, fmaf(m1.y, r.dy, m1.z * r.dz));
+    float v = fmaf(tt, dv, ov);
+    if (!(v > 0 && u + v <= 1)) return false;
+    t_out = tt;
+    u_out = u;
+    v_out = v;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t as_u32(float f) { return __float_as_uint(f); }
+
+template <int TRI>
+__device__ __forceinline__ bool test_slot(const scene_desc &sc, uint32_t slot, const dray &r, float best, float &t,
+                                          float &u, float &v) {
+    const float4 *p = reinterpret_cast<const float4 *>(sc.tris + slot);
+    float4 r0 = p[0], r1 = p[1], r2 = p[2];
+    if (TRI == TRI_EXACT) return tri_exact(r0, r1, r2, r, best, t, u, v);
+    return tri_woop(r0, r1, r2, r, best, t, u, v);
+}
+
+template <int TRI>
+__device__ __forceinline__ uint32_t slot_tri_id(const scene_desc &sc, uint32_t slot) {
+    if (TRI == TRI_EXACT) return as_u32(reinterpret_cast<const float4 *>(sc.tris + slot)[0].w);
+    return sc.slot_ids[slot];
+}
+
+// Result of one ray.
+struct dhit {
+    float t, u, v;
+    uint32_t slot;
+    bool found;
+};
+
+__device__ __forceinline__ void store_hit(const hit_soa &h, uint32_t i, float t, float u, float v, uint32_t id) {
+    h.t[i] = t;
+    h.u[i] = u;
+    h.v[i] = v;
+    h.id[i] = id;
+}
+
+} // namespace dev
+} // namespace rt

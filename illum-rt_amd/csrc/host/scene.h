@@ -1,0 +1,74 @@
+// Host-side triangle scene: positions, index triples, per-triangle material,
+// a small diffuse material table, plus the seeded synthetic generators and a
+// minimal OBJ reader/writer.
+//
+// The reference keeps vertices as {pos, norm, tc} and triangles as
+// {a, b, c, material_id} (libgi/rt.h:35-47); only positions and indices take
+// part in intersection, so that is all the tracer stores.
+#pragma once
+
+#include "../core/vec.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace rt {
+
+struct material {
+    vec3f albedo;
+    vec3f emissive;
+};
+
+struct scene {
+    std::vector<vec3f> pos;        // vertex positions
+    std::vector<uint32_t> idx;     // 3 per triangle
+    std::vector<uint32_t> mat;     // material id per triangle
+    std::vector<material> mats;
+
+    uint32_t n_tris() const { return (uint32_t)(idx.size() / 3); }
+    vec3f a(uint32_t t) const { return pos[idx[3 * t + 0]]; }
+    vec3f b(uint32_t t) const { return pos[idx[3 * t + 1]]; }
+    vec3f c(uint32_t t) const { return pos[idx[3 * t + 2]]; }
+};
+
+// Camera as in libgi/camera.h:7-25: fovy is the vertical field of view in
+// degrees, near_h = tanf(pi*fovy/360), near_w = aspect*near_h.
+struct camera {
+    vec3f pos, dir, up;
+    float fovy;
+    int w, h;
+};
+
+// Configuration of a synthetic workload (scene + camera), by number:
+//   1: 8,192 random triangles, camera (0.5,0.5,-1) -> +z, 60 deg, 640x480
+//   2: interior hall, ~74k triangles, 1920x1080 camera inside
+//   3: hall at 2x tessellation + debris, exactly 262,144 triangles, 1920x1080
+// Config 4's scenes come from gen_random_tris with per-scene sizes.
+struct workload {
+    scene sc;
+    camera cam;
+    vec3f light;   // point light for the shadow-ray workload
+};
+
+// n triangles with centroids uniform in [0,1]^3 and vertices displaced from
+// the centroid by N(0, sigma^2) per coordinate; PCG32 stream `seed`.
+scene gen_random_tris(uint32_t n, float sigma, uint64_t seed);
+
+// The procedural hall.  `dense` selects the config-3 tessellation; `debris`
+// debris triangles (log-normal edge length, median 0.05) are appended.
+scene gen_hall(bool dense, uint32_t debris, uint64_t seed);
+
+workload make_workload(int config);
+
+// Number of debris triangles that brings the dense hall to exactly 2^18.
+uint32_t hall_debris_for_config3();
+
+// OBJ: "v x y z" with %.9g (round-trips every float) and "f a b c" (1-based),
+// one object, no normals, materials as "usemtl m<id>" groups written in
+// triangle order.  The reader accepts v/f (triangulating polygons as fans)
+// and usemtl, ignores everything else, and keeps face order.
+bool write_obj(const scene &s, const std::string &path);
+bool read_obj(const std::string &path, scene &out, std::string &err);
+
+} // namespace rt

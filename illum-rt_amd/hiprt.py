@@ -1,0 +1,308 @@
+"""ctypes binding of libillum_hiprt.so (the C ABI in include/illum_hiprt.h).
+
+Host objects (scenes, BVHs) come back as numpy arrays; device buffers are
+passed as torch CUDA tensors, whose memory and streams are plumbing only --
+every trace runs in the library's own gfx950 kernels.  Loading fails loudly
+when the shared library is missing; there is no fallback path.
+
+Import torch before this module in processes that use both, so that the one
+HIP runtime of the process is torch's.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_int32, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libillum_hiprt.so")
+
+BVH2, BVH4 = 0, 1
+TRI_EXACT, TRI_WOOP = 0, 1
+SPLIT_OM, SPLIT_SM, SPLIT_SAH = 0, 1, 2
+
+TRACERS = ("while-while", "persistent-while-while", "persistent-speculative-while-while",
+           "dynamic-while-while")
+
+# numpy view of the reference's 56-byte bbvh_node (rt/cpu/bvh.h:42-52)
+BBVH_NODE = np.dtype([("box_l", np.float32, (2, 3)), ("box_r", np.float32, (2, 3)),
+                      ("link_l", np.int32), ("link_r", np.int32)])
+assert BBVH_NODE.itemsize == 56
+
+
+class HiprtError(RuntimeError):
+    pass
+
+
+class _BvhParams(ctypes.Structure):
+    _fields_ = [("split", c_int32), ("max_tris", c_int32), ("planes", c_int32), ("esc", c_int32)]
+
+
+class _Rays(ctypes.Structure):
+    _fields_ = [(n, c_void_p) for n in ("ox", "oy", "oz", "tmin", "dx", "dy", "dz", "tmax")]
+
+
+class _Hits(ctypes.Structure):
+    _fields_ = [("t", c_void_p), ("u", c_void_p), ("v", c_void_p), ("id", c_void_p)]
+
+
+class _TracerOpts(ctypes.Structure):
+    _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HiprtError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "hiprt_last_error": (c_char_p, []),
+            "hiprt_version": (c_char_p, []),
+            "hiprt_camera_rays_host": (c_int, [POINTER(c_float), POINTER(c_void_p)]),
+            "hiprt_scene_generate": (c_int, [c_int, POINTER(c_void_p), POINTER(c_float)]),
+            "hiprt_scene_random": (c_int, [c_uint32, c_float, c_uint64, POINTER(c_void_p)]),
+            "hiprt_scene_from_arrays": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_void_p,
+                                                POINTER(c_void_p)]),
+            "hiprt_scene_load_obj": (c_int, [c_char_p, POINTER(c_void_p)]),
+            "hiprt_scene_write_obj": (c_int, [c_void_p, c_char_p]),
+            "hiprt_scene_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
+            "hiprt_scene_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+            "hiprt_scene_free": (None, [c_void_p]),
+            "hiprt_bvh_build": (c_int, [c_void_p, POINTER(_BvhParams), POINTER(c_void_p)]),
+            "hiprt_bvh_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
+                                       POINTER(c_uint32)]),
+            "hiprt_bvh_copy": (c_int, [c_void_p, c_void_p, c_void_p]),
+            "hiprt_bvh_validate": (c_int, [c_void_p, c_void_p, c_int32]),
+            "hiprt_bvh_free": (None, [c_void_p]),
+            "hiprt_create": (c_int, [c_int32, POINTER(c_void_p)]),
+            "hiprt_destroy": (c_int, [c_void_p]),
+            "hiprt_scene_commit": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, POINTER(c_void_p)]),
+            "hiprt_scene_upload": (c_int, [c_void_p, c_void_p, c_uint32, c_int32, c_void_p, c_void_p, c_uint32,
+                                           c_int32, c_int32, POINTER(c_void_p)]),
+            "hiprt_scene_layout": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32),
+                                           POINTER(c_uint32), POINTER(c_int32), POINTER(c_int32)]),
+            "hiprt_scene_download": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+            "hiprt_scene_destroy": (c_int, [c_void_p]),
+            "hiprt_tracer_id": (c_int, [c_char_p]),
+            "hiprt_tracer_name": (c_char_p, [c_int]),
+            "hiprt_trace_closest": (c_int, [c_void_p, POINTER(_Rays), c_uint32, POINTER(_Hits), c_int32,
+                                            POINTER(_TracerOpts), c_void_p]),
+            "hiprt_trace_any": (c_int, [c_void_p, POINTER(_Rays), c_uint32, c_void_p, POINTER(_Hits), c_int32,
+                                        POINTER(_TracerOpts), c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names the C header declares (parsed from include/illum_hiprt.h)."""
+    import re
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "illum_hiprt.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"\b(hiprt_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(status):
+    if status != 0:
+        raise HiprtError(lib().hiprt_last_error().decode())
+
+
+def _ptr(a):
+    return c_void_p(a.ctypes.data) if a is not None else None
+
+
+class HostScene:
+    """Host triangle scene (positions, index triples, materials)."""
+
+    def __init__(self, handle, cam=None):
+        self.h = c_void_p(handle) if not isinstance(handle, c_void_p) else handle
+        self.cam = cam
+
+    @classmethod
+    def generate(cls, config):
+        h = c_void_p()
+        cam = (c_float * 15)()
+        check(lib().hiprt_scene_generate(config, ctypes.byref(h), cam))
+        return cls(h, np.array(cam, dtype=np.float32))
+
+    @classmethod
+    def random(cls, n, sigma=0.02, seed=1):
+        h = c_void_p()
+        check(lib().hiprt_scene_random(n, sigma, seed, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, pos, idx, mat=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+        m = None if mat is None else np.ascontiguousarray(mat, dtype=np.uint32)
+        h = c_void_p()
+        check(lib().hiprt_scene_from_arrays(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(m), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_obj(cls, path):
+        h = c_void_p()
+        check(lib().hiprt_scene_load_obj(str(path).encode(), ctypes.byref(h)))
+        return cls(h)
+
+    def write_obj(self, path):
+        check(lib().hiprt_scene_write_obj(self.h, str(path).encode()))
+
+    def arrays(self):
+        nv, nt = c_uint32(), c_uint32()
+        check(lib().hiprt_scene_info(self.h, ctypes.byref(nv), ctypes.byref(nt)))
+        pos = np.empty((nv.value, 3), np.float32)
+        idx = np.empty((nt.value, 3), np.uint32)
+        mat = np.empty(nt.value, np.uint32)
+        check(lib().hiprt_scene_copy(self.h, _ptr(pos), _ptr(idx), _ptr(mat)))
+        return pos, idx, mat
+
+    @property
+    def n_tris(self):
+        nv, nt = c_uint32(), c_uint32()
+        check(lib().hiprt_scene_info(self.h, ctypes.byref(nv), ctypes.byref(nt)))
+        return nt.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.hiprt_scene_free(self.h)
+            self.h = None
+
+
+class Bvh:
+    """Host BVH in the reference's node layout plus its leaf-slot index."""
+
+    def __init__(self, scene, split=SPLIT_SAH, max_tris=4, planes=16, esc=True):
+        p = _BvhParams(split, max_tris, planes, 1 if esc else 0)
+        self.h = c_void_p()
+        check(lib().hiprt_bvh_build(scene.h, ctypes.byref(p), ctypes.byref(self.h)))
+        self.scene = scene
+        self.max_tris = max_tris
+        nn, ni, root, depth = c_uint32(), c_uint32(), c_uint32(), c_uint32()
+        check(lib().hiprt_bvh_info(self.h, ctypes.byref(nn), ctypes.byref(ni), ctypes.byref(root),
+                                   ctypes.byref(depth)))
+        self.root, self.depth = root.value, depth.value
+        self.nodes = np.zeros(nn.value, BBVH_NODE)
+        self.index = np.zeros(ni.value, np.uint32)
+        check(lib().hiprt_bvh_copy(self.h, _ptr(self.nodes), _ptr(self.index)))
+
+    def validate(self):
+        return lib().hiprt_bvh_validate(self.scene.h, self.h, self.max_tris), lib().hiprt_last_error().decode()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.hiprt_bvh_free(self.h)
+            self.h = None
+
+
+def camera_rays(cam):
+    """Host camera rays (8, w*h) float32 for cam = pos, dir, up, fovy, w, h."""
+    cam = np.ascontiguousarray(cam[:12], dtype=np.float32)
+    w, h = int(cam[10]), int(cam[11])
+    out = np.empty((8, w * h), np.float32)
+    planes = (c_void_p * 8)(*[out[k].ctypes.data for k in range(8)])
+    check(lib().hiprt_camera_rays_host(cam.ctypes.data_as(POINTER(c_float)), planes))
+    return out
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = c_void_p()
+        check(lib().hiprt_create(device, ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hiprt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+
+class DeviceScene:
+    def __init__(self, ctx, scene, bvh, bvh_format=BVH2, tri_format=TRI_EXACT):
+        self.ctx = ctx
+        self.h = c_void_p()
+        check(lib().hiprt_scene_commit(ctx.h, scene.h, bvh.h, bvh_format, tri_format, ctypes.byref(self.h)))
+        nn, ns, root, depth, bf, tf = c_uint32(), c_uint32(), c_int32(), c_uint32(), c_int32(), c_int32()
+        check(lib().hiprt_scene_layout(self.h, ctypes.byref(nn), ctypes.byref(ns), ctypes.byref(root),
+                                       ctypes.byref(depth), ctypes.byref(bf), ctypes.byref(tf)))
+        self.n_nodes, self.n_slots, self.root_ref, self.depth = nn.value, ns.value, root.value, depth.value
+        self.bvh_format, self.tri_format = bf.value, tf.value
+
+    def download(self):
+        node_bytes = 64 if self.bvh_format == BVH2 else 128
+        nodes = np.zeros(self.n_nodes * node_bytes // 4, np.uint32)
+        tris = np.zeros(self.n_slots * 12, np.float32)
+        ids = np.zeros(self.n_slots, np.uint32)
+        check(lib().hiprt_scene_download(self.h, _ptr(nodes), _ptr(tris), _ptr(ids)))
+        return nodes, tris, ids
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hiprt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+
+def _rays_struct(rays):
+    """rays: torch float32 CUDA tensor of shape (8, n), contiguous."""
+    assert rays.dim() == 2 and rays.shape[0] == 8 and rays.is_contiguous()
+    return _Rays(*[rays[k].data_ptr() for k in range(8)])
+
+
+def _hits_struct(hits):
+    """hits: dict with t, u, v (float32) and id (int32) CUDA tensors."""
+    return _Hits(hits["t"].data_ptr(), hits["u"].data_ptr(), hits["v"].data_ptr(), hits["id"].data_ptr())
+
+
+def tracer_id(name):
+    i = lib().hiprt_tracer_id(name.encode())
+    if i < 0:
+        raise HiprtError(f"No tracer {name}")
+    return i
+
+
+def _opts(refill_threshold=0, blocks=0):
+    return _TracerOpts(refill_threshold, blocks)
+
+
+def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_threshold=0, blocks=0):
+    n = rays.shape[1]
+    r, h, o = _rays_struct(rays), _hits_struct(hits), _opts(refill_threshold, blocks)
+    s = c_void_p(stream) if stream else None
+    check(lib().hiprt_trace_closest(dscene.h, ctypes.byref(r), n, ctypes.byref(h), tracer_id(tracer),
+                                    ctypes.byref(o), s))
+
+
+def trace_any(dscene, rays, occluded, hits=None, tracer="while-while", stream=None, refill_threshold=0, blocks=0):
+    n = rays.shape[1]
+    r, o = _rays_struct(rays), _opts(refill_threshold, blocks)
+    hp = ctypes.byref(_hits_struct(hits)) if hits is not None else None
+    occ = c_void_p(occluded.data_ptr()) if occluded is not None else None
+    s = c_void_p(stream) if stream else None
+    check(lib().hiprt_trace_any(dscene.h, ctypes.byref(r), n, occ, hp, tracer_id(tracer), ctypes.byref(o), s))
+
+
+def new_hits(n, device="cuda"):
+    import torch
+    return {"t": torch.empty(n, dtype=torch.float32, device=device),
+            "u": torch.empty(n, dtype=torch.float32, device=device),
+            "v": torch.empty(n, dtype=torch.float32, device=device),
+            "id": torch.empty(n, dtype=torch.int32, device=device)}

@@ -1,0 +1,180 @@
+/*
+ * illum_hiprt.h -- C ABI of the MI355X (gfx950) batch ray tracer.
+ *
+ * This is the drop-in boundary for the reference's batch ray-tracer path:
+ * the C++ platform classes (wf::hip::platform and its tracers, mirroring
+ * libgi/wavefront-rt.h) and the Python test/bench harness call only these
+ * functions.  All entry points return 0 (HIPRT_OK) on success or a nonzero
+ * status; the message of the last failure on the calling thread is available
+ * from hiprt_last_error().  Nothing throws across this boundary.
+ *
+ * Conventions
+ *   - Pointers named d_* are device pointers; all others are host pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Trace calls are asynchronous on that stream.
+ *   - Rays are structure-of-arrays fp32: origin, t_min, direction, t_max
+ *     (32 B per ray); hits are structure-of-arrays t, beta, gamma, triangle id
+ *     (16 B per ray).  A miss is t = FLT_MAX, id = 0, as
+ *     triangle_intersection::reset() (libgi/rt.h:57-67).
+ */
+#ifndef ILLUM_HIPRT_H
+#define ILLUM_HIPRT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HIPRT_OK = 0,
+    HIPRT_ERR_ARG = 1,         /* invalid argument or handle */
+    HIPRT_ERR_HIP = 2,         /* a HIP runtime call failed */
+    HIPRT_ERR_UNSUPPORTED = 3, /* valid request this build cannot serve */
+    HIPRT_ERR_IO = 4           /* file could not be read or written */
+};
+
+/* Message of the last failing call on this thread ("" if none).
+ * Replaces CHECK_CUDA_ERROR's stderr+throw (rt/cuda/cuda-helpers.cpp:5-12). */
+const char *hiprt_last_error(void);
+
+/* Version string of this library. */
+const char *hiprt_version(void);
+
+/* ------------------------------------------------------------------ host rays
+ * Camera rays of cam_ray (libgi/camera.cpp:44-61) for every pixel of a w x h
+ * image, pixel-centre offset 0, t_min 1e-4, t_max FLT_MAX, written row-major
+ * into eight host planes (ox, oy, oz, tmin, dx, dy, dz, tmax) of w*h floats.
+ * cam = pos.xyz, dir.xyz, up.xyz, fovy (vertical, degrees), w, h. */
+int hiprt_camera_rays_host(const float *cam, float *const *planes);
+
+/* ---------------------------------------------------------------- host scene
+ * A triangle scene in host memory: positions, index triples, per-triangle
+ * material.  Stands for ::scene's vertices/triangles (libgi/scene.h) as far as
+ * tracing needs them.  No GPU is touched. */
+typedef struct hiprt_host_scene_s *hiprt_host_scene;
+
+/* Synthetic workloads by number: 1 = 8,192 random triangles, 2 = interior hall
+ * (~74k), 3 = dense hall + debris (262,144).  When cam is non-NULL it receives
+ * 15 floats: pos.xyz, dir.xyz, up.xyz, fovy (degrees, vertical), w, h, and the
+ * point light's position xyz. */
+int hiprt_scene_generate(int config, hiprt_host_scene *out, float *cam);
+/* n random triangles, centroids U[0,1]^3, vertex offsets N(0, sigma^2), PCG32 seed. */
+int hiprt_scene_random(uint32_t n_tris, float sigma, uint64_t seed, hiprt_host_scene *out);
+/* Copy of caller arrays: pos[3*n_vert], idx[3*n_tris], mat[n_tris] (mat may be NULL). */
+int hiprt_scene_from_arrays(const float *pos, uint32_t n_vert, const uint32_t *idx, uint32_t n_tris,
+                            const uint32_t *mat, hiprt_host_scene *out);
+/* Minimal OBJ reader/writer ("load" of the reference, libgi/scene.cpp:291-364). */
+int hiprt_scene_load_obj(const char *path, hiprt_host_scene *out);
+int hiprt_scene_write_obj(hiprt_host_scene s, const char *path);
+int hiprt_scene_info(hiprt_host_scene s, uint32_t *n_vert, uint32_t *n_tris);
+/* Any output pointer may be NULL. */
+int hiprt_scene_copy(hiprt_host_scene s, float *pos, uint32_t *idx, uint32_t *mat);
+void hiprt_scene_free(hiprt_host_scene s);
+
+/* ------------------------------------------------------------------ host BVH
+ * The reference's CPU BVH builders (rt/cpu/bvh-ctor.h): object median (0),
+ * spatial median (1) or plane-sweep SAH (2), with optional early split
+ * clipping.  Defaults of the reference's GPU platforms: {2, 4, 16, 1}
+ * (rt/cuda/base.cpp:94-105). */
+typedef struct {
+    int32_t split;     /* 0 om, 1 sm, 2 sah */
+    int32_t max_tris;  /* triangles per leaf */
+    int32_t planes;    /* SAH candidate planes */
+    int32_t esc;       /* early split clipping on/off */
+} hiprt_bvh_params;
+
+typedef struct hiprt_bvh_s *hiprt_bvh;
+
+int hiprt_bvh_build(hiprt_host_scene s, const hiprt_bvh_params *params, hiprt_bvh *out);
+/* n_nodes: entries of the reference node array; n_index: leaf slots;
+ * depth: inner nodes on the longest root-to-leaf path. */
+int hiprt_bvh_info(hiprt_bvh b, uint32_t *n_nodes, uint32_t *n_index, uint32_t *root, uint32_t *depth);
+/* nodes: n_nodes records of 56 B in the reference's bbvh_node layout
+ * (rt/cpu/bvh.h:42-52: box_l.min/max, box_r.min/max, link_l, link_r);
+ * index: n_index uint32.  Either pointer may be NULL. */
+int hiprt_bvh_copy(hiprt_bvh b, void *nodes, uint32_t *index);
+/* Structural check of the tree against the scene; 0 if every invariant holds,
+ * else HIPRT_ERR_ARG with the broken invariant in hiprt_last_error(). */
+int hiprt_bvh_validate(hiprt_host_scene s, hiprt_bvh b, int32_t max_tris);
+void hiprt_bvh_free(hiprt_bvh b);
+
+/* ------------------------------------------------------------------- context */
+typedef struct hiprt_context_s *hiprt_context;
+
+/* Selects HIP device `device` (cudaSetDevice(0) of rt/cuda/platform.cpp:29). */
+int hiprt_create(int32_t device, hiprt_context *out);
+int hiprt_destroy(hiprt_context ctx);
+
+/* ------------------------------------------------------------ device scenes
+ * Layouts: bvh_format 0 = binary (64-B Aila-Laine nodes), 1 = collapsed 4-wide
+ * (128-B nodes); tri_format 0 = exact (A, A-B, A-C records: bit-identical to
+ * the CPU Shirley test), 1 = Woop (affine rows).  48-B triangle records. */
+enum { HIPRT_BVH2 = 0, HIPRT_BVH4 = 1 };
+enum { HIPRT_TRI_EXACT = 0, HIPRT_TRI_WOOP = 1 };
+
+typedef struct hiprt_scene_s *hiprt_scene;
+
+/* Flattens (scene, bvh) on the host and uploads it: the work of
+ * batch_rt::build + compact_bvh_node_builder (rt/cuda/base.cpp:88-117,157-178). */
+int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32_t bvh_format,
+                       int32_t tri_format, hiprt_scene *out);
+/* Uploads an already flattened scene (host arrays): n_nodes nodes of 64 B
+ * (bvh_format 0) or 128 B (1); n_slots triangle records of 48 B; slot_ids[n_slots]
+ * triangle id per slot; root_ref = index of the root inner node or the leaf
+ * reference of a one-leaf tree.  The arrays are checked on the host (child
+ * indices increase along every edge, leaf slots in range) before upload. */
+int hiprt_scene_upload(hiprt_context ctx, const void *nodes, uint32_t n_nodes, int32_t bvh_format,
+                       const void *tri_records, const uint32_t *slot_ids, uint32_t n_slots, int32_t tri_format,
+                       int32_t root_ref, hiprt_scene *out);
+/* Host copy of the flattened arrays (sizes from hiprt_scene_layout). */
+int hiprt_scene_layout(hiprt_scene sc, uint32_t *n_nodes, uint32_t *n_slots, int32_t *root_ref, uint32_t *depth,
+                       int32_t *bvh_format, int32_t *tri_format);
+int hiprt_scene_download(hiprt_scene sc, void *nodes, void *tri_records, uint32_t *slot_ids);
+int hiprt_scene_destroy(hiprt_scene sc);
+
+/* --------------------------------------------------------------------- trace */
+typedef struct {
+    const float *d_ox, *d_oy, *d_oz, *d_tmin, *d_dx, *d_dy, *d_dz, *d_tmax;
+} hiprt_rays;
+
+typedef struct {
+    float *d_t, *d_u, *d_v;
+    uint32_t *d_id;
+} hiprt_hits;
+
+/* Tracer variants, named as the reference registers its CUDA tracers
+ * (rt/cuda/platform.cpp:34-56).  hiprt_tracer_id maps a name to an id, -1 if
+ * unknown; "default" is "dynamic-while-while". */
+enum {
+    HIPRT_TRACER_WHILE_WHILE = 0,                        /* plain: one ray per lane, grid = batch */
+    HIPRT_TRACER_PERSISTENT_WHILE_WHILE = 1,             /* persistent waves, one atomic per fetch */
+    HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE = 2, /* + leaf postponement */
+    HIPRT_TRACER_DYNAMIC_WHILE_WHILE = 3,                /* + leaf postponement + dynamic refill */
+    HIPRT_TRACER_COUNT = 4
+};
+int hiprt_tracer_id(const char *name);
+const char *hiprt_tracer_name(int tracer);
+
+/* Tunables for the persistent tracers (zero = default). */
+typedef struct {
+    int32_t refill_threshold; /* refill when fewer lanes of 64 are active (default 40) */
+    int32_t blocks;           /* grid size (default: occupancy x CU count) */
+} hiprt_tracer_opts;
+
+/* compute_closest_hit of batch_ray_tracer (libgi/wavefront-rt.h:145-149,
+ * rt/cpu/wavefront.cpp:53-59): hits[i] = closest hit of ray i. */
+int hiprt_trace_closest(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, const hiprt_hits *hits, int32_t tracer,
+                        const hiprt_tracer_opts *opts, void *stream);
+/* compute_any_hit (rt/cpu/wavefront.cpp:60-70): d_occluded[i] = 1 if any
+ * triangle is hit in (t_min, t_max), else 0.  When hits is non-NULL its
+ * arrays are also written the way the reference's CPU adapter does:
+ * t = ref = 0xFFFFFFFF (as float 4294967295) if occluded, reset() otherwise. */
+int hiprt_trace_any(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *d_occluded, const hiprt_hits *hits,
+                    int32_t tracer, const hiprt_tracer_opts *opts, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ILLUM_HIPRT_H */

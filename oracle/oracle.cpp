@@ -1,0 +1,471 @@
+// CPU oracle for the batch ray-tracer path.  TEST INFRASTRUCTURE ONLY: it is
+// loaded by the tests, by __graft_entry__.smoke() and by bench.py's CPU
+// baseline as the checker; the product never links or calls it.
+//
+// It restates, in plain C++ compiled with -ffp-contract=off (as the reference
+// is built: g++ -O2 on x86-64, no FMA):
+//   ray construction         libgi/rt.h:24-31        (id = 1/d, t_min 1e-4)
+//   Shirley triangle test    libgi/intersect.h:19-70
+//   slab test intersect4     libgi/intersect.h:230-263
+//   BVH closest/any hit      rt/cpu/bvh2.cpp:72-189  (binary_bvh_tracer)
+//   brute force              rt/cpu/seq.cpp:9-30     (seq_tri_is)
+//   batch adapter            rt/cpu/wavefront.cpp:53-70 (OpenMP over rows)
+//   camera rays              libgi/camera.h:18-24, libgi/camera.cpp:44-61
+//   cosine bounce rays       rt/cpu/bounce.cpp:41-72, libgi/sampling.h:13-36,
+//                            libgi/util.h:212-233, libgi/rt.cpp:7-15
+// plus checks the reference does not have: an FP64 intersection of a given
+// ray/triangle pair and counters of nodes visited and triangles tested.
+//
+// The tree comes in the reference's own bbvh_node layout (56-byte records,
+// rt/cpu/bvh.h:42-52) with its leaf-slot index, so the oracle walks exactly
+// the array the reference's tracer would.  One deliberate deviation: the
+// reference's traversal stack is a fixed uint32_t[25] without a bounds check
+// (bvh2.cpp:76), undefined behaviour on trees deeper than 24 inner levels;
+// here it grows, which matches the reference wherever the reference is defined.
+//
+// Parity pins (tests/test_oracle.py): brute force == BVH traversal on small
+// scenes; closed-form hits (centroid, edges, t bounds, degenerate and
+// edge-on triangles); slab-test special cases; FP64 agreement; camera rays
+// against the pinhole closed form; cosine rays against the hemisphere and
+// cos-theta distribution.
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+struct v3 {
+    float x, y, z;
+};
+inline v3 add(v3 a, v3 b) { return v3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline v3 sub(v3 a, v3 b) { return v3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline v3 mul(v3 a, float s) { return v3{a.x * s, a.y * s, a.z * s}; }
+inline v3 smul(float s, v3 a) { return v3{s * a.x, s * a.y, s * a.z}; }
+inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline v3 cross(v3 a, v3 b) { return v3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+inline v3 normalize(v3 v) { return mul(v, 1.0f / sqrtf(dot(v, v))); }
+
+struct ray {
+    v3 o, d, id;
+    float t_min, t_max;
+};
+inline ray make_ray(v3 o, v3 d, float tmin, float tmax) {
+    ray r;
+    r.o = o;
+    r.d = d;
+    r.id = v3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    r.t_min = tmin;
+    r.t_max = tmax;
+    return r;
+}
+
+struct hit {
+    float t, beta, gamma;
+    uint32_t ref;
+};
+
+struct box {
+    v3 min, max;
+};
+// bbvh_node of rt/cpu/bvh.h:42-52
+struct node {
+    box l, r;
+    int32_t link_l, link_r;
+    bool inner() const { return link_r >= 0; }
+};
+static_assert(sizeof(node) == 56, "bbvh_node layout");
+
+struct scene_view {
+    const float *pos;
+    const uint32_t *idx;
+    uint32_t n_tris;
+    v3 p(uint32_t v) const { return v3{pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]}; }
+};
+
+// libgi/intersect.h:19-70
+bool intersect_tri(const scene_view &s, uint32_t t, const ray &ray, hit &info) {
+    v3 pa = s.p(s.idx[3 * t]), pb = s.p(s.idx[3 * t + 1]), pc = s.p(s.idx[3 * t + 2]);
+    const float a_x = pa.x, a_y = pa.y, a_z = pa.z;
+    const float a = a_x - pb.x, b = a_y - pb.y, c = a_z - pb.z;
+    const float d = a_x - pc.x, e = a_y - pc.y, f = a_z - pc.z;
+    const float g = ray.d.x, h = ray.d.y, i = ray.d.z;
+    const float j = a_x - ray.o.x, k = a_y - ray.o.y, l = a_z - ray.o.z;
+    float common1 = e * i - h * f;
+    float common2 = g * f - d * i;
+    float common3 = d * h - e * g;
+    float M = a * common1 + b * common2 + c * common3;
+    float beta = j * common1 + k * common2 + l * common3;
+    common1 = a * k - j * b;
+    common2 = j * c - a * l;
+    common3 = b * l - k * c;
+    float gamma = i * common1 + h * common2 + g * common3;
+    float tt = -(f * common1 + e * common2 + d * common3);
+    beta /= M;
+    gamma /= M;
+    tt /= M;
+    if (tt > ray.t_min && tt < ray.t_max)
+        if (beta > 0 && gamma > 0 && beta + gamma <= 1) {
+            info.t = tt;
+            info.beta = beta;
+            info.gamma = gamma;
+            return true;
+        }
+    return false;
+}
+
+// libgi/intersect.h:230-263
+bool intersect4(const box &bx, const ray &ray, float &is) {
+    float idx = ray.id.x, idy = ray.id.y, idz = ray.id.z;
+    float t1x_tmp = (bx.min.x - ray.o.x) * idx;
+    float t2x_tmp = (bx.max.x - ray.o.x) * idx;
+    float t1x = (t1x_tmp < t2x_tmp) ? t1x_tmp : t2x_tmp;
+    float t2x = (t2x_tmp < t1x_tmp) ? t1x_tmp : t2x_tmp;
+    float t1y_tmp = (bx.min.y - ray.o.y) * idy;
+    float t2y_tmp = (bx.max.y - ray.o.y) * idy;
+    float t1y = (t1y_tmp < t2y_tmp) ? t1y_tmp : t2y_tmp;
+    float t2y = (t2y_tmp < t1y_tmp) ? t1y_tmp : t2y_tmp;
+    float t1z_tmp = (bx.min.z - ray.o.z) * idz;
+    float t2z_tmp = (bx.max.z - ray.o.z) * idz;
+    float t1z = (t1z_tmp < t2z_tmp) ? t1z_tmp : t2z_tmp;
+    float t2z = (t2z_tmp < t1z_tmp) ? t1z_tmp : t2z_tmp;
+    float t1 = (t1x < t1y) ? t1y : t1x;
+    t1 = (t1z < t1) ? t1 : t1z;
+    float t2 = (t2x < t2y) ? t2x : t2y;
+    t2 = (t2z < t2) ? t2z : t2;
+    if (t1 > t2) return false;
+    if (t2 < ray.t_min) return false;
+    if (t1 > ray.t_max) return false;
+    is = t1;
+    return true;
+}
+
+struct tree {
+    const node *nodes;
+    uint32_t n_nodes, root;
+    const uint32_t *index;
+};
+
+// binary_bvh_tracer::closest_hit, rt/cpu/bvh2.cpp:72-137.  counts[0] += inner
+// nodes visited, counts[1] += triangles tested (when counts is non-null).
+hit closest_hit(const scene_view &s, const tree &t, const ray &ray, uint32_t *counts) {
+    hit closest{FLT_MAX, 0, 0, 0}, intersection{FLT_MAX, 0, 0, 0};
+    if (t.n_nodes == 0) return closest;
+    std::vector<uint32_t> stack;
+    stack.reserve(64);
+    stack.push_back(t.root);
+    uint32_t n_inner = 0, n_tri = 0;
+    while (!stack.empty()) {
+        const node &nd = t.nodes[stack.back()];
+        stack.pop_back();
+        if (nd.inner()) {
+            ++n_inner;
+            float dist_l, dist_r;
+            bool hit_l = intersect4(nd.l, ray, dist_l) && dist_l < closest.t;
+            bool hit_r = intersect4(nd.r, ray, dist_r) && dist_r < closest.t;
+            if (hit_l && hit_r) {
+                if (dist_l < dist_r) {
+                    stack.push_back((uint32_t)nd.link_r);
+                    stack.push_back((uint32_t)nd.link_l);
+                } else {
+                    stack.push_back((uint32_t)nd.link_l);
+                    stack.push_back((uint32_t)nd.link_r);
+                }
+            } else if (hit_l) {
+                stack.push_back((uint32_t)nd.link_l);
+            } else if (hit_r) {
+                stack.push_back((uint32_t)nd.link_r);
+            }
+        } else {
+            int32_t off = -nd.link_l, cnt = -nd.link_r;
+            for (int32_t i = 0; i < cnt; ++i) {
+                uint32_t tri = t.index[off + i];
+                ++n_tri;
+                if (intersect_tri(s, tri, ray, intersection) && intersection.t < closest.t) {
+                    closest = intersection;
+                    closest.ref = tri;
+                }
+            }
+        }
+    }
+    if (counts) {
+        counts[0] += n_inner;
+        counts[1] += n_tri;
+    }
+    return closest;
+}
+
+// binary_bvh_tracer::any_hit, rt/cpu/bvh2.cpp:140-189
+bool any_hit(const scene_view &s, const tree &t, const ray &ray) {
+    hit intersection{FLT_MAX, 0, 0, 0};
+    if (t.n_nodes == 0) return false;
+    std::vector<uint32_t> stack;
+    stack.reserve(64);
+    stack.push_back(t.root);
+    while (!stack.empty()) {
+        const node &nd = t.nodes[stack.back()];
+        stack.pop_back();
+        if (nd.inner()) {
+            float dist_l, dist_r;
+            bool hit_l = intersect4(nd.l, ray, dist_l);
+            bool hit_r = intersect4(nd.r, ray, dist_r);
+            if (hit_l && hit_r) {
+                if (dist_l < dist_r) {
+                    stack.push_back((uint32_t)nd.link_r);
+                    stack.push_back((uint32_t)nd.link_l);
+                } else {
+                    stack.push_back((uint32_t)nd.link_l);
+                    stack.push_back((uint32_t)nd.link_r);
+                }
+            } else if (hit_l) {
+                stack.push_back((uint32_t)nd.link_l);
+            } else if (hit_r) {
+                stack.push_back((uint32_t)nd.link_r);
+            }
+        } else {
+            int32_t off = -nd.link_l, cnt = -nd.link_r;
+            for (int32_t i = 0; i < cnt; ++i)
+                if (intersect_tri(s, t.index[off + i], ray, intersection)) return true;
+        }
+    }
+    return false;
+}
+
+// seq_tri_is, rt/cpu/seq.cpp:9-30
+hit seq_closest(const scene_view &s, const ray &ray) {
+    hit closest{FLT_MAX, 0, 0, 0}, intersection{FLT_MAX, 0, 0, 0};
+    for (uint32_t i = 0; i < s.n_tris; ++i)
+        if (intersect_tri(s, i, ray, intersection) && intersection.t < closest.t) {
+            closest = intersection;
+            closest.ref = i;
+        }
+    return closest;
+}
+bool seq_any(const scene_view &s, const ray &ray) {
+    hit intersection{FLT_MAX, 0, 0, 0};
+    for (uint32_t i = 0; i < s.n_tris; ++i)
+        if (intersect_tri(s, i, ray, intersection)) return true;
+    return false;
+}
+
+struct rays_in {
+    const float *p[8];   // ox oy oz tmin dx dy dz tmax
+    ray get(uint32_t i) const {
+        return make_ray(v3{p[0][i], p[1][i], p[2][i]}, v3{p[4][i], p[5][i], p[6][i]}, p[3][i], p[7][i]);
+    }
+};
+
+int set_threads(int threads) {
+    int n = threads > 0 ? threads : omp_get_max_threads();
+    return n;
+}
+
+} // namespace
+
+extern "C" {
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
+// Batch closest hit over `n` rays in rows of `row` rays, the OpenMP schedule
+// of batch_rt_adapter::compute_closest_hit.  counts (2 per ray) may be null.
+void orc_trace_closest(const float *pos, const uint32_t *idx, uint32_t n_tris, const void *nodes, uint32_t n_nodes,
+                       uint32_t root, const uint32_t *index, const float *const *ray_planes, uint32_t n, uint32_t row,
+                       float *t, float *u, float *v, uint32_t *id, uint32_t *counts, int threads) {
+    scene_view s{pos, idx, n_tris};
+    tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
+    rays_in R;
+    for (int k = 0; k < 8; ++k) R.p[k] = ray_planes[k];
+    if (row == 0) row = n ? n : 1;
+    int64_t rows = ((int64_t)n + row - 1) / row;
+#pragma omp parallel for schedule(static) num_threads(set_threads(threads))
+    for (int64_t y = 0; y < rows; ++y)
+        for (uint32_t x = 0; x < row; ++x) {
+            uint64_t i = (uint64_t)y * row + x;
+            if (i >= n) break;
+            uint32_t c[2] = {0, 0};
+            hit h = closest_hit(s, tr, R.get((uint32_t)i), counts ? c : nullptr);
+            t[i] = h.t;
+            u[i] = h.t != FLT_MAX ? h.beta : 0.0f;
+            v[i] = h.t != FLT_MAX ? h.gamma : 0.0f;
+            id[i] = h.ref;
+            if (counts) {
+                counts[2 * i] = c[0];
+                counts[2 * i + 1] = c[1];
+            }
+        }
+}
+
+// Batch any hit: occluded[i] = 1 when any triangle is hit in (t_min, t_max).
+void orc_trace_any(const float *pos, const uint32_t *idx, uint32_t n_tris, const void *nodes, uint32_t n_nodes,
+                   uint32_t root, const uint32_t *index, const float *const *ray_planes, uint32_t n, uint8_t *occluded,
+                   int threads) {
+    scene_view s{pos, idx, n_tris};
+    tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
+    rays_in R;
+    for (int k = 0; k < 8; ++k) R.p[k] = ray_planes[k];
+#pragma omp parallel for schedule(static) num_threads(set_threads(threads))
+    for (int64_t i = 0; i < (int64_t)n; ++i) occluded[i] = any_hit(s, tr, R.get((uint32_t)i)) ? 1 : 0;
+}
+
+// Brute force over all triangles.
+void orc_seq_closest(const float *pos, const uint32_t *idx, uint32_t n_tris, const float *const *ray_planes, uint32_t n,
+                     float *t, float *u, float *v, uint32_t *id, int threads) {
+    scene_view s{pos, idx, n_tris};
+    rays_in R;
+    for (int k = 0; k < 8; ++k) R.p[k] = ray_planes[k];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(set_threads(threads))
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        hit h = seq_closest(s, R.get((uint32_t)i));
+        t[i] = h.t;
+        u[i] = h.t != FLT_MAX ? h.beta : 0.0f;
+        v[i] = h.t != FLT_MAX ? h.gamma : 0.0f;
+        id[i] = h.ref;
+    }
+}
+
+void orc_seq_any(const float *pos, const uint32_t *idx, uint32_t n_tris, const float *const *ray_planes, uint32_t n,
+                 uint8_t *occluded, int threads) {
+    scene_view s{pos, idx, n_tris};
+    rays_in R;
+    for (int k = 0; k < 8; ++k) R.p[k] = ray_planes[k];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(set_threads(threads))
+    for (int64_t i = 0; i < (int64_t)n; ++i) occluded[i] = seq_any(s, R.get((uint32_t)i)) ? 1 : 0;
+}
+
+// FP64 intersection of ray i with triangle tri[i]: Cramer's rule on the fp32
+// inputs widened to double.  Writes t, beta, gamma (NaN when the system is
+// singular) and inside[i] = 1 when beta >= 0, gamma >= 0, beta + gamma <= 1
+// (closed, so edge hits count as inside).
+void orc_fp64_intersect(const float *pos, const uint32_t *idx, const uint32_t *tri, const float *const *ray_planes,
+                        uint32_t n, double *t, double *beta, double *gamma, uint8_t *inside) {
+    for (uint32_t r = 0; r < n; ++r) {
+        const float *const *P = ray_planes;
+        uint32_t tr = tri[r];
+        double A[3], B[3], C[3];
+        for (int k = 0; k < 3; ++k) {
+            A[k] = pos[3 * idx[3 * tr] + k];
+            B[k] = pos[3 * idx[3 * tr + 1] + k];
+            C[k] = pos[3 * idx[3 * tr + 2] + k];
+        }
+        double o[3] = {P[0][r], P[1][r], P[2][r]}, d[3] = {P[4][r], P[5][r], P[6][r]};
+        // Solve o + t d = A + beta (B-A) + gamma (C-A):
+        // [B-A, C-A, -d] (beta, gamma, t)^T = o - A
+        double e1[3], e2[3], q[3], m[3];
+        for (int k = 0; k < 3; ++k) {
+            e1[k] = B[k] - A[k];
+            e2[k] = C[k] - A[k];
+            q[k] = o[k] - A[k];
+            m[k] = -d[k];
+        }
+        auto det3 = [](const double *c0, const double *c1, const double *c2) {
+            return c0[0] * (c1[1] * c2[2] - c1[2] * c2[1]) - c1[0] * (c0[1] * c2[2] - c0[2] * c2[1]) +
+                   c2[0] * (c0[1] * c1[2] - c0[2] * c1[1]);
+        };
+        double D = det3(e1, e2, m);
+        if (D == 0.0) {
+            t[r] = beta[r] = gamma[r] = NAN;
+            inside[r] = 0;
+            continue;
+        }
+        double b = det3(q, e2, m) / D, g = det3(e1, q, m) / D, tt = det3(e1, e2, q) / D;
+        t[r] = tt;
+        beta[r] = b;
+        gamma[r] = g;
+        inside[r] = (b >= 0 && g >= 0 && b + g <= 1) ? 1 : 0;
+    }
+}
+
+// Camera rays of libgi/camera.cpp:44-61 for a w x h image, pixel-centre
+// offset 0, t_min 1e-4, t_max FLT_MAX; out planes as ray_planes.
+void orc_camera_rays(const float *pos, const float *dir, const float *up, float fovy, int w, int h,
+                     float *const *out) {
+    v3 P{pos[0], pos[1], pos[2]}, D{dir[0], dir[1], dir[2]}, Up{up[0], up[1], up[2]};
+    float aspect = float(w) / h;
+    float near_h = tanf(float(M_PI) * fovy * 0.5f / 180.0f);
+    float near_w = aspect * near_h;
+    v3 U = normalize(cross(D, Up));
+    v3 V = normalize(cross(U, D));
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            float uu = ((float)x + 0.5f + 0.0f) / (float)w * 2.0f - 1.0f;
+            float vv = ((float)y + 0.5f + 0.0f) / (float)h * 2.0f - 1.0f;
+            uu = near_w * uu;
+            vv = near_h * vv;
+            v3 d = normalize(add(add(D, mul(U, uu)), mul(V, vv)));
+            size_t i = (size_t)y * w + x;
+            out[0][i] = P.x;
+            out[1][i] = P.y;
+            out[2][i] = P.z;
+            out[3][i] = 1e-4f;
+            out[4][i] = d.x;
+            out[5][i] = d.y;
+            out[6][i] = d.z;
+            out[7][i] = FLT_MAX;
+        }
+}
+
+// Cosine-weighted bounce ray from hit (t, beta, gamma, id) of incoming ray i,
+// driven by the given uniform numbers xi0/xi1 (rt/cpu/bounce.cpp:41-72).
+// Rays without a hit get the reference's disabled ray: o = 0, d = (1,1,1),
+// t_max = -FLT_MAX.
+void orc_cosine_rays(const float *pos, const uint32_t *idx, const float *const *in_planes, const float *hit_t,
+                     const float *hit_u, const float *hit_v, const uint32_t *hit_id, const float *xi0,
+                     const float *xi1, uint32_t n, float *const *out) {
+    const float pi = float(M_PI);
+    for (uint32_t i = 0; i < n; ++i) {
+        float o[3] = {0, 0, 0}, d[3] = {1, 1, 1}, tmax = -FLT_MAX;
+        if (hit_t[i] != FLT_MAX) {
+            uint32_t tr = hit_id[i];
+            v3 A{pos[3 * idx[3 * tr]], pos[3 * idx[3 * tr] + 1], pos[3 * idx[3 * tr] + 2]};
+            v3 B{pos[3 * idx[3 * tr + 1]], pos[3 * idx[3 * tr + 1] + 1], pos[3 * idx[3 * tr + 1] + 2]};
+            v3 C{pos[3 * idx[3 * tr + 2]], pos[3 * idx[3 * tr + 2] + 1], pos[3 * idx[3 * tr + 2] + 2]};
+            float beta = hit_u[i], gamma = hit_v[i];
+            v3 x = add(add(smul(1.0f - beta - gamma, A), smul(beta, B)), smul(gamma, C));
+            v3 ng = normalize(cross(sub(B, A), sub(C, A)));
+            v3 rd{in_planes[4][i], in_planes[5][i], in_planes[6][i]};
+            if (dot(rd, ng) > 0) ng = v3{-ng.x, -ng.y, -ng.z};
+            // uniform_sample_disk + cosine_sample_hemisphere
+            float r = sqrtf(xi0[i]);
+            float theta = 2 * pi * xi1[i];
+            float dx = r * cosf(theta), dy = r * sinf(theta);
+            float z = 1 - dx * dx - dy * dy;
+            v3 s{dx, dy, z > 0 ? sqrtf(z) : 0.f};
+            // align(s, ng)
+            float sg = copysignf(1.f, ng.z);
+            v3 w{s.x, s.y, s.z * sg};
+            v3 hv{ng.x, ng.y, ng.z + sg};
+            float k = dot(w, hv) / (1.f + (ng.z < 0 ? -ng.z : ng.z));
+            v3 wi = sub(smul(k, hv), w);
+            o[0] = x.x;
+            o[1] = x.y;
+            o[2] = x.z;
+            d[0] = wi.x;
+            d[1] = wi.y;
+            d[2] = wi.z;
+            tmax = FLT_MAX;
+        }
+        out[0][i] = o[0];
+        out[1][i] = o[1];
+        out[2][i] = o[2];
+        out[3][i] = 1e-4f;
+        out[4][i] = d[0];
+        out[5][i] = d[1];
+        out[6][i] = d[2];
+        out[7][i] = tmax;
+    }
+}
+
+// Wall-clock seconds of one orc_trace_closest call (the CPU baseline leg).
+double orc_time_closest(const float *pos, const uint32_t *idx, uint32_t n_tris, const void *nodes, uint32_t n_nodes,
+                        uint32_t root, const uint32_t *index, const float *const *ray_planes, uint32_t n, uint32_t row,
+                        float *t, float *u, float *v, uint32_t *id, int threads) {
+    double t0 = omp_get_wtime();
+    orc_trace_closest(pos, idx, n_tris, nodes, n_nodes, root, index, ray_planes, n, row, t, u, v, id, nullptr,
+                      threads);
+    return omp_get_wtime() - t0;
+}
+
+} // extern "C"

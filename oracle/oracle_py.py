@@ -1,0 +1,162 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker.  The product never imports it.
+
+Scenes are (pos (nv,3) float32, idx (nt,3) uint32); trees are the reference's
+bbvh_node records (numpy structured array, 56 B each) with a root and the
+leaf-slot index; rays are (8, n) float32 planes ox, oy, oz, tmin, dx, dy, dz,
+tmax.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_uint32, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+FLT_MAX = np.float32(3.4028234663852886e38)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run make in oracle/")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_max_threads.restype = c_int
+        L.orc_time_closest.restype = c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _planes(rays):
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    arr = (c_void_p * 8)(*[rays[k].ctypes.data for k in range(8)])
+    return rays, arr
+
+
+def _scene(pos, idx):
+    return (np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3),
+            np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3))
+
+
+def trace_closest(pos, idx, nodes, root, index, rays, row=0, threads=0, counts=False):
+    """binary_bvh_tracer::closest_hit over every ray. Returns t, u, v, id[, counts (n,2)]."""
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    t = np.empty(n, np.float32)
+    u = np.empty(n, np.float32)
+    v = np.empty(n, np.float32)
+    i = np.empty(n, np.uint32)
+    c = np.zeros((n, 2), np.uint32) if counts else None
+    nodes = np.ascontiguousarray(nodes)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    lib().orc_trace_closest(_p(pos), _p(idx), c_uint32(len(idx)), _p(nodes), c_uint32(len(nodes)), c_uint32(root),
+                            _p(index), planes, c_uint32(n), c_uint32(row), _p(t), _p(u), _p(v), _p(i), _p(c),
+                            c_int(threads))
+    return (t, u, v, i, c) if counts else (t, u, v, i)
+
+
+def trace_any(pos, idx, nodes, root, index, rays, threads=0):
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    occ = np.empty(n, np.uint8)
+    nodes = np.ascontiguousarray(nodes)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    lib().orc_trace_any(_p(pos), _p(idx), c_uint32(len(idx)), _p(nodes), c_uint32(len(nodes)), c_uint32(root),
+                        _p(index), planes, c_uint32(n), _p(occ), c_int(threads))
+    return occ
+
+
+def seq_closest(pos, idx, rays, threads=0):
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    t = np.empty(n, np.float32)
+    u = np.empty(n, np.float32)
+    v = np.empty(n, np.float32)
+    i = np.empty(n, np.uint32)
+    lib().orc_seq_closest(_p(pos), _p(idx), c_uint32(len(idx)), planes, c_uint32(n), _p(t), _p(u), _p(v), _p(i),
+                          c_int(threads))
+    return t, u, v, i
+
+
+def seq_any(pos, idx, rays, threads=0):
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    occ = np.empty(n, np.uint8)
+    lib().orc_seq_any(_p(pos), _p(idx), c_uint32(len(idx)), planes, c_uint32(n), _p(occ), c_int(threads))
+    return occ
+
+
+def fp64_intersect(pos, idx, tri, rays):
+    """Double-precision intersection of ray i with triangle tri[i]: t, beta, gamma, inside."""
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    tri = np.ascontiguousarray(tri, dtype=np.uint32)
+    t = np.empty(n, np.float64)
+    b = np.empty(n, np.float64)
+    g = np.empty(n, np.float64)
+    ins = np.empty(n, np.uint8)
+    lib().orc_fp64_intersect(_p(pos), _p(idx), _p(tri), planes, c_uint32(n), _p(t), _p(b), _p(g), _p(ins))
+    return t, b, g, ins
+
+
+def camera_rays(pos, dir, up, fovy, w, h):
+    out = np.empty((8, w * h), np.float32)
+    arr = (c_void_p * 8)(*[out[k].ctypes.data for k in range(8)])
+    P = np.asarray(pos, np.float32)
+    D = np.asarray(dir, np.float32)
+    U = np.asarray(up, np.float32)
+    lib().orc_camera_rays(_p(P), _p(D), _p(U), c_float(fovy), c_int(w), c_int(h), arr)
+    return out
+
+
+def cosine_rays(pos, idx, in_rays, t, u, v, ids, xi0, xi1):
+    pos, idx = _scene(pos, idx)
+    in_rays, inp = _planes(in_rays)
+    n = in_rays.shape[1]
+    out = np.empty((8, n), np.float32)
+    arr = (c_void_p * 8)(*[out[k].ctypes.data for k in range(8)])
+    t = np.ascontiguousarray(t, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    xi0 = np.ascontiguousarray(xi0, np.float32)
+    xi1 = np.ascontiguousarray(xi1, np.float32)
+    lib().orc_cosine_rays(_p(pos), _p(idx), inp, _p(t), _p(u), _p(v), _p(ids), _p(xi0), _p(xi1), c_uint32(n), arr)
+    return out
+
+
+def time_closest(pos, idx, nodes, root, index, rays, row=0, threads=0):
+    """Seconds of one batch closest-hit call (OpenMP over rows)."""
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    t = np.empty(n, np.float32)
+    u = np.empty(n, np.float32)
+    v = np.empty(n, np.float32)
+    i = np.empty(n, np.uint32)
+    nodes = np.ascontiguousarray(nodes)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    return lib().orc_time_closest(_p(pos), _p(idx), c_uint32(len(idx)), _p(nodes), c_uint32(len(nodes)),
+                                  c_uint32(root), _p(index), planes, c_uint32(n), c_uint32(row), _p(t), _p(u), _p(v),
+                                  _p(i), c_int(threads))
+
+
+def max_threads():
+    return lib().orc_max_threads()
