@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: closest-hit Mrays/s of the 4-bounce diffuse path tracer on the
+262,144-triangle hall at 1920x1080 -- batched (persistent kernel, compacted
+queues, hipGraph replay) against plain per-bounce launches.
+
+One step = one path-traced frame: camera rays plus up to four cosine-weighted
+diffuse bounces (Russian roulette p = 0.8 at bounces 3 and 4), every level's
+rays traced by the gfx950 closest-hit kernel.  `value` counts every traced
+ray of the timed frames over their wall time on the GPU stream (inputs -- the
+flattened BVH and triangle records -- already resident in HBM).
+
+Prints one JSON line.  Also reported: the plain schedule's rate and the ratio,
+the trace kernels alone on primary and on diffuse rays, the roofline of the
+dominant kernel (algorithmic bytes per launch from the counting kernel
+variant, time from HIP events on the launch stream) and the CPU oracle's
+throughput on a bounded sample of the same diffuse rays.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s closest-hit, 262k-tri scene, diffuse rays: batched vs plain kernel"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_frames(torch, pt, steps, warmup, stream):
+    for f in range(warmup):
+        pt.render(f, stream.cuda_stream)
+    torch.cuda.synchronize()
+    before = pt.ray_totals()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for f in range(steps):
+        pt.render(warmup + f, stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    after = pt.ray_totals()
+    ms = e0.elapsed_time(e1) / steps
+    rays = float((after - before).sum()) / steps
+    return ms, rays, (after - before) / steps
+
+
+def time_trace(torch, hiprt, ds, rays, tracer, stream, reps=10):
+    n = rays.shape[1]
+    hits = hiprt.new_hits(n)
+    for _ in range(2):
+        hiprt.trace_closest(ds, rays, hits, tracer=tracer, stream=stream.cuda_stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        hiprt.trace_closest(ds, rays, hits, tracer=tracer, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps, hits
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays are independent, so "
+              "the path shards by image tiles across GPUs; see the sharding paragraph of the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "illum-rt_amd"))
+    import hiprt
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    stream = torch.cuda.current_stream()
+    t0 = time.time()
+    ctx = hiprt.Context(0)
+    sc = hiprt.HostScene.generate(3)
+    pos, idx, _ = sc.arrays()
+    bvh = hiprt.Bvh(sc)
+    ds = hiprt.DeviceScene(ctx, sc, bvh)
+    cam = sc.cam
+    w, h = int(cam[10]), int(cam[11])
+    log(f"scene: {len(idx)} triangles, {ds.n_nodes} inner nodes, {ds.n_slots} leaf slots, depth {ds.depth}; "
+        f"setup {time.time() - t0:.1f}s")
+
+    batched = hiprt.PathTracer(ds, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True)
+    plain = hiprt.PathTracer(ds, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=False)
+    ms_b, rays_b, per_level_b = time_frames(torch, batched, args.steps, args.warmup, stream)
+    log(f"batched: {ms_b:.3f} ms/frame, {rays_b / 1e6:.3f} Mrays/frame, per level {per_level_b.tolist()}")
+    ms_p, rays_p, per_level_p = time_frames(torch, plain, args.steps, args.warmup, stream)
+    log(f"plain:   {ms_p:.3f} ms/frame, {rays_p / 1e6:.3f} Mrays/frame")
+    value = rays_b / ms_b / 1e3
+    plain_value = rays_p / ms_p / 1e3
+
+    # Dominant kernel: the persistent closest-hit trace.  Per level: HIP-event
+    # time of the launch, and the counting variant's fetches on the same rays.
+    ms_lvl, fetch, rays_lvl = batched.profile(frame=10_000, stream=stream.cuda_stream)
+    bytes_lvl = 48.0 * rays_lvl + 64.0 * fetch[:, 0] + 48.0 * fetch[:, 1]
+    achieved = float(bytes_lvl.sum()) / (float(ms_lvl.sum()) * 1e-3) / 1e9
+    launches = int((rays_lvl > 0).sum())
+    footprint = ds.n_nodes * 64 + ds.n_slots * 48
+    compulsory = float(48.0 * rays_lvl.sum() + footprint * launches) / (float(ms_lvl.sum()) * 1e-3) / 1e9
+    log(f"trace per level ms {np.round(ms_lvl, 3).tolist()}, nodes/ray "
+        f"{np.round(fetch[:, 0] / np.maximum(rays_lvl, 1), 1).tolist()}, tris/ray "
+        f"{np.round(fetch[:, 1] / np.maximum(rays_lvl, 1), 1).tolist()}")
+
+    # The trace kernels alone: primary rays and first-bounce diffuse rays.
+    d_cam = torch.empty((8, w * h), dtype=torch.float32, device="cuda")
+    hiprt.camera_rays_device(cam, d_cam, stream.cuda_stream)
+    kernels = {}
+    ms, hits = time_trace(torch, hiprt, ds, d_cam, "while-while", stream)
+    d_diff = torch.empty_like(d_cam)
+    hiprt.bounce_cosine(ds, d_cam, hits, d_diff, seed=12345, level=0, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    for name, rays in (("primary", d_cam), ("diffuse", d_diff)):
+        for tr in ("while-while", "dynamic-while-while"):
+            ms, _ = time_trace(torch, hiprt, ds, rays, tr, stream)
+            kernels[f"{name}/{tr}"] = round(rays.shape[1] / ms / 1e3, 1)
+    log(f"trace kernels alone (Mrays/s): {kernels}")
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as orc
+        threads = min(16, os.cpu_count() or 1)
+        sample_n = 500_000
+        sample = d_diff[:, :sample_n].cpu().numpy()
+        orc.time_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, sample[:, :50_000], row=w, threads=threads)
+        secs = sorted(orc.time_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, sample, row=w, threads=threads)
+                      for _ in range(3))
+        one = orc.time_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, sample[:, :50_000], row=w, threads=1)
+        cpu = {"value": round(sample_n / secs[1] / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"first {sample_n} first-bounce cosine rays of the 1920x1080 frame, CPU restatement of "
+                         f"binary_bvh_tracer (same BVH), OpenMP over rows, median of 3; 1 thread on 50k: "
+                         f"{50_000 / one / 1e6:.2f} Mrays/s"}
+        log(f"cpu baseline: {cpu}")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "trace_hall_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_b, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "hall 262,144 triangles, 1920x1080, 4-bounce cosine path tracing, "
+                               "RR p=0.8 at bounces 3-4, batched persistent+graph",
+                   "triangles": int(len(idx)), "width": w, "height": h, "max_depth": 4,
+                   "bvh": "SAH 16 planes, 4 tris/leaf, ESC", "rays_per_frame": int(round(rays_b))},
+        "plain_value": round(plain_value, 1),
+        "plain_ms_per_step": round(ms_p, 4),
+        "batched_over_plain": round(value / plain_value, 3),
+        "trace_kernels_mrays": kernels,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_trace_persistent (dynamic-while-while)",
+                     "avg_launch_ms": round(float(ms_lvl.sum()) / max(launches, 1), 4),
+                     "bytes_per_launch": round(float(bytes_lvl.sum()) / max(launches, 1)),
+                     "compulsory_GBs": round(compulsory, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    batched.close()
+    plain.close()
+    ds.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

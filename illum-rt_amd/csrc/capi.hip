@@ -5,16 +5,10 @@
 // increase along every edge (so traversal terminates) and its depth fits the
 // chosen kernel's stack, and ray/hit pointers must be non-null for every
 // plane the kernel reads or writes.
-#include "../../include/illum_hiprt.h"
+#include "capi_internal.h"
 
-#include "core/layout.h"
 #include "core/raygen.h"
 #include "device/kernels.h"
-#include "host/bvh.h"
-#include "host/flatten.h"
-#include "host/scene.h"
-
-#include <hip/hip_runtime.h>
 
 #include <cstring>
 #include <memory>
@@ -23,68 +17,19 @@
 #include <vector>
 
 namespace {
-
 thread_local std::string g_err;
+}
 
+namespace hiprt_detail {
 int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
-int ok() { return HIPRT_OK; }
+} // namespace hiprt_detail
 
-#define HIP_TRY(call)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (call);                                                                    \
-        if (e_ != hipSuccess) return fail(HIPRT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-// Owned device allocation.
-struct dbuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    dbuf() = default;
-    dbuf(const dbuf &) = delete;
-    dbuf &operator=(const dbuf &) = delete;
-    ~dbuf() { release(); }
-    void release() {
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t alloc(size_t n) {
-        release();
-        bytes = n;
-        return n ? hipMalloc(&p, n) : hipSuccess;
-    }
-    hipError_t upload(const void *src, size_t n) {
-        hipError_t e = alloc(n);
-        if (e != hipSuccess || n == 0) return e;
-        return hipMemcpy(p, src, n, hipMemcpyHostToDevice);
-    }
-};
-
-} // namespace
-
-struct hiprt_host_scene_s {
-    rt::scene sc;
-};
-
-struct hiprt_bvh_s {
-    rt::bvh b;
-    rt::bvh_stats st;
-};
-
-struct hiprt_context_s {
-    int device = 0;
-    dbuf counter;   // work counter of the persistent kernels
-};
-
-struct hiprt_scene_s {
-    hiprt_context ctx = nullptr;
-    rt::flat_scene host;   // host copy of the uploaded layout
-    dbuf nodes, tris, slot_ids;
-    rt::scene_desc desc{};
-};
+using hiprt_detail::dbuf;
+using hiprt_detail::fail;
+using hiprt_detail::ok;
 
 extern "C" {
 
@@ -373,7 +318,36 @@ int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32
     rt::flat_scene f;
     std::string e = rt::flatten(s->sc, b->b, bvh_format, tri_format, f);
     if (!e.empty()) return fail(HIPRT_ERR_UNSUPPORTED, e);
-    return upload_flat(ctx, std::move(f), out);
+    hiprt_scene ds = nullptr;
+    int st = upload_flat(ctx, std::move(f), &ds);
+    if (st != HIPRT_OK) return st;
+    std::unique_ptr<hiprt_scene_s> guard(ds);
+    // Shading data: per triangle A.xyz + material id, B.xyz, C.xyz; per
+    // material albedo and emission.
+    const rt::scene &sc = s->sc;
+    uint32_t nt = sc.n_tris();
+    std::vector<float> geo(12ull * nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        rt::vec3f A = sc.a(t), B = sc.b(t), C = sc.c(t);
+        uint32_t m = sc.mat.empty() ? 0u : sc.mat[t];
+        float mb;
+        std::memcpy(&mb, &m, 4);
+        float v[12] = {A.x, A.y, A.z, mb, B.x, B.y, B.z, 0.0f, C.x, C.y, C.z, 0.0f};
+        std::memcpy(&geo[12ull * t], v, sizeof v);
+    }
+    std::vector<float> mats(8 * (sc.mats.empty() ? 1 : sc.mats.size()), 0.0f);
+    for (size_t m = 0; m < sc.mats.size(); ++m) {
+        const rt::material &mt = sc.mats[m];
+        float v[8] = {mt.albedo.x, mt.albedo.y, mt.albedo.z, 0.0f, mt.emissive.x, mt.emissive.y, mt.emissive.z, 0.0f};
+        std::memcpy(&mats[8 * m], v, sizeof v);
+    }
+    for (uint32_t m : sc.mat)
+        if (m >= sc.mats.size()) return fail(HIPRT_ERR_ARG, "triangle material id out of range");
+    HIP_TRY(ds->tri_geo.upload(geo.data(), geo.size() * 4));
+    HIP_TRY(ds->mats.upload(mats.data(), mats.size() * 4));
+    ds->n_tris = nt;
+    *out = guard.release();
+    return ok();
 }
 
 int hiprt_scene_upload(hiprt_context ctx, const void *nodes, uint32_t n_nodes, int32_t bvh_format,
@@ -480,7 +454,9 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     if (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits) {
         if (depth > rt::dev::kPlainStackCap - 1)
             return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack (" + std::to_string(depth) + " levels)");
-        HIP_TRY(rt::dev::launch_trace_plain(sc->desc, r, h, occluded, n, any, s));
+        HIP_TRY(rt::dev::launch_trace_plain(sc->desc, r, h, occluded, n, any, s,
+                                            opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats)
+                                                 : nullptr));
         return ok();
     }
     rt::dev::persistent_cfg cfg{};
@@ -489,6 +465,7 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.refill_thr = (opts && opts->refill_threshold > 0) ? opts->refill_threshold : 40;
     if (cfg.refill_thr > 64) cfg.refill_thr = 64;
     cfg.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
+    cfg.stats = opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats) : nullptr;
     HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
                                              static_cast<uint32_t *>(sc->ctx->counter.p), any, cfg, s));
     return ok();

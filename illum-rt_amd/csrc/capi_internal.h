@@ -1,0 +1,78 @@
+// Internal state behind the C ABI's opaque handles, shared by the .hip files
+// that implement include/illum_hiprt.h.
+#pragma once
+
+#include "../../include/illum_hiprt.h"
+
+#include "core/layout.h"
+#include "host/bvh.h"
+#include "host/flatten.h"
+#include "host/scene.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace hiprt_detail {
+
+// Records `msg` as the calling thread's last error and returns `code`.
+int fail(int code, const std::string &msg);
+inline int ok() { return HIPRT_OK; }
+
+#define HIP_TRY(call)                                                                                          \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess)                                                                                  \
+            return ::hiprt_detail::fail(HIPRT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// Owned device allocation.
+struct dbuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    dbuf() = default;
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    ~dbuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        bytes = n;
+        return n ? hipMalloc(&p, n) : hipSuccess;
+    }
+    hipError_t upload(const void *src, size_t n) {
+        hipError_t e = alloc(n);
+        if (e != hipSuccess || n == 0) return e;
+        return hipMemcpy(p, src, n, hipMemcpyHostToDevice);
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+} // namespace hiprt_detail
+
+struct hiprt_host_scene_s {
+    rt::scene sc;
+};
+
+struct hiprt_bvh_s {
+    rt::bvh b;
+    rt::bvh_stats st;
+};
+
+struct hiprt_context_s {
+    int device = 0;
+    hiprt_detail::dbuf counter;   // work counter of the persistent kernels
+};
+
+struct hiprt_scene_s {
+    hiprt_context ctx = nullptr;
+    rt::flat_scene host;                         // host copy of the uploaded layout
+    hiprt_detail::dbuf nodes, tris, slot_ids;    // traversal data
+    hiprt_detail::dbuf tri_geo, mats;            // shading data (scenes committed from a host scene)
+    uint32_t n_tris = 0;
+    rt::scene_desc desc{};
+};

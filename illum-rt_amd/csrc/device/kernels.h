@@ -22,8 +22,10 @@ constexpr int kLdsStack = 28;
 constexpr int kPersistentStackCap = kRegStack + kLdsStack;
 constexpr int kPersistentWavesPerBlock = 4;
 
+// With `stats` non-null the counting variant runs and adds the number of inner
+// nodes fetched to stats[0] and of triangle records tested to stats[1].
 hipError_t launch_trace_plain(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
-                              uint32_t n, bool any, hipStream_t stream);
+                              uint32_t n, bool any, hipStream_t stream, unsigned long long *stats = nullptr);
 
 // Tunables of the persistent kernel.
 struct persistent_cfg {
@@ -31,6 +33,7 @@ struct persistent_cfg {
     int refill;           // fetch new rays when active lanes drop below refill_thr
     int refill_thr;       // active-lane threshold for refill (out of 64)
     int blocks;           // grid size; 0 = occupancy-derived default
+    unsigned long long *stats = nullptr;   // non-null: counting variant (inner nodes, triangle tests)
 };
 
 // `counter` is one device uint32 that must be zero at launch; `count` is the
@@ -39,6 +42,10 @@ struct persistent_cfg {
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
                                    uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,
                                    bool any, const persistent_cfg &cfg, hipStream_t stream);
+
+// Grid size (blocks) of the persistent kernel instantiation for these
+// settings: blocks resident per CU times the CU count.
+hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks);
 
 // Fills `n` uint32 words with `value` (used instead of hipMemsetAsync so the
 // clearing can be captured into graphs).

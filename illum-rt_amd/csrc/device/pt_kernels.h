@@ -1,0 +1,50 @@
+// Types and launchers of the wavefront path-tracing kernels (device/pt.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../core/layout.h"
+#include "../core/raygen.h"
+
+#include <cstdint>
+
+namespace rt {
+namespace dev {
+
+// Writable ray planes.
+struct ray_out {
+    float *ox, *oy, *oz, *tmin, *dx, *dy, *dz, *tmax;
+    ray_soa as_in() const { return ray_soa{ox, oy, oz, tmin, dx, dy, dz, tmax}; }
+};
+
+// Shading data of a device scene: three float4 per triangle
+// (A.xyz + material id bits, B.xyz, C.xyz) and two float4 per material
+// (albedo, emission).
+struct pt_scene {
+    const float4 *tri_geo;
+    const float4 *mats;
+};
+
+struct pt_params {
+    int max_depth;      // last trace level (primary = 0)
+    int rr_start;       // first level whose ray is subject to Russian roulette
+    float rr_survive;   // survival probability
+};
+
+hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
+                              bool clear_radiance, hipStream_t s);
+hipError_t launch_shade_plain(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
+                              const ray_out &rays, const hit_soa &hits, float *thr, float *rad, uint32_t *counts,
+                              uint32_t n, hipStream_t s);
+hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
+                                const ray_out &rays_in, const hit_soa &hits, const uint32_t *path_in,
+                                const ray_out &rays_out, uint32_t *path_out, float *thr, float *rad,
+                                uint32_t *counts, uint32_t n_max, hipStream_t s);
+hipError_t launch_set_u64(uint64_t *p, uint64_t v, hipStream_t s);
+// totals[i] += counts[i] for i < levels.
+hipError_t launch_add_counts(unsigned long long *totals, const uint32_t *counts, int levels, hipStream_t s);
+hipError_t launch_cosine_bounce(const pt_scene &ps, const ray_soa &in, const hit_soa &hits, const ray_out &out,
+                                uint64_t seed, int level, uint32_t n, hipStream_t s);
+
+} // namespace dev
+} // namespace rt

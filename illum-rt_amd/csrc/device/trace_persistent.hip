@@ -67,11 +67,11 @@ struct lane_stack {
 
 } // namespace
 
-template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG>
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits,
                                                           uint8_t *occluded, uint32_t n_static,
                                                           const uint32_t *count_ptr, uint32_t *counter,
-                                                          int refill_thr) {
+                                                          int refill_thr, unsigned long long *stats) {
     __shared__ int32_t lds_stack[kPersistentWavesPerBlock * kLdsStack * 64];
     const uint32_t n = count_ptr ? *count_ptr : n_static;
     const int lane = threadIdx.x & 63;
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
     dhit h{};
     bool alive = false;
     bool exhausted = false;   // wave-uniform: the counter has run past n
+    uint32_t c_nodes = 0, c_tris = 0;
 
     for (;;) {
         // ---- fetch: one atomic per wave for all idle lanes ----
@@ -122,6 +123,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
             for (;;) {
                 bool searching = true;
                 while (cur >= 0 && cur != kInvalidRef) {
+                    if (COUNT) ++c_nodes;
                     const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
                     float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
                     float dl, dr;
@@ -159,6 +161,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                     uint32_t slot = leaf_slot(leaf), cnt = leaf_count(leaf);
                     for (uint32_t k = 0; k < cnt; ++k) {
                         float t, u, v;
+                        if (COUNT) ++c_tris;
                         if (test_slot<TRI>(sc, slot + k, r, h.t, t, u, v)) {
                             h.found = true;
                             h.slot = slot + k;
@@ -196,6 +199,10 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
             }
         }
     }
+    if (COUNT) {
+        atomicAdd(stats, (unsigned long long)c_nodes);
+        atomicAdd(stats + 1, (unsigned long long)c_tris);
+    }
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t value, uint32_t n) {
@@ -211,32 +218,54 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t 
 
 namespace {
 
+// Blocks of one instantiation that fit the device at once (occupancy x CUs),
+// asked once per device.
+template <int TRI, bool ANY, bool SPEC, bool REFILL>
+hipError_t grid_of(int *blocks) {
+    static int cached[64] = {0};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (cached[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, false>,
+                                                         256, 0);
+        if (e != hipSuccess) return e;
+        cached[dev] = cus * (per_cu > 0 ? per_cu : 1);
+    }
+    *blocks = cached[dev];
+    return hipSuccess;
+}
+
 template <int TRI, bool ANY, bool SPEC, bool REFILL>
 hipError_t launch_variant(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
                           uint32_t n, const uint32_t *count_ptr, uint32_t *counter, const persistent_cfg &cfg,
                           hipStream_t stream) {
-    auto kern = k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack>;
     int blocks = cfg.blocks;
     if (blocks <= 0) {
-        // Occupancy of one instantiation does not change; ask once per device.
-        static int cached_blocks[64] = {0};
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
+        hipError_t e = grid_of<TRI, ANY, SPEC, REFILL>(&blocks);
         if (e != hipSuccess) return e;
-        if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
-        if (cached_blocks[dev] == 0) {
-            int cus = 0, per_cu = 0;
-            e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            if (e != hipSuccess) return e;
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0);
-            if (e != hipSuccess) return e;
-            cached_blocks[dev] = cus * (per_cu > 0 ? per_cu : 1);
-        }
-        blocks = cached_blocks[dev];
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, stream, sc, rays, hits, occluded, n, count_ptr, counter,
-                       cfg.refill_thr);
+    if (cfg.stats)
+        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, true>), dim3(blocks), dim3(256), 0,
+                           stream, sc, rays, hits, occluded, n, count_ptr, counter, cfg.refill_thr, cfg.stats);
+    else
+        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, false>), dim3(blocks), dim3(256), 0,
+                           stream, sc, rays, hits, occluded, n, count_ptr, counter, cfg.refill_thr, cfg.stats);
     return hipGetLastError();
+}
+
+template <int TRI, bool ANY>
+hipError_t grid_tri_any(const persistent_cfg &cfg, int *blocks) {
+    if (cfg.speculative) {
+        if (cfg.refill) return grid_of<TRI, ANY, true, true>(blocks);
+        return grid_of<TRI, ANY, true, false>(blocks);
+    }
+    if (cfg.refill) return grid_of<TRI, ANY, false, true>(blocks);
+    return grid_of<TRI, ANY, false, false>(blocks);
 }
 
 template <int TRI, bool ANY>
@@ -252,6 +281,11 @@ hipError_t launch_tri_any(const scene_desc &sc, const ray_soa &rays, const hit_s
 }
 
 } // namespace
+
+hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks) {
+    if (sc.tri == TRI_EXACT) return any ? grid_tri_any<TRI_EXACT, true>(cfg, blocks) : grid_tri_any<TRI_EXACT, false>(cfg, blocks);
+    return any ? grid_tri_any<TRI_WOOP, true>(cfg, blocks) : grid_tri_any<TRI_WOOP, false>(cfg, blocks);
+}
 
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
                                    uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,

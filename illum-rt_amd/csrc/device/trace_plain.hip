@@ -14,8 +14,9 @@
 namespace rt {
 namespace dev {
 
-template <int TRI, bool ANY>
-__device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray &r, dhit &h) {
+template <int TRI, bool ANY, bool COUNT>
+__device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray &r, dhit &h, uint32_t &n_nodes,
+                                               uint32_t &n_tris) {
     int32_t stack[kPlainStackCap];
     int sp = 0;
     stack[0] = kInvalidRef;
@@ -25,6 +26,7 @@ __device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray 
     h.t = ANY ? INFINITY : FLT_MAX;
     while (cur != kInvalidRef) {
         while (cur >= 0 && cur != kInvalidRef) {
+            if (COUNT) ++n_nodes;
             const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
             float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
             float dl, dr;
@@ -48,6 +50,7 @@ __device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray 
         uint32_t slot = leaf_slot(cur), cnt = leaf_count(cur);
         for (uint32_t k = 0; k < cnt; ++k) {
             float t, u, v;
+            if (COUNT) ++n_tris;
             if (test_slot<TRI>(sc, slot + k, r, h.t, t, u, v)) {
                 h.found = true;
                 h.slot = slot + k;
@@ -61,14 +64,19 @@ __device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray 
     }
 }
 
-template <int TRI, bool ANY>
+template <int TRI, bool ANY, bool COUNT>
 __global__ __launch_bounds__(256) void k_trace_plain(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occluded,
-                                                     uint32_t n) {
+                                                     uint32_t n, unsigned long long *stats) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     dray r = load_ray(rays, i);
     dhit h;
-    traverse_plain<TRI, ANY>(sc, r, h);
+    uint32_t c_nodes = 0, c_tris = 0;
+    traverse_plain<TRI, ANY, COUNT>(sc, r, h, c_nodes, c_tris);
+    if (COUNT) {
+        atomicAdd(stats, (unsigned long long)c_nodes);
+        atomicAdd(stats + 1, (unsigned long long)c_tris);
+    }
     if (ANY) {
         if (occluded) occluded[i] = h.found ? 1 : 0;
         // The reference's CPU adapter marks an occluded ray with
@@ -81,16 +89,25 @@ __global__ __launch_bounds__(256) void k_trace_plain(scene_desc sc, ray_soa rays
     }
 }
 
-hipError_t launch_trace_plain(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
-                              uint32_t n, bool any, hipStream_t stream) {
-    if (n == 0) return hipSuccess;
+namespace {
+template <int TRI, bool ANY>
+void launch_tri_any(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded, uint32_t n,
+                    unsigned long long *stats, hipStream_t stream) {
     dim3 grid((n + 255u) / 256u), block(256);
+    if (stats) hipLaunchKernelGGL((k_trace_plain<TRI, ANY, true>), grid, block, 0, stream, sc, rays, hits, occluded, n, stats);
+    else hipLaunchKernelGGL((k_trace_plain<TRI, ANY, false>), grid, block, 0, stream, sc, rays, hits, occluded, n, stats);
+}
+} // namespace
+
+hipError_t launch_trace_plain(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
+                              uint32_t n, bool any, hipStream_t stream, unsigned long long *stats) {
+    if (n == 0) return hipSuccess;
     if (sc.tri == TRI_EXACT) {
-        if (any) hipLaunchKernelGGL((k_trace_plain<TRI_EXACT, true>), grid, block, 0, stream, sc, rays, hits, occluded, n);
-        else hipLaunchKernelGGL((k_trace_plain<TRI_EXACT, false>), grid, block, 0, stream, sc, rays, hits, occluded, n);
+        if (any) launch_tri_any<TRI_EXACT, true>(sc, rays, hits, occluded, n, stats, stream);
+        else launch_tri_any<TRI_EXACT, false>(sc, rays, hits, occluded, n, stats, stream);
     } else {
-        if (any) hipLaunchKernelGGL((k_trace_plain<TRI_WOOP, true>), grid, block, 0, stream, sc, rays, hits, occluded, n);
-        else hipLaunchKernelGGL((k_trace_plain<TRI_WOOP, false>), grid, block, 0, stream, sc, rays, hits, occluded, n);
+        if (any) launch_tri_any<TRI_WOOP, true>(sc, rays, hits, occluded, n, stats, stream);
+        else launch_tri_any<TRI_WOOP, false>(sc, rays, hits, occluded, n, stats, stream);
     }
     return hipGetLastError();
 }

@@ -1,0 +1,323 @@
+// C ABI of the wavefront steps and the wavefront path tracer.
+//
+// The batched schedule records one frame -- counter resets, camera rays, and
+// per level a persistent trace that reads the live-ray count from device
+// memory followed by a compacting shade step -- into a hipGraph once, and
+// replays it per frame.  Counters are cleared by fill kernels, never by
+// memset nodes, and the frame's random seed is set by a one-thread kernel
+// launched before the graph, so nothing host-side changes between replays.
+#include "capi_internal.h"
+
+#include "core/raygen.h"
+#include "device/kernels.h"
+#include "device/pt_kernels.h"
+
+#include <memory>
+#include <new>
+#include <vector>
+
+using hiprt_detail::dbuf;
+using hiprt_detail::fail;
+using hiprt_detail::ok;
+
+namespace {
+
+rt::cam_frame frame_from(const float *cam) {
+    return rt::make_cam_frame(rt::vec3f{cam[0], cam[1], cam[2]}, rt::vec3f{cam[3], cam[4], cam[5]},
+                              rt::vec3f{cam[6], cam[7], cam[8]}, cam[9], (int)cam[10], (int)cam[11]);
+}
+
+// Eight planes of n floats carved from one allocation.
+struct ray_planes {
+    dbuf mem;
+    rt::dev::ray_out out{};
+    hipError_t alloc(uint32_t n) {
+        hipError_t e = mem.alloc(8ull * n * 4);
+        if (e != hipSuccess) return e;
+        float *b = mem.as<float>();
+        out = rt::dev::ray_out{b, b + n, b + 2ull * n, b + 3ull * n, b + 4ull * n, b + 5ull * n, b + 6ull * n,
+                               b + 7ull * n};
+        return hipSuccess;
+    }
+};
+
+} // namespace
+
+struct hiprt_pt_s {
+    hiprt_scene sc = nullptr;
+    rt::cam_frame f{};
+    uint32_t n = 0;
+    rt::dev::pt_params pp{};
+    uint64_t seed = 0;
+    int tracer = 0;
+    bool batched = false;
+    rt::dev::persistent_cfg pcfg{};
+    ray_planes rays[2];
+    dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work;
+    rt::hit_soa h{};
+    hipStream_t cap = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool clear_next = true;
+
+    ~hiprt_pt_s() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+
+    rt::dev::pt_scene ps() const {
+        return rt::dev::pt_scene{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    }
+
+    // The frame's launches on stream s (frame seed already set).  When
+    // `ev` is non-null, events ev[2L], ev[2L+1] bracket level L's trace;
+    // when `stats` is non-null the counting trace variant adds level L's
+    // fetch counts to stats[2L], stats[2L+1].
+    hipError_t record(hipStream_t s, bool clear_radiance, hipEvent_t *ev = nullptr,
+                      unsigned long long *stats = nullptr) {
+        using namespace rt::dev;
+        uint32_t *cnt = counts.as<uint32_t>();
+        int levels = pp.max_depth + 1;
+        hipError_t e = launch_fill_u32(cnt, 0u, (uint32_t)levels + 1, s);
+        if (e != hipSuccess) return e;
+        e = launch_fill_u32(cnt, n, 1u, s);
+        if (e != hipSuccess) return e;
+        e = launch_camera_rays(f, rays[0].out, batched ? paths[0].as<uint32_t>() : nullptr, thr.as<float>(),
+                               rad.as<float>(), clear_radiance, s);
+        if (e != hipSuccess) return e;
+        const uint64_t *fs = frame_seed.as<const uint64_t>();
+        for (int L = 0; L < levels; ++L) {
+            if (ev) {
+                e = hipEventRecord(ev[2 * L], s);
+                if (e != hipSuccess) return e;
+            }
+            unsigned long long *lstats = stats ? stats + 2 * L : nullptr;
+            if (!batched) {
+                e = launch_trace_plain(sc->desc, rays[0].out.as_in(), h, nullptr, n, false, s, lstats);
+                if (e != hipSuccess) return e;
+                if (ev) {
+                    e = hipEventRecord(ev[2 * L + 1], s);
+                    if (e != hipSuccess) return e;
+                }
+                e = launch_shade_plain(ps(), pp, L, fs, rays[0].out, h, thr.as<float>(), rad.as<float>(), cnt, n, s);
+            } else {
+                // persistent tracers read the queue length from counts[L]
+                const ray_planes &in = rays[L & 1], &outq = rays[(L + 1) & 1];
+                persistent_cfg cfg = pcfg;
+                cfg.stats = lstats;
+                e = launch_trace_persistent(sc->desc, in.out.as_in(), h, nullptr, n, cnt + L, work.as<uint32_t>(), false,
+                                            cfg, s);
+                if (e != hipSuccess) return e;
+                if (ev) {
+                    e = hipEventRecord(ev[2 * L + 1], s);
+                    if (e != hipSuccess) return e;
+                }
+                e = launch_shade_compact(ps(), pp, L, fs, in.out, h, paths[L & 1].as<const uint32_t>(), outq.out,
+                                         paths[(L + 1) & 1].as<uint32_t>(), thr.as<float>(), rad.as<float>(), cnt, n,
+                                         s);
+            }
+            if (e != hipSuccess) return e;
+        }
+        return launch_add_counts(totals.as<unsigned long long>(), cnt, levels, s);
+    }
+};
+
+extern "C" {
+
+int hiprt_camera_rays(const float *cam, const hiprt_ray_buffers *out, void *stream) {
+    if (!cam || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!out->d_ox || !out->d_oy || !out->d_oz || !out->d_tmin || !out->d_dx || !out->d_dy || !out->d_dz ||
+        !out->d_tmax)
+        return fail(HIPRT_ERR_ARG, "null ray plane");
+    if (cam[10] < 1 || cam[11] < 1) return fail(HIPRT_ERR_ARG, "image size must be positive");
+    rt::cam_frame f = frame_from(cam);
+    rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
+    HIP_TRY(rt::dev::launch_camera_rays(f, o, nullptr, nullptr, nullptr, false, static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, uint64_t seed,
+                        int32_t level, const hiprt_ray_buffers *out, void *stream) {
+    if (!sc || !rays || !hits || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    if (n == 0) return ok();
+    if (!rays->d_dx || !rays->d_dy || !rays->d_dz || !rays->d_tmax || !hits->d_t || !hits->d_u || !hits->d_v ||
+        !hits->d_id)
+        return fail(HIPRT_ERR_ARG, "null input plane");
+    if (!out->d_ox || !out->d_oy || !out->d_oz || !out->d_tmin || !out->d_dx || !out->d_dy || !out->d_dz ||
+        !out->d_tmax)
+        return fail(HIPRT_ERR_ARG, "null output plane");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::ray_soa in{rays->d_ox, rays->d_oy, rays->d_oz, rays->d_tmin, rays->d_dx, rays->d_dy, rays->d_dz, rays->d_tmax};
+    rt::hit_soa h{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
+    rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    HIP_TRY(rt::dev::launch_cosine_bounce(ps, in, h, o, seed, level, n, static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *params, hiprt_pt *out) {
+    if (!sc || !cam || !params || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    if (sc->desc.bvh != rt::BVH2) return fail(HIPRT_ERR_UNSUPPORTED, "path tracer needs a binary BVH scene");
+    if (cam[10] < 1 || cam[11] < 1) return fail(HIPRT_ERR_ARG, "image size must be positive");
+    if (params->max_depth < 0 || params->max_depth > 64) return fail(HIPRT_ERR_ARG, "max_depth must be 0..64");
+    if (!(params->rr_survive > 0.0f && params->rr_survive <= 1.0f)) return fail(HIPRT_ERR_ARG, "rr_survive must be in (0, 1]");
+    double npix = (double)cam[10] * (double)cam[11];
+    if (npix > 0x40000000) return fail(HIPRT_ERR_ARG, "image too large");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    std::unique_ptr<hiprt_pt_s> p(new (std::nothrow) hiprt_pt_s);
+    if (!p) return fail(HIPRT_ERR_ARG, "out of host memory");
+    p->sc = sc;
+    p->f = frame_from(cam);
+    p->n = (uint32_t)npix;
+    p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive};
+    p->seed = params->seed;
+    p->batched = params->batched != 0;
+    p->tracer = params->tracer >= 0 ? params->tracer
+                                    : (p->batched ? HIPRT_TRACER_DYNAMIC_WHILE_WHILE : HIPRT_TRACER_WHILE_WHILE);
+    if (p->tracer >= HIPRT_TRACER_COUNT) return fail(HIPRT_ERR_ARG, "unknown tracer id");
+    if (p->batched && p->tracer == HIPRT_TRACER_WHILE_WHILE)
+        return fail(HIPRT_ERR_ARG, "the batched schedule needs a persistent tracer");
+    if (!p->batched && p->tracer != HIPRT_TRACER_WHILE_WHILE)
+        return fail(HIPRT_ERR_ARG, "the plain schedule uses the while-while tracer");
+    int depth = sc->desc.depth;
+    if (depth > (p->batched ? rt::dev::kPersistentStackCap : rt::dev::kPlainStackCap - 1))
+        return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
+    uint32_t n = p->n;
+    HIP_TRY(p->rays[0].alloc(n));
+    if (p->batched) {
+        HIP_TRY(p->rays[1].alloc(n));
+        HIP_TRY(p->paths[0].alloc(4ull * n));
+        HIP_TRY(p->paths[1].alloc(4ull * n));
+    }
+    HIP_TRY(p->hits.alloc(16ull * n));
+    float *hb = p->hits.as<float>();
+    p->h = rt::hit_soa{hb, hb + n, hb + 2ull * n, reinterpret_cast<uint32_t *>(hb + 3ull * n)};
+    HIP_TRY(p->thr.alloc(12ull * n));
+    HIP_TRY(p->rad.alloc(12ull * n));
+    HIP_TRY(hipMemset(p->rad.p, 0, 12ull * n));
+    HIP_TRY(p->counts.alloc(4ull * (p->pp.max_depth + 2)));
+    HIP_TRY(hipMemset(p->counts.p, 0, 4ull * (p->pp.max_depth + 2)));
+    HIP_TRY(p->totals.alloc(8ull * (p->pp.max_depth + 1)));
+    HIP_TRY(hipMemset(p->totals.p, 0, 8ull * (p->pp.max_depth + 1)));
+    HIP_TRY(p->frame_seed.alloc(8));
+    HIP_TRY(p->work.alloc(256));
+    p->pcfg.speculative = p->tracer >= HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE;
+    p->pcfg.refill = p->tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
+    p->pcfg.refill_thr = 40;
+    p->pcfg.blocks = 0;
+    if (p->batched) {
+        // Resolve the persistent grid size outside the capture, then record
+        // the frame once.  The graph never clears the accumulator: frame 0
+        // after a reset is rendered eagerly with clear_radiance set.
+        HIP_TRY(rt::dev::persistent_grid(sc->desc, false, p->pcfg, &p->pcfg.blocks));
+        HIP_TRY(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+        hipError_t rec = p->record(p->cap, false);
+        hipGraph_t g = nullptr;
+        hipError_t end = hipStreamEndCapture(p->cap, &g);
+        if (rec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(HIPRT_ERR_HIP, std::string("recording the frame graph: ") + hipGetErrorString(rec));
+        }
+        HIP_TRY(end);
+        hipError_t inst = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(inst);
+    }
+    *out = p.release();
+    return ok();
+}
+
+int hiprt_pt_render(hiprt_pt pt, uint32_t frame, void *stream) {
+    if (!pt) return fail(HIPRT_ERR_ARG, "null path tracer");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipSetDevice(pt->sc->ctx->device));
+    HIP_TRY(rt::dev::launch_set_u64(pt->frame_seed.as<uint64_t>(),
+                                    pt->seed * 0x100000001B3ULL + (uint64_t)frame, s));
+    if (pt->batched && !pt->clear_next) {
+        HIP_TRY(hipGraphLaunch(pt->exec, s));
+    } else {
+        HIP_TRY(pt->record(s, pt->clear_next));
+    }
+    pt->clear_next = false;
+    return ok();
+}
+
+int hiprt_pt_ray_counts(hiprt_pt pt, uint32_t *counts, int32_t n_levels) {
+    if (!pt || !counts) return fail(HIPRT_ERR_ARG, "null argument");
+    int levels = pt->pp.max_depth + 1;
+    if (n_levels < levels) return fail(HIPRT_ERR_ARG, "counts array shorter than max_depth + 1");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(counts, pt->counts.p, 4ull * levels, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+int hiprt_pt_ray_totals(hiprt_pt pt, uint64_t *totals, int32_t n_levels) {
+    if (!pt || !totals) return fail(HIPRT_ERR_ARG, "null argument");
+    int levels = pt->pp.max_depth + 1;
+    if (n_levels < levels) return fail(HIPRT_ERR_ARG, "totals array shorter than max_depth + 1");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(totals, pt->totals.p, 8ull * levels, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_level, uint64_t *fetch_stats,
+                     uint32_t *rays_per_level, int32_t n_levels) {
+    if (!pt) return fail(HIPRT_ERR_ARG, "null path tracer");
+    int levels = pt->pp.max_depth + 1;
+    if (n_levels < levels) return fail(HIPRT_ERR_ARG, "arrays shorter than max_depth + 1");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipSetDevice(pt->sc->ctx->device));
+    std::vector<hipEvent_t> ev(2 * levels, nullptr);
+    struct ev_guard {
+        std::vector<hipEvent_t> &v;
+        ~ev_guard() {
+            for (hipEvent_t e : v)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } guard{ev};
+    for (hipEvent_t &e : ev) HIP_TRY(hipEventCreate(&e));
+    uint64_t seed = pt->seed * 0x100000001B3ULL + (uint64_t)frame;
+    HIP_TRY(rt::dev::launch_set_u64(pt->frame_seed.as<uint64_t>(), seed, s));
+    HIP_TRY(pt->record(s, pt->clear_next, ev.data(), nullptr));
+    pt->clear_next = false;
+    HIP_TRY(hipStreamSynchronize(s));
+    if (ms_per_level)
+        for (int L = 0; L < levels; ++L) HIP_TRY(hipEventElapsedTime(&ms_per_level[L], ev[2 * L], ev[2 * L + 1]));
+    if (rays_per_level) HIP_TRY(hipMemcpy(rays_per_level, pt->counts.p, 4ull * levels, hipMemcpyDeviceToHost));
+    if (fetch_stats) {
+        dbuf st;
+        HIP_TRY(st.alloc(16ull * levels));
+        HIP_TRY(hipMemset(st.p, 0, 16ull * levels));
+        HIP_TRY(rt::dev::launch_set_u64(pt->frame_seed.as<uint64_t>(), seed, s));
+        HIP_TRY(pt->record(s, false, nullptr, st.as<unsigned long long>()));
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(fetch_stats, st.p, 16ull * levels, hipMemcpyDeviceToHost));
+    }
+    return ok();
+}
+
+int hiprt_pt_image(hiprt_pt pt, float *rgb) {
+    if (!pt || !rgb) return fail(HIPRT_ERR_ARG, "null argument");
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(rgb, pt->rad.p, 12ull * pt->n, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+int hiprt_pt_reset(hiprt_pt pt, void *stream) {
+    if (!pt) return fail(HIPRT_ERR_ARG, "null path tracer");
+    (void)stream;
+    pt->clear_next = true;
+    return ok();
+}
+
+int hiprt_pt_destroy(hiprt_pt pt) {
+    if (!pt) return fail(HIPRT_ERR_ARG, "null path tracer");
+    (void)hipSetDevice(pt->sc->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete pt;
+    return ok();
+}
+
+} // extern "C"

@@ -49,7 +49,12 @@ class _Hits(ctypes.Structure):
 
 
 class _TracerOpts(ctypes.Structure):
-    _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32)]
+    _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32), ("d_fetch_stats", c_void_p)]
+
+
+class _PtParams(ctypes.Structure):
+    _fields_ = [("max_depth", c_int32), ("rr_start", c_int32), ("rr_survive", c_float), ("seed", c_uint64),
+                ("tracer", c_int32), ("batched", c_int32)]
 
 
 _lib = None
@@ -96,6 +101,18 @@ def lib():
                                             POINTER(_TracerOpts), c_void_p]),
             "hiprt_trace_any": (c_int, [c_void_p, POINTER(_Rays), c_uint32, c_void_p, POINTER(_Hits), c_int32,
                                         POINTER(_TracerOpts), c_void_p]),
+            "hiprt_camera_rays": (c_int, [POINTER(c_float), POINTER(_Rays), c_void_p]),
+            "hiprt_bounce_cosine": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), c_uint32, c_uint64, c_int32,
+                                            POINTER(_Rays), c_void_p]),
+            "hiprt_pt_create": (c_int, [c_void_p, POINTER(c_float), POINTER(_PtParams), POINTER(c_void_p)]),
+            "hiprt_pt_render": (c_int, [c_void_p, c_uint32, c_void_p]),
+            "hiprt_pt_ray_counts": (c_int, [c_void_p, POINTER(c_uint32), c_int32]),
+            "hiprt_pt_image": (c_int, [c_void_p, c_void_p]),
+            "hiprt_pt_ray_totals": (c_int, [c_void_p, POINTER(c_uint64), c_int32]),
+            "hiprt_pt_profile": (c_int, [c_void_p, c_uint32, c_void_p, POINTER(c_float), POINTER(c_uint64),
+                                         POINTER(c_uint32), c_int32]),
+            "hiprt_pt_reset": (c_int, [c_void_p, c_void_p]),
+            "hiprt_pt_destroy": (c_int, [c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -279,13 +296,16 @@ def tracer_id(name):
     return i
 
 
-def _opts(refill_threshold=0, blocks=0):
-    return _TracerOpts(refill_threshold, blocks)
+def _opts(refill_threshold=0, blocks=0, fetch_stats=None):
+    return _TracerOpts(refill_threshold, blocks, fetch_stats.data_ptr() if fetch_stats is not None else None)
 
 
-def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_threshold=0, blocks=0):
+def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
+                  fetch_stats=None):
+    """Closest hits of `rays` (8, n) into `hits`.  fetch_stats: optional int64
+    CUDA tensor of 2 that receives inner-node fetches and triangle tests."""
     n = rays.shape[1]
-    r, h, o = _rays_struct(rays), _hits_struct(hits), _opts(refill_threshold, blocks)
+    r, h, o = _rays_struct(rays), _hits_struct(hits), _opts(refill_threshold, blocks, fetch_stats)
     s = c_void_p(stream) if stream else None
     check(lib().hiprt_trace_closest(dscene.h, ctypes.byref(r), n, ctypes.byref(h), tracer_id(tracer),
                                     ctypes.byref(o), s))
@@ -306,3 +326,75 @@ def new_hits(n, device="cuda"):
             "u": torch.empty(n, dtype=torch.float32, device=device),
             "v": torch.empty(n, dtype=torch.float32, device=device),
             "id": torch.empty(n, dtype=torch.int32, device=device)}
+
+
+def _stream(stream):
+    return c_void_p(stream) if stream else None
+
+
+def camera_rays_device(cam, out, stream=None):
+    """GPU camera rays into `out`, a (8, w*h) float32 CUDA tensor."""
+    c = np.ascontiguousarray(cam[:12], dtype=np.float32)
+    r = _rays_struct(out)
+    check(lib().hiprt_camera_rays(c.ctypes.data_as(POINTER(c_float)), ctypes.byref(r), _stream(stream)))
+
+
+def bounce_cosine(dscene, rays, hits, out, seed, level, stream=None):
+    """Cosine-weighted bounce rays off `hits` of `rays` into `out` (both (8, n))."""
+    n = rays.shape[1]
+    r, h, o = _rays_struct(rays), _hits_struct(hits), _rays_struct(out)
+    check(lib().hiprt_bounce_cosine(dscene.h, ctypes.byref(r), ctypes.byref(h), n, seed, level, ctypes.byref(o),
+                                    _stream(stream)))
+
+
+class PathTracer:
+    """Wavefront diffuse path tracer over a device scene (plain or batched)."""
+
+    def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None):
+        self.dscene = dscene
+        c = np.ascontiguousarray(cam[:12], dtype=np.float32)
+        self.w, self.h = int(c[10]), int(c[11])
+        self.levels = max_depth + 1
+        tid = -1 if tracer is None else tracer_id(tracer)
+        p = _PtParams(max_depth, rr_start, rr_survive, seed, tid, 1 if batched else 0)
+        self.h_ = c_void_p()
+        check(lib().hiprt_pt_create(dscene.h, c.ctypes.data_as(POINTER(c_float)), ctypes.byref(p),
+                                    ctypes.byref(self.h_)))
+
+    def render(self, frame=0, stream=None):
+        check(lib().hiprt_pt_render(self.h_, frame, _stream(stream)))
+
+    def ray_counts(self):
+        out = (c_uint32 * self.levels)()
+        check(lib().hiprt_pt_ray_counts(self.h_, out, self.levels))
+        return np.array(out, dtype=np.uint64)
+
+    def ray_totals(self):
+        out = (c_uint64 * self.levels)()
+        check(lib().hiprt_pt_ray_totals(self.h_, out, self.levels))
+        return np.array(out, dtype=np.uint64)
+
+    def profile(self, frame=0, stream=None):
+        """Per-level trace milliseconds, (inner-node fetches, triangle tests), live rays."""
+        ms = (c_float * self.levels)()
+        st = (c_uint64 * (2 * self.levels))()
+        rays = (c_uint32 * self.levels)()
+        check(lib().hiprt_pt_profile(self.h_, frame, _stream(stream), ms, st, rays, self.levels))
+        return (np.array(ms, np.float64), np.array(st, np.uint64).reshape(-1, 2), np.array(rays, np.uint64))
+
+    def image(self):
+        img = np.empty((self.h, self.w, 3), np.float32)
+        check(lib().hiprt_pt_image(self.h_, c_void_p(img.ctypes.data)))
+        return img
+
+    def reset(self):
+        check(lib().hiprt_pt_reset(self.h_, None))
+
+    def close(self):
+        if getattr(self, "h_", None):
+            lib().hiprt_pt_destroy(self.h_)
+            self.h_ = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()

@@ -160,6 +160,9 @@ const char *hiprt_tracer_name(int tracer);
 typedef struct {
     int32_t refill_threshold; /* refill when fewer lanes of 64 are active (default 40) */
     int32_t blocks;           /* grid size (default: occupancy x CU count) */
+    uint64_t *d_fetch_stats;  /* non-NULL: run the counting variant, which adds the inner
+                                 nodes it fetches to [0] and the triangle records it
+                                 tests to [1] (device uint64[2]) */
 } hiprt_tracer_opts;
 
 /* compute_closest_hit of batch_ray_tracer (libgi/wavefront-rt.h:145-149,
@@ -172,6 +175,64 @@ int hiprt_trace_closest(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, cons
  * t = ref = 0xFFFFFFFF (as float 4294967295) if occluded, reset() otherwise. */
 int hiprt_trace_any(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *d_occluded, const hiprt_hits *hits,
                     int32_t tracer, const hiprt_tracer_opts *opts, void *stream);
+
+/* ----------------------------------------------------------- wavefront steps
+ * GPU forms of the reference's wavefront steps around the trace:
+ *   "sample camera rays"        (rt/cuda/rni.cpp:31-101, cam_ray semantics)
+ *   "sample cos weighted dir"   (rt/cpu/bounce.cpp:41-72)
+ * Output rays go to writable device planes. */
+typedef struct {
+    float *d_ox, *d_oy, *d_oz, *d_tmin, *d_dx, *d_dy, *d_dz, *d_tmax;
+} hiprt_ray_buffers;
+
+/* w*h camera rays (cam as for hiprt_camera_rays_host); bit-identical to it. */
+int hiprt_camera_rays(const float *cam, const hiprt_ray_buffers *out, void *stream);
+/* out[i] = cosine-weighted bounce ray off hit i of ray i (disabled ray
+ * o = 0, d = (1,1,1), t_max = -FLT_MAX for misses).  PCG32 stream per
+ * (seed, i, level).  Needs a scene committed from a host scene. */
+int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, uint64_t seed,
+                        int32_t level, const hiprt_ray_buffers *out, void *stream);
+
+/* ------------------------------------------------------- wavefront path tracer
+ * A diffuse path tracer over one committed scene and camera: camera rays,
+ * closest hit, shading (emission added, cosine-weighted bounce, throughput
+ * times albedo), Russian roulette from level rr_start on.  Two schedules with
+ * identical per-path results:
+ *   batched = 0: one launch per step per level over all w*h slots, dead paths
+ *                kept as disabled rays (the reference's wavefront form);
+ *   batched = 1: live paths compacted into a queue after every level, traced
+ *                by a persistent kernel that reads the queue length on the
+ *                device, the whole frame captured once in a hipGraph. */
+typedef struct {
+    int32_t max_depth;   /* last trace level; 4 = primary + 4 bounces */
+    int32_t rr_start;    /* first level whose ray is subject to roulette (3) */
+    float rr_survive;    /* survival probability (0.8) */
+    uint64_t seed;
+    int32_t tracer;      /* tracer id; -1 = while-while (plain) / default (batched) */
+    int32_t batched;
+} hiprt_pt_params;
+
+typedef struct hiprt_pt_s *hiprt_pt;
+
+int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *params, hiprt_pt *out);
+/* Renders one frame (one sample per pixel) into the accumulator, async. */
+int hiprt_pt_render(hiprt_pt pt, uint32_t frame, void *stream);
+/* Live rays traced per level in the last frame (synchronizes the device). */
+int hiprt_pt_ray_counts(hiprt_pt pt, uint32_t *counts, int32_t n_levels);
+/* Live rays traced per level summed over every frame since creation
+ * (synchronizes the device). */
+int hiprt_pt_ray_totals(hiprt_pt pt, uint64_t *totals, int32_t n_levels);
+/* Runs frame `frame` twice outside the graph: once timing each level's trace
+ * launch with HIP events on `stream` (ms_per_level), once with the counting
+ * trace variant (fetch_stats: inner-node fetches and triangle tests, 2 per
+ * level).  rays_per_level receives the live rays of each level.  Both frames
+ * add to the radiance accumulator.  Synchronizes the device. */
+int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_level, uint64_t *fetch_stats,
+                     uint32_t *rays_per_level, int32_t n_levels);
+/* Accumulated radiance, w*h*3 floats (synchronizes the device). */
+int hiprt_pt_image(hiprt_pt pt, float *rgb);
+int hiprt_pt_reset(hiprt_pt pt, void *stream);
+int hiprt_pt_destroy(hiprt_pt pt);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,218 @@
+"""Host side of the product: generators, the restated BVH builders, the
+flattener and the C ABI's host entry points.  No GPU is touched."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hiprt
+import oracle_py as orc
+from parity import FLT_MAX, uniform_sphere_rays
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ---------------------------------------------------------------- C ABI surface
+
+def test_library_exports_every_declared_symbol():
+    L = hiprt.lib()
+    names = hiprt.exported_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_errors_are_reported_not_thrown():
+    L = hiprt.lib()
+    h = ctypes.c_void_p()
+    assert L.hiprt_scene_generate(9, ctypes.byref(h), None) == 1
+    assert b"config" in L.hiprt_last_error()
+    assert L.hiprt_scene_load_obj(b"/nonexistent.obj", ctypes.byref(h)) == 4
+    assert hiprt.tracer_id("default") == 3
+    with pytest.raises(hiprt.HiprtError):
+        hiprt.tracer_id("no-such-tracer")
+    bad = np.array([[0, 1, 5]], np.uint32)
+    with pytest.raises(hiprt.HiprtError, match="out of range"):
+        hiprt.HostScene.from_arrays(np.zeros((3, 3), np.float32), bad)
+
+
+# ---------------------------------------------------------------- generators
+
+def test_workload_sizes():
+    assert hiprt.HostScene.generate(1).n_tris == 8192
+    assert hiprt.HostScene.generate(2).n_tris == 74068
+    assert hiprt.HostScene.generate(3).n_tris == 262144
+
+
+def test_random_scene_statistics_and_determinism():
+    a = hiprt.HostScene.random(8192, 0.02, 1)
+    b = hiprt.HostScene.random(8192, 0.02, 1)
+    pa, ia, _ = a.arrays()
+    pb, _, _ = b.arrays()
+    assert np.array_equal(pa, pb)
+    tri = pa[ia]                                   # (n, 3, 3)
+    cent = tri.mean(axis=1)
+    off = (tri - cent[:, None, :]).reshape(-1, 3)
+    # centroids U[0,1]^3 (within the per-vertex noise), offsets ~ N(0, 0.02^2)
+    assert abs(cent.mean() - 0.5) < 0.01 and cent.min() > -0.1 and cent.max() < 1.1
+    # offsets from the sample centroid have variance 2/3 sigma^2
+    assert abs(off.std() - 0.02 * np.sqrt(2 / 3)) < 0.001
+
+
+def test_obj_round_trip(tmp_path):
+    sc = hiprt.HostScene.generate(1)
+    p = tmp_path / "c1.obj"
+    sc.write_obj(p)
+    back = hiprt.HostScene.load_obj(p)
+    a, b = sc.arrays(), back.arrays()
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))   # %.9g round-trips floats
+    assert np.array_equal(a[1], b[1])
+
+
+def test_obj_reader_fans_polygons(tmp_path):
+    p = tmp_path / "quad.obj"
+    p.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1/1 2/2 3/3 4/4\n")
+    pos, idx, _ = hiprt.HostScene.load_obj(p).arrays()
+    assert idx.tolist() == [[0, 1, 2], [0, 2, 3]]
+
+
+# ---------------------------------------------------------------- BVH builders
+
+@pytest.mark.parametrize("split", [hiprt.SPLIT_OM, hiprt.SPLIT_SM, hiprt.SPLIT_SAH])
+@pytest.mark.parametrize("esc", [False, True])
+@pytest.mark.parametrize("max_tris", [1, 4, 7])
+def test_builder_invariants(split, esc, max_tris):
+    sc = hiprt.HostScene.random(3000, 0.03, 5)
+    b = hiprt.Bvh(sc, split=split, max_tris=max_tris, planes=16, esc=esc)
+    code, msg = b.validate()
+    assert code == 0, msg
+    # root is node 0 and node ids are a pre-order: children after parents
+    inner = b.nodes["link_r"] >= 0
+    ids = np.nonzero(inner)[0]
+    assert b.root == 0
+    assert np.all(b.nodes["link_l"][ids] > ids) and np.all(b.nodes["link_r"][ids] > ids)
+    leaves = ~inner
+    assert np.all(-b.nodes["link_r"][leaves] <= max_tris)
+    if esc:
+        assert len(b.index) >= sc.n_tris
+    else:
+        assert np.array_equal(np.sort(b.index), np.arange(sc.n_tris))
+
+
+def _box_contains(outer, inner):
+    return np.all(outer[0] <= inner[0]) and np.all(outer[1] >= inner[1])
+
+
+def test_sah_children_enclose_their_triangles():
+    # without ESC every inner node's child box holds its subtree's triangle boxes
+    sc = hiprt.HostScene.random(2000, 0.03, 2)
+    pos, idx, _ = sc.arrays()
+    b = hiprt.Bvh(sc, esc=False)
+    tri_box = np.stack([pos[idx].min(axis=1), pos[idx].max(axis=1)], axis=1)   # (n, 2, 3)
+
+    def subtree_box(nid):
+        nd = b.nodes[nid]
+        if nd["link_r"] < 0:
+            s = b.index[-nd["link_l"]: -nd["link_l"] - nd["link_r"]]
+            return np.stack([tri_box[s, 0].min(0), tri_box[s, 1].max(0)])
+        l, r = subtree_box(nd["link_l"]), subtree_box(nd["link_r"])
+        assert _box_contains(nd["box_l"], l) and _box_contains(nd["box_r"], r)
+        return np.stack([np.minimum(l[0], r[0]), np.maximum(l[1], r[1])])
+
+    whole = subtree_box(b.root)
+    assert np.allclose(whole[0], tri_box[:, 0].min(0)) and np.allclose(whole[1], tri_box[:, 1].max(0))
+
+
+def test_sah_split_separates_clusters():
+    # two clusters of 8 tiny triangles far apart on x: the root split must
+    # put each cluster in its own child
+    rng = np.random.default_rng(0)
+    tris = []
+    for cx in (0.0, 10.0):
+        for _ in range(8):
+            c = np.array([cx, 0, 0]) + rng.uniform(-0.5, 0.5, 3)
+            tris.append(c + rng.normal(0, 0.05, (3, 3)))
+    pos = np.concatenate(tris).astype(np.float32)
+    idx = np.arange(len(pos), dtype=np.uint32).reshape(-1, 3)
+    sc = hiprt.HostScene.from_arrays(pos, idx)
+    b = hiprt.Bvh(sc, max_tris=4, esc=False)
+    root = b.nodes[b.root]
+    assert root["box_l"][1][0] < 5 and root["box_r"][0][0] > 5
+
+
+def test_n_minus_one_split_is_treated_as_degenerate():
+    # 5 triangles: 4 clustered at x~0 and one at x~10.  Every SAH plane puts
+    # the four on the left and the loner on the right -- mid == end-1 -- which
+    # the reference treats as degenerate (bvh-ctor.h:441) and replaces by the
+    # object median (2 | 3), although (4 | 1) would be cheaper.
+    pts = [[0, 0, 0], [0.1, 0, 0], [0.2, 0, 0], [0.3, 0, 0], [10, 0, 0]]
+    tris = [np.array(p) + np.array([[0, 0, 0], [0.05, 0.1, 0], [0, 0.1, 0.05]]) for p in pts]
+    pos = np.concatenate(tris).astype(np.float32)
+    idx = np.arange(15, dtype=np.uint32).reshape(-1, 3)
+    sc = hiprt.HostScene.from_arrays(pos, idx)
+    b = hiprt.Bvh(sc, max_tris=1, planes=16, esc=False)
+    root = b.nodes[b.root]
+    left = b.nodes[root["link_l"]]
+    # object median of 5: left child gets 2 primitives (an inner node of 2 leaves)
+    assert left["link_r"] >= 0
+    assert b.nodes[left["link_l"]]["link_r"] < 0 and b.nodes[left["link_r"]]["link_r"] < 0
+
+
+def test_root_leaf_for_tiny_scene():
+    sc = hiprt.HostScene.random(3, 0.02, 1)
+    b = hiprt.Bvh(sc, max_tris=4)
+    assert len(b.nodes) == 1 and b.nodes[0]["link_r"] == -3 and b.nodes[0]["link_l"] == 0
+    pos, idx, _ = sc.arrays()
+    rays = uniform_sphere_rays(3000, 4)
+    a = orc.trace_closest(pos, idx, b.nodes, b.root, b.index, rays)
+    s = orc.seq_closest(pos, idx, rays)
+    assert np.array_equal(a[3], s[3]) and np.array_equal(a[0], s[0])
+
+
+def test_empty_scene():
+    sc = hiprt.HostScene.from_arrays(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32))
+    b = hiprt.Bvh(sc)
+    assert len(b.nodes) == 0 and len(b.index) == 0
+
+
+def test_esc_fragments_cover_triangles():
+    # ESC boxes bound fragments, not triangles: every point of a triangle lies
+    # in the union of its fragments' leaf boxes.  Checked by sampling points
+    # and finding, for each, a leaf that lists the triangle and whose parent
+    # box contains the point.
+    sc = hiprt.HostScene.generate(2)   # long wall strips get clipped
+    pos, idx, _ = sc.arrays()
+    b = hiprt.Bvh(sc)
+    assert len(b.index) > sc.n_tris      # clipping happened
+    parent_box = {}
+    for nid, nd in enumerate(b.nodes):
+        if nd["link_r"] >= 0:
+            parent_box[int(nd["link_l"])] = nd["box_l"]
+            parent_box[int(nd["link_r"])] = nd["box_r"]
+    boxes_of = {}
+    for nid, nd in enumerate(b.nodes):
+        if nd["link_r"] < 0:
+            for s in range(-nd["link_l"], -nd["link_l"] - nd["link_r"]):
+                boxes_of.setdefault(int(b.index[s]), []).append(parent_box[nid])
+    rng = np.random.default_rng(0)
+    dup = [t for t, bx in boxes_of.items() if len(bx) > 1][:200]
+    assert dup
+    for t in dup:
+        A, B, C = pos[idx[t]].astype(np.float64)
+        for _ in range(20):
+            r1, r2 = rng.random(2)
+            s = np.sqrt(r1)
+            p = (1 - s) * A + s * (1 - r2) * B + s * r2 * C
+            assert any(np.all(bx[0] - 1e-5 <= p) and np.all(p <= bx[1] + 1e-5) for bx in boxes_of[t])
+
+
+# ---------------------------------------------------------------- ray generation
+
+def test_config_camera_rays_and_depth_fit_the_kernels():
+    for cfg in (1, 2, 3):
+        sc = hiprt.HostScene.generate(cfg)
+        b = hiprt.Bvh(sc)
+        # persistent kernels keep 32 stack entries; the plain kernel 63
+        assert b.depth <= 32, (cfg, b.depth)
