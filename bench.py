@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays are independent, so "
@@ -148,6 +150,15 @@ def main():
                          f"binary_bvh_tracer (same BVH), OpenMP over rows, median of 3; 1 thread on 50k: "
                          f"{50_000 / one / 1e6:.2f} Mrays/s"}
         log(f"cpu baseline: {cpu}")
+
+    if args.copy_mib > 0:
+        src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "trace_hall_traffic.json")
