@@ -11,6 +11,7 @@
 #include "core/raygen.h"
 #include "device/kernels.h"
 #include "device/pt_kernels.h"
+#include "device/steps_kernels.h"
 
 #include <memory>
 #include <new>
@@ -152,6 +153,59 @@ int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits
     rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
     rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
     HIP_TRY(rt::dev::launch_cosine_bounce(ps, in, h, o, seed, level, n, static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+namespace {
+bool full_rays(const hiprt_rays *r) {
+    return r && r->d_ox && r->d_oy && r->d_oz && r->d_tmin && r->d_dx && r->d_dy && r->d_dz && r->d_tmax;
+}
+bool full_hits(const hiprt_hits *h) { return h && h->d_t && h->d_u && h->d_v && h->d_id; }
+rt::ray_soa soa(const hiprt_rays *r) {
+    return rt::ray_soa{r->d_ox, r->d_oy, r->d_oz, r->d_tmin, r->d_dx, r->d_dy, r->d_dz, r->d_tmax};
+}
+rt::hit_soa hsoa(const hiprt_hits *h) { return rt::hit_soa{h->d_t, h->d_u, h->d_v, h->d_id}; }
+} // namespace
+
+int hiprt_add_hitpoint_albedo(hiprt_scene sc, const hiprt_hits *hits, uint32_t n, float *d_framebuffer, void *stream) {
+    if (!sc || !full_hits(hits) || (n && !d_framebuffer)) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    HIP_TRY(rt::dev::launch_add_albedo(ps, hsoa(hits), reinterpret_cast<float4 *>(d_framebuffer), n,
+                                       static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_shadow_rays(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, const float *light,
+                      const hiprt_ray_buffers *out, void *stream) {
+    if (!sc || !full_rays(rays) || !full_hits(hits) || !light || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!out->d_ox || !out->d_oy || !out->d_oz || !out->d_tmin || !out->d_dx || !out->d_dy || !out->d_dz ||
+        !out->d_tmax)
+        return fail(HIPRT_ERR_ARG, "null output plane");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
+    HIP_TRY(rt::dev::launch_shadow_rays(ps, soa(rays), hsoa(hits), rt::vec3f{light[0], light[1], light[2]}, o, n,
+                                        static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *hits,
+                          const hiprt_rays *shadow_rays, const uint8_t *d_occluded, uint32_t n, const float *light,
+                          const float *power, float *d_framebuffer, void *stream) {
+    if (!sc || !full_rays(cam_rays) || !full_hits(hits) || !full_rays(shadow_rays) || !light || !power ||
+        (n && (!d_occluded || !d_framebuffer)))
+        return fail(HIPRT_ERR_ARG, "null argument");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    HIP_TRY(rt::dev::launch_integrate_light(ps, soa(cam_rays), hsoa(hits), soa(shadow_rays), d_occluded,
+                                            rt::vec3f{light[0], light[1], light[2]},
+                                            rt::vec3f{power[0], power[1], power[2]},
+                                            reinterpret_cast<float4 *>(d_framebuffer), n,
+                                            static_cast<hipStream_t>(stream)));
     return ok();
 }
 

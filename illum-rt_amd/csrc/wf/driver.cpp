@@ -1,0 +1,280 @@
+// rtgi-hip: RTGI's script front end for the commands on the batch-tracing
+// path (driver/interaction.cpp:247-856), driving the hip platform.  A script
+// such as
+//
+//     platform hip
+//     raytracer dynamic-while-while
+//     load scene.obj            (or: generate 3)
+//     add-scene-step build accel struct
+//     commit
+//     algo primary-wf
+//     run
+//
+// runs as it would on RTGI's cuda platform.  Errors print
+// "command (<name>): <message>" and the script continues, as in the
+// reference; "run" checks the same preconditions.
+//
+// Commands: at, look, up, fovy, resolution, sppx, load, generate, pointlight,
+// platform, raytracer, bvh, triangles, add-scene-step, commit, algo, run,
+// outfile, stats, echo, quit.  Images are written as PFM (float) or, for any
+// other extension, 8-bit PPM with a 2.2 gamma.
+#include "hip-platform.h"
+
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <string>
+
+using namespace std;
+
+namespace {
+
+struct update_counters {
+    int cmdid = 0;
+    int scene_touched_at = 0, tracer_touched_at = 0, accel_touched_at = 0;
+    bool valid_platform = false;
+};
+
+update_counters uc;
+unique_ptr<wf::simple_algorithm> algo;
+string algo_name;
+string outfile = "out.pfm";
+
+#define error(x)                                                              \
+    {                                                                         \
+        cerr << "command (" << command << "): " << x << endl;                  \
+        return;                                                               \
+    }
+#define check_in_complete(x)                                                  \
+    {                                                                         \
+        if (in.bad() || in.fail() || !in.eof()) error(x);                     \
+    }
+
+bool write_image(const string &path) {
+    const auto &fb = wf::rc->framebuffer;
+    int w = wf::rc->w, h = wf::rc->h;
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) return false;
+    bool pfm = path.size() > 4 && path.substr(path.size() - 4) == ".pfm";
+    if (pfm) {
+        fprintf(f, "PF\n%d %d\n-1.0\n", w, h);   // little endian, bottom row first
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) fwrite(&fb[4 * ((size_t)y * w + x)], 4, 3, f);
+    } else {
+        fprintf(f, "P6\n%d %d\n255\n", w, h);
+        for (int y = h - 1; y >= 0; --y)
+            for (int x = 0; x < w; ++x)
+                for (int k = 0; k < 3; ++k) {
+                    float v = fb[4 * ((size_t)y * w + x) + k];
+                    v = v < 0 ? 0 : (v > 1 ? 1 : v);
+                    fputc((int)lround(255.0 * pow(v, 1.0 / 2.2)), f);
+                }
+    }
+    return fclose(f) == 0;
+}
+
+bool read3(istringstream &in, float *v) {
+    in >> v[0] >> v[1] >> v[2];
+    return !in.fail();
+}
+
+void run() {
+    algo->prepare_data();
+    algo->prepare_frame();
+    while (algo->compute_sample(wf::rc->sppx, wf::rc->platform->timer)) {
+    }
+    algo->finalize_frame(wf::rc->platform->timer);
+    if (!write_image(outfile)) cerr << "cannot write " << outfile << endl;
+}
+
+void eval(const string &line) {
+    istringstream in(line);
+    string command;
+    in >> command;
+    ++uc.cmdid;
+    wf::render_context *rc = wf::rc;
+    try {
+        if (command.empty() || command[0] == '#' || command.substr(0, 2) == "//") {
+            return;
+        } else if (command == "at" || command == "look" || command == "up") {
+            float v[3];
+            if (!read3(in, v)) error("Syntax error, requires 3 numerical components");
+            float *dst = command == "at" ? rc->scene.cam_pos : (command == "look" ? rc->scene.cam_dir : rc->scene.cam_up);
+            for (int k = 0; k < 3; ++k) dst[k] = v[k];
+        } else if (command == "fovy") {
+            float f;
+            in >> f;
+            check_in_complete("Syntax error, requires one angle in degrees");
+            rc->scene.fovy = f;
+        } else if (command == "resolution") {
+            int w, h;
+            in >> w >> h;
+            check_in_complete("Syntax error, requires 2 integral values");
+            if (w <= 0 || h <= 0) error("Resolution must be positive");
+            rc->w = w;
+            rc->h = h;
+            algo.reset();   // ray data is sized by the resolution
+            algo_name.clear();
+        } else if (command == "sppx") {
+            int s;
+            in >> s;
+            check_in_complete("Syntax error, requires exactly one positive integral value");
+            rc->sppx = s;
+        } else if (command == "load") {
+            string file;
+            in >> file;
+            if (in.fail()) error("Syntax error, requires a file name (no spaces, sorry) and (optionally) a name");
+            hiprt_host_scene s = nullptr;
+            if (hiprt_scene_load_obj(file.c_str(), &s) != HIPRT_OK) error(hiprt_last_error());
+            if (rc->scene.geometry) hiprt_scene_free(rc->scene.geometry);
+            rc->scene.geometry = s;
+            uc.scene_touched_at = uc.cmdid;
+        } else if (command == "generate") {
+            int cfg;
+            in >> cfg;
+            check_in_complete("Syntax error, requires a workload number (1-3)");
+            hiprt_host_scene s = nullptr;
+            float cam[15];
+            if (hiprt_scene_generate(cfg, &s, cam) != HIPRT_OK) error(hiprt_last_error());
+            if (rc->scene.geometry) hiprt_scene_free(rc->scene.geometry);
+            rc->scene.geometry = s;
+            for (int k = 0; k < 3; ++k) {
+                rc->scene.cam_pos[k] = cam[k];
+                rc->scene.cam_dir[k] = cam[3 + k];
+                rc->scene.cam_up[k] = cam[6 + k];
+            }
+            rc->scene.fovy = cam[9];
+            rc->scene.lights = {wf::pointlight{{cam[12], cam[13], cam[14]}, {400.0f, 400.0f, 400.0f}}};
+            uc.scene_touched_at = uc.cmdid;
+        } else if (command == "pointlight") {
+            // pointlight [replace] pos x y z col r g b (interaction.cpp:723-753)
+            string cmd;
+            in >> cmd;
+            bool replace = false;
+            if (cmd == "replace") {
+                replace = true;
+                in >> cmd;
+            }
+            wf::pointlight pl;
+            if (cmd != "pos" || !read3(in, pl.pos)) error("Syntax error: poinlight [replace] pos x y z col x y z");
+            in >> cmd;
+            if (cmd != "col" || !read3(in, pl.col)) error("Syntax error: poinlight [replace] pos x y z col x y z");
+            check_in_complete("Syntax error: poinlight [replace] pos x y z col x y z");
+            if (replace && !rc->scene.lights.empty()) rc->scene.lights[0] = pl;
+            else rc->scene.lights.push_back(pl);
+        } else if (command == "platform") {
+            string name;
+            in >> name;
+            if (name.empty()) {
+                cout << "platform: " << (rc->platform ? rc->platform->name : string("none")) << endl;
+                return;
+            }
+            vector<string> args;
+            string a;
+            while (in >> a) args.push_back(a);
+            algo.reset();
+            algo_name.clear();
+            delete rc->platform;
+            rc->platform = nullptr;
+            uc.valid_platform = false;
+            if (name == "hip") {
+                try {
+                    rc->platform = new wf::hip::platform(args);
+                } catch (const std::exception &e) {
+                    error("The hip platform is not available: " << e.what());
+                }
+                uc.valid_platform = true;
+            } else if (name == "none") {
+                uc.valid_platform = true;
+            } else {
+                error("There is no platform called '" << name << "'");
+            }
+            uc.tracer_touched_at = uc.cmdid;
+        } else if (command == "raytracer" || command == "bvh" || command == "triangles") {
+            if (!rc->platform) error("No platform selected, this command needs the hip platform");
+            if (!rc->platform->interprete(command, in)) error("unknown command");
+            if (command == "raytracer") uc.tracer_touched_at = uc.cmdid;
+        } else if (command == "add-scene-step") {
+            string name;
+            while (isspace(in.peek())) in.get();
+            getline(in, name);
+            if (!rc->platform) error("No platform selected, steps are available only in wavefront mode");
+            if (auto *s = rc->platform->step(name)) rc->platform->append_setup_step(s);
+            else error("Platform " << rc->platform->name << " does not support step " << name);
+        } else if (command == "commit") {
+            if (!uc.valid_platform) error("Invalid platform");
+            if (!rc->scene.geometry) error("There is no scene data to work with");
+            if (!rc->platform) error("There is no ray traversal scheme to commit the scene data to");
+            rc->platform->commit_scene(&rc->scene);
+            uc.accel_touched_at = uc.cmdid;
+        } else if (command == "algo") {
+            string name;
+            in >> name;
+            check_in_complete("Syntax error, requires an algorithm name");
+            auto a = wf::hip::make_algorithm(name);
+            if (!a) error("There is no wavefront algorithm called '" << name << "'");
+            algo = std::move(a);
+            algo_name = name;
+        } else if (command == "run") {
+            if (!uc.valid_platform) error("Invalid platform");
+            if (uc.scene_touched_at == 0 || uc.tracer_touched_at == 0 || uc.accel_touched_at == 0 || !algo)
+                error("We have to have a scene loaded, a ray tracer set, an acceleration structure built and an "
+                      "algorithm set prior to running");
+            if (uc.accel_touched_at < uc.scene_touched_at) error("The current acceleration structure is out-dated");
+            auto *pf = dynamic_cast<wf::hip::platform *>(rc->platform);
+            if (!pf || !pf->ds) error("The committed scene has no acceleration structure (add-scene-step build accel struct)");
+            run();
+        } else if (command == "outfile") {
+            string name;
+            in >> name;
+            check_in_complete("Syntax error, only accepts a single file name (no spaces, sorry)");
+            outfile = name;
+        } else if (command == "stats") {
+            string sub;
+            in >> sub;
+            if (sub == "clear") wf::hip::clear_stats();
+            else if (sub == "print") wf::hip::print_stats(cout);
+            else error("No such stats subcommand");
+        } else if (command == "echo") {
+            string text;
+            getline(in, text);
+            cout << (text.empty() ? string(" ") : text.substr(text[0] == ' ' ? 1 : 0)) << endl;
+        } else if (command == "quit" || command == "exit") {
+            exit(0);
+        } else {
+            error("Unknown command");
+        }
+    } catch (const std::exception &e) {
+        error(e.what());
+    }
+}
+
+} // namespace
+
+int main(int argc, char **argv) {
+    wf::render_context ctx;
+    wf::rc = &ctx;
+    auto process = [](istream &in) {
+        string line;
+        while (getline(in, line)) eval(line);
+    };
+    if (argc < 2) {
+        process(cin);
+    } else {
+        for (int i = 1; i < argc; ++i) {
+            ifstream f(argv[i]);
+            if (!f) {
+                cerr << "cannot open script " << argv[i] << endl;
+                return 1;
+            }
+            process(f);
+        }
+    }
+    algo.reset();
+    delete ctx.platform;
+    ctx.platform = nullptr;
+    return 0;
+}

@@ -1,0 +1,503 @@
+#include "hip-platform.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <iostream>
+#include <map>
+
+namespace wf {
+
+render_context *rc = nullptr;
+
+void render_context::camera(float *cam) const {
+    for (int k = 0; k < 3; ++k) {
+        cam[k] = scene.cam_pos[k];
+        cam[3 + k] = scene.cam_dir[k];
+        cam[6 + k] = scene.cam_up[k];
+    }
+    cam[9] = scene.fovy;
+    cam[10] = (float)w;
+    cam[11] = (float)h;
+}
+
+namespace hip {
+
+platform *pf = nullptr;
+
+void check(int status, const char *what) {
+    if (status != HIPRT_OK) throw std::runtime_error(std::string(what) + ": " + hiprt_last_error());
+}
+
+static void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+device_buffer::~device_buffer() {
+    if (p) (void)hipFree(p);
+}
+
+void device_buffer::resize(size_t n) {
+    if (n == bytes) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = n;
+    if (n) hip_check(hipMalloc(&p, n), "hipMalloc");
+}
+
+// ---------------------------------------------------------------- timer
+
+// HIP-event pairs per step name, folded into totals at synchronize()
+// (rt/cuda/base.cpp:17-45).
+struct event_timer : public wf::timer {
+    std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> events;
+    static std::map<std::string, std::pair<double, long>> &totals() {
+        static std::map<std::string, std::pair<double, long>> t;
+        return t;
+    }
+    void start(const std::string &name) override {
+        auto it = events.find(name);
+        if (it == events.end()) {
+            hipEvent_t a, b;
+            hip_check(hipEventCreate(&a), "hipEventCreate");
+            hip_check(hipEventCreate(&b), "hipEventCreate");
+            it = events.emplace(name, std::make_pair(a, b)).first;
+        }
+        hip_check(hipEventRecord(it->second.first, nullptr), "hipEventRecord");
+    }
+    void stop(const std::string &name) override {
+        hip_check(hipEventRecord(events[name].second, nullptr), "hipEventRecord");
+    }
+    void synchronize() override {
+        for (auto &kv : events) {
+            float ms = 0;
+            hip_check(hipEventSynchronize(kv.second.second), "hipEventSynchronize");
+            hip_check(hipEventElapsedTime(&ms, kv.second.first, kv.second.second), "hipEventElapsedTime");
+            auto &t = totals()[kv.first];
+            t.first += ms;
+            t.second += 1;
+            (void)hipEventDestroy(kv.second.first);
+            (void)hipEventDestroy(kv.second.second);
+        }
+        events.clear();
+    }
+    ~event_timer() override {
+        for (auto &kv : events) {
+            (void)hipEventDestroy(kv.second.first);
+            (void)hipEventDestroy(kv.second.second);
+        }
+    }
+};
+
+#define time_this_wf_step wf::raii_timer local_wf_raii_timer(id_str, *pf->timer)
+
+// ---------------------------------------------------------------- raydata
+
+raydata::raydata(int w_, int h_) { resize(w_, h_); }
+
+void raydata::resize(int w_, int h_) {
+    w = w_;
+    h = h_;
+    rays.resize(32ull * n());
+    hits.resize(16ull * n());
+    occluded.resize(n());
+}
+
+hiprt_rays raydata::in() const {
+    const float *b = rays.as<const float>();
+    size_t m = n();
+    return hiprt_rays{b, b + m, b + 2 * m, b + 3 * m, b + 4 * m, b + 5 * m, b + 6 * m, b + 7 * m};
+}
+hiprt_ray_buffers raydata::out() const {
+    float *b = rays.as<float>();
+    size_t m = n();
+    return hiprt_ray_buffers{b, b + m, b + 2 * m, b + 3 * m, b + 4 * m, b + 5 * m, b + 6 * m, b + 7 * m};
+}
+hiprt_hits raydata::hit() const {
+    float *b = hits.as<float>();
+    size_t m = n();
+    return hiprt_hits{b, b + m, b + 2 * m, reinterpret_cast<uint32_t *>(b + 3 * m)};
+}
+
+// ---------------------------------------------------------------- tracers
+
+void batch_rt::compute_closest_hit() {
+    if (!rd) throw std::logic_error("tracer used without ray data");
+    if (!pf->ds) throw std::logic_error("no committed scene");
+    hiprt_rays r = rd->in();
+    hiprt_hits h = rd->hit();
+    check(hiprt_trace_closest(pf->ds, &r, rd->n(), &h, tracer, nullptr, nullptr), "find closest hit");
+}
+
+void batch_rt::compute_any_hit() {
+    if (!rd) throw std::logic_error("tracer used without ray data");
+    if (!pf->ds) throw std::logic_error("no committed scene");
+    hiprt_rays r = rd->in();
+    hiprt_hits h = rd->hit();
+    check(hiprt_trace_any(pf->ds, &r, rd->n(), rd->occluded.as<uint8_t>(), &h, tracer, nullptr, nullptr),
+          "find any hit");
+}
+
+// ---------------------------------------------------------------- steps
+
+struct rd_step : public wf::step {
+    raydata *rd = nullptr;
+    void use(raydata *r) { rd = r; }
+};
+
+struct initialize_framebuffer : public wf::step {
+    static constexpr char id[] = "initialize framebuffer";
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        pf->framebuffer.resize(16ull * rc->w * rc->h);
+        hip_check(hipMemsetAsync(pf->framebuffer.p, 0, pf->framebuffer.bytes, nullptr), "hipMemsetAsync");
+    }
+};
+
+struct sample_camera_rays : public rd_step {
+    static constexpr char id[] = "sample camera rays";
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        float cam[12];
+        rc->camera(cam);
+        hiprt_ray_buffers o = rd->out();
+        check(hiprt_camera_rays(cam, &o, nullptr), id);
+    }
+};
+
+struct find_closest_hits : public wf::find_closest_hits {
+    std::string id_str = id;
+    raydata *rd = nullptr;
+    find_closest_hits() : wf::find_closest_hits(nullptr) {}
+    void use(wf::raydata *r) override { rd = dynamic_cast<raydata *>(r); }
+    void run() override {
+        time_this_wf_step;
+        pf->rt->use(rd);
+        pf->rt->compute_closest_hit();
+    }
+};
+
+struct find_any_hits : public wf::find_any_hits {
+    std::string id_str = id;
+    raydata *rd = nullptr;
+    find_any_hits() : wf::find_any_hits(nullptr) {}
+    void use(wf::raydata *r) override { rd = dynamic_cast<raydata *>(r); }
+    void run() override {
+        time_this_wf_step;
+        pf->rt->use(rd);
+        pf->rt->compute_any_hit();
+    }
+};
+
+struct add_hitpoint_albedo : public rd_step {
+    static constexpr char id[] = "add hitpoint albedo";
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        hiprt_hits h = rd->hit();
+        check(hiprt_add_hitpoint_albedo(pf->ds, &h, rd->n(), pf->framebuffer.as<float>(), nullptr), id);
+    }
+};
+
+struct sample_light_dir : public wf::step {
+    static constexpr char id[] = "sample light dir";
+    std::string id_str = id;
+    raydata *cam = nullptr, *shadow = nullptr;
+    void run() override {
+        time_this_wf_step;
+        if (rc->scene.lights.empty()) throw std::logic_error("direct-wf needs a pointlight");
+        hiprt_rays r = cam->in();
+        hiprt_hits h = cam->hit();
+        hiprt_ray_buffers o = shadow->out();
+        check(hiprt_shadow_rays(pf->ds, &r, &h, cam->n(), rc->scene.lights[0].pos, &o, nullptr), id);
+    }
+};
+
+struct integrate_light_sample : public wf::step {
+    static constexpr char id[] = "integrate light sample";
+    std::string id_str = id;
+    raydata *cam = nullptr, *shadow = nullptr;
+    void run() override {
+        time_this_wf_step;
+        const pointlight &l = rc->scene.lights[0];
+        hiprt_rays cr = cam->in(), sr = shadow->in();
+        hiprt_hits h = cam->hit();
+        check(hiprt_integrate_light(pf->ds, &cr, &h, &sr, shadow->occluded.as<uint8_t>(), cam->n(), l.pos, l.col,
+                                    pf->framebuffer.as<float>(), nullptr),
+              id);
+    }
+};
+
+struct sample_cos_weighted_dir : public wf::step {
+    static constexpr char id[] = "sample cos weighted dir";
+    std::string id_str = id;
+    raydata *cam = nullptr, *bounce = nullptr;
+    uint64_t seed = 1;
+    int level = 0;
+    void run() override {
+        time_this_wf_step;
+        hiprt_rays r = cam->in();
+        hiprt_hits h = cam->hit();
+        hiprt_ray_buffers o = bounce->out();
+        check(hiprt_bounce_cosine(pf->ds, &r, &h, cam->n(), seed++, level, &o, nullptr), id);
+    }
+};
+
+struct download_framebuffer : public wf::step {
+    static constexpr char id[] = "download framebuffer";
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        size_t n = (size_t)rc->w * rc->h;
+        rc->framebuffer.resize(4 * n);
+        hip_check(hipMemcpy(rc->framebuffer.data(), pf->framebuffer.p, 16 * n, hipMemcpyDeviceToHost), "hipMemcpy");
+        for (size_t i = 0; i < n; ++i) {
+            float wgt = rc->framebuffer[4 * i + 3];
+            if (wgt > 0)
+                for (int k = 0; k < 4; ++k) rc->framebuffer[4 * i + k] /= wgt;
+        }
+    }
+};
+
+struct build_accel_struct : public wf::build_accel_struct {
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        pf->build();
+    }
+};
+
+// ---------------------------------------------------------------- platform
+
+platform::platform(const std::vector<std::string> &args) : wf::platform("hip") {
+    if (pf) throw std::logic_error("The " + name + " platform is already set up");
+    for (auto &a : args) std::cerr << "Platform hip does not support the argument " << a << std::endl;
+    check(hiprt_create(0, &ctx), "platform hip");
+    pf = this;
+    for (int t = 0; t < HIPRT_TRACER_COUNT; ++t) {
+        std::string n = hiprt_tracer_name(t);
+        tracers[n] = [t]() -> wf::batch_ray_tracer * { return new batch_rt(t); };
+    }
+    link_tracer(hiprt_tracer_name(hiprt_tracer_id("default")), "default");
+    register_wf_step_by_id(, initialize_framebuffer);
+    register_wf_step_by_id(, sample_camera_rays);
+    register_wf_step_by_id(, hip::find_closest_hits);
+    register_wf_step_by_id(, hip::find_any_hits);
+    register_wf_step_by_id(, add_hitpoint_albedo);
+    register_wf_step_by_id(, sample_light_dir);
+    register_wf_step_by_id(, integrate_light_sample);
+    register_wf_step_by_id(, sample_cos_weighted_dir);
+    register_wf_step_by_id(, download_framebuffer);
+    register_wf_step_by_id(, hip::build_accel_struct);
+    timer = new event_timer;
+}
+
+platform::~platform() {
+    (void)hipDeviceSynchronize();
+    if (ds) hiprt_scene_destroy(ds);
+    if (bvh) hiprt_bvh_free(bvh);
+    framebuffer.resize(0);
+    if (ctx) hiprt_destroy(ctx);
+    pf = nullptr;
+}
+
+raydata *platform::allocate_raydata() { return new raydata(rc->w, rc->h); }
+
+void platform::commit_scene(wf::scene *s) {
+    sd = s;
+    if (!rt) rt = dynamic_cast<batch_rt *>(select("default"));
+    for (auto *st : scene_steps) st->run();
+}
+
+void platform::build() {
+    if (!sd || !sd->geometry) throw std::logic_error("There is no scene data to work with");
+    if (ds) hiprt_scene_destroy(ds);
+    if (bvh) hiprt_bvh_free(bvh);
+    ds = nullptr;
+    bvh = nullptr;
+    check(hiprt_bvh_build(sd->geometry, &bvh_params, &bvh), "build accel struct");
+    check(hiprt_scene_commit(ctx, sd->geometry, bvh, HIPRT_BVH2, tri_format, &ds), "build accel struct");
+}
+
+#define error(x)                                                                  \
+    {                                                                             \
+        std::cerr << "command (" << command << "): " << x << std::endl;            \
+        return true;                                                              \
+    }
+#define check_in_complete(x)                                                      \
+    {                                                                             \
+        if (in.bad() || in.fail() || !in.eof()) error(x);                         \
+    }
+
+// "raytracer <variant>" (rt/cuda/platform.cpp:102-131) and the BVH settings
+// of batch_rt::interprete (rt/cuda/base.cpp:119-155), which here really are
+// routed to the platform.
+bool platform::interprete(const std::string &command, std::istringstream &in) {
+    if (command == "raytracer") {
+        std::string variant;
+        in >> variant;
+        check_in_complete("Syntax error, requires a hip ray tracer variant name");
+        if (tracers.count(variant) == 0) error("There is no such hip ray tracer variant");
+        rt = dynamic_cast<batch_rt *>(select(variant));
+        return true;
+    }
+    if (command == "bvh") {
+        std::string sub;
+        in >> sub;
+        if (sub == "type") {
+            std::string t;
+            in >> t;
+            check_in_complete("Syntax error, \"bvh type\" requires exactly one string value");
+            if (t == "om") bvh_params.split = 0;
+            else if (t == "sm") bvh_params.split = 1;
+            else if (t == "sah") bvh_params.split = 2;
+            else error("Parameter error, \"bvh type\" must be one of \"sm\", \"om\", \"sah\"");
+            return true;
+        }
+        if (sub == "max_tris" || sub == "planes") {
+            int v;
+            in >> v;
+            check_in_complete("Syntax error, \"bvh " << sub << "\" requires exactly one positive integer value");
+            if (v < 1) error("Parameter error, value must be positive");
+            (sub == "max_tris" ? bvh_params.max_tris : bvh_params.planes) = v;
+            return true;
+        }
+        if (sub == "esc") {
+            std::string v;
+            in >> v;
+            check_in_complete("Syntax error, \"bvh esc\" requires on or off");
+            if (v != "on" && v != "off") error("Syntax error, \"bvh esc\" requires on or off");
+            bvh_params.esc = v == "on";
+            return true;
+        }
+        error("Unknown bvh subcommand " << sub);
+    }
+    if (command == "triangles") {
+        std::string v;
+        in >> v;
+        check_in_complete("Syntax error, \"triangles\" requires exact or woop");
+        if (v == "exact") tri_format = HIPRT_TRI_EXACT;
+        else if (v == "woop") tri_format = HIPRT_TRI_WOOP;
+        else error("Syntax error, \"triangles\" requires exact or woop");
+        return true;
+    }
+    return false;
+}
+
+#undef error
+#undef check_in_complete
+
+// ---------------------------------------------------------------- algorithms
+
+namespace {
+
+// primary-wf: camera rays, closest hit, hit-point albedo (gi/primary-hit.cpp:85-117).
+class primary_wf : public simple_algorithm {
+    std::unique_ptr<raydata> rd;
+
+public:
+    primary_wf() {
+        rd.reset(pf->allocate_raydata());
+        auto *cam = pf->step<sample_camera_rays>();
+        auto *hit = pf->step<hip::find_closest_hits>();
+        auto *alb = pf->step<add_hitpoint_albedo>();
+        cam->use(rd.get());
+        hit->use(rd.get());
+        alb->use(rd.get());
+        frame_preparation_steps.push_back(pf->step<initialize_framebuffer>());
+        sampling_steps = {cam, hit, alb};
+        frame_finalization_steps.push_back(pf->step<download_framebuffer>());
+    }
+};
+
+// direct-wf with a point light: camera rays, closest hit, shadow rays, any
+// hit ("find occluders"), integrate (gi/direct.cpp:527-552).
+class direct_wf : public simple_algorithm {
+    std::unique_ptr<raydata> cam_rd, shadow_rd;
+
+public:
+    direct_wf() {
+        cam_rd.reset(pf->allocate_raydata());
+        shadow_rd.reset(pf->allocate_raydata());
+        auto *cam = pf->step<sample_camera_rays>();
+        auto *hit = pf->step<hip::find_closest_hits>();
+        auto *light = pf->step<sample_light_dir>();
+        auto *occ = pf->step<hip::find_any_hits>("find occluders");
+        auto *integ = pf->step<integrate_light_sample>();
+        cam->use(cam_rd.get());
+        hit->use(cam_rd.get());
+        light->cam = cam_rd.get();
+        light->shadow = shadow_rd.get();
+        occ->use(shadow_rd.get());
+        integ->cam = cam_rd.get();
+        integ->shadow = shadow_rd.get();
+        frame_preparation_steps.push_back(pf->step<initialize_framebuffer>());
+        sampling_steps = {cam, hit, light, occ, integ};
+        frame_finalization_steps.push_back(pf->step<download_framebuffer>());
+    }
+};
+
+// pt-wf: the batched wavefront path tracer of the library, one frame per
+// sample, the accumulator averaged into the framebuffer at the end.
+class pt_wf : public simple_algorithm {
+    hiprt_pt pt = nullptr;
+    int frames = 0;
+
+public:
+    ~pt_wf() override {
+        if (pt) hiprt_pt_destroy(pt);
+    }
+    void prepare_frame() override {
+        simple_algorithm::prepare_frame();
+        if (pt) hiprt_pt_destroy(pt);
+        pt = nullptr;
+        float cam[12];
+        rc->camera(cam);
+        int tracer = pf->rt ? pf->rt->tracer : -1;
+        if (tracer == HIPRT_TRACER_WHILE_WHILE) tracer = -1;   // batched needs a persistent tracer
+        hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1};
+        check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
+        frames = 0;
+    }
+    bool compute_sample(int sppx, wf::timer *t) override {
+        if (current_sample_index >= sppx) return false;
+        ++current_sample_index;
+        t->start("path trace frame");
+        check(hiprt_pt_render(pt, (uint32_t)frames++, nullptr), "pt-wf");
+        t->stop("path trace frame");
+        t->synchronize();
+        return current_sample_index < sppx;
+    }
+    void finalize_frame(wf::timer *t) override {
+        size_t n = (size_t)rc->w * rc->h;
+        std::vector<float> rgb(3 * n);
+        check(hiprt_pt_image(pt, rgb.data()), "pt-wf");
+        rc->framebuffer.assign(4 * n, 1.0f);
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) rc->framebuffer[4 * i + k] = rgb[3 * i + k] / (float)(frames ? frames : 1);
+        if (t) t->synchronize();
+    }
+};
+
+} // namespace
+
+void print_stats(std::ostream &out) {
+    for (auto &kv : event_timer::totals())
+        out << kv.first << ": " << kv.second.first << " ms in " << kv.second.second << " calls ("
+            << kv.second.first / (kv.second.second ? kv.second.second : 1) << " ms avg)" << std::endl;
+}
+
+void clear_stats() { event_timer::totals().clear(); }
+
+std::unique_ptr<simple_algorithm> make_algorithm(const std::string &name) {
+    if (!pf) throw std::logic_error("wavefront algorithms need the hip platform");
+    if (name == "primary-wf") return std::make_unique<primary_wf>();
+    if (name == "direct-wf") return std::make_unique<direct_wf>();
+    if (name == "pt-wf") return std::make_unique<pt_wf>();
+    return nullptr;
+}
+
+} // namespace hip
+} // namespace wf

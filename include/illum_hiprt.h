@@ -193,6 +193,23 @@ int hiprt_camera_rays(const float *cam, const hiprt_ray_buffers *out, void *stre
 int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, uint64_t seed,
                         int32_t level, const hiprt_ray_buffers *out, void *stream);
 
+/* framebuffer[i] += (albedo of hit i's material, 1), (0,0,0,1) on a miss:
+ * "add hitpoint albedo" (rt/cpu/wavefront.cpp:105-119).  d_framebuffer holds
+ * 4 floats per ray (rgb, weight). */
+int hiprt_add_hitpoint_albedo(hiprt_scene sc, const hiprt_hits *hits, uint32_t n, float *d_framebuffer, void *stream);
+/* Shadow ray from hit i towards the point light: origin = hit point, unit
+ * direction, t_max = distance - 1e-4 (ray::length_exclusive, libgi/rt.h:30);
+ * the disabled ray for misses.  The point-light case of "sample light dir"
+ * (rt/cpu/bounce.cpp:74-106). */
+int hiprt_shadow_rays(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, const float *light,
+                      const hiprt_ray_buffers *out, void *stream);
+/* framebuffer[i] += diffuse direct light of the point light (power rgb) at
+ * hit i when its shadow ray is unoccluded, weight += 1: "integrate light
+ * sample" (rt/cpu/bounce.cpp:132-151) for a Lambertian surface. */
+int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *hits,
+                          const hiprt_rays *shadow_rays, const uint8_t *d_occluded, uint32_t n, const float *light,
+                          const float *power, float *d_framebuffer, void *stream);
+
 /* ------------------------------------------------------- wavefront path tracer
  * A diffuse path tracer over one committed scene and camera: camera rays,
  * closest hit, shading (emission added, cosine-weighted bounce, throughput

@@ -1,0 +1,145 @@
+"""The script front end (illum-rt_amd/rtgi-hip): RTGI's command protocol on
+the hip platform.  CPU tests check the error behaviour the reference's
+interaction.cpp has; GPU tests run whole scripts and compare the written
+images with the oracle / the library."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hiprt
+import oracle_py as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "illum-rt_amd", "rtgi-hip")
+
+
+def run_script(text, tmp_path, timeout=120):
+    p = tmp_path / "script.rtgi"
+    p.write_text(text)
+    r = subprocess.run([DRIVER, str(p)], capture_output=True, text=True, timeout=timeout, cwd=tmp_path)
+    return r
+
+
+def read_pfm(path):
+    with open(path, "rb") as f:
+        assert f.readline().strip() == b"PF"
+        w, h = map(int, f.readline().split())
+        scale = float(f.readline())
+        data = np.frombuffer(f.read(), dtype="<f4" if scale < 0 else ">f4")
+    return data.reshape(h, w, 3)
+
+
+def test_driver_errors_follow_the_reference(tmp_path):
+    r = run_script("echo hi\nplatform cuda\nraytracer while-while\nrun\nresolution 0 5\nsppx x\nbogus\n"
+                   "generate 9\nbvh type sah\n# comment\n// comment\n", tmp_path)
+    assert r.returncode == 0
+    err = r.stderr.splitlines()
+    assert "command (platform): There is no platform called 'cuda'" in err
+    assert any(e.startswith("command (raytracer): No platform selected") for e in err)
+    assert "command (run): Invalid platform" in err
+    assert "command (resolution): Resolution must be positive" in err
+    assert "command (sppx): Syntax error, requires exactly one positive integral value" in err
+    assert "command (bogus): Unknown command" in err
+    assert r.stdout.strip() == "hi"
+    assert len(err) == 8
+
+
+SCRIPT_HEAD = """platform hip
+generate {cfg}
+resolution {w} {h}
+raytracer {tracer}
+add-scene-step build accel struct
+commit
+"""
+
+
+@pytest.mark.gpu
+def test_primary_wf_image_matches_oracle(tmp_path, torch_cuda):
+    print("case: rtgi-hip primary-wf, 8k random triangles, 160x120", flush=True)
+    w, h = 160, 120
+    r = run_script(SCRIPT_HEAD.format(cfg=1, w=w, h=h, tracer="while-while") +
+                   "algo primary-wf\noutfile p.pfm\nrun\nstats print\n", tmp_path)
+    assert r.returncode == 0 and not r.stderr.strip(), r.stderr
+    img = read_pfm(tmp_path / "p.pfm")
+    sc = hiprt.HostScene.generate(1)
+    pos, idx, _ = sc.arrays()
+    bvh = hiprt.Bvh(sc)
+    cam = sc.cam.copy()
+    cam[10], cam[11] = w, h
+    rays = hiprt.camera_rays(cam)
+    t, *_ = orc.trace_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, rays)
+    want = np.where(t != np.float32(3.4028234663852886e38), np.float32(0.8), np.float32(0)).reshape(h, w)
+    assert np.array_equal(img[..., 0], want) and np.array_equal(img[..., 2], want)
+    assert "find closest hit" in r.stdout
+
+
+@pytest.mark.gpu
+def test_direct_wf_shadows_match_oracle(tmp_path, torch_cuda):
+    print("case: rtgi-hip direct-wf, hall 192x108, point light", flush=True)
+    w, h = 192, 108
+    r = run_script(SCRIPT_HEAD.format(cfg=2, w=w, h=h, tracer="dynamic-while-while") +
+                   "pointlight replace pos 20 13.5 7.5 col 400 400 400\nalgo direct-wf\noutfile d.pfm\nrun\n",
+                   tmp_path)
+    assert r.returncode == 0 and not r.stderr.strip(), r.stderr
+    img = read_pfm(tmp_path / "d.pfm")
+    # the lit set must be exactly the pixels whose shadow ray the oracle finds unoccluded
+    sc = hiprt.HostScene.generate(2)
+    pos, idx, mat = sc.arrays()
+    bvh = hiprt.Bvh(sc)
+    cam = sc.cam.copy()
+    cam[10], cam[11] = w, h
+    rays = hiprt.camera_rays(cam)
+    t, u, v, i = orc.trace_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, rays)
+    hit = t != np.float32(3.4028234663852886e38)
+    A, B, C = (pos[idx[i, k]].astype(np.float32) for k in range(3))
+    x = (np.float32(1) - u - v)[:, None] * A + u[:, None] * B + v[:, None] * C
+    light = np.array([20, 13.5, 7.5], np.float32)
+    to = light - x
+    dist = np.sqrt((to * to).sum(1))
+    shadow = np.zeros((8, len(t)), np.float32)
+    shadow[0:3] = x.T
+    shadow[3] = 1e-4
+    shadow[4:7] = (to / dist[:, None]).T
+    shadow[7] = np.where(hit, dist - np.float32(1e-4), -np.float32(3.4e38))
+    occ = orc.trace_any(pos, idx, bvh.nodes, bvh.root, bvh.index, shadow)
+    lit_img = img.reshape(-1, 3).sum(1) > 0
+    candidates = hit & (occ == 0)
+    # lit pixels are unoccluded hits (the numpy shadow rays here may differ
+    # from the device's in a last bit, so a grazing pixel or two may flip);
+    # unlit unoccluded hits face away from the light
+    assert (lit_img & ~candidates).sum() <= max(2, 1e-4 * len(t))
+    assert lit_img.sum() > 0.2 * candidates.sum()
+
+
+@pytest.mark.gpu
+def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda):
+    print("case: rtgi-hip pt-wf, hall 128x72, 2 spp vs hiprt.PathTracer", flush=True)
+    w, h = 128, 72
+    r = run_script(SCRIPT_HEAD.format(cfg=3, w=w, h=h, tracer="dynamic-while-while") +
+                   "sppx 2\nalgo pt-wf\noutfile pt.pfm\nrun\n", tmp_path, timeout=300)
+    assert r.returncode == 0 and not r.stderr.strip(), r.stderr
+    img = read_pfm(tmp_path / "pt.pfm")
+    sc = hiprt.HostScene.generate(3)
+    bvh = hiprt.Bvh(sc)
+    ctx = hiprt.Context(0)
+    ds = hiprt.DeviceScene(ctx, sc, bvh)
+    cam = sc.cam.copy()
+    cam[10], cam[11] = w, h
+    pt = hiprt.PathTracer(ds, cam, seed=1, batched=True)
+    pt.render(0)
+    pt.render(1)
+    want = pt.image() / np.float32(2)
+    pt.close()
+    ds.close()
+    ctx.close()
+    assert np.array_equal(img.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_unknown_tracer_and_stale_bvh(tmp_path, torch_cuda):
+    r = run_script("platform hip\nraytracer if-if\ngenerate 1\nalgo primary-wf\nrun\n", tmp_path)
+    err = r.stderr.splitlines()
+    assert "command (raytracer): There is no such hip ray tracer variant" in err
+    assert any(e.startswith("command (run): We have to have a scene loaded") for e in err)
