@@ -230,7 +230,8 @@ namespace {
 // Checks a flattened tree and computes its depth.  Children must have larger
 // indices than their parent, which rules out cycles; leaf slot ranges must lie
 // inside the triangle records.
-std::string check_layout(rt::flat_scene &f) {
+std::string check_layout(rt::flat_scene &f, uint32_t &stack_need) {
+    stack_need = 0;
     uint32_t n_slots = (uint32_t)f.tris.size();
     auto leaf_ok = [&](int32_t ref) {
         uint64_t s = rt::leaf_slot(ref), c = rt::leaf_count(ref);
@@ -248,7 +249,7 @@ std::string check_layout(rt::flat_scene &f) {
     }
     size_t n = f.bvh_kind == rt::BVH2 ? f.nodes2.size() : f.nodes4.size();
     if (f.root_ref < 0 || (size_t)f.root_ref >= n) return "root index out of range";
-    std::vector<uint32_t> depth(n, 1);
+    std::vector<uint32_t> depth(n, 1), need(n, 0);
     for (size_t i = n; i-- > 0;) {
         int32_t refs[4];
         int nref = 0;
@@ -276,13 +277,20 @@ std::string check_layout(rt::flat_scene &f) {
             depth[i] = std::max(depth[i], depth[r] + 1);
         }
         if (valid == 0) return "node without children";
+        // a visit of node i pushes at most valid-1 entries, then descends
+        uint32_t below = 0;
+        for (int k = 0; k < nref; ++k)
+            if (refs[k] != rt::kInvalidRef && !rt::ref_is_leaf(refs[k])) below = std::max(below, need[refs[k]]);
+        need[i] = (uint32_t)(valid - 1) + below;
     }
     f.depth = depth[f.root_ref];
+    stack_need = need[f.root_ref];
     return "";
 }
 
 int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
-    std::string e = check_layout(f);
+    uint32_t stack_need = 0;
+    std::string e = check_layout(f, stack_need);
     if (!e.empty()) return fail(HIPRT_ERR_ARG, "invalid BVH layout: " + e);
     HIP_TRY(hipSetDevice(ctx->device));
     std::unique_ptr<hiprt_scene_s> s(new (std::nothrow) hiprt_scene_s);
@@ -304,6 +312,7 @@ int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
     s->desc.bvh = h.bvh_kind;
     s->desc.tri = h.tri_kind;
     s->desc.depth = (int32_t)h.depth;
+    s->stack_need = (int)stack_need;
     *out = s.release();
     return ok();
 }
@@ -345,7 +354,9 @@ int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32
         if (m >= sc.mats.size()) return fail(HIPRT_ERR_ARG, "triangle material id out of range");
     HIP_TRY(ds->tri_geo.upload(geo.data(), geo.size() * 4));
     HIP_TRY(ds->mats.upload(mats.data(), mats.size() * 4));
+    HIP_TRY(ds->tri_vidx.upload(sc.idx.data(), sc.idx.size() * 4));
     ds->n_tris = nt;
+    ds->n_vert = (uint32_t)sc.pos.size();
     *out = guard.release();
     return ok();
 }
@@ -443,17 +454,19 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
         return fail(HIPRT_ERR_ARG, "any-hit needs an occlusion array or hit planes");
     if (any && hits && hits->d_t && (!hits->d_u || !hits->d_v || !hits->d_id))
         return fail(HIPRT_ERR_ARG, "null hit plane");
-    if (sc->desc.bvh != rt::BVH2) return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH traversal is not available for this tracer");
     HIP_TRY(hipSetDevice(sc->ctx->device));
     rt::ray_soa r{rays->d_ox, rays->d_oy, rays->d_oz, rays->d_tmin, rays->d_dx, rays->d_dy, rays->d_dz, rays->d_tmax};
     rt::hit_soa h{nullptr, nullptr, nullptr, nullptr};
     if (hits) h = rt::hit_soa{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int depth = sc->desc.depth;
-    bool persistent_fits = depth <= rt::dev::kPersistentStackCap;
+    int need = sc->stack_need;
+    bool persistent_fits = need <= rt::dev::persistent_stack_cap(sc->desc.bvh);
+    if (sc->desc.bvh == rt::BVH4 && (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits))
+        return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH needs a persistent tracer with a stack of " +
+                                               std::to_string(need) + " entries");
     if (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits) {
-        if (depth > rt::dev::kPlainStackCap - 1)
-            return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack (" + std::to_string(depth) + " levels)");
+        if (need > rt::dev::kPlainStackCap - 1)
+            return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack (" + std::to_string(need) + " entries)");
         HIP_TRY(rt::dev::launch_trace_plain(sc->desc, r, h, occluded, n, any, s,
                                             opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats)
                                                  : nullptr));

@@ -65,7 +65,8 @@ struct hiprt_bvh_s {
 
 struct hiprt_context_s {
     int device = 0;
-    hiprt_detail::dbuf counter;   // work counter of the persistent kernels
+    hiprt_detail::dbuf counter;       // work counter of the persistent kernels
+    hiprt_detail::dbuf batch_table;   // descriptor table of the last batched trace
 };
 
 struct hiprt_scene_s {
@@ -73,6 +74,8 @@ struct hiprt_scene_s {
     rt::flat_scene host;                         // host copy of the uploaded layout
     hiprt_detail::dbuf nodes, tris, slot_ids;    // traversal data
     hiprt_detail::dbuf tri_geo, mats;            // shading data (scenes committed from a host scene)
-    uint32_t n_tris = 0;
+    hiprt_detail::dbuf tri_vidx;                 // vertex indices per triangle (adjoint scatter)
+    uint32_t n_tris = 0, n_vert = 0;
+    int stack_need = 0;                          // most stack entries any traversal can hold
     rt::scene_desc desc{};
 };

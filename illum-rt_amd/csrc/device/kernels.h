@@ -16,11 +16,25 @@ namespace dev {
 constexpr int kPlainStackCap = 64;
 
 // Persistent kernels: per-lane stack of kRegStack entries in VGPRs backed by
-// kLdsStack entries per lane in LDS.  Total capacity kRegStack + kLdsStack.
+// kLdsStack (binary BVH) or kLdsStack4 (4-wide BVH) entries per lane in LDS.
 constexpr int kRegStack = 4;
 constexpr int kLdsStack = 28;
+constexpr int kLdsStack4 = 44;
 constexpr int kPersistentStackCap = kRegStack + kLdsStack;
 constexpr int kPersistentWavesPerBlock = 4;
+
+// Stack entries the persistent kernels hold for a BVH of the given format.
+int persistent_stack_cap(int bvh);
+
+// One entry of a multi-scene batch: a scene and its own ray and hit buffers;
+// its rays occupy [first, first + n) of the launch's index space.
+struct batch_desc {
+    scene_desc scene;
+    ray_soa rays;
+    hit_soa hits;
+    uint8_t *occluded;
+    uint32_t first, n;
+};
 
 // With `stats` non-null the counting variant runs and adds the number of inner
 // nodes fetched to stats[0] and of triangle records tested to stats[1].
@@ -42,6 +56,12 @@ struct persistent_cfg {
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
                                    uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,
                                    bool any, const persistent_cfg &cfg, hipStream_t stream);
+
+// One persistent launch over a descriptor table (device memory, sorted by
+// `first`, first entry at 0, `total` rays in all).  All scenes must share the
+// triangle and BVH formats.
+hipError_t launch_trace_batch(const batch_desc *d_table, uint32_t n_desc, uint32_t total, int tri, int bvh, bool any,
+                              uint32_t *counter, const persistent_cfg &cfg, hipStream_t stream);
 
 // Grid size (blocks) of the persistent kernel instantiation for these
 // settings: blocks resident per CU times the CU count.

@@ -4,21 +4,28 @@
 // takes work from one global counter with a single atomicAdd per fetch (the
 // first lane adds the number of idle lanes and the wave splits the range by
 // lane rank).  Per lane the traversal keeps the reference's decisions -- both
-// child boxes tested, nearer child first, strict "<" on the best t -- and adds:
+// child boxes tested with intersect4's arithmetic, nearer child first, strict
+// "<" on the best t -- and adds:
 //
-//   * a stack whose newest kRegStack entries live in VGPRs (a shift register
-//     with static indices, so no scratch) and whose older entries spill to an
-//     LDS column owned by the lane (level-major, lane-minor: conflict free);
+//   * a stack whose newest REG entries live in VGPRs (a shift register with
+//     static indices, so no scratch) and whose older entries spill to an LDS
+//     column owned by the lane (level-major, lane-minor: conflict free);
 //   * leaf postponement (SPEC): a lane that reaches a leaf parks it and keeps
 //     descending until every lane still searching has a parked leaf, so the
 //     wave runs triangle tests together instead of one lane at a time;
 //   * dynamic refill (REFILL): when fewer than `refill_thr` lanes still have a
 //     ray, the wave leaves the traversal loop and hands the idle lanes new
-//     rays, so long rays do not hold 63 idle lanes hostage.
+//     rays, so long rays do not hold 63 idle lanes hostage;
+//   * 4-wide nodes (BVH == 4): four child boxes per 128-byte node, hit
+//     children sorted by entry distance, the nearest entered, the others
+//     pushed farthest first;
+//   * a descriptor table (MULTI): the rays of many scenes form one index
+//     space; each lane binds the scene of the ray it fetched, so one launch
+//     traces uneven batches of many scenes.
 //
-// Postponement and refill change the order in which boxes see the running
-// best t, so the result can differ from the plain kernel only between
-// triangles with exactly equal t.
+// Postponement, refill and 4-wide nodes change the order in which boxes see
+// the running best t, so results can differ from the plain kernel only
+// between triangles with exactly equal t.
 #include "kernels.h"
 #include "trace_core.h"
 
@@ -27,11 +34,11 @@ namespace dev {
 
 namespace {
 
-template <int REG>
+template <int REG, int LDS>
 struct lane_stack {
     int32_t s[REG > 0 ? REG : 1];
-    int rs;   // entries held in registers
-    int ls;   // entries held in LDS
+    int rs;         // entries held in registers
+    int ls;         // entries held in LDS
     int32_t *col;   // this lane's LDS column, entries 64 words apart
 
     __device__ __forceinline__ void reset() {
@@ -40,11 +47,11 @@ struct lane_stack {
     }
     __device__ __forceinline__ void push(int32_t x) {
         if (REG == 0) {
-            if (ls < kLdsStack) col[64 * ls++] = x;
+            if (ls < LDS) col[64 * ls++] = x;
             return;
         }
         if (rs == REG) {
-            if (ls < kLdsStack) col[64 * ls++] = s[REG - 1];
+            if (ls < LDS) col[64 * ls++] = s[REG - 1];
         } else {
             ++rs;
         }
@@ -65,23 +72,76 @@ struct lane_stack {
     }
 };
 
+// The scene a lane traces.  For single-scene launches these are copies of
+// the kernel arguments and stay wave-uniform.
+struct lane_scene {
+    const void *nodes;
+    const tri_rec *tris;
+    const uint32_t *slot_ids;
+    int32_t root;
+};
+
+__device__ __forceinline__ lane_scene scene_of(const scene_desc &sc) {
+    return lane_scene{sc.nodes, sc.tris, sc.slot_ids, sc.root_ref};
+}
+
+template <int TRI>
+__device__ __forceinline__ bool test_rec(const lane_scene &ls, uint32_t slot, const dray &r, float best, float &t,
+                                         float &u, float &v) {
+    const float4 *p = reinterpret_cast<const float4 *>(ls.tris + slot);
+    float4 r0 = p[0], r1 = p[1], r2 = p[2];
+    if (TRI == TRI_EXACT) return tri_exact(r0, r1, r2, r, best, t, u, v);
+    return tri_woop(r0, r1, r2, r, best, t, u, v);
+}
+
+template <int TRI>
+__device__ __forceinline__ uint32_t rec_tri_id(const lane_scene &ls, uint32_t slot) {
+    if (TRI == TRI_EXACT) return __float_as_uint(reinterpret_cast<const float4 *>(ls.tris + slot)[0].w);
+    return ls.slot_ids[slot];
+}
+
+__device__ __forceinline__ void cswap(float &ka, int32_t &ra, float &kb, int32_t &rb) {
+    if (kb < ka) {
+        float tk = ka;
+        ka = kb;
+        kb = tk;
+        int32_t tr = ra;
+        ra = rb;
+        rb = tr;
+    }
+}
+
+// Index of the descriptor whose ray range holds global ray `idx`
+// (descriptors sorted by `first`, the first starting at 0).
+__device__ __forceinline__ uint32_t find_desc(const batch_desc *table, uint32_t n_desc, uint32_t idx) {
+    uint32_t lo = 0, hi = n_desc;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (table[mid].first <= idx) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 } // namespace
 
-template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, bool COUNT>
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, int LDS, bool COUNT, int BVH, bool MULTI>
 __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits,
                                                           uint8_t *occluded, uint32_t n_static,
                                                           const uint32_t *count_ptr, uint32_t *counter,
-                                                          int refill_thr, unsigned long long *stats) {
-    __shared__ int32_t lds_stack[kPersistentWavesPerBlock * kLdsStack * 64];
+                                                          int refill_thr, unsigned long long *stats,
+                                                          const batch_desc *table, uint32_t n_desc) {
+    __shared__ int32_t lds_stack[kPersistentWavesPerBlock * LDS * 64];
     const uint32_t n = count_ptr ? *count_ptr : n_static;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const node2 *nodes = static_cast<const node2 *>(sc.nodes);
 
-    lane_stack<REG> st;
-    st.col = lds_stack + wave * kLdsStack * 64 + lane;
+    lane_stack<REG, LDS> st;
+    st.col = lds_stack + wave * LDS * 64 + lane;
     st.reset();
 
+    lane_scene ls = scene_of(sc);
+    uint32_t desc = 0;
     dray r{};
     int32_t cur = kInvalidRef, leaf = kInvalidRef;
     uint32_t ray_id = 0;
@@ -105,9 +165,17 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                 uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
                 uint32_t idx = base + rank;
                 if (idx < n) {
-                    r = load_ray(rays, idx);
-                    ray_id = idx;
-                    cur = sc.root_ref;
+                    if (MULTI) {
+                        desc = find_desc(table, n_desc, idx);
+                        const batch_desc &bd = table[desc];
+                        ls = scene_of(bd.scene);
+                        ray_id = idx - bd.first;
+                        r = load_ray(bd.rays, ray_id);
+                    } else {
+                        ray_id = idx;
+                        r = load_ray(rays, idx);
+                    }
+                    cur = ls.root;
                     leaf = kInvalidRef;
                     st.reset();
                     h.found = false;
@@ -124,22 +192,56 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                 bool searching = true;
                 while (cur >= 0 && cur != kInvalidRef) {
                     if (COUNT) ++c_nodes;
-                    const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
-                    float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-                    float dl, dr;
-                    bool hl = box_exact<ANY>(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, h.t, dl);
-                    bool hr = box_exact<ANY>(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, h.t, dr);
-                    int32_t rl = __float_as_int(n3.x), rr = __float_as_int(n3.y);
-                    if (hl && hr) {
-                        bool left_first = dl < dr;
-                        st.push(left_first ? rr : rl);
-                        cur = left_first ? rl : rr;
-                    } else if (hl) {
-                        cur = rl;
-                    } else if (hr) {
-                        cur = rr;
+                    if (BVH == 2) {
+                        const float4 *np =
+                            reinterpret_cast<const float4 *>(static_cast<const node2 *>(ls.nodes) + cur);
+                        float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+                        float dl, dr;
+                        bool hl = box_exact<ANY>(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, h.t, dl);
+                        bool hr = box_exact<ANY>(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, h.t, dr);
+                        int32_t rl = __float_as_int(n3.x), rr = __float_as_int(n3.y);
+                        if (hl && hr) {
+                            bool left_first = dl < dr;
+                            st.push(left_first ? rr : rl);
+                            cur = left_first ? rl : rr;
+                        } else if (hl) {
+                            cur = rl;
+                        } else if (hr) {
+                            cur = rr;
+                        } else {
+                            cur = st.pop();
+                        }
                     } else {
-                        cur = st.pop();
+                        const float4 *np =
+                            reinterpret_cast<const float4 *>(static_cast<const node4 *>(ls.nodes) + cur);
+                        float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5], rf = np[6];
+                        float k0, k1, k2, k3;
+                        int32_t c0 = __float_as_int(rf.x), c1 = __float_as_int(rf.y), c2 = __float_as_int(rf.z),
+                                c3 = __float_as_int(rf.w);
+                        bool b0 = box_exact<ANY>(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, h.t, k0) && c0 != kInvalidRef;
+                        bool b1 = box_exact<ANY>(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, h.t, k1) && c1 != kInvalidRef;
+                        bool b2 = box_exact<ANY>(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, h.t, k2) && c2 != kInvalidRef;
+                        bool b3 = box_exact<ANY>(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, h.t, k3) && c3 != kInvalidRef;
+                        int nh = (int)b0 + (int)b1 + (int)b2 + (int)b3;
+                        // misses sort after hits; a NaN entry distance (axis-parallel
+                        // ray on a slab plane) sorts last among the hits
+                        k0 = b0 ? ((k0 == k0) ? k0 : FLT_MAX) : INFINITY;
+                        k1 = b1 ? ((k1 == k1) ? k1 : FLT_MAX) : INFINITY;
+                        k2 = b2 ? ((k2 == k2) ? k2 : FLT_MAX) : INFINITY;
+                        k3 = b3 ? ((k3 == k3) ? k3 : FLT_MAX) : INFINITY;
+                        cswap(k0, c0, k1, c1);
+                        cswap(k2, c2, k3, c3);
+                        cswap(k0, c0, k2, c2);
+                        cswap(k1, c1, k3, c3);
+                        cswap(k1, c1, k2, c2);
+                        if (nh == 0) {
+                            cur = st.pop();
+                        } else {
+                            if (nh == 4) st.push(c3);
+                            if (nh >= 3) st.push(c2);
+                            if (nh >= 2) st.push(c1);
+                            cur = c0;
+                        }
                     }
                     if (SPEC) {
                         if (cur < 0 && leaf >= 0) {
@@ -162,7 +264,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                     for (uint32_t k = 0; k < cnt; ++k) {
                         float t, u, v;
                         if (COUNT) ++c_tris;
-                        if (test_slot<TRI>(sc, slot + k, r, h.t, t, u, v)) {
+                        if (test_rec<TRI>(ls, slot + k, r, h.t, t, u, v)) {
                             h.found = true;
                             h.slot = slot + k;
                             if (!ANY) {
@@ -185,15 +287,21 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
             }
             if (cur == kInvalidRef) {
                 // ---- terminated: write the result ----
+                hit_soa ho = hits;
+                uint8_t *occ = occluded;
+                if (MULTI) {
+                    ho = table[desc].hits;
+                    occ = table[desc].occluded;
+                }
                 if (ANY) {
-                    if (occluded) occluded[ray_id] = h.found ? 1 : 0;
-                    if (hits.t)
-                        store_hit(hits, ray_id, h.found ? 4294967295.0f : FLT_MAX, 0.0f, 0.0f,
+                    if (occ) occ[ray_id] = h.found ? 1 : 0;
+                    if (ho.t)
+                        store_hit(ho, ray_id, h.found ? 4294967295.0f : FLT_MAX, 0.0f, 0.0f,
                                   h.found ? 0xffffffffu : 0u);
                 } else if (h.found) {
-                    store_hit(hits, ray_id, h.t, h.u, h.v, slot_tri_id<TRI>(sc, h.slot));
+                    store_hit(ho, ray_id, h.t, h.u, h.v, rec_tri_id<TRI>(ls, h.slot));
                 } else {
-                    store_hit(hits, ray_id, FLT_MAX, 0.0f, 0.0f, 0u);
+                    store_hit(ho, ray_id, FLT_MAX, 0.0f, 0.0f, 0u);
                 }
                 alive = false;
             }
@@ -218,9 +326,11 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t 
 
 namespace {
 
+constexpr int lds_for(int bvh) { return bvh == 2 ? kLdsStack : kLdsStack4; }
+
 // Blocks of one instantiation that fit the device at once (occupancy x CUs),
 // asked once per device.
-template <int TRI, bool ANY, bool SPEC, bool REFILL>
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI>
 hipError_t grid_of(int *blocks) {
     static int cached[64] = {0};
     int dev = 0;
@@ -231,8 +341,8 @@ hipError_t grid_of(int *blocks) {
         int cus = 0, per_cu = 0;
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, false>,
-                                                         256, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, lds_for(BVH), false, BVH, MULTI>, 256, 0);
         if (e != hipSuccess) return e;
         cached[dev] = cus * (per_cu > 0 ? per_cu : 1);
     }
@@ -240,51 +350,74 @@ hipError_t grid_of(int *blocks) {
     return hipSuccess;
 }
 
-template <int TRI, bool ANY, bool SPEC, bool REFILL>
-hipError_t launch_variant(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
-                          uint32_t n, const uint32_t *count_ptr, uint32_t *counter, const persistent_cfg &cfg,
-                          hipStream_t stream) {
+struct launch_args {
+    scene_desc sc;
+    ray_soa rays;
+    hit_soa hits;
+    uint8_t *occluded;
+    uint32_t n;
+    const uint32_t *count_ptr;
+    uint32_t *counter;
+    const batch_desc *table;
+    uint32_t n_desc;
+};
+
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI>
+hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipStream_t stream, bool grid_only,
+                          int *grid_out) {
     int blocks = cfg.blocks;
     if (blocks <= 0) {
-        hipError_t e = grid_of<TRI, ANY, SPEC, REFILL>(&blocks);
+        hipError_t e = grid_of<TRI, ANY, SPEC, REFILL, BVH, MULTI>(&blocks);
         if (e != hipSuccess) return e;
     }
-    if (cfg.stats)
-        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, true>), dim3(blocks), dim3(256), 0,
-                           stream, sc, rays, hits, occluded, n, count_ptr, counter, cfg.refill_thr, cfg.stats);
+    if (grid_out) *grid_out = blocks;
+    if (grid_only) return hipSuccess;
+    constexpr int L = lds_for(BVH);
+    if (cfg.stats && !MULTI)
+        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, L, true, BVH, MULTI>), dim3(blocks),
+                           dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
+                           cfg.refill_thr, cfg.stats, a.table, a.n_desc);
     else
-        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, false>), dim3(blocks), dim3(256), 0,
-                           stream, sc, rays, hits, occluded, n, count_ptr, counter, cfg.refill_thr, cfg.stats);
+        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, L, false, BVH, MULTI>), dim3(blocks),
+                           dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
+                           cfg.refill_thr, cfg.stats, a.table, a.n_desc);
     return hipGetLastError();
 }
 
-template <int TRI, bool ANY>
-hipError_t grid_tri_any(const persistent_cfg &cfg, int *blocks) {
+template <int TRI, bool ANY, int BVH, bool MULTI>
+hipError_t dispatch_flags(const launch_args &a, const persistent_cfg &cfg, hipStream_t s, bool grid_only, int *g) {
     if (cfg.speculative) {
-        if (cfg.refill) return grid_of<TRI, ANY, true, true>(blocks);
-        return grid_of<TRI, ANY, true, false>(blocks);
+        if (cfg.refill) return launch_variant<TRI, ANY, true, true, BVH, MULTI>(a, cfg, s, grid_only, g);
+        return launch_variant<TRI, ANY, true, false, BVH, MULTI>(a, cfg, s, grid_only, g);
     }
-    if (cfg.refill) return grid_of<TRI, ANY, false, true>(blocks);
-    return grid_of<TRI, ANY, false, false>(blocks);
+    if (cfg.refill) return launch_variant<TRI, ANY, false, true, BVH, MULTI>(a, cfg, s, grid_only, g);
+    return launch_variant<TRI, ANY, false, false, BVH, MULTI>(a, cfg, s, grid_only, g);
 }
 
-template <int TRI, bool ANY>
-hipError_t launch_tri_any(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
-                          uint32_t n, const uint32_t *count_ptr, uint32_t *counter, const persistent_cfg &cfg,
-                          hipStream_t stream) {
-    if (cfg.speculative) {
-        if (cfg.refill) return launch_variant<TRI, ANY, true, true>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
-        return launch_variant<TRI, ANY, true, false>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
+template <bool MULTI>
+hipError_t dispatch(int tri, bool any, int bvh, const launch_args &a, const persistent_cfg &cfg, hipStream_t s,
+                    bool grid_only, int *g) {
+    if (tri == TRI_EXACT) {
+        if (bvh == BVH2)
+            return any ? dispatch_flags<TRI_EXACT, true, 2, MULTI>(a, cfg, s, grid_only, g)
+                       : dispatch_flags<TRI_EXACT, false, 2, MULTI>(a, cfg, s, grid_only, g);
+        return any ? dispatch_flags<TRI_EXACT, true, 4, MULTI>(a, cfg, s, grid_only, g)
+                   : dispatch_flags<TRI_EXACT, false, 4, MULTI>(a, cfg, s, grid_only, g);
     }
-    if (cfg.refill) return launch_variant<TRI, ANY, false, true>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
-    return launch_variant<TRI, ANY, false, false>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
+    if (bvh == BVH2)
+        return any ? dispatch_flags<TRI_WOOP, true, 2, MULTI>(a, cfg, s, grid_only, g)
+                   : dispatch_flags<TRI_WOOP, false, 2, MULTI>(a, cfg, s, grid_only, g);
+    return any ? dispatch_flags<TRI_WOOP, true, 4, MULTI>(a, cfg, s, grid_only, g)
+               : dispatch_flags<TRI_WOOP, false, 4, MULTI>(a, cfg, s, grid_only, g);
 }
 
 } // namespace
 
+int persistent_stack_cap(int bvh) { return kRegStack + (bvh == BVH4 ? kLdsStack4 : kLdsStack); }
+
 hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks) {
-    if (sc.tri == TRI_EXACT) return any ? grid_tri_any<TRI_EXACT, true>(cfg, blocks) : grid_tri_any<TRI_EXACT, false>(cfg, blocks);
-    return any ? grid_tri_any<TRI_WOOP, true>(cfg, blocks) : grid_tri_any<TRI_WOOP, false>(cfg, blocks);
+    launch_args a{};
+    return dispatch<false>(sc.tri, any, sc.bvh, a, cfg, nullptr, true, blocks);
 }
 
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
@@ -292,12 +425,20 @@ hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, co
                                    bool any, const persistent_cfg &cfg, hipStream_t stream) {
     hipError_t e = launch_fill_u32(counter, 0u, 1u, stream);
     if (e != hipSuccess) return e;
-    if (sc.tri == TRI_EXACT) {
-        if (any) return launch_tri_any<TRI_EXACT, true>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
-        return launch_tri_any<TRI_EXACT, false>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
-    }
-    if (any) return launch_tri_any<TRI_WOOP, true>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
-    return launch_tri_any<TRI_WOOP, false>(sc, rays, hits, occluded, n, count_ptr, counter, cfg, stream);
+    launch_args a{sc, rays, hits, occluded, n, count_ptr, counter, nullptr, 0};
+    return dispatch<false>(sc.tri, any, sc.bvh, a, cfg, stream, false, nullptr);
+}
+
+hipError_t launch_trace_batch(const batch_desc *d_table, uint32_t n_desc, uint32_t total, int tri, int bvh, bool any,
+                              uint32_t *counter, const persistent_cfg &cfg, hipStream_t stream) {
+    hipError_t e = launch_fill_u32(counter, 0u, 1u, stream);
+    if (e != hipSuccess) return e;
+    launch_args a{};
+    a.n = total;
+    a.counter = counter;
+    a.table = d_table;
+    a.n_desc = n_desc;
+    return dispatch<true>(tri, any, bvh, a, cfg, stream, false, nullptr);
 }
 
 } // namespace dev

@@ -12,6 +12,7 @@
 #include "device/kernels.h"
 #include "device/pt_kernels.h"
 #include "device/steps_kernels.h"
+#include "device/adjoint_kernels.h"
 
 #include <memory>
 #include <new>
@@ -209,10 +210,39 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
     return ok();
 }
 
+int hiprt_adjoint_closest(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n,
+                          const float *d_grad_t, const float *d_grad_u, const float *d_grad_v, float *d_grad_vertices,
+                          float *d_grad_origin, float *d_grad_dir, float *d_grad_corners, void *stream) {
+    if (!sc || !full_rays(rays) || !hits || !hits->d_t || !hits->d_id) return fail(HIPRT_ERR_ARG, "null argument");
+    if (n && (!d_grad_t || !d_grad_u || !d_grad_v)) return fail(HIPRT_ERR_ARG, "null upstream gradient");
+    if (!sc->tri_geo.p || !sc->tri_vidx.p)
+        return fail(HIPRT_ERR_UNSUPPORTED, "scene has no vertex data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::dev::adjoint_args a{soa(rays),
+                            rt::hit_soa{hits->d_t, hits->d_u, hits->d_v, hits->d_id},
+                            sc->tri_geo.as<const float4>(),
+                            sc->tri_vidx.as<const uint32_t>(),
+                            d_grad_t,
+                            d_grad_u,
+                            d_grad_v,
+                            d_grad_vertices,
+                            d_grad_origin,
+                            d_grad_dir,
+                            d_grad_corners};
+    HIP_TRY(rt::dev::launch_adjoint_closest(a, n, static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_scene_counts(hiprt_scene sc, uint32_t *n_vert, uint32_t *n_tris) {
+    if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
+    if (n_vert) *n_vert = sc->n_vert;
+    if (n_tris) *n_tris = sc->n_tris;
+    return ok();
+}
+
 int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *params, hiprt_pt *out) {
     if (!sc || !cam || !params || !out) return fail(HIPRT_ERR_ARG, "null argument");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
-    if (sc->desc.bvh != rt::BVH2) return fail(HIPRT_ERR_UNSUPPORTED, "path tracer needs a binary BVH scene");
     if (cam[10] < 1 || cam[11] < 1) return fail(HIPRT_ERR_ARG, "image size must be positive");
     if (params->max_depth < 0 || params->max_depth > 64) return fail(HIPRT_ERR_ARG, "max_depth must be 0..64");
     if (!(params->rr_survive > 0.0f && params->rr_survive <= 1.0f)) return fail(HIPRT_ERR_ARG, "rr_survive must be in (0, 1]");
@@ -234,8 +264,9 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
         return fail(HIPRT_ERR_ARG, "the batched schedule needs a persistent tracer");
     if (!p->batched && p->tracer != HIPRT_TRACER_WHILE_WHILE)
         return fail(HIPRT_ERR_ARG, "the plain schedule uses the while-while tracer");
-    int depth = sc->desc.depth;
-    if (depth > (p->batched ? rt::dev::kPersistentStackCap : rt::dev::kPlainStackCap - 1))
+    if (!p->batched && sc->desc.bvh != rt::BVH2)
+        return fail(HIPRT_ERR_UNSUPPORTED, "the plain schedule needs a binary BVH scene");
+    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh) : rt::dev::kPlainStackCap - 1))
         return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     uint32_t n = p->n;
     HIP_TRY(p->rays[0].alloc(n));

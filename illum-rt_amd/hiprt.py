@@ -52,6 +52,15 @@ class _TracerOpts(ctypes.Structure):
     _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32), ("d_fetch_stats", c_void_p)]
 
 
+class _BatchEntry(ctypes.Structure):
+    _fields_ = [("scene", c_void_p), ("rays", _Rays), ("hits", _Hits), ("occluded", c_void_p), ("n", c_uint32)]
+
+
+class _TuneResult(ctypes.Structure):
+    _fields_ = [("tracer", c_int32), ("refill_threshold", c_int32), ("blocks", c_int32), ("bvh_format", c_int32),
+                ("mrays_per_s", c_float), ("candidates", c_int32)]
+
+
 class _PtParams(ctypes.Structure):
     _fields_ = [("max_depth", c_int32), ("rr_start", c_int32), ("rr_survive", c_float), ("seed", c_uint64),
                 ("tracer", c_int32), ("batched", c_int32)]
@@ -113,6 +122,18 @@ def lib():
                                          POINTER(c_uint32), c_int32]),
             "hiprt_pt_reset": (c_int, [c_void_p, c_void_p]),
             "hiprt_pt_destroy": (c_int, [c_void_p]),
+            "hiprt_adjoint_closest": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), c_uint32, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+            "hiprt_scene_counts": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
+            "hiprt_trace_batch": (c_int, [POINTER(_BatchEntry), c_uint32, c_int32, c_int32, POINTER(_TracerOpts),
+                                          c_void_p]),
+            "hiprt_autotune": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(_Rays), c_uint32, c_int32,
+                                       POINTER(_TuneResult), POINTER(c_void_p)]),
+            "hiprt_add_hitpoint_albedo": (c_int, [c_void_p, POINTER(_Hits), c_uint32, c_void_p, c_void_p]),
+            "hiprt_shadow_rays": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), c_uint32, POINTER(c_float),
+                                          POINTER(_Rays), c_void_p]),
+            "hiprt_integrate_light": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), POINTER(_Rays), c_void_p,
+                                              c_uint32, POINTER(c_float), POINTER(c_float), c_void_p, c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -250,10 +271,13 @@ class Context:
 
 
 class DeviceScene:
-    def __init__(self, ctx, scene, bvh, bvh_format=BVH2, tri_format=TRI_EXACT):
+    def __init__(self, ctx, scene, bvh, bvh_format=BVH2, tri_format=TRI_EXACT, handle=None):
         self.ctx = ctx
-        self.h = c_void_p()
-        check(lib().hiprt_scene_commit(ctx.h, scene.h, bvh.h, bvh_format, tri_format, ctypes.byref(self.h)))
+        if handle is not None:
+            self.h = handle
+        else:
+            self.h = c_void_p()
+            check(lib().hiprt_scene_commit(ctx.h, scene.h, bvh.h, bvh_format, tri_format, ctypes.byref(self.h)))
         nn, ns, root, depth, bf, tf = c_uint32(), c_uint32(), c_int32(), c_uint32(), c_int32(), c_int32()
         check(lib().hiprt_scene_layout(self.h, ctypes.byref(nn), ctypes.byref(ns), ctypes.byref(root),
                                        ctypes.byref(depth), ctypes.byref(bf), ctypes.byref(tf)))
@@ -398,3 +422,54 @@ class PathTracer:
     def __del__(self):
         if _lib is not None:
             self.close()
+
+
+def _dp(t):
+    return c_void_p(t.data_ptr()) if t is not None else None
+
+
+def scene_counts(dscene):
+    nv, nt = c_uint32(), c_uint32()
+    check(lib().hiprt_scene_counts(dscene.h, ctypes.byref(nv), ctypes.byref(nt)))
+    return nv.value, nt.value
+
+
+def adjoint_closest(dscene, rays, hits, grad_t, grad_u, grad_v, grad_vertices=None, grad_origin=None,
+                    grad_dir=None, grad_corners=None, stream=None):
+    """Backward of the closest hit with the hit set of `hits` frozen.  Upstream
+    gradients are float32 (n,) CUDA tensors; outputs (all optional): vertex
+    gradients (n_vert, 3) accumulated, origin / direction gradients (n, 3) and
+    per-ray corner gradients (n, 9), overwritten."""
+    n = rays.shape[1]
+    r, h = _rays_struct(rays), _hits_struct(hits)
+    check(lib().hiprt_adjoint_closest(dscene.h, ctypes.byref(r), ctypes.byref(h), n, _dp(grad_t), _dp(grad_u),
+                                      _dp(grad_v), _dp(grad_vertices), _dp(grad_origin), _dp(grad_dir),
+                                      _dp(grad_corners), _stream(stream)))
+
+
+def trace_batch(entries, any=False, tracer="dynamic-while-while", stream=None, refill_threshold=0, blocks=0):
+    """entries: list of (dscene, rays (8,n), hits dict or None, occluded tensor or None)."""
+    arr = (_BatchEntry * len(entries))()
+    for k, (ds, rays, hits, occ) in enumerate(entries):
+        arr[k].scene = ds.h
+        arr[k].rays = _rays_struct(rays)
+        arr[k].hits = _hits_struct(hits) if hits is not None else _Hits(None, None, None, None)
+        arr[k].occluded = occ.data_ptr() if occ is not None else None
+        arr[k].n = rays.shape[1]
+    o = _opts(refill_threshold, blocks)
+    check(lib().hiprt_trace_batch(arr, len(entries), 1 if any else 0, tracer_id(tracer), ctypes.byref(o),
+                                  _stream(stream)))
+
+
+def autotune(ctx, scene, bvh, sample_rays, any=False):
+    """Returns (result dict, DeviceScene committed in the chosen layout)."""
+    res = _TuneResult()
+    h = c_void_p()
+    r = _rays_struct(sample_rays)
+    check(lib().hiprt_autotune(ctx.h, scene.h, bvh.h, ctypes.byref(r), sample_rays.shape[1], 1 if any else 0,
+                               ctypes.byref(res), ctypes.byref(h)))
+    ds = DeviceScene(ctx, None, None, handle=h)
+    out = {"tracer": TRACERS[res.tracer], "refill_threshold": res.refill_threshold, "blocks": res.blocks,
+           "bvh_format": "bvh4" if res.bvh_format == BVH4 else "bvh2", "mrays_per_s": res.mrays_per_s,
+           "candidates": res.candidates}
+    return out, ds

@@ -176,6 +176,45 @@ int hiprt_trace_closest(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, cons
 int hiprt_trace_any(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *d_occluded, const hiprt_hits *hits,
                     int32_t tracer, const hiprt_tracer_opts *opts, void *stream);
 
+/* ------------------------------------------------------------ batched trace
+ * Many scenes, each with its own ray batch, traced by ONE persistent launch
+ * through a descriptor table: the batches form one index space from which
+ * every wavefront fetches rays, and each lane binds the scene of the ray it
+ * fetched.  Entries whose scenes differ in BVH or triangle format are traced
+ * by one launch per format group.  closest hit: hits required; any hit:
+ * d_occluded and/or hits.  No reference counterpart (the reference traces one
+ * scene per launch, rt/cuda/tracers.cu). */
+typedef struct {
+    hiprt_scene scene;
+    hiprt_rays rays;
+    hiprt_hits hits;
+    uint8_t *d_occluded;
+    uint32_t n;
+} hiprt_batch_entry;
+
+int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int32_t any, int32_t tracer,
+                      const hiprt_tracer_opts *opts, void *stream);
+
+/* --------------------------------------------------------------- autotuner
+ * Times candidate configurations on a sample of a scene's rays -- tracer
+ * variant (persistent / speculative / dynamic), refill threshold, waves per
+ * CU (grid size) and BVH layout (binary vs collapsed 4-wide) -- and returns
+ * the fastest.  Every candidate's hits on the sample are compared with the
+ * plain kernel's; a candidate that differs other than at equal t is
+ * rejected.  Results never depend on the choice.  The scene is committed in
+ * the chosen layout into *tuned (caller destroys it). */
+typedef struct {
+    int32_t tracer;
+    int32_t refill_threshold;
+    int32_t blocks;
+    int32_t bvh_format;
+    float mrays_per_s;     /* measured on the sample */
+    int32_t candidates;    /* configurations timed */
+} hiprt_tune_result;
+
+int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hiprt_rays *sample, uint32_t n,
+                   int32_t any, hiprt_tune_result *result, hiprt_scene *tuned);
+
 /* ----------------------------------------------------------- wavefront steps
  * GPU forms of the reference's wavefront steps around the trace:
  *   "sample camera rays"        (rt/cuda/rni.cpp:31-101, cam_ray semantics)
@@ -250,6 +289,22 @@ int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_le
 int hiprt_pt_image(hiprt_pt pt, float *rgb);
 int hiprt_pt_reset(hiprt_pt pt, void *stream);
 int hiprt_pt_destroy(hiprt_pt pt);
+
+/* -------------------------------------------------------- closest-hit adjoint
+ * Backward pass of the closest hit with the hit set frozen: for every ray
+ * with a valid hit in `hits` (the forward result; only t and id are read),
+ * the intersection (t, beta, gamma) is recomputed from its triangle's
+ * vertices and the upstream gradients dL/dt, dL/dbeta, dL/dgamma are pulled
+ * back to the ray and the triangle's vertices.  No reference counterpart.
+ *   d_grad_vertices  (n_vert x 3) floats, ACCUMULATED (+=), may be NULL
+ *   d_grad_origin, d_grad_dir  (n x 3) floats, overwritten (zero for misses), may be NULL
+ *   d_grad_corners   (n x 9) floats: dL/dA, dL/dB, dL/dC per ray, may be NULL
+ * Needs a scene committed from a host scene. */
+int hiprt_adjoint_closest(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n,
+                          const float *d_grad_t, const float *d_grad_u, const float *d_grad_v, float *d_grad_vertices,
+                          float *d_grad_origin, float *d_grad_dir, float *d_grad_corners, void *stream);
+/* Vertex and triangle counts of a committed scene (for sizing gradients). */
+int hiprt_scene_counts(hiprt_scene sc, uint32_t *n_vert, uint32_t *n_tris);
 
 #ifdef __cplusplus
 }

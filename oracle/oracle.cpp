@@ -378,6 +378,111 @@ void orc_fp64_intersect(const float *pos, const uint32_t *idx, const uint32_t *t
     }
 }
 
+// ---- closest-hit adjoint checks (no reference counterpart) ----------------
+//
+// solve64: o + t d = A + beta (B-A) + gamma (C-A) by Cramer's rule in double.
+static bool solve64(const double *A, const double *B, const double *C, const double *o, const double *d, double &t,
+                    double &beta, double &gamma) {
+    double e1[3], e2[3], q[3], m[3];
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = B[k] - A[k];
+        e2[k] = C[k] - A[k];
+        q[k] = o[k] - A[k];
+        m[k] = -d[k];
+    }
+    auto det3 = [](const double *c0, const double *c1, const double *c2) {
+        return c0[0] * (c1[1] * c2[2] - c1[2] * c2[1]) - c1[0] * (c0[1] * c2[2] - c0[2] * c2[1]) +
+               c2[0] * (c0[1] * c1[2] - c0[2] * c1[1]);
+    };
+    double D = det3(e1, e2, m);
+    if (D == 0.0) return false;
+    beta = det3(q, e2, m) / D;
+    gamma = det3(e1, q, m) / D;
+    t = det3(e1, e2, q) / D;
+    return true;
+}
+
+static void load15(const float *pos, const uint32_t *idx, uint32_t tri, const float *const *P, uint32_t r,
+                   double *x) {   // A B C o d
+    for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) x[3 * v + k] = pos[3 * idx[3 * tri + v] + k];
+    x[9] = P[0][r];
+    x[10] = P[1][r];
+    x[11] = P[2][r];
+    x[12] = P[4][r];
+    x[13] = P[5][r];
+    x[14] = P[6][r];
+}
+
+// Analytic FP64 gradient of L = gt*t + gb*beta + gg*gamma with respect to
+// (A, B, C, o, d), 15 doubles per ray, via lambda = M^-T g (M = [B-A, C-A, -d]).
+void orc_adjoint_fp64(const float *pos, const uint32_t *idx, const uint32_t *tri, const float *const *ray_planes,
+                      uint32_t n, const double *gt, const double *gb, const double *gg, double *out) {
+    for (uint32_t r = 0; r < n; ++r) {
+        double x[15];
+        load15(pos, idx, tri[r], ray_planes, r, x);
+        const double *A = x, *B = x + 3, *C = x + 6, *o = x + 9, *d = x + 12;
+        double t, b, g;
+        double *y = out + 15 * (size_t)r;
+        if (!solve64(A, B, C, o, d, t, b, g)) {
+            for (int k = 0; k < 15; ++k) y[k] = NAN;
+            continue;
+        }
+        // M^T lambda = (gb, gg, gt): rows of M^T are e1, e2, -d
+        double M[3][3];
+        for (int k = 0; k < 3; ++k) {
+            M[0][k] = B[k] - A[k];
+            M[1][k] = C[k] - A[k];
+            M[2][k] = -d[k];
+        }
+        double rhs[3] = {gb[r], gg[r], gt[r]}, lam[3];
+        double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                     M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                     M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+        for (int c = 0; c < 3; ++c) {   // Cramer on the rows of M (system M lam = rhs)
+            double Mc[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) Mc[i][j] = j == c ? rhs[i] : M[i][j];
+            lam[c] = (Mc[0][0] * (Mc[1][1] * Mc[2][2] - Mc[1][2] * Mc[2][1]) -
+                      Mc[0][1] * (Mc[1][0] * Mc[2][2] - Mc[1][2] * Mc[2][0]) +
+                      Mc[0][2] * (Mc[1][0] * Mc[2][1] - Mc[1][1] * Mc[2][0])) /
+                     det;
+        }
+        for (int k = 0; k < 3; ++k) {
+            y[k] = -(1.0 - b - g) * lam[k];
+            y[3 + k] = -b * lam[k];
+            y[6 + k] = -g * lam[k];
+            y[9 + k] = lam[k];
+            y[12 + k] = t * lam[k];
+        }
+    }
+}
+
+// Central differences of the same L with step h on each of the 15 inputs.
+void orc_adjoint_fd(const float *pos, const uint32_t *idx, const uint32_t *tri, const float *const *ray_planes,
+                    uint32_t n, const double *gt, const double *gb, const double *gg, double h, double *out) {
+    for (uint32_t r = 0; r < n; ++r) {
+        double x[15];
+        load15(pos, idx, tri[r], ray_planes, r, x);
+        double *y = out + 15 * (size_t)r;
+        for (int k = 0; k < 15; ++k) {
+            double v[2];
+            for (int s = 0; s < 2; ++s) {
+                double xp[15];
+                for (int j = 0; j < 15; ++j) xp[j] = x[j];
+                xp[k] += s == 0 ? h : -h;
+                double t, b, g;
+                if (!solve64(xp, xp + 3, xp + 6, xp + 9, xp + 12, t, b, g)) {
+                    v[s] = NAN;
+                    continue;
+                }
+                v[s] = gt[r] * t + gb[r] * b + gg[r] * g;
+            }
+            y[k] = (v[0] - v[1]) / (2 * h);
+        }
+    }
+}
+
 // Camera rays of libgi/camera.cpp:44-61 for a w x h image, pixel-centre
 // offset 0, t_min 1e-4, t_max FLT_MAX; out planes as ray_planes.
 void orc_camera_rays(const float *pos, const float *dir, const float *up, float fovy, int w, int h,

@@ -160,3 +160,21 @@ def time_closest(pos, idx, nodes, root, index, rays, row=0, threads=0):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+def adjoint_fp64(pos, idx, tri, rays, gt, gb, gg, fd_step=None):
+    """FP64 gradient of sum(gt*t + gb*beta + gg*gamma) per ray w.r.t. (A, B, C, o, d):
+    analytic (fd_step None) or by central differences with step fd_step.  (n, 15)."""
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    tri = np.ascontiguousarray(tri, dtype=np.uint32)
+    g = [np.ascontiguousarray(x, dtype=np.float64) for x in (gt, gb, gg)]
+    out = np.empty((n, 15), np.float64)
+    if fd_step is None:
+        lib().orc_adjoint_fp64(_p(pos), _p(idx), _p(tri), planes, c_uint32(n), _p(g[0]), _p(g[1]), _p(g[2]),
+                               _p(out))
+    else:
+        lib().orc_adjoint_fd(_p(pos), _p(idx), _p(tri), planes, c_uint32(n), _p(g[0]), _p(g[1]), _p(g[2]),
+                             c_double(fd_step), _p(out))
+    return out
