@@ -28,6 +28,7 @@ rt::dev::persistent_cfg cfg_for(int tracer, const hiprt_tracer_opts *opts) {
     c.refill_thr = (opts && opts->refill_threshold > 0) ? std::min(opts->refill_threshold, 64) : 40;
     c.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
     c.stats = nullptr;
+    c.stack = (opts && opts->stack_config == 1) ? 1 : 0;
     return c;
 }
 
@@ -83,7 +84,7 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
     size_t off = 0;
     for (auto &g : groups) {
         int bvh = g.first.first, tri = g.first.second;
-        if (need[g.first] > rt::dev::persistent_stack_cap(bvh))
+        if (need[g.first] > rt::dev::persistent_stack_cap(bvh, cfg_for(tracer, opts).stack))
             return fail(HIPRT_ERR_UNSUPPORTED, "a scene's BVH is deeper than the persistent traversal stack");
         auto *dst = reinterpret_cast<rt::dev::batch_desc *>(static_cast<char *>(table.p) + off);
         HIP_TRY(hipMemcpy(dst, g.second.data(), g.second.size() * sizeof(rt::dev::batch_desc), hipMemcpyHostToDevice));
@@ -144,17 +145,19 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
     hiprt_tune_result best{};
     float best_ms = 1e30f;
     int count = 0;
-    for (int f = 0; f < 2; ++f) {
-        if (layouts[f]->stack_need > rt::dev::persistent_stack_cap(f)) continue;
+    for (int f = 0; f < 2; ++f)
+    for (int stack = 0; stack < 2; ++stack) {
+        if (layouts[f]->stack_need > rt::dev::persistent_stack_cap(f, stack)) continue;
         for (int tracer = HIPRT_TRACER_PERSISTENT_WHILE_WHILE; tracer < HIPRT_TRACER_COUNT; ++tracer) {
             std::vector<int> thresholds = tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE ? std::vector<int>{24, 40, 56}
                                                                                      : std::vector<int>{40};
-            rt::dev::persistent_cfg pc = cfg_for(tracer, nullptr);
+            hiprt_tracer_opts base{40, 0, nullptr, stack};
+            rt::dev::persistent_cfg pc = cfg_for(tracer, &base);
             int full = 0;
             HIP_TRY(rt::dev::persistent_grid(layouts[f]->desc, any != 0, pc, &full));
             for (int thr : thresholds)
                 for (int frac = 0; frac < 2; ++frac) {
-                    hiprt_tracer_opts o{thr, frac == 0 ? full : std::max(1, full / 2), nullptr};
+                    hiprt_tracer_opts o{thr, frac == 0 ? full : std::max(1, full / 2), nullptr, stack};
                     if (int st = trace(layouts[f], tracer, &o, &h)) {
                         (void)hipEventDestroy(e0);
                         (void)hipEventDestroy(e1);
@@ -181,7 +184,7 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
                     if (!same) continue;
                     if (ms < best_ms) {
                         best_ms = ms;
-                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(n / (ms * 1e3)), 0};
+                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(n / (ms * 1e3)), 0, stack};
                     }
                 }
         }

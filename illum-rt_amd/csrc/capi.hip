@@ -10,6 +10,7 @@
 #include "core/raygen.h"
 #include "device/kernels.h"
 
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -313,6 +314,17 @@ int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
     s->desc.tri = h.tri_kind;
     s->desc.depth = (int32_t)h.depth;
     s->stack_need = (int)stack_need;
+    {
+        // box bounds of absent 4-wide children are +-inf, never NaN
+        const float *fv = h.bvh_kind == rt::BVH2 ? reinterpret_cast<const float *>(h.nodes2.data())
+                                                 : reinterpret_cast<const float *>(h.nodes4.data());
+        size_t nf = h.bvh_kind == rt::BVH2 ? h.nodes2.size() * 16 : h.nodes4.size() * 32;
+        size_t row = h.bvh_kind == rt::BVH2 ? 16 : 32, boxes = h.bvh_kind == rt::BVH2 ? 12 : 24;
+        bool nan_free = true;
+        for (size_t k = 0; k < nf && nan_free; ++k)
+            if (k % row < boxes && std::isnan(fv[k])) nan_free = false;
+        s->desc.nan_free = nan_free ? 1 : 0;
+    }
     *out = s.release();
     return ok();
 }
@@ -460,7 +472,8 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     if (hits) h = rt::hit_soa{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
     hipStream_t s = static_cast<hipStream_t>(stream);
     int need = sc->stack_need;
-    bool persistent_fits = need <= rt::dev::persistent_stack_cap(sc->desc.bvh);
+    int stack = (opts && opts->stack_config == 1) ? 1 : 0;
+    bool persistent_fits = need <= rt::dev::persistent_stack_cap(sc->desc.bvh, stack);
     if (sc->desc.bvh == rt::BVH4 && (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits))
         return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH needs a persistent tracer with a stack of " +
                                                std::to_string(need) + " entries");
@@ -479,6 +492,7 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     if (cfg.refill_thr > 64) cfg.refill_thr = 64;
     cfg.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
     cfg.stats = opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats) : nullptr;
+    cfg.stack = stack;
     HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
                                              static_cast<uint32_t *>(sc->ctx->counter.p), any, cfg, s));
     return ok();

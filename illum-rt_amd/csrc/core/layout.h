@@ -86,6 +86,7 @@ struct scene_desc {
     int32_t bvh;              // bvh_format
     int32_t tri;              // tri_format
     int32_t depth;            // longest root-to-leaf path, in inner nodes
+    int32_t nan_free;         // no node box bound is NaN (kernels may use the min/max box test)
 };
 
 } // namespace rt

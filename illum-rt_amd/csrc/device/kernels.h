@@ -23,8 +23,9 @@ constexpr int kLdsStack4 = 44;
 constexpr int kPersistentStackCap = kRegStack + kLdsStack;
 constexpr int kPersistentWavesPerBlock = 4;
 
-// Stack entries the persistent kernels hold for a BVH of the given format.
-int persistent_stack_cap(int bvh);
+// Stack entries the persistent kernels hold for a BVH of the given format
+// and stack configuration (0: 4 VGPR + 28/44 LDS entries, 1: 8 VGPR + 16/32).
+int persistent_stack_cap(int bvh, int stack = 0);
 
 // One entry of a multi-scene batch: a scene and its own ray and hit buffers;
 // its rays occupy [first, first + n) of the launch's index space.
@@ -48,6 +49,7 @@ struct persistent_cfg {
     int refill_thr;       // active-lane threshold for refill (out of 64)
     int blocks;           // grid size; 0 = occupancy-derived default
     unsigned long long *stats = nullptr;   // non-null: counting variant (inner nodes, triangle tests)
+    int stack = 0;        // stack configuration, see persistent_stack_cap
 };
 
 // `counter` is one device uint32 that must be zero at launch; `count` is the

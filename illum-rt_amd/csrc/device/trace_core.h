@@ -67,6 +67,41 @@ __device__ __forceinline__ bool box_exact(float xlo, float xhi, float ylo, float
     return hit;
 }
 
+// intersect4 when no slab value can be NaN: the operands are the same
+// separately rounded (box - o) * (1/d) products, and for NaN-free inputs the
+// select chains of intersect4 equal IEEE min/max (min3/max3 on gfx950),
+// differing at most in the sign of a zero, which no comparison sees.  NaN
+// slab values need a non-finite ray component or box bound; kernels take
+// this path only for wavefronts whose rays and scene are all finite.
+template <bool ANY>
+__device__ __forceinline__ bool box_fast(float xlo, float xhi, float ylo, float yhi, float zlo, float zhi,
+                                         const dray &r, float best, float &dist) {
+    float ax = (xlo - r.ox) * r.ix, bx = (xhi - r.ox) * r.ix;
+    float ay = (ylo - r.oy) * r.iy, by = (yhi - r.oy) * r.iy;
+    float az = (zlo - r.oz) * r.iz, bz = (zhi - r.oz) * r.iz;
+    float t1 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    float t2 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    dist = t1;
+    bool hit = (t1 <= t2) && (t2 >= r.tmin) && (t1 <= r.tmax);
+    if (!ANY) hit = hit && (t1 < best);
+    return hit;
+}
+
+// True when every component of the ray and of its inverse direction is
+// finite, i.e. box_fast decides exactly like box_exact for this ray.
+__device__ __forceinline__ bool ray_is_finite(const dray &r) {
+    return isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz) && isfinite(r.dx) && isfinite(r.dy) &&
+           isfinite(r.dz) && isfinite(r.ix) && isfinite(r.iy) && isfinite(r.iz) && !isnan(r.tmin) &&
+           !isnan(r.tmax);
+}
+
+template <bool ANY>
+__device__ __forceinline__ bool box_test(bool exact, float xlo, float xhi, float ylo, float yhi, float zlo,
+                                         float zhi, const dray &r, float best, float &dist) {
+    if (exact) return box_exact<ANY>(xlo, xhi, ylo, yhi, zlo, zhi, r, best, dist);
+    return box_fast<ANY>(xlo, xhi, ylo, yhi, zlo, zhi, r, best, dist);
+}
+
 // Shirley/Cramer test on an exact record (A, A-B, A-C).  Accepts iff
 // tmin < t < tmax, beta > 0, gamma > 0, beta + gamma <= 1, and t < best.
 __device__ __forceinline__ bool tri_exact(const float4 &r0, const float4 &r1, const float4 &r2, const dray &r,
@@ -85,6 +120,15 @@ __device__ __forceinline__ bool tri_exact(const float4 &r0, const float4 &r1, co
     c3 = b * l - k * c;
     float gamma = i * c1 + h * c2 + g * c3;
     float tt = -(f * c1 + e * c2 + d * c3);
+    // Sign pre-filter: a quotient whose numerator and M have opposite signs
+    // (or a zero numerator) is <= 0, so "beta > 0" / "gamma > 0" fail -- and
+    // with t_min >= 0 so does "t > t_min" -- whatever the division would
+    // round to.  Only the three divisions are skipped; accepted hits are the
+    // quotients of the reference's expressions.
+    bool neg_m = M < 0;
+    if ((beta > 0) == neg_m || (gamma > 0) == neg_m || ((tt > 0) == neg_m && r.tmin >= 0)) {
+        if (M == M && M != 0) return false;   // (a NaN or zero M falls through to the IEEE divisions)
+    }
     tt /= M;
     if (!(tt > r.tmin && tt < r.tmax && tt < best)) return false;
     beta /= M;

@@ -148,6 +148,8 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
     dhit h{};
     bool alive = false;
     bool exhausted = false;   // wave-uniform: the counter has run past n
+    bool finite = true;       // this lane's ray has only finite components
+    bool exact = !sc.nan_free;   // wave-uniform: use the select-based box test
     uint32_t c_nodes = 0, c_tris = 0;
 
     for (;;) {
@@ -180,11 +182,14 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                     st.reset();
                     h.found = false;
                     h.t = ANY ? INFINITY : FLT_MAX;
+                    finite = ray_is_finite(r);
                     alive = true;
                 }
             }
         }
         if (__ballot(alive) == 0) break;
+        exact = __ballot(alive && !finite) != 0 || (MULTI ? false : !sc.nan_free);
+        if (MULTI && !exact) exact = __ballot(alive && !table[desc].scene.nan_free) != 0;
 
         if (alive) {
             // ---- traversal, while-while ----
@@ -197,8 +202,8 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                             reinterpret_cast<const float4 *>(static_cast<const node2 *>(ls.nodes) + cur);
                         float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
                         float dl, dr;
-                        bool hl = box_exact<ANY>(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, h.t, dl);
-                        bool hr = box_exact<ANY>(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, h.t, dr);
+                        bool hl = box_test<ANY>(exact, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, h.t, dl);
+                        bool hr = box_test<ANY>(exact, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, h.t, dr);
                         int32_t rl = __float_as_int(n3.x), rr = __float_as_int(n3.y);
                         if (hl && hr) {
                             bool left_first = dl < dr;
@@ -218,10 +223,10 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                         float k0, k1, k2, k3;
                         int32_t c0 = __float_as_int(rf.x), c1 = __float_as_int(rf.y), c2 = __float_as_int(rf.z),
                                 c3 = __float_as_int(rf.w);
-                        bool b0 = box_exact<ANY>(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, h.t, k0) && c0 != kInvalidRef;
-                        bool b1 = box_exact<ANY>(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, h.t, k1) && c1 != kInvalidRef;
-                        bool b2 = box_exact<ANY>(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, h.t, k2) && c2 != kInvalidRef;
-                        bool b3 = box_exact<ANY>(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, h.t, k3) && c3 != kInvalidRef;
+                        bool b0 = box_test<ANY>(exact, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, h.t, k0) && c0 != kInvalidRef;
+                        bool b1 = box_test<ANY>(exact, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, h.t, k1) && c1 != kInvalidRef;
+                        bool b2 = box_test<ANY>(exact, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, h.t, k2) && c2 != kInvalidRef;
+                        bool b3 = box_test<ANY>(exact, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, h.t, k3) && c3 != kInvalidRef;
                         int nh = (int)b0 + (int)b1 + (int)b2 + (int)b3;
                         // misses sort after hits; a NaN entry distance (axis-parallel
                         // ray on a slab plane) sorts last among the hits
@@ -326,11 +331,18 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t 
 
 namespace {
 
-constexpr int lds_for(int bvh) { return bvh == 2 ? kLdsStack : kLdsStack4; }
+// Stack configurations: (VGPR entries, LDS entries) per lane.  Config 0 keeps
+// more in LDS (fewer VGPRs, 28 KB of LDS per block for binary trees: 5
+// blocks per CU); config 1 keeps 8 entries in VGPRs and 16 in LDS (16 KB per
+// block: occupancy bound by VGPRs instead of LDS).
+constexpr int reg_for(int stk) { return stk == 0 ? kRegStack : 8; }
+constexpr int lds_for(int bvh, int stk) {
+    return bvh == 2 ? (stk == 0 ? kLdsStack : 16) : (stk == 0 ? kLdsStack4 : 32);
+}
 
 // Blocks of one instantiation that fit the device at once (occupancy x CUs),
 // asked once per device.
-template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI>
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK>
 hipError_t grid_of(int *blocks) {
     static int cached[64] = {0};
     int dev = 0;
@@ -342,7 +354,8 @@ hipError_t grid_of(int *blocks) {
         e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, lds_for(BVH), false, BVH, MULTI>, 256, 0);
+            &per_cu,
+            k_trace_persistent<TRI, ANY, SPEC, REFILL, reg_for(STK), lds_for(BVH, STK), false, BVH, MULTI>, 256, 0);
         if (e != hipSuccess) return e;
         cached[dev] = cus * (per_cu > 0 ? per_cu : 1);
     }
@@ -362,36 +375,45 @@ struct launch_args {
     uint32_t n_desc;
 };
 
-template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI>
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK>
 hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipStream_t stream, bool grid_only,
                           int *grid_out) {
     int blocks = cfg.blocks;
     if (blocks <= 0) {
-        hipError_t e = grid_of<TRI, ANY, SPEC, REFILL, BVH, MULTI>(&blocks);
+        hipError_t e = grid_of<TRI, ANY, SPEC, REFILL, BVH, MULTI, STK>(&blocks);
         if (e != hipSuccess) return e;
     }
     if (grid_out) *grid_out = blocks;
     if (grid_only) return hipSuccess;
-    constexpr int L = lds_for(BVH);
-    if (cfg.stats && !MULTI)
-        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, L, true, BVH, MULTI>), dim3(blocks),
-                           dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
-                           cfg.refill_thr, cfg.stats, a.table, a.n_desc);
-    else
-        hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, kRegStack, L, false, BVH, MULTI>), dim3(blocks),
-                           dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
-                           cfg.refill_thr, cfg.stats, a.table, a.n_desc);
+    constexpr int R = reg_for(STK), L = lds_for(BVH, STK);
+    if constexpr (!MULTI) {
+        if (cfg.stats) {
+            hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, true, BVH, MULTI>), dim3(blocks),
+                               dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
+                               cfg.refill_thr, cfg.stats, a.table, a.n_desc);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, false, BVH, MULTI>), dim3(blocks), dim3(256),
+                       0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter, cfg.refill_thr,
+                       cfg.stats, a.table, a.n_desc);
     return hipGetLastError();
 }
 
-template <int TRI, bool ANY, int BVH, bool MULTI>
+template <int TRI, bool ANY, int BVH, bool MULTI, int STK>
 hipError_t dispatch_flags(const launch_args &a, const persistent_cfg &cfg, hipStream_t s, bool grid_only, int *g) {
     if (cfg.speculative) {
-        if (cfg.refill) return launch_variant<TRI, ANY, true, true, BVH, MULTI>(a, cfg, s, grid_only, g);
-        return launch_variant<TRI, ANY, true, false, BVH, MULTI>(a, cfg, s, grid_only, g);
+        if (cfg.refill) return launch_variant<TRI, ANY, true, true, BVH, MULTI, STK>(a, cfg, s, grid_only, g);
+        return launch_variant<TRI, ANY, true, false, BVH, MULTI, STK>(a, cfg, s, grid_only, g);
     }
-    if (cfg.refill) return launch_variant<TRI, ANY, false, true, BVH, MULTI>(a, cfg, s, grid_only, g);
-    return launch_variant<TRI, ANY, false, false, BVH, MULTI>(a, cfg, s, grid_only, g);
+    if (cfg.refill) return launch_variant<TRI, ANY, false, true, BVH, MULTI, STK>(a, cfg, s, grid_only, g);
+    return launch_variant<TRI, ANY, false, false, BVH, MULTI, STK>(a, cfg, s, grid_only, g);
+}
+
+template <int TRI, bool ANY, int BVH, bool MULTI>
+hipError_t dispatch_stack(const launch_args &a, const persistent_cfg &cfg, hipStream_t s, bool grid_only, int *g) {
+    if (cfg.stack == 1) return dispatch_flags<TRI, ANY, BVH, MULTI, 1>(a, cfg, s, grid_only, g);
+    return dispatch_flags<TRI, ANY, BVH, MULTI, 0>(a, cfg, s, grid_only, g);
 }
 
 template <bool MULTI>
@@ -399,21 +421,24 @@ hipError_t dispatch(int tri, bool any, int bvh, const launch_args &a, const pers
                     bool grid_only, int *g) {
     if (tri == TRI_EXACT) {
         if (bvh == BVH2)
-            return any ? dispatch_flags<TRI_EXACT, true, 2, MULTI>(a, cfg, s, grid_only, g)
-                       : dispatch_flags<TRI_EXACT, false, 2, MULTI>(a, cfg, s, grid_only, g);
-        return any ? dispatch_flags<TRI_EXACT, true, 4, MULTI>(a, cfg, s, grid_only, g)
-                   : dispatch_flags<TRI_EXACT, false, 4, MULTI>(a, cfg, s, grid_only, g);
+            return any ? dispatch_stack<TRI_EXACT, true, 2, MULTI>(a, cfg, s, grid_only, g)
+                       : dispatch_stack<TRI_EXACT, false, 2, MULTI>(a, cfg, s, grid_only, g);
+        return any ? dispatch_stack<TRI_EXACT, true, 4, MULTI>(a, cfg, s, grid_only, g)
+                   : dispatch_stack<TRI_EXACT, false, 4, MULTI>(a, cfg, s, grid_only, g);
     }
     if (bvh == BVH2)
-        return any ? dispatch_flags<TRI_WOOP, true, 2, MULTI>(a, cfg, s, grid_only, g)
-                   : dispatch_flags<TRI_WOOP, false, 2, MULTI>(a, cfg, s, grid_only, g);
-    return any ? dispatch_flags<TRI_WOOP, true, 4, MULTI>(a, cfg, s, grid_only, g)
-               : dispatch_flags<TRI_WOOP, false, 4, MULTI>(a, cfg, s, grid_only, g);
+        return any ? dispatch_stack<TRI_WOOP, true, 2, MULTI>(a, cfg, s, grid_only, g)
+                   : dispatch_stack<TRI_WOOP, false, 2, MULTI>(a, cfg, s, grid_only, g);
+    return any ? dispatch_stack<TRI_WOOP, true, 4, MULTI>(a, cfg, s, grid_only, g)
+               : dispatch_stack<TRI_WOOP, false, 4, MULTI>(a, cfg, s, grid_only, g);
 }
 
 } // namespace
 
-int persistent_stack_cap(int bvh) { return kRegStack + (bvh == BVH4 ? kLdsStack4 : kLdsStack); }
+int persistent_stack_cap(int bvh, int stack) {
+    int b = bvh == BVH4 ? 4 : 2;
+    return stack == 1 ? reg_for(1) + lds_for(b, 1) : reg_for(0) + lds_for(b, 0);
+}
 
 hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks) {
     launch_args a{};

@@ -15,8 +15,8 @@ namespace rt {
 namespace dev {
 
 template <int TRI, bool ANY, bool COUNT>
-__device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray &r, dhit &h, uint32_t &n_nodes,
-                                               uint32_t &n_tris) {
+__device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray &r, bool exact, dhit &h,
+                                               uint32_t &n_nodes, uint32_t &n_tris) {
     int32_t stack[kPlainStackCap];
     int sp = 0;
     stack[0] = kInvalidRef;
@@ -30,8 +30,8 @@ __device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray 
             const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
             float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
             float dl, dr;
-            bool hl = box_exact<ANY>(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, h.t, dl);
-            bool hr = box_exact<ANY>(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, h.t, dr);
+            bool hl = box_test<ANY>(exact, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, h.t, dl);
+            bool hr = box_test<ANY>(exact, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, h.t, dr);
             int32_t rl = __float_as_int(n3.x), rr = __float_as_int(n3.y);
             if (hl && hr) {
                 bool left_first = dl < dr;
@@ -70,9 +70,12 @@ __global__ __launch_bounds__(256) void k_trace_plain(scene_desc sc, ray_soa rays
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     dray r = load_ray(rays, i);
+    // the select-based box test only where some ray of the wavefront (or the
+    // scene) has a non-finite component; both decide identically otherwise
+    const bool exact = !sc.nan_free || __ballot(!ray_is_finite(r)) != 0;
     dhit h;
     uint32_t c_nodes = 0, c_tris = 0;
-    traverse_plain<TRI, ANY, COUNT>(sc, r, h, c_nodes, c_tris);
+    traverse_plain<TRI, ANY, COUNT>(sc, r, exact, h, c_nodes, c_tris);
     if (COUNT) {
         atomicAdd(stats, (unsigned long long)c_nodes);
         atomicAdd(stats + 1, (unsigned long long)c_tris);

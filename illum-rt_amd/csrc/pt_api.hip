@@ -266,7 +266,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
         return fail(HIPRT_ERR_ARG, "the plain schedule uses the while-while tracer");
     if (!p->batched && sc->desc.bvh != rt::BVH2)
         return fail(HIPRT_ERR_UNSUPPORTED, "the plain schedule needs a binary BVH scene");
-    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh) : rt::dev::kPlainStackCap - 1))
+    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh, 0) : rt::dev::kPlainStackCap - 1))
         return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     uint32_t n = p->n;
     HIP_TRY(p->rays[0].alloc(n));

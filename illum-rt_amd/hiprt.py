@@ -49,7 +49,8 @@ class _Hits(ctypes.Structure):
 
 
 class _TracerOpts(ctypes.Structure):
-    _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32), ("d_fetch_stats", c_void_p)]
+    _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32), ("d_fetch_stats", c_void_p),
+                ("stack_config", c_int32)]
 
 
 class _BatchEntry(ctypes.Structure):
@@ -58,7 +59,7 @@ class _BatchEntry(ctypes.Structure):
 
 class _TuneResult(ctypes.Structure):
     _fields_ = [("tracer", c_int32), ("refill_threshold", c_int32), ("blocks", c_int32), ("bvh_format", c_int32),
-                ("mrays_per_s", c_float), ("candidates", c_int32)]
+                ("mrays_per_s", c_float), ("candidates", c_int32), ("stack_config", c_int32)]
 
 
 class _PtParams(ctypes.Structure):
@@ -320,24 +321,26 @@ def tracer_id(name):
     return i
 
 
-def _opts(refill_threshold=0, blocks=0, fetch_stats=None):
-    return _TracerOpts(refill_threshold, blocks, fetch_stats.data_ptr() if fetch_stats is not None else None)
+def _opts(refill_threshold=0, blocks=0, fetch_stats=None, stack_config=0):
+    return _TracerOpts(refill_threshold, blocks, fetch_stats.data_ptr() if fetch_stats is not None else None,
+                       stack_config)
 
 
 def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
-                  fetch_stats=None):
+                  fetch_stats=None, stack_config=0):
     """Closest hits of `rays` (8, n) into `hits`.  fetch_stats: optional int64
     CUDA tensor of 2 that receives inner-node fetches and triangle tests."""
     n = rays.shape[1]
-    r, h, o = _rays_struct(rays), _hits_struct(hits), _opts(refill_threshold, blocks, fetch_stats)
+    r, h, o = _rays_struct(rays), _hits_struct(hits), _opts(refill_threshold, blocks, fetch_stats, stack_config)
     s = c_void_p(stream) if stream else None
     check(lib().hiprt_trace_closest(dscene.h, ctypes.byref(r), n, ctypes.byref(h), tracer_id(tracer),
                                     ctypes.byref(o), s))
 
 
-def trace_any(dscene, rays, occluded, hits=None, tracer="while-while", stream=None, refill_threshold=0, blocks=0):
+def trace_any(dscene, rays, occluded, hits=None, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
+              stack_config=0):
     n = rays.shape[1]
-    r, o = _rays_struct(rays), _opts(refill_threshold, blocks)
+    r, o = _rays_struct(rays), _opts(refill_threshold, blocks, None, stack_config)
     hp = ctypes.byref(_hits_struct(hits)) if hits is not None else None
     occ = c_void_p(occluded.data_ptr()) if occluded is not None else None
     s = c_void_p(stream) if stream else None
@@ -447,7 +450,8 @@ def adjoint_closest(dscene, rays, hits, grad_t, grad_u, grad_v, grad_vertices=No
                                       _dp(grad_corners), _stream(stream)))
 
 
-def trace_batch(entries, any=False, tracer="dynamic-while-while", stream=None, refill_threshold=0, blocks=0):
+def trace_batch(entries, any=False, tracer="dynamic-while-while", stream=None, refill_threshold=0, blocks=0,
+                stack_config=0):
     """entries: list of (dscene, rays (8,n), hits dict or None, occluded tensor or None)."""
     arr = (_BatchEntry * len(entries))()
     for k, (ds, rays, hits, occ) in enumerate(entries):
@@ -456,7 +460,7 @@ def trace_batch(entries, any=False, tracer="dynamic-while-while", stream=None, r
         arr[k].hits = _hits_struct(hits) if hits is not None else _Hits(None, None, None, None)
         arr[k].occluded = occ.data_ptr() if occ is not None else None
         arr[k].n = rays.shape[1]
-    o = _opts(refill_threshold, blocks)
+    o = _opts(refill_threshold, blocks, None, stack_config)
     check(lib().hiprt_trace_batch(arr, len(entries), 1 if any else 0, tracer_id(tracer), ctypes.byref(o),
                                   _stream(stream)))
 
@@ -471,5 +475,5 @@ def autotune(ctx, scene, bvh, sample_rays, any=False):
     ds = DeviceScene(ctx, None, None, handle=h)
     out = {"tracer": TRACERS[res.tracer], "refill_threshold": res.refill_threshold, "blocks": res.blocks,
            "bvh_format": "bvh4" if res.bvh_format == BVH4 else "bvh2", "mrays_per_s": res.mrays_per_s,
-           "candidates": res.candidates}
+           "candidates": res.candidates, "stack_config": res.stack_config}
     return out, ds

@@ -163,6 +163,8 @@ typedef struct {
     uint64_t *d_fetch_stats;  /* non-NULL: run the counting variant, which adds the inner
                                  nodes it fetches to [0] and the triangle records it
                                  tests to [1] (device uint64[2]) */
+    int32_t stack_config;     /* traversal stack per lane: 0 = 4 VGPR + 28 LDS entries
+                                 (44 for 4-wide), 1 = 8 VGPR + 16 LDS (32 for 4-wide) */
 } hiprt_tracer_opts;
 
 /* compute_closest_hit of batch_ray_tracer (libgi/wavefront-rt.h:145-149,
@@ -198,7 +200,8 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
 /* --------------------------------------------------------------- autotuner
  * Times candidate configurations on a sample of a scene's rays -- tracer
  * variant (persistent / speculative / dynamic), refill threshold, waves per
- * CU (grid size) and BVH layout (binary vs collapsed 4-wide) -- and returns
+ * CU (grid size), stack configuration and BVH layout (binary vs collapsed
+ * 4-wide) -- and returns
  * the fastest.  Every candidate's hits on the sample are compared with the
  * plain kernel's; a candidate that differs other than at equal t is
  * rejected.  Results never depend on the choice.  The scene is committed in
@@ -210,6 +213,7 @@ typedef struct {
     int32_t bvh_format;
     float mrays_per_s;     /* measured on the sample */
     int32_t candidates;    /* configurations timed */
+    int32_t stack_config;
 } hiprt_tune_result;
 
 int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hiprt_rays *sample, uint32_t n,

@@ -71,6 +71,19 @@ def test_persistent_exact_matches_up_to_ties(torch_cuda, c1, tracer):
         assert np.array_equal(a[same].view(np.uint32), b[same].view(np.uint32))
 
 
+@pytest.mark.parametrize("tracer", TRACERS[1:])
+def test_second_stack_config_matches_up_to_ties(torch_cuda, c1, tracer):
+    """Stack configuration 1 (8 register entries, 16 LDS entries per lane)
+    changes only where entries live, never the traversal order."""
+    print(f"case: C1 camera+random rays, {tracer}, exact records, stack config 1", flush=True)
+    got0 = _run_closest(torch_cuda, c1["exact"], c1["rays"], tracer)
+    got1 = _run_closest(torch_cuda, c1["exact"], c1["rays"], tracer, stack_config=1)
+    rep = classify(orc, c1["pos"], c1["idx"], c1["rays"], got1, c1["ref"])
+    assert rep["unexplained"] == 0, rep
+    same_t = got0[0].view(np.uint32) == got1[0].view(np.uint32)
+    assert same_t.all()
+
+
 @pytest.mark.parametrize("tracer", ["while-while", "dynamic-while-while"])
 def test_woop_within_tie_and_grazing_bar(torch_cuda, c1, tracer):
     """Woop records: hit agreement >= 99.999% and every mismatch a tie or a
