@@ -65,12 +65,98 @@ def time_trace(torch, hiprt, ds, rays, tracer, stream, reps=10):
     return e0.elapsed_time(e1) / reps, hits
 
 
+def time_calls(torch, fn, stream, reps=5):
+    fn()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_shadow(torch, hiprt, ctx, stream):
+    """Hall of 74,068 triangles at 1920x1080: primary closest hits, then one
+    shadow ray per hit to the point light (t_max = distance - 1e-4), traced
+    with any hit."""
+    sc = hiprt.HostScene.generate(2)
+    bvh = hiprt.Bvh(sc)
+    ds = hiprt.DeviceScene(ctx, sc, bvh)
+    cam = sc.cam
+    n = int(cam[10]) * int(cam[11])
+    prim = torch.empty((8, n), dtype=torch.float32, device="cuda")
+    hiprt.camera_rays_device(cam, prim, stream.cuda_stream)
+    hits = hiprt.new_hits(n)
+    hiprt.trace_closest(ds, prim, hits, tracer="while-while", stream=stream.cuda_stream)
+    shadow = torch.empty_like(prim)
+    hiprt.shadow_rays(ds, prim, hits, shadow, cam[12:15], stream.cuda_stream)
+    occ = torch.empty(n, dtype=torch.uint8, device="cuda")
+    out = {"triangles": sc.n_tris, "rays": n}
+    for tr in ("while-while", "dynamic-while-while"):
+        ms = time_calls(torch, lambda: hiprt.trace_any(ds, shadow, occ, tracer=tr, stream=stream.cuda_stream), stream)
+        out[f"any/{tr}"] = round(n / ms / 1e3, 1)
+    ms = time_calls(torch, lambda: hiprt.trace_closest(ds, prim, hits, tracer="dynamic-while-while",
+                                                      stream=stream.cuda_stream), stream)
+    out["primary/dynamic-while-while"] = round(n / ms / 1e3, 1)
+    out["occluded_frac"] = round(float(occ.float().mean()), 4)
+    ds.close()
+    return out
+
+
+def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
+    """64 random-triangle scenes of log-uniform size with their own rays: one
+    descriptor-table launch against one launch per scene, both with the
+    dynamic tracer; each scene's layout (binary or 4-wide) picked by the
+    autotuner on a sample of its rays."""
+    import workloads
+    t0 = time.time()
+    work = workloads.multi_scene(n_scenes, seed=4)
+    build_s = time.time() - t0
+    entries, scenes, layouts = [], [], []
+    total = 0
+    n_tris = sum(sc.n_tris for sc, _, _ in work)
+    for sc, bvh, rays in work:
+        d_rays = torch.from_numpy(rays).cuda()
+        sample = d_rays[:, ::max(1, rays.shape[1] // 32768)].contiguous()
+        res, ds = hiprt.autotune(ctx, sc, bvh, sample)
+        layouts.append(res["bvh_format"])
+        hits = hiprt.new_hits(rays.shape[1])
+        entries.append((ds, d_rays, hits, None))
+        scenes.append(ds)
+        total += rays.shape[1]
+    del work
+    tune_s = time.time() - t0 - build_s
+
+    def separate():
+        for ds, r, h, _ in entries:
+            hiprt.trace_closest(ds, r, h, tracer="dynamic-while-while", stream=stream.cuda_stream)
+
+    def batched():
+        hiprt.trace_batch(entries, tracer="dynamic-while-while", stream=stream.cuda_stream)
+
+    ms_sep = time_calls(torch, separate, stream, reps=3)
+    ms_bat = time_calls(torch, batched, stream, reps=3)
+    out = {"scenes": n_scenes, "triangles": int(n_tris), "rays": int(total),
+           "batched_mrays": round(total / ms_bat / 1e3, 1), "separate_mrays": round(total / ms_sep / 1e3, 1),
+           "batched_ms": round(ms_bat, 3), "separate_ms": round(ms_sep, 3),
+           "bvh4_scenes": sum(1 for l in layouts if l == "bvh4"),
+           "host_build_s": round(build_s, 1), "autotune_s": round(tune_s, 1)}
+    for ds in scenes:
+        ds.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the shadow-ray and 64-scene legs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -134,6 +220,29 @@ def main():
             kernels[f"{name}/{tr}"] = round(rays.shape[1] / ms / 1e3, 1)
     log(f"trace kernels alone (Mrays/s): {kernels}")
 
+    # Closest-hit adjoint on 1M of the diffuse rays: vertex (atomic), origin
+    # and direction gradients of sum(a t + b beta + c gamma), hit set frozen.
+    n_adj = 1_000_000
+    adj_rays = d_diff[:, :n_adj].contiguous()
+    adj_hits = hiprt.new_hits(n_adj)
+    hiprt.trace_closest(ds, adj_rays, adj_hits, tracer="dynamic-while-while", stream=stream.cuda_stream)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    g_t, g_u, g_v = (torch.randn(n_adj, device="cuda", generator=gen) for _ in range(3))
+    g_vert = torch.zeros((hiprt.scene_counts(ds)[0], 3), dtype=torch.float32, device="cuda")
+    g_o = torch.empty((n_adj, 3), dtype=torch.float32, device="cuda")
+    g_d = torch.empty((n_adj, 3), dtype=torch.float32, device="cuda")
+    ms_adj = time_calls(torch, lambda: hiprt.adjoint_closest(ds, adj_rays, adj_hits, g_t, g_u, g_v, g_vert, g_o, g_d,
+                                                             stream=stream.cuda_stream), stream, reps=10)
+    adjoint = {"rays": n_adj, "ms": round(ms_adj, 4), "mrays": round(n_adj / ms_adj / 1e3, 1)}
+    log(f"adjoint: {adjoint}")
+
+    extra = {}
+    if not args.no_extra:
+        extra["shadow_any_hit_mrays"] = bench_shadow(torch, hiprt, ctx, stream)
+        log(f"shadow rays: {extra['shadow_any_hit_mrays']}")
+        extra["multi_scene"] = bench_multi_scene(torch, hiprt, ctx, stream)
+        log(f"multi-scene: {extra['multi_scene']}")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -193,7 +302,9 @@ def main():
                      "bytes_per_launch": round(float(bytes_lvl.sum()) / max(launches, 1)),
                      "compulsory_GBs": round(compulsory, 1)},
         "cpu_baseline": cpu,
+        "adjoint_closest": adjoint,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
     batched.close()
     plain.close()

@@ -7,6 +7,8 @@
 // plane the kernel reads or writes.
 #include "capi_internal.h"
 
+#include "device/refit_kernels.h"
+
 #include "core/raygen.h"
 #include "device/kernels.h"
 
@@ -219,12 +221,14 @@ int hiprt_create(int32_t device, hiprt_context *out) {
 
 int hiprt_destroy(hiprt_context ctx) {
     if (!ctx) return fail(HIPRT_ERR_ARG, "null context");
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     delete ctx;
     return ok();
 }
 
 // ---------------------------------------------------------------- device scene
+
+} // extern "C"
 
 namespace {
 
@@ -331,11 +335,15 @@ int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
 
 } // namespace
 
+extern "C" {
+
 int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32_t bvh_format, int32_t tri_format,
                        hiprt_scene *out) {
     if (!ctx || !s || !b || !out) return fail(HIPRT_ERR_ARG, "null argument");
     if (bvh_format != HIPRT_BVH2 && bvh_format != HIPRT_BVH4) return fail(HIPRT_ERR_ARG, "unknown bvh_format");
     if (tri_format != HIPRT_TRI_EXACT && tri_format != HIPRT_TRI_WOOP) return fail(HIPRT_ERR_ARG, "unknown tri_format");
+    for (uint32_t t : b->b.index)
+        if (t >= s->sc.n_tris()) return fail(HIPRT_ERR_ARG, "the BVH names a triangle the scene does not have");
     rt::flat_scene f;
     std::string e = rt::flatten(s->sc, b->b, bvh_format, tri_format, f);
     if (!e.empty()) return fail(HIPRT_ERR_UNSUPPORTED, e);
@@ -412,21 +420,77 @@ int hiprt_scene_layout(hiprt_scene sc, uint32_t *n_nodes, uint32_t *n_slots, int
 
 int hiprt_scene_download(hiprt_scene sc, void *nodes, void *tri_records, uint32_t *slot_ids) {
     if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
-    const rt::flat_scene &h = sc->host;
-    if (nodes) {
-        if (h.bvh_kind == rt::BVH2 && !h.nodes2.empty())
-            std::memcpy(nodes, h.nodes2.data(), h.nodes2.size() * sizeof(rt::node2));
-        if (h.bvh_kind == rt::BVH4 && !h.nodes4.empty())
-            std::memcpy(nodes, h.nodes4.data(), h.nodes4.size() * sizeof(rt::node4));
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    if (nodes && sc->nodes.bytes) HIP_TRY(hipMemcpy(nodes, sc->nodes.p, sc->nodes.bytes, hipMemcpyDeviceToHost));
+    if (tri_records && sc->tris.bytes)
+        HIP_TRY(hipMemcpy(tri_records, sc->tris.p, sc->tris.bytes, hipMemcpyDeviceToHost));
+    if (slot_ids && sc->slot_ids.bytes)
+        HIP_TRY(hipMemcpy(slot_ids, sc->slot_ids.p, sc->slot_ids.bytes, hipMemcpyDeviceToHost));
+    return ok();
+}
+
+int hiprt_scene_refit(hiprt_scene sc, const float *d_positions, void *stream) {
+    if (!sc || !d_positions) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!sc->tri_vidx.p) {
+        if (sc->host.tris.empty()) return ok();
+        return fail(HIPRT_ERR_UNSUPPORTED, "refit needs a scene committed from a host scene");
     }
-    if (tri_records && !h.tris.empty()) std::memcpy(tri_records, h.tris.data(), h.tris.size() * sizeof(rt::tri_rec));
-    if (slot_ids && !h.slot_ids.empty()) std::memcpy(slot_ids, h.slot_ids.data(), h.slot_ids.size() * 4);
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    const rt::flat_scene &h = sc->host;
+    if (sc->refit_off.empty()) {
+        // inner nodes grouped by depth, deepest level first
+        std::vector<std::vector<int32_t>> levels;
+        if (h.root_ref >= 0) {
+            std::vector<int32_t> cur{h.root_ref};
+            while (!cur.empty()) {
+                levels.push_back(cur);
+                std::vector<int32_t> next;
+                for (int32_t id : cur) {
+                    int32_t refs[4] = {rt::kInvalidRef, rt::kInvalidRef, rt::kInvalidRef, rt::kInvalidRef};
+                    if (h.bvh_kind == rt::BVH2) {
+                        refs[0] = h.nodes2[id].ref_l;
+                        refs[1] = h.nodes2[id].ref_r;
+                    } else {
+                        for (int c = 0; c < 4; ++c) refs[c] = h.nodes4[id].ref[c];
+                    }
+                    for (int32_t r : refs)
+                        if (r >= 0 && r != rt::kInvalidRef) next.push_back(r);
+                }
+                cur.swap(next);
+            }
+        }
+        std::vector<int32_t> list;
+        sc->refit_off.push_back(0);
+        for (auto it = levels.rbegin(); it != levels.rend(); ++it) {
+            list.insert(list.end(), it->begin(), it->end());
+            sc->refit_off.push_back((uint32_t)list.size());
+        }
+        HIP_TRY(sc->refit_list.upload(list.data(), list.size() * 4));
+    }
+    rt::dev::refit_args a{};
+    a.nodes = sc->nodes.p;
+    a.bvh = h.bvh_kind;
+    a.tris = sc->tris.as<rt::tri_rec>();
+    a.slot_ids = sc->slot_ids.as<uint32_t>();
+    a.tri = h.tri_kind;
+    a.n_slots = (uint32_t)h.tris.size();
+    a.tri_geo = sc->tri_geo.as<float>();
+    a.n_tris = sc->n_tris;
+    a.pos = d_positions;
+    a.vidx = sc->tri_vidx.as<uint32_t>();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(rt::dev::launch_refit_records(a, s));
+    const int32_t *list = sc->refit_list.as<int32_t>();
+    for (size_t k = 0; k + 1 < sc->refit_off.size(); ++k)
+        HIP_TRY(rt::dev::launch_refit_level(a, list + sc->refit_off[k], sc->refit_off[k + 1] - sc->refit_off[k], s));
+    sc->desc.nan_free = 1;   // select-based min/max from an empty box never store NaN
     return ok();
 }
 
 int hiprt_scene_destroy(hiprt_scene sc) {
     if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
-    hipSetDevice(sc->ctx->device);
+    (void)hipSetDevice(sc->ctx->device);
     delete sc;
     return ok();
 }

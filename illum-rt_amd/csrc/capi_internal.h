@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 namespace hiprt_detail {
 
@@ -77,5 +78,7 @@ struct hiprt_scene_s {
     hiprt_detail::dbuf tri_vidx;                 // vertex indices per triangle (adjoint scatter)
     uint32_t n_tris = 0, n_vert = 0;
     int stack_need = 0;                          // most stack entries any traversal can hold
+    hiprt_detail::dbuf refit_list;               // inner nodes by level, deepest first (refit)
+    std::vector<uint32_t> refit_off;             // level boundaries in refit_list
     rt::scene_desc desc{};
 };

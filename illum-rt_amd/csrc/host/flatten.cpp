@@ -9,20 +9,6 @@ namespace rt {
 
 namespace {
 
-float bits_as_float(uint32_t u) {
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
-
-struct dvec {
-    double x, y, z;
-};
-dvec dv(vec3f v) { return dvec{v.x, v.y, v.z}; }
-dvec dsub(dvec a, dvec b) { return dvec{a.x - b.x, a.y - b.y, a.z - b.z}; }
-dvec dcross(dvec a, dvec b) { return dvec{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-double ddot(dvec a, dvec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-
 // Reference into the flattened tree for bbvh node `id`.
 struct ref_map {
     const bvh &b;
@@ -37,48 +23,6 @@ struct ref_map {
 aabb child_box(const bbvh_node &n, int side) { return side == 0 ? n.box_l : n.box_r; }
 
 } // namespace
-
-tri_rec make_exact_rec(vec3f A, vec3f B, vec3f C, uint32_t tri_id) {
-    tri_rec r{};
-    r.r0[0] = A.x;
-    r.r0[1] = A.y;
-    r.r0[2] = A.z;
-    r.r0[3] = bits_as_float(tri_id);
-    r.r1[0] = A.x - B.x;
-    r.r1[1] = A.y - B.y;
-    r.r1[2] = A.z - B.z;
-    r.r2[0] = A.x - C.x;
-    r.r2[1] = A.y - C.y;
-    r.r2[2] = A.z - C.z;
-    return r;
-}
-
-// Rows of inverse([B-A, C-A, N]) with N = (B-A)x(C-A), in FP64.  For a point
-// p, (row0.(p-A), row1.(p-A), row2.(p-A)) = (beta, gamma, s): beta and gamma
-// are the barycentric weights of B and C (the reference's beta/gamma) and s
-// the signed offset along N.  Stored as (row.xyz, -row.A) and rounded to fp32.
-tri_rec make_woop_rec(vec3f A, vec3f B, vec3f C) {
-    dvec a = dv(A), e1 = dsub(dv(B), a), e2 = dsub(dv(C), a);
-    dvec n = dcross(e1, e2);
-    double det = ddot(n, n);   // det([e1 e2 n]) = |e1 x e2|^2
-    tri_rec r{};
-    if (!(det > 0) || !std::isfinite(det)) {
-        // Degenerate: row2 = (0,0,0,1) makes every t = -inf, i.e. no hit.
-        r.r2[3] = 1.0f;
-        return r;
-    }
-    // inverse of a matrix with columns c0,c1,c2: rows (c1xc2, c2xc0, c0xc1)/det
-    dvec rows[3] = {dcross(e2, n), dcross(n, e1), dcross(e1, e2)};
-    float *out[3] = {r.r0, r.r1, r.r2};
-    for (int k = 0; k < 3; ++k) {
-        dvec q{rows[k].x / det, rows[k].y / det, rows[k].z / det};
-        out[k][0] = (float)q.x;
-        out[k][1] = (float)q.y;
-        out[k][2] = (float)q.z;
-        out[k][3] = (float)(-ddot(q, a));
-    }
-    return r;
-}
 
 std::string flatten(const scene &s, const bvh &b, int bvh_kind, int tri_kind, flat_scene &out) {
     out = flat_scene{};

@@ -10,6 +10,7 @@
 #pragma once
 
 #include "../core/layout.h"
+#include "../core/records.h"
 #include "bvh.h"
 #include "scene.h"
 
@@ -31,9 +32,5 @@ struct flat_scene {
 
 // Returns "" on success, else a message (leaf too large, too many slots).
 std::string flatten(const scene &s, const bvh &b, int bvh_kind, int tri_kind, flat_scene &out);
-
-// Triangle records for one triangle in each format.
-tri_rec make_exact_rec(vec3f A, vec3f B, vec3f C, uint32_t tri_id);
-tri_rec make_woop_rec(vec3f A, vec3f B, vec3f C);
 
 } // namespace rt

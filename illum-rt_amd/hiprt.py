@@ -104,6 +104,7 @@ def lib():
             "hiprt_scene_layout": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32),
                                            POINTER(c_uint32), POINTER(c_int32), POINTER(c_int32)]),
             "hiprt_scene_download": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+            "hiprt_scene_refit": (c_int, [c_void_p, c_void_p, c_void_p]),
             "hiprt_scene_destroy": (c_int, [c_void_p]),
             "hiprt_tracer_id": (c_int, [c_char_p]),
             "hiprt_tracer_name": (c_char_p, [c_int]),
@@ -293,6 +294,15 @@ class DeviceScene:
         check(lib().hiprt_scene_download(self.h, _ptr(nodes), _ptr(tris), _ptr(ids)))
         return nodes, tris, ids
 
+    def refit(self, positions, stream=None):
+        """Refit to `positions`, a contiguous (n_vert, 3) float32 CUDA tensor:
+        same triangles and topology, new records and boxes."""
+        assert positions.dtype.is_floating_point and positions.element_size() == 4 and positions.is_contiguous()
+        nv, _ = scene_counts(self)
+        if tuple(positions.shape) != (nv, 3) or not positions.is_cuda:
+            raise HiprtError(f"refit needs a ({nv}, 3) CUDA tensor, got {tuple(positions.shape)}")
+        check(lib().hiprt_scene_refit(self.h, c_void_p(positions.data_ptr()), _stream(stream)))
+
     def close(self):
         if getattr(self, "h", None):
             lib().hiprt_scene_destroy(self.h)
@@ -372,6 +382,16 @@ def bounce_cosine(dscene, rays, hits, out, seed, level, stream=None):
     r, h, o = _rays_struct(rays), _hits_struct(hits), _rays_struct(out)
     check(lib().hiprt_bounce_cosine(dscene.h, ctypes.byref(r), ctypes.byref(h), n, seed, level, ctypes.byref(o),
                                     _stream(stream)))
+
+
+def shadow_rays(dscene, rays, hits, out, light, stream=None):
+    """Shadow rays from `hits` of `rays` towards the point `light` into `out`
+    ((8, n) each); misses give the disabled ray."""
+    n = rays.shape[1]
+    r, h, o = _rays_struct(rays), _hits_struct(hits), _rays_struct(out)
+    lp = np.ascontiguousarray(light[:3], dtype=np.float32)
+    check(lib().hiprt_shadow_rays(dscene.h, ctypes.byref(r), ctypes.byref(h), n,
+                                  lp.ctypes.data_as(POINTER(c_float)), ctypes.byref(o), _stream(stream)))
 
 
 class PathTracer:

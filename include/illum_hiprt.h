@@ -130,7 +130,18 @@ int hiprt_scene_upload(hiprt_context ctx, const void *nodes, uint32_t n_nodes, i
 /* Host copy of the flattened arrays (sizes from hiprt_scene_layout). */
 int hiprt_scene_layout(hiprt_scene sc, uint32_t *n_nodes, uint32_t *n_slots, int32_t *root_ref, uint32_t *depth,
                        int32_t *bvh_format, int32_t *tri_format);
+/* Device arrays of the scene (sizes from hiprt_scene_layout), after waiting
+ * for the device: reflects refits. */
 int hiprt_scene_download(hiprt_scene sc, void *nodes, void *tri_records, uint32_t *slot_ids);
+/* Refit to new vertex positions (d_positions: 3 floats per vertex, as many
+ * vertices as the committed host scene): same triangles and tree topology,
+ * triangle records rebuilt, node boxes recomputed bottom-up on the device,
+ * one launch per tree level.  Leaf boxes bound whole triangles.  The device
+ * counterpart of rebuilding on every commit (rt/cuda/base.cpp:88-117) when
+ * only vertices move -- scene-view transforms (rt/cuda/preprocessing.cpp:41-72)
+ * or gradient steps on vertices.  Needs a scene from hiprt_scene_commit;
+ * stream-ordered, no host synchronisation. */
+int hiprt_scene_refit(hiprt_scene sc, const float *d_positions, void *stream);
 int hiprt_scene_destroy(hiprt_scene sc);
 
 /* --------------------------------------------------------------------- trace */

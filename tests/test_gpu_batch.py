@@ -6,6 +6,7 @@ import pytest
 
 import hiprt
 import oracle_py as orc
+import workloads
 from parity import FLT_MAX, classify, uniform_sphere_rays
 
 pytestmark = pytest.mark.gpu
@@ -60,32 +61,13 @@ def test_plain_kernel_refuses_bvh4(torch_cuda, hip_ctx):
         hiprt.trace_closest(ds, rays, hiprt.new_hits(100), tracer="while-while")
 
 
-def _c4_like(n_scenes, seed, max_tris=60_000, max_rays=150_000):
-    """Random-triangle scenes of log-uniform sizes, each with its own rays:
-    half pinhole camera rays, half uniform-sphere rays from U[0,1]^3."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for s in range(n_scenes):
-        nt = int(np.exp(rng.uniform(np.log(1000), np.log(max_tris))))
-        nr = int(np.exp(rng.uniform(np.log(4000), np.log(max_rays))))
-        sigma = 0.02 * (8192.0 / nt) ** (1.0 / 3.0)
-        sc = hiprt.HostScene.random(nt, sigma, 100 + s)
-        w = max(1, int(np.sqrt(nr / 2 * 4 / 3)))
-        h = max(1, (nr // 2) // w)
-        cam = np.array([0.5, 0.5, -1, 0, 0, 1, 0, 1, 0, 60, w, h], np.float32)
-        rays = np.concatenate([hiprt.camera_rays(cam), uniform_sphere_rays(nr - w * h, 1000 + s)], 1)
-        out.append((sc, rays))
-    return out
-
-
 @pytest.mark.parametrize("layout", [hiprt.BVH2, hiprt.BVH4])
 def test_batch_of_uneven_scenes_one_launch(torch_cuda, hip_ctx, layout):
     print(f"case: 12 uneven random scenes, one batched launch, layout {layout}", flush=True)
     torch = torch_cuda
-    work = _c4_like(12, seed=4)
+    work = workloads.multi_scene(12, seed=4, tri_hi=60_000, ray_hi=150_000)
     entries, refs = [], []
-    for sc, rays in work:
-        bvh = hiprt.Bvh(sc)
+    for sc, bvh, rays in work:
         ds = hiprt.DeviceScene(hip_ctx, sc, bvh, layout)
         d_rays = torch.from_numpy(rays).cuda()
         h = hiprt.new_hits(rays.shape[1])
@@ -105,7 +87,7 @@ def test_batch_of_uneven_scenes_one_launch(torch_cuda, hip_ctx, layout):
     hiprt.trace_batch([(e[0], e[1], None, o) for e, o in zip(entries, occs)], any=True,
                       tracer="persistent-speculative-while-while")
     torch.cuda.synchronize()
-    for (sc, rays), o, want in zip(work, occs, refs):
+    for o, want in zip(occs, refs):
         assert np.array_equal(o.cpu().numpy().astype(bool), want[0] != FLT_MAX)
 
 

@@ -1,0 +1,250 @@
+"""GPU parity on the cases the reference's tracer handles by construction and
+a careless kernel would not: rays with zero, infinite or NaN components,
+empty and inverted t intervals, a root that is a leaf, exact ties
+from duplicated triangles, degenerate triangles, empty batches; plus the
+kernels' fetch counters against the oracle's counting traversal and the C2
+any-hit shadow-ray workload."""
+import numpy as np
+import pytest
+
+import hiprt
+import oracle_py as orc
+from parity import FLT_MAX, bit_exact, classify, uniform_sphere_rays
+
+pytestmark = pytest.mark.gpu
+
+TRACERS = list(hiprt.TRACERS)
+INF = np.float32(np.inf)
+NAN = np.float32(np.nan)
+
+
+def _special_rays():
+    """Rays whose slab or Cramer arithmetic meets zeros, infinities or NaN."""
+    rows = []
+
+    def add(o, d, tmin=1e-4, tmax=FLT_MAX):
+        rows.append([o[0], o[1], o[2], tmin, d[0], d[1], d[2], tmax])
+
+    inside = (0.5, 0.5, 0.5)
+    front = (0.5, 0.5, -1.0)
+    for o in (inside, front):
+        for d in ((0, 0, 1), (0, 0, -1), (-0.0, 0, 1), (0, -0.0, 1), (1, 0, 0), (-1, 0, 0), (0, 1, 0),
+                  (0, -1, 0), (1e-30, 1e-30, 1), (1e-40, 0, 1), (0.3, 0.4, 0.0), (0, 0, 0), (3e38, 0, 1e-3)):
+            add(o, d)
+    add(front, (0, 0, 1), tmax=0.0)                 # empty interval
+    add(front, (0, 0, 1), tmin=1e-4, tmax=1e-4)     # t_min == t_max
+    add(front, (0, 0, 1), tmin=2.0, tmax=1.0)       # inverted
+    add(inside, (0, 0, 1), tmin=-1.0)               # negative t_min (hits behind the origin count)
+    add(inside, (0.2, 0.3, 0.9), tmin=-INF)
+    add(front, (0, 0, 1), tmax=INF)
+    add(front, (NAN, 0, 1))
+    add((NAN, 0.5, -1), (0, 0, 1))
+    add((INF, 0.5, 0.5), (-1, 0, 0))
+    add(front, (0, 0, 1), tmin=NAN)
+    add(front, (0, 0, 1), tmax=NAN)
+    add(front, (INF, INF, INF))
+    return np.array(rows, np.float32).T.copy()
+
+
+def _mixed_rays(n_random, seed):
+    """Random rays with the special rays spread over them: some wavefronts
+    hold one special ray among finite ones, one wavefront holds only special
+    rays."""
+    sp = _special_rays()
+    rnd = uniform_sphere_rays(n_random, seed)
+    k = sp.shape[1]
+    pos = np.arange(k) * 97 + 13
+    out = rnd.copy()
+    out[:, pos] = sp
+    block = np.tile(sp, (1, (64 + k - 1) // k))[:, :64]
+    return np.concatenate([out, block], 1), np.concatenate([pos, np.arange(out.shape[1], out.shape[1] + 64)])
+
+
+def _closest(torch, ds, rays, tracer, **kw):
+    h = hiprt.new_hits(rays.shape[1])
+    for x in h.values():
+        x.fill_(-5)
+    hiprt.trace_closest(ds, torch.from_numpy(rays).cuda(), h, tracer=tracer, **kw)
+    torch.cuda.synchronize()
+    return (h["t"].cpu().numpy(), h["u"].cpu().numpy(), h["v"].cpu().numpy(),
+            h["id"].cpu().numpy().view(np.uint32))
+
+
+def _any(torch, ds, rays, tracer, **kw):
+    occ = torch.full((rays.shape[1],), 7, dtype=torch.uint8, device="cuda")
+    hiprt.trace_any(ds, torch.from_numpy(rays).cuda(), occ, tracer=tracer, **kw)
+    torch.cuda.synchronize()
+    return occ.cpu().numpy()
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def c1s(hip_ctx):
+    sc = hiprt.HostScene.generate(1)
+    pos, idx, _ = sc.arrays()
+    bvh = hiprt.Bvh(sc)
+    ds = hiprt.DeviceScene(hip_ctx, sc, bvh)
+    ds4 = hiprt.DeviceScene(hip_ctx, sc, bvh, hiprt.BVH4)
+    return dict(sc=sc, pos=pos, idx=idx, bvh=bvh, ds=ds, ds4=ds4)
+
+
+def test_special_rays_every_tracer(torch_cuda, c1s):
+    print("case: C1, special rays mixed into random rays, every tracer and stack config", flush=True)
+    torch = torch_cuda
+    rays, special = _mixed_rays(20_000, 21)
+    b = c1s["bvh"]
+    want = orc.trace_closest(c1s["pos"], c1s["idx"], b.nodes, b.root, b.index, rays)
+    want_any = orc.trace_any(c1s["pos"], c1s["idx"], b.nodes, b.root, b.index, rays)
+    got = _closest(torch, c1s["ds"], rays, "while-while")
+    assert bit_exact(got, want)
+    assert np.array_equal(_any(torch, c1s["ds"], rays, "while-while"), want_any)
+    for tracer in TRACERS[1:]:
+        for stack in (0, 1):
+            for ds in (c1s["ds"], c1s["ds4"]):
+                got = _closest(torch, ds, rays, tracer, stack_config=stack)
+                rep = classify(orc, c1s["pos"], c1s["idx"], rays, got, want)
+                assert rep["unexplained"] == 0, (tracer, stack, rep)
+                # the special rays themselves: same t, to the bit
+                assert np.array_equal(_bits(got[0][special]), _bits(want[0][special])), (tracer, stack)
+                assert np.array_equal(_any(torch, ds, rays, tracer, stack_config=stack), want_any), (tracer, stack)
+
+
+@pytest.mark.parametrize("n_tris", [1, 3])
+def test_root_leaf_scene(torch_cuda, hip_ctx, n_tris):
+    print(f"case: {n_tris}-triangle scene, root is a leaf", flush=True)
+    torch = torch_cuda
+    rng = np.random.default_rng(n_tris)
+    pos = (rng.uniform(0.3, 0.7, (n_tris, 1, 3)) + rng.normal(0, 0.8, (n_tris, 3, 3))).reshape(-1, 3).astype(np.float32)
+    idx = np.arange(3 * n_tris, dtype=np.uint32).reshape(-1, 3)
+    sc = hiprt.HostScene.from_arrays(pos, idx)
+    bvh = hiprt.Bvh(sc)
+    assert bvh.nodes.shape[0] == 1 and bvh.nodes[0]["link_l"] <= 0
+    rays = np.concatenate([uniform_sphere_rays(5000, 7), _special_rays()], 1)
+    want = orc.trace_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, rays)
+    assert (want[0] != FLT_MAX).sum() > 20
+    want_any = orc.trace_any(pos, idx, bvh.nodes, bvh.root, bvh.index, rays)
+    for layout in (hiprt.BVH2, hiprt.BVH4):
+        ds = hiprt.DeviceScene(hip_ctx, sc, bvh, layout)
+        for tracer in TRACERS[(1 if layout == hiprt.BVH4 else 0):]:
+            got = _closest(torch, ds, rays, tracer)
+            assert bit_exact(got, want), (layout, tracer)
+            assert np.array_equal(_any(torch, ds, rays, tracer), want_any), (layout, tracer)
+        ds.close()
+
+
+def test_duplicated_and_degenerate_triangles(torch_cuda, hip_ctx):
+    """Every triangle twice (each hit an exact tie between two ids) plus
+    zero-area triangles: points, collinear vertices and repeated vertices."""
+    print("case: duplicated + degenerate triangles", flush=True)
+    torch = torch_cuda
+    base = hiprt.HostScene.random(3000, 0.03, 17)
+    pos, idx, _ = base.arrays()
+    rng = np.random.default_rng(5)
+    extra = []
+    for k in range(200):
+        p = rng.uniform(0, 1, 3)
+        q = rng.uniform(0, 1, 3)
+        kind = k % 3
+        if kind == 0:
+            extra.append([p, p, p])
+        elif kind == 1:
+            extra.append([p, 0.5 * (p + q), q])
+        else:
+            extra.append([p, p, q])
+    extra = np.array(extra, np.float32).reshape(-1, 3)
+    all_pos = np.concatenate([pos, extra]).astype(np.float32)
+    n0 = len(idx)
+    idx_all = np.concatenate([idx, idx, (np.arange(len(extra)) + len(pos)).reshape(-1, 3)]).astype(np.uint32)
+    sc = hiprt.HostScene.from_arrays(all_pos, idx_all)
+    bvh = hiprt.Bvh(sc)
+    rays = np.concatenate([uniform_sphere_rays(40_000, 9), _special_rays()], 1)
+    want = orc.trace_closest(all_pos, idx_all, bvh.nodes, bvh.root, bvh.index, rays)
+    hit = want[0] != FLT_MAX
+    assert hit.sum() > 1000
+    assert (want[3][hit] < 2 * n0).all()   # degenerate triangles are never hit
+    ds = hiprt.DeviceScene(hip_ctx, sc, bvh)
+    assert bit_exact(_closest(torch, ds, rays, "while-while"), want)
+    for tracer in TRACERS[1:]:
+        got = _closest(torch, ds, rays, tracer)
+        rep = classify(orc, all_pos, idx_all, rays, got, want)
+        assert rep["unexplained"] == 0, (tracer, rep)
+        # where t agrees, a different id is the other copy of the same triangle
+        same_t = _bits(got[0]) == _bits(want[0])
+        assert same_t.mean() >= 0.9999, tracer
+        diff = same_t & (got[3] != want[3])
+        assert ((got[3][diff] % n0) == (want[3][diff] % n0)).all(), tracer
+        if tracer == "persistent-while-while":   # the reference's order: the same copy wins
+            assert bit_exact(got, want)
+    ds.close()
+
+
+def test_empty_batches(torch_cuda, c1s):
+    torch = torch_cuda
+    empty = torch.empty((8, 0), dtype=torch.float32, device="cuda")
+    for tracer in TRACERS:
+        hiprt.trace_closest(c1s["ds"], empty, hiprt.new_hits(0), tracer=tracer)
+        hiprt.trace_any(c1s["ds"], empty, torch.empty(0, dtype=torch.uint8, device="cuda"), tracer=tracer)
+    hiprt.trace_batch([(c1s["ds"], empty, hiprt.new_hits(0), None)])
+    torch.cuda.synchronize()
+
+
+def test_fetch_counts_match_the_oracle(torch_cuda, c1s):
+    """Inner-node fetches and triangle tests counted by the kernels against the
+    oracle's counting traversal over the same tree and rays: equal for the
+    plain kernel, within 2% for the persistent ones (speculation tests a few
+    triangles early, which can change what later culling removes)."""
+    print("case: C1 fetch counters vs oracle counts", flush=True)
+    torch = torch_cuda
+    cam = hiprt.camera_rays(c1s["sc"].cam)
+    rays = np.concatenate([cam, uniform_sphere_rays(100_000, 33)], 1)
+    b = c1s["bvh"]
+    *_, counts = orc.trace_closest(c1s["pos"], c1s["idx"], b.nodes, b.root, b.index, rays, counts=True)
+    want = counts.astype(np.int64).sum(0)
+    for tracer in TRACERS:
+        st = torch.zeros(2, dtype=torch.int64, device="cuda")
+        _closest(torch, c1s["ds"], rays, tracer, fetch_stats=st)
+        got = st.cpu().numpy()
+        print(tracer, got, want)
+        if tracer == "while-while":
+            assert np.array_equal(got, want)
+        else:
+            assert np.all(np.abs(got / want - 1) <= 0.02), (tracer, got, want)
+
+
+def test_c2_shadow_rays_any_hit(torch_cuda, hip_ctx):
+    """The C2 workload at quarter resolution: primary closest hits, shadow rays
+    to the point light built on the GPU, any hit with every tracer."""
+    print("case: C2 hall, 480x270 primary + shadow rays", flush=True)
+    torch = torch_cuda
+    sc = hiprt.HostScene.generate(2)
+    pos, idx, _ = sc.arrays()
+    bvh = hiprt.Bvh(sc)
+    ds = hiprt.DeviceScene(hip_ctx, sc, bvh)
+    cam = sc.cam.copy()
+    cam[10], cam[11] = 480, 270
+    n = 480 * 270
+    prim = torch.empty((8, n), dtype=torch.float32, device="cuda")
+    hiprt.camera_rays_device(cam, prim)
+    h = hiprt.new_hits(n)
+    hiprt.trace_closest(ds, prim, h, tracer="while-while")
+    shadow = torch.empty_like(prim)
+    hiprt.shadow_rays(ds, prim, h, shadow, sc.cam[12:15])
+    torch.cuda.synchronize()
+    srays = shadow.cpu().numpy()
+    hit = h["t"].cpu().numpy() != FLT_MAX
+    assert hit.mean() > 0.99
+    # shadow rays: origin = hit point, unit direction, t_max = distance - 1e-4
+    light = sc.cam[12:15].astype(np.float64)
+    o = srays[0:3, hit].astype(np.float64)
+    dist = np.linalg.norm(light[:, None] - o, axis=0)
+    assert np.allclose(srays[7, hit], dist - 1e-4, rtol=1e-5, atol=1e-5)
+    assert np.allclose(np.linalg.norm(srays[4:7, hit], axis=0), 1, atol=1e-6)
+    want = orc.trace_any(pos, idx, bvh.nodes, bvh.root, bvh.index, srays)
+    assert 0.05 < want[hit].mean() < 0.95
+    for tracer in TRACERS:
+        got = _any(torch, ds, srays, tracer)
+        assert np.array_equal(got, want), tracer
+    ds.close()
