@@ -49,17 +49,17 @@ def time_frames(torch, pt, steps, warmup, stream):
     return ms, rays, (after - before) / steps
 
 
-def time_trace(torch, hiprt, ds, rays, tracer, stream, reps=10):
+def time_trace(torch, hiprt, ds, rays, tracer, stream, reps=10, **kw):
     n = rays.shape[1]
     hits = hiprt.new_hits(n)
     for _ in range(2):
-        hiprt.trace_closest(ds, rays, hits, tracer=tracer, stream=stream.cuda_stream)
+        hiprt.trace_closest(ds, rays, hits, tracer=tracer, stream=stream.cuda_stream, **kw)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(reps):
-        hiprt.trace_closest(ds, rays, hits, tracer=tracer, stream=stream.cuda_stream)
+        hiprt.trace_closest(ds, rays, hits, tracer=tracer, stream=stream.cuda_stream, **kw)
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps, hits
@@ -214,10 +214,14 @@ def main():
     d_diff = torch.empty_like(d_cam)
     hiprt.bounce_cosine(ds, d_cam, hits, d_diff, seed=12345, level=0, stream=stream.cuda_stream)
     torch.cuda.synchronize()
+    variants = (("while-while", {}), ("dynamic-while-while", {}),
+                ("dynamic-while-while", {"stack_config": 1}), ("dynamic-while-while", {"work_chunks": 8}),
+                ("dynamic-while-while", {"stack_config": 1, "work_chunks": 8}))
     for name, rays in (("primary", d_cam), ("diffuse", d_diff)):
-        for tr in ("while-while", "dynamic-while-while"):
-            ms, _ = time_trace(torch, hiprt, ds, rays, tr, stream)
-            kernels[f"{name}/{tr}"] = round(rays.shape[1] / ms / 1e3, 1)
+        for tr, kw in variants:
+            ms, _ = time_trace(torch, hiprt, ds, rays, tr, stream, **kw)
+            tag = "".join(f",{k}={v}" for k, v in kw.items())
+            kernels[f"{name}/{tr}{tag}"] = round(rays.shape[1] / ms / 1e3, 1)
     log(f"trace kernels alone (Mrays/s): {kernels}")
 
     # Closest-hit adjoint on 1M of the diffuse rays: vertex (atomic), origin
@@ -235,6 +239,16 @@ def main():
                                                              stream=stream.cuda_stream), stream, reps=10)
     adjoint = {"rays": n_adj, "ms": round(ms_adj, 4), "mrays": round(n_adj / ms_adj / 1e3, 1)}
     log(f"adjoint: {adjoint}")
+
+    # Device refit of the hall to its own vertices (records and boxes rebuilt;
+    # leaf boxes then bound whole triangles instead of clipped fragments), and
+    # the diffuse trace over the refitted tree.  Runs last on this scene.
+    d_pos = torch.from_numpy(pos).cuda()
+    ms_refit = time_calls(torch, lambda: ds.refit(d_pos, stream.cuda_stream), stream, reps=10)
+    ms_after, _ = time_trace(torch, hiprt, ds, d_diff, "dynamic-while-while", stream)
+    refit = {"ms": round(ms_refit, 4), "levels_launched": int(ds.depth),
+             "diffuse_after_refit_mrays": round(d_diff.shape[1] / ms_after / 1e3, 1)}
+    log(f"refit: {refit}")
 
     extra = {}
     if not args.no_extra:
@@ -303,6 +317,7 @@ def main():
                      "compulsory_GBs": round(compulsory, 1)},
         "cpu_baseline": cpu,
         "adjoint_closest": adjoint,
+        "refit": refit,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

@@ -29,6 +29,7 @@ rt::dev::persistent_cfg cfg_for(int tracer, const hiprt_tracer_opts *opts) {
     c.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
     c.stats = nullptr;
     c.stack = (opts && opts->stack_config == 1) ? 1 : 0;
+    c.chunks = (opts && opts->work_chunks == 8) ? 8 : 1;
     return c;
 }
 
@@ -151,13 +152,14 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
         for (int tracer = HIPRT_TRACER_PERSISTENT_WHILE_WHILE; tracer < HIPRT_TRACER_COUNT; ++tracer) {
             std::vector<int> thresholds = tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE ? std::vector<int>{24, 40, 56}
                                                                                      : std::vector<int>{40};
-            hiprt_tracer_opts base{40, 0, nullptr, stack};
+            hiprt_tracer_opts base{40, 0, nullptr, stack, 1};
             rt::dev::persistent_cfg pc = cfg_for(tracer, &base);
             int full = 0;
             HIP_TRY(rt::dev::persistent_grid(layouts[f]->desc, any != 0, pc, &full));
             for (int thr : thresholds)
-                for (int frac = 0; frac < 2; ++frac) {
-                    hiprt_tracer_opts o{thr, frac == 0 ? full : std::max(1, full / 2), nullptr, stack};
+                for (int frac = 0; frac < 2; ++frac)
+                for (int chunks : {1, 8}) {
+                    hiprt_tracer_opts o{thr, frac == 0 ? full : std::max(1, full / 2), nullptr, stack, chunks};
                     if (int st = trace(layouts[f], tracer, &o, &h)) {
                         (void)hipEventDestroy(e0);
                         (void)hipEventDestroy(e1);
@@ -184,7 +186,7 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
                     if (!same) continue;
                     if (ms < best_ms) {
                         best_ms = ms;
-                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(n / (ms * 1e3)), 0, stack};
+                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(n / (ms * 1e3)), 0, stack, chunks};
                     }
                 }
         }

@@ -213,8 +213,8 @@ int hiprt_create(int32_t device, hiprt_context *out) {
     std::unique_ptr<hiprt_context_s> c(new (std::nothrow) hiprt_context_s);
     if (!c) return fail(HIPRT_ERR_ARG, "out of host memory");
     c->device = device;
-    HIP_TRY(c->counter.alloc(256));
-    HIP_TRY(hipMemset(c->counter.p, 0, 256));
+    HIP_TRY(c->counter.alloc(rt::dev::kCounterBytes));
+    HIP_TRY(hipMemset(c->counter.p, 0, rt::dev::kCounterBytes));
     *out = c.release();
     return ok();
 }
@@ -557,6 +557,7 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
     cfg.stats = opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats) : nullptr;
     cfg.stack = stack;
+    cfg.chunks = (opts && opts->work_chunks == 8) ? 8 : 1;
     HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
                                              static_cast<uint32_t *>(sc->ctx->counter.p), any, cfg, s));
     return ok();

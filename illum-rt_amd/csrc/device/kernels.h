@@ -22,6 +22,11 @@ constexpr int kLdsStack = 28;
 constexpr int kLdsStack4 = 44;
 constexpr int kPersistentStackCap = kRegStack + kLdsStack;
 constexpr int kPersistentWavesPerBlock = 4;
+// Work counters of the persistent kernels: one per chunk, each in its own
+// 256-byte line; kCounterBytes of device memory hold all of them.
+constexpr uint32_t kMaxChunks = 8;
+constexpr uint32_t kCounterStride = 64;
+constexpr size_t kCounterBytes = kMaxChunks * kCounterStride * 4;
 
 // Stack entries the persistent kernels hold for a BVH of the given format
 // and stack configuration (0: 4 VGPR + 28/44 LDS entries, 1: 8 VGPR + 16/32).
@@ -50,9 +55,11 @@ struct persistent_cfg {
     int blocks;           // grid size; 0 = occupancy-derived default
     unsigned long long *stats = nullptr;   // non-null: counting variant (inner nodes, triangle tests)
     int stack = 0;        // stack configuration, see persistent_stack_cap
+    int chunks = 1;       // 1: one work counter; 8: one contiguous chunk of rays per XCD
 };
 
-// `counter` is one device uint32 that must be zero at launch; `count` is the
+// `counter` points to kCounterBytes of device memory (cleared by the launch);
+// `count` is the
 // number of rays, read from device memory when count_ptr is non-null (so a
 // captured graph can trace a batch whose size an earlier kernel produced).
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,

@@ -21,7 +21,13 @@
 //     pushed farthest first;
 //   * a descriptor table (MULTI): the rays of many scenes form one index
 //     space; each lane binds the scene of the ray it fetched, so one launch
-//     traces uneven batches of many scenes.
+//     traces uneven batches of many scenes;
+//   * work chunks (n_chunks = 8): the ray range is split into one contiguous
+//     chunk per XCD, each with its own counter; a block starts on the chunk
+//     of the XCD it runs on (the dispatcher places workgroup b on XCD b % 8)
+//     and moves on to the next chunk when its own runs dry.  Neighbouring
+//     rays -- neighbouring pixels, neighbouring path origins -- then share
+//     one XCD's L2 instead of being spread over all eight.
 //
 // Postponement, refill and 4-wide nodes change the order in which boxes see
 // the running best t, so results can differ from the plain kernel only
@@ -130,9 +136,12 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                                                           uint8_t *occluded, uint32_t n_static,
                                                           const uint32_t *count_ptr, uint32_t *counter,
                                                           int refill_thr, unsigned long long *stats,
-                                                          const batch_desc *table, uint32_t n_desc) {
+                                                          const batch_desc *table, uint32_t n_desc,
+                                                          uint32_t n_chunks) {
     __shared__ int32_t lds_stack[kPersistentWavesPerBlock * LDS * 64];
     const uint32_t n = count_ptr ? *count_ptr : n_static;
+    uint32_t chunk = blockIdx.x % n_chunks;   // wave-uniform: the chunk this wave takes rays from
+    uint32_t dry = 0;                         // chunks this wave has found exhausted
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 
@@ -147,7 +156,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
     uint32_t ray_id = 0;
     dhit h{};
     bool alive = false;
-    bool exhausted = false;   // wave-uniform: the counter has run past n
+    bool exhausted = false;   // wave-uniform: every chunk's counter has run past its end
     bool finite = true;       // this lane's ray has only finite components
     bool exact = !sc.nan_free;   // wave-uniform: use the select-based box test
     uint32_t c_nodes = 0, c_tris = 0;
@@ -159,14 +168,22 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
         bool fetch = !exhausted && (REFILL ? (n_alive < refill_thr) : (n_alive == 0));
         if (fetch) {
             uint32_t want = (uint32_t)__popcll(idle);
+            const uint32_t lo = (uint32_t)((uint64_t)n * chunk / n_chunks);
+            const uint32_t hi = (uint32_t)((uint64_t)n * (chunk + 1) / n_chunks);
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(counter, want);
+            if (lane == 0) base = atomicAdd(counter + chunk * kCounterStride, want);
             base = __shfl(base, 0);
-            if (base + want >= n) exhausted = true;
+            // base counts rays taken from this chunk; it only grows
+            const uint64_t first = (uint64_t)lo + base;
+            if (first + want >= hi) {
+                chunk = chunk + 1 == n_chunks ? 0 : chunk + 1;
+                if (++dry >= n_chunks) exhausted = true;
+            }
             if (!alive) {
                 uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                uint32_t idx = base + rank;
-                if (idx < n) {
+                uint64_t idx64 = first + rank;
+                uint32_t idx = (uint32_t)idx64;
+                if (idx64 < hi) {
                     if (MULTI) {
                         desc = find_desc(table, n_desc, idx);
                         const batch_desc &bd = table[desc];
@@ -187,7 +204,11 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                 }
             }
         }
-        if (__ballot(alive) == 0) break;
+        if (__ballot(alive) == 0) {
+            // nothing fetched: try the next chunk, or leave once every chunk is dry
+            if (exhausted) break;
+            continue;
+        }
         exact = __ballot(alive && !finite) != 0 || (MULTI ? false : !sc.nan_free);
         if (MULTI && !exact) exact = __ballot(alive && !table[desc].scene.nan_free) != 0;
 
@@ -375,6 +396,8 @@ struct launch_args {
     uint32_t n_desc;
 };
 
+__host__ __device__ constexpr uint32_t chunks_of(int c) { return c == 8 ? 8u : 1u; }
+
 template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK>
 hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipStream_t stream, bool grid_only,
                           int *grid_out) {
@@ -390,13 +413,13 @@ hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipSt
         if (cfg.stats) {
             hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, true, BVH, MULTI>), dim3(blocks),
                                dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
-                               cfg.refill_thr, cfg.stats, a.table, a.n_desc);
+                               cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks));
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, false, BVH, MULTI>), dim3(blocks), dim3(256),
                        0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter, cfg.refill_thr,
-                       cfg.stats, a.table, a.n_desc);
+                       cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks));
     return hipGetLastError();
 }
 
@@ -448,7 +471,7 @@ hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg 
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
                                    uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,
                                    bool any, const persistent_cfg &cfg, hipStream_t stream) {
-    hipError_t e = launch_fill_u32(counter, 0u, 1u, stream);
+    hipError_t e = launch_fill_u32(counter, 0u, kMaxChunks * kCounterStride, stream);
     if (e != hipSuccess) return e;
     launch_args a{sc, rays, hits, occluded, n, count_ptr, counter, nullptr, 0};
     return dispatch<false>(sc.tri, any, sc.bvh, a, cfg, stream, false, nullptr);
@@ -456,7 +479,7 @@ hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, co
 
 hipError_t launch_trace_batch(const batch_desc *d_table, uint32_t n_desc, uint32_t total, int tri, int bvh, bool any,
                               uint32_t *counter, const persistent_cfg &cfg, hipStream_t stream) {
-    hipError_t e = launch_fill_u32(counter, 0u, 1u, stream);
+    hipError_t e = launch_fill_u32(counter, 0u, kMaxChunks * kCounterStride, stream);
     if (e != hipSuccess) return e;
     launch_args a{};
     a.n = total;

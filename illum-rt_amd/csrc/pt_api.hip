@@ -266,7 +266,8 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
         return fail(HIPRT_ERR_ARG, "the plain schedule uses the while-while tracer");
     if (!p->batched && sc->desc.bvh != rt::BVH2)
         return fail(HIPRT_ERR_UNSUPPORTED, "the plain schedule needs a binary BVH scene");
-    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh, 0) : rt::dev::kPlainStackCap - 1))
+    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh, params->stack_config == 1 ? 1 : 0)
+                                     : rt::dev::kPlainStackCap - 1))
         return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     uint32_t n = p->n;
     HIP_TRY(p->rays[0].alloc(n));
@@ -286,11 +287,13 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     HIP_TRY(p->totals.alloc(8ull * (p->pp.max_depth + 1)));
     HIP_TRY(hipMemset(p->totals.p, 0, 8ull * (p->pp.max_depth + 1)));
     HIP_TRY(p->frame_seed.alloc(8));
-    HIP_TRY(p->work.alloc(256));
+    HIP_TRY(p->work.alloc(rt::dev::kCounterBytes));
     p->pcfg.speculative = p->tracer >= HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE;
     p->pcfg.refill = p->tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
     p->pcfg.refill_thr = 40;
     p->pcfg.blocks = 0;
+    p->pcfg.stack = params->stack_config == 1 ? 1 : 0;
+    p->pcfg.chunks = params->work_chunks == 8 ? 8 : 1;
     if (p->batched) {
         // Resolve the persistent grid size outside the capture, then record
         // the frame once.  The graph never clears the accumulator: frame 0

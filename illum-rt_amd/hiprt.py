@@ -50,7 +50,7 @@ class _Hits(ctypes.Structure):
 
 class _TracerOpts(ctypes.Structure):
     _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32), ("d_fetch_stats", c_void_p),
-                ("stack_config", c_int32)]
+                ("stack_config", c_int32), ("work_chunks", c_int32)]
 
 
 class _BatchEntry(ctypes.Structure):
@@ -59,12 +59,13 @@ class _BatchEntry(ctypes.Structure):
 
 class _TuneResult(ctypes.Structure):
     _fields_ = [("tracer", c_int32), ("refill_threshold", c_int32), ("blocks", c_int32), ("bvh_format", c_int32),
-                ("mrays_per_s", c_float), ("candidates", c_int32), ("stack_config", c_int32)]
+                ("mrays_per_s", c_float), ("candidates", c_int32), ("stack_config", c_int32),
+                ("work_chunks", c_int32)]
 
 
 class _PtParams(ctypes.Structure):
     _fields_ = [("max_depth", c_int32), ("rr_start", c_int32), ("rr_survive", c_float), ("seed", c_uint64),
-                ("tracer", c_int32), ("batched", c_int32)]
+                ("tracer", c_int32), ("batched", c_int32), ("stack_config", c_int32), ("work_chunks", c_int32)]
 
 
 _lib = None
@@ -298,8 +299,8 @@ class DeviceScene:
         """Refit to `positions`, a contiguous (n_vert, 3) float32 CUDA tensor:
         same triangles and topology, new records and boxes."""
         assert positions.dtype.is_floating_point and positions.element_size() == 4 and positions.is_contiguous()
-        nv, _ = scene_counts(self)
-        if tuple(positions.shape) != (nv, 3) or not positions.is_cuda:
+        nv, _ = scene_counts(self)   # 0 for scenes uploaded without a host scene: the library refuses those
+        if (nv and tuple(positions.shape) != (nv, 3)) or not positions.is_cuda:
             raise HiprtError(f"refit needs a ({nv}, 3) CUDA tensor, got {tuple(positions.shape)}")
         check(lib().hiprt_scene_refit(self.h, c_void_p(positions.data_ptr()), _stream(stream)))
 
@@ -331,26 +332,27 @@ def tracer_id(name):
     return i
 
 
-def _opts(refill_threshold=0, blocks=0, fetch_stats=None, stack_config=0):
+def _opts(refill_threshold=0, blocks=0, fetch_stats=None, stack_config=0, work_chunks=1):
     return _TracerOpts(refill_threshold, blocks, fetch_stats.data_ptr() if fetch_stats is not None else None,
-                       stack_config)
+                       stack_config, work_chunks)
 
 
 def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
-                  fetch_stats=None, stack_config=0):
+                  fetch_stats=None, stack_config=0, work_chunks=1):
     """Closest hits of `rays` (8, n) into `hits`.  fetch_stats: optional int64
     CUDA tensor of 2 that receives inner-node fetches and triangle tests."""
     n = rays.shape[1]
-    r, h, o = _rays_struct(rays), _hits_struct(hits), _opts(refill_threshold, blocks, fetch_stats, stack_config)
+    r, h = _rays_struct(rays), _hits_struct(hits)
+    o = _opts(refill_threshold, blocks, fetch_stats, stack_config, work_chunks)
     s = c_void_p(stream) if stream else None
     check(lib().hiprt_trace_closest(dscene.h, ctypes.byref(r), n, ctypes.byref(h), tracer_id(tracer),
                                     ctypes.byref(o), s))
 
 
 def trace_any(dscene, rays, occluded, hits=None, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
-              stack_config=0):
+              stack_config=0, work_chunks=1):
     n = rays.shape[1]
-    r, o = _rays_struct(rays), _opts(refill_threshold, blocks, None, stack_config)
+    r, o = _rays_struct(rays), _opts(refill_threshold, blocks, None, stack_config, work_chunks)
     hp = ctypes.byref(_hits_struct(hits)) if hits is not None else None
     occ = c_void_p(occluded.data_ptr()) if occluded is not None else None
     s = c_void_p(stream) if stream else None
@@ -397,13 +399,14 @@ def shadow_rays(dscene, rays, hits, out, light, stream=None):
 class PathTracer:
     """Wavefront diffuse path tracer over a device scene (plain or batched)."""
 
-    def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None):
+    def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None,
+                 stack_config=0, work_chunks=1):
         self.dscene = dscene
         c = np.ascontiguousarray(cam[:12], dtype=np.float32)
         self.w, self.h = int(c[10]), int(c[11])
         self.levels = max_depth + 1
         tid = -1 if tracer is None else tracer_id(tracer)
-        p = _PtParams(max_depth, rr_start, rr_survive, seed, tid, 1 if batched else 0)
+        p = _PtParams(max_depth, rr_start, rr_survive, seed, tid, 1 if batched else 0, stack_config, work_chunks)
         self.h_ = c_void_p()
         check(lib().hiprt_pt_create(dscene.h, c.ctypes.data_as(POINTER(c_float)), ctypes.byref(p),
                                     ctypes.byref(self.h_)))
@@ -471,7 +474,7 @@ def adjoint_closest(dscene, rays, hits, grad_t, grad_u, grad_v, grad_vertices=No
 
 
 def trace_batch(entries, any=False, tracer="dynamic-while-while", stream=None, refill_threshold=0, blocks=0,
-                stack_config=0):
+                stack_config=0, work_chunks=1):
     """entries: list of (dscene, rays (8,n), hits dict or None, occluded tensor or None)."""
     arr = (_BatchEntry * len(entries))()
     for k, (ds, rays, hits, occ) in enumerate(entries):
@@ -480,7 +483,7 @@ def trace_batch(entries, any=False, tracer="dynamic-while-while", stream=None, r
         arr[k].hits = _hits_struct(hits) if hits is not None else _Hits(None, None, None, None)
         arr[k].occluded = occ.data_ptr() if occ is not None else None
         arr[k].n = rays.shape[1]
-    o = _opts(refill_threshold, blocks, None, stack_config)
+    o = _opts(refill_threshold, blocks, None, stack_config, work_chunks)
     check(lib().hiprt_trace_batch(arr, len(entries), 1 if any else 0, tracer_id(tracer), ctypes.byref(o),
                                   _stream(stream)))
 
@@ -495,5 +498,5 @@ def autotune(ctx, scene, bvh, sample_rays, any=False):
     ds = DeviceScene(ctx, None, None, handle=h)
     out = {"tracer": TRACERS[res.tracer], "refill_threshold": res.refill_threshold, "blocks": res.blocks,
            "bvh_format": "bvh4" if res.bvh_format == BVH4 else "bvh2", "mrays_per_s": res.mrays_per_s,
-           "candidates": res.candidates, "stack_config": res.stack_config}
+           "candidates": res.candidates, "stack_config": res.stack_config, "work_chunks": res.work_chunks}
     return out, ds

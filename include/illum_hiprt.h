@@ -176,6 +176,8 @@ typedef struct {
                                  tests to [1] (device uint64[2]) */
     int32_t stack_config;     /* traversal stack per lane: 0 = 4 VGPR + 28 LDS entries
                                  (44 for 4-wide), 1 = 8 VGPR + 16 LDS (32 for 4-wide) */
+    int32_t work_chunks;      /* 1 = one work counter; 8 = one contiguous chunk of the
+                                 rays per XCD (blocks start on their own XCD's chunk) */
 } hiprt_tracer_opts;
 
 /* compute_closest_hit of batch_ray_tracer (libgi/wavefront-rt.h:145-149,
@@ -225,6 +227,7 @@ typedef struct {
     float mrays_per_s;     /* measured on the sample */
     int32_t candidates;    /* configurations timed */
     int32_t stack_config;
+    int32_t work_chunks;
 } hiprt_tune_result;
 
 int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hiprt_rays *sample, uint32_t n,
@@ -281,6 +284,8 @@ typedef struct {
     uint64_t seed;
     int32_t tracer;      /* tracer id; -1 = while-while (plain) / default (batched) */
     int32_t batched;
+    int32_t stack_config;   /* persistent tracer options (batched), as in hiprt_tracer_opts */
+    int32_t work_chunks;
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;

@@ -194,8 +194,10 @@ def test_empty_batches(torch_cuda, c1s):
 def test_fetch_counts_match_the_oracle(torch_cuda, c1s):
     """Inner-node fetches and triangle tests counted by the kernels against the
     oracle's counting traversal over the same tree and rays: equal for the
-    plain kernel, within 2% for the persistent ones (speculation tests a few
-    triangles early, which can change what later culling removes)."""
+    kernels that keep the reference's order (plain, persistent); the
+    speculative ones postpone leaves, so the best t arrives later and culls
+    less -- measured +1.8% nodes and +4.0% triangle tests on these rays --
+    and are held to 2% and 5%."""
     print("case: C1 fetch counters vs oracle counts", flush=True)
     torch = torch_cuda
     cam = hiprt.camera_rays(c1s["sc"].cam)
@@ -208,10 +210,10 @@ def test_fetch_counts_match_the_oracle(torch_cuda, c1s):
         _closest(torch, c1s["ds"], rays, tracer, fetch_stats=st)
         got = st.cpu().numpy()
         print(tracer, got, want)
-        if tracer == "while-while":
+        if tracer in ("while-while", "persistent-while-while"):
             assert np.array_equal(got, want)
         else:
-            assert np.all(np.abs(got / want - 1) <= 0.02), (tracer, got, want)
+            assert 1.0 <= got[0] / want[0] <= 1.02 and 1.0 <= got[1] / want[1] <= 1.05, (tracer, got, want)
 
 
 def test_c2_shadow_rays_any_hit(torch_cuda, hip_ctx):

@@ -75,13 +75,13 @@ def test_bounce_rays_match_oracle_and_trace_exactly(torch_cuda, hall):
     assert bit_exact(_hits_np(hits2), ref)
 
 
-@pytest.mark.parametrize("res", [(320, 180)])
-def test_batched_equals_plain(torch_cuda, hall, res):
-    print(f"case: hall path tracer {res}, plain vs batched, 2 frames", flush=True)
+@pytest.mark.parametrize("res,stack,chunks", [((320, 180), 0, 1), ((320, 180), 1, 8)])
+def test_batched_equals_plain(torch_cuda, hall, res, stack, chunks):
+    print(f"case: hall path tracer {res}, plain vs batched (stack {stack}, chunks {chunks}), 2 frames", flush=True)
     cam = hall["cam"].copy()
     cam[10], cam[11] = res
     plain = hiprt.PathTracer(hall["ds"], cam, batched=False, seed=5)
-    batched = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=5)
+    batched = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=5, stack_config=stack, work_chunks=chunks)
     counts = []
     for pt in (plain, batched):
         pt.render(0)

@@ -84,6 +84,20 @@ def test_second_stack_config_matches_up_to_ties(torch_cuda, c1, tracer):
     assert same_t.all()
 
 
+@pytest.mark.parametrize("blocks", [0, 3])
+@pytest.mark.parametrize("tracer", TRACERS[1:])
+def test_xcd_work_chunks_match_up_to_ties(torch_cuda, c1, tracer, blocks):
+    """Eight work chunks, one per XCD, also with fewer blocks than chunks
+    (every block then serves several chunks): every ray traced once, same
+    hits up to ties."""
+    print(f"case: C1 camera+random rays, {tracer}, 8 work chunks, blocks {blocks}", flush=True)
+    got = _run_closest(torch_cuda, c1["exact"], c1["rays"], tracer, work_chunks=8, blocks=blocks)
+    assert (got[0].view(np.uint32) != np.float32(-7).view(np.uint32)).all()   # every hit slot written
+    rep = classify(orc, c1["pos"], c1["idx"], c1["rays"], got, c1["ref"])
+    assert rep["unexplained"] == 0, rep
+    assert rep["agree_frac"] >= 0.99999, rep
+
+
 @pytest.mark.parametrize("tracer", ["while-while", "dynamic-while-while"])
 def test_woop_within_tie_and_grazing_bar(torch_cuda, c1, tracer):
     """Woop records: hit agreement >= 99.999% and every mismatch a tie or a
