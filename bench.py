@@ -49,7 +49,7 @@ def time_frames(torch, pt, steps, warmup, stream):
     return ms, rays, (after - before) / steps
 
 
-def time_trace(torch, hiprt, ds, rays, tracer, stream, reps=10, **kw):
+def time_trace(torch, hiprt, ds, rays, tracer="dynamic-while-while", stream=None, reps=10, **kw):
     n = rays.shape[1]
     hits = hiprt.new_hits(n)
     for _ in range(2):
@@ -120,7 +120,7 @@ def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
     n_tris = sum(sc.n_tris for sc, _, _ in work)
     for sc, bvh, rays in work:
         d_rays = torch.from_numpy(rays).cuda()
-        sample = d_rays[:, ::max(1, rays.shape[1] // 32768)].contiguous()
+        sample = d_rays[:, ::max(1, rays.shape[1] // 262144)].contiguous()
         res, ds = hiprt.autotune(ctx, sc, bvh, sample)
         layouts.append(res["bvh_format"])
         hits = hiprt.new_hits(rays.shape[1])
@@ -185,7 +185,27 @@ def main():
     log(f"scene: {len(idx)} triangles, {ds.n_nodes} inner nodes, {ds.n_slots} leaf slots, depth {ds.depth}; "
         f"setup {time.time() - t0:.1f}s")
 
-    batched = hiprt.PathTracer(ds, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True)
+    # Camera rays and first-bounce diffuse rays of the frame.
+    d_cam = torch.empty((8, w * h), dtype=torch.float32, device="cuda")
+    hiprt.camera_rays_device(cam, d_cam, stream.cuda_stream)
+    cam_hits = hiprt.new_hits(w * h)
+    hiprt.trace_closest(ds, d_cam, cam_hits, tracer="while-while", stream=stream.cuda_stream)
+    d_diff = torch.empty_like(d_cam)
+    hiprt.bounce_cosine(ds, d_cam, cam_hits, d_diff, seed=12345, level=0, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+
+    # Autotune the persistent tracer on the frame's first-bounce diffuse rays
+    # (tracer, refill threshold, stack configuration, work chunks, BVH
+    # layout); the batched schedule runs with the chosen configuration.
+    t_tune = time.time()
+    tuned, ds_tuned = hiprt.autotune(ctx, sc, bvh, d_diff)
+    tuned["seconds"] = round(time.time() - t_tune, 1)
+    log(f"autotuned: {tuned}")
+    pt_kw = dict(tracer=tuned["tracer"], stack_config=tuned["stack_config"], work_chunks=tuned["work_chunks"],
+                 refill_threshold=tuned["refill_threshold"], blocks=tuned["blocks"])
+
+    batched = hiprt.PathTracer(ds_tuned, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True,
+                               **pt_kw)
     plain = hiprt.PathTracer(ds, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=False)
     ms_b, rays_b, per_level_b = time_frames(torch, batched, args.steps, args.warmup, stream)
     log(f"batched: {ms_b:.3f} ms/frame, {rays_b / 1e6:.3f} Mrays/frame, per level {per_level_b.tolist()}")
@@ -193,27 +213,27 @@ def main():
     log(f"plain:   {ms_p:.3f} ms/frame, {rays_p / 1e6:.3f} Mrays/frame")
     value = rays_b / ms_b / 1e3
     plain_value = rays_p / ms_p / 1e3
+    # the batched schedule with the fixed defaults (binary BVH, dynamic tracer, stack 0, one chunk)
+    fixed = hiprt.PathTracer(ds, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True)
+    ms_f, rays_f, _ = time_frames(torch, fixed, args.steps, args.warmup, stream)
+    fixed.close()
+    log(f"batched, fixed defaults: {ms_f:.3f} ms/frame")
 
     # Dominant kernel: the persistent closest-hit trace.  Per level: HIP-event
     # time of the launch, and the counting variant's fetches on the same rays.
+    node_bytes = 64.0 if tuned["bvh_format"] == "bvh2" else 128.0
     ms_lvl, fetch, rays_lvl = batched.profile(frame=10_000, stream=stream.cuda_stream)
-    bytes_lvl = 48.0 * rays_lvl + 64.0 * fetch[:, 0] + 48.0 * fetch[:, 1]
+    bytes_lvl = 48.0 * rays_lvl + node_bytes * fetch[:, 0] + 48.0 * fetch[:, 1]
     achieved = float(bytes_lvl.sum()) / (float(ms_lvl.sum()) * 1e-3) / 1e9
     launches = int((rays_lvl > 0).sum())
-    footprint = ds.n_nodes * 64 + ds.n_slots * 48
+    footprint = ds_tuned.n_nodes * node_bytes + ds_tuned.n_slots * 48
     compulsory = float(48.0 * rays_lvl.sum() + footprint * launches) / (float(ms_lvl.sum()) * 1e-3) / 1e9
     log(f"trace per level ms {np.round(ms_lvl, 3).tolist()}, nodes/ray "
         f"{np.round(fetch[:, 0] / np.maximum(rays_lvl, 1), 1).tolist()}, tris/ray "
         f"{np.round(fetch[:, 1] / np.maximum(rays_lvl, 1), 1).tolist()}")
 
     # The trace kernels alone: primary rays and first-bounce diffuse rays.
-    d_cam = torch.empty((8, w * h), dtype=torch.float32, device="cuda")
-    hiprt.camera_rays_device(cam, d_cam, stream.cuda_stream)
     kernels = {}
-    ms, hits = time_trace(torch, hiprt, ds, d_cam, "while-while", stream)
-    d_diff = torch.empty_like(d_cam)
-    hiprt.bounce_cosine(ds, d_cam, hits, d_diff, seed=12345, level=0, stream=stream.cuda_stream)
-    torch.cuda.synchronize()
     variants = (("while-while", {}), ("dynamic-while-while", {}),
                 ("dynamic-while-while", {"stack_config": 1}), ("dynamic-while-while", {"work_chunks": 8}),
                 ("dynamic-while-while", {"stack_config": 1, "work_chunks": 8}))
@@ -222,6 +242,8 @@ def main():
             ms, _ = time_trace(torch, hiprt, ds, rays, tr, stream, **kw)
             tag = "".join(f",{k}={v}" for k, v in kw.items())
             kernels[f"{name}/{tr}{tag}"] = round(rays.shape[1] / ms / 1e3, 1)
+        ms, _ = time_trace(torch, hiprt, ds_tuned, rays, stream=stream, **pt_kw)
+        kernels[f"{name}/autotuned"] = round(rays.shape[1] / ms / 1e3, 1)
     log(f"trace kernels alone (Mrays/s): {kernels}")
 
     # Closest-hit adjoint on 1M of the diffuse rays: vertex (atomic), origin
@@ -302,16 +324,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "hall 262,144 triangles, 1920x1080, 4-bounce cosine path tracing, "
-                               "RR p=0.8 at bounces 3-4, batched persistent+graph",
+                               "RR p=0.8 at bounces 3-4, batched persistent+graph, autotuned tracer",
                    "triangles": int(len(idx)), "width": w, "height": h, "max_depth": 4,
-                   "bvh": "SAH 16 planes, 4 tris/leaf, ESC", "rays_per_frame": int(round(rays_b))},
+                   "bvh": "SAH 16 planes, 4 tris/leaf, ESC", "layout": tuned["bvh_format"],
+                   "rays_per_frame": int(round(rays_b))},
         "plain_value": round(plain_value, 1),
         "plain_ms_per_step": round(ms_p, 4),
         "batched_over_plain": round(value / plain_value, 3),
         "trace_kernels_mrays": kernels,
+        "batched_fixed_defaults_value": round(rays_f / ms_f / 1e3, 1),
+        "autotuned": tuned,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_trace_persistent (dynamic-while-while)",
+                     "kernel": f"k_trace_persistent ({tuned['tracer']}, {tuned['bvh_format']})",
                      "avg_launch_ms": round(float(ms_lvl.sum()) / max(launches, 1), 4),
                      "bytes_per_launch": round(float(bytes_lvl.sum()) / max(launches, 1)),
                      "compulsory_GBs": round(compulsory, 1)},
@@ -323,6 +348,7 @@ def main():
     print(json.dumps(out), flush=True)
     batched.close()
     plain.close()
+    ds_tuned.close()
     ds.close()
     ctx.close()
 

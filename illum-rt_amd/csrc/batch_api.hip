@@ -114,22 +114,52 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
             return st;
         }
     }
+    // A persistent grid holds ~300k lanes on MI355X: a batch much smaller than
+    // that times tail latency, not throughput, and ranks configurations
+    // wrongly.  Small samples are therefore tiled up to kMinTuneRays rays;
+    // results are checked on the first n (the sample itself).
+    constexpr uint32_t kMinTuneRays = 1u << 20;
+    const uint32_t reps = n >= kMinTuneRays ? 1u : (kMinTuneRays + n - 1) / n;
+    const uint64_t nt64 = (uint64_t)n * reps;
+    if (nt64 > 0xF0000000ull) {
+        cleanup();
+        return fail(HIPRT_ERR_ARG, "sample too large");
+    }
+    const uint32_t nt = (uint32_t)nt64;
+    dbuf tiled;
+    hiprt_rays tr = *sample;
+    if (reps > 1) {
+        if (tiled.alloc(32ull * nt) != hipSuccess) {
+            cleanup();
+            return fail(HIPRT_ERR_HIP, "cannot allocate the tiled sample");
+        }
+        const float *src[8] = {sample->d_ox, sample->d_oy, sample->d_oz, sample->d_tmin,
+                               sample->d_dx, sample->d_dy, sample->d_dz, sample->d_tmax};
+        const float *dst[8];
+        for (int k = 0; k < 8; ++k) {
+            float *plane = tiled.as<float>() + (uint64_t)k * nt;
+            dst[k] = plane;
+            for (uint32_t r = 0; r < reps; ++r)
+                HIP_TRY(hipMemcpy(plane + (uint64_t)r * n, src[k], 4ull * n, hipMemcpyDeviceToDevice));
+        }
+        tr = hiprt_rays{dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dst[6], dst[7]};
+    }
     // reference hits of the sample: plain kernel, binary layout
     dbuf hb, ref_hb, occ;
-    if (hb.alloc(16ull * n) != hipSuccess || ref_hb.alloc(16ull * n) != hipSuccess || occ.alloc(n) != hipSuccess) {
+    if (hb.alloc(16ull * nt) != hipSuccess || ref_hb.alloc(16ull * n) != hipSuccess || occ.alloc(nt) != hipSuccess) {
         cleanup();
         return fail(HIPRT_ERR_HIP, "cannot allocate the sample's hit buffers");
     }
-    auto hits_of = [n](dbuf &d) {
+    auto hits_of = [](dbuf &d, uint32_t m) {
         float *p = d.as<float>();
-        return hiprt_hits{p, p + n, p + 2ull * n, reinterpret_cast<uint32_t *>(p + 3ull * n)};
+        return hiprt_hits{p, p + m, p + 2ull * m, reinterpret_cast<uint32_t *>(p + 3ull * m)};
     };
-    hiprt_hits ref_h = hits_of(ref_hb), h = hits_of(hb);
-    auto trace = [&](hiprt_scene sc, int tracer, const hiprt_tracer_opts *o, hiprt_hits *hh) {
-        return any ? hiprt_trace_any(sc, sample, n, occ.as<uint8_t>(), hh, tracer, o, nullptr)
-                   : hiprt_trace_closest(sc, sample, n, hh, tracer, o, nullptr);
+    hiprt_hits ref_h = hits_of(ref_hb, n), h = hits_of(hb, nt);
+    auto trace = [&](hiprt_scene sc, int tracer, const hiprt_tracer_opts *o, hiprt_hits *hh, uint32_t m) {
+        return any ? hiprt_trace_any(sc, &tr, m, occ.as<uint8_t>(), hh, tracer, o, nullptr)
+                   : hiprt_trace_closest(sc, &tr, m, hh, tracer, o, nullptr);
     };
-    if (int st = trace(layouts[0], HIPRT_TRACER_WHILE_WHILE, nullptr, &ref_h)) {
+    if (int st = trace(layouts[0], HIPRT_TRACER_WHILE_WHILE, nullptr, &ref_h, n)) {
         cleanup();
         return st;
     }
@@ -157,19 +187,18 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
             int full = 0;
             HIP_TRY(rt::dev::persistent_grid(layouts[f]->desc, any != 0, pc, &full));
             for (int thr : thresholds)
-                for (int frac = 0; frac < 2; ++frac)
                 for (int chunks : {1, 8}) {
-                    hiprt_tracer_opts o{thr, frac == 0 ? full : std::max(1, full / 2), nullptr, stack, chunks};
-                    if (int st = trace(layouts[f], tracer, &o, &h)) {
+                    hiprt_tracer_opts o{thr, full, nullptr, stack, chunks};
+                    if (int st = trace(layouts[f], tracer, &o, &h, nt)) {
                         (void)hipEventDestroy(e0);
                         (void)hipEventDestroy(e1);
                         cleanup();
                         return st;
                     }
                     float ms = 1e30f;
-                    for (int rep = 0; rep < 3; ++rep) {
+                    for (int rep = 0; rep < 2; ++rep) {
                         HIP_TRY(hipEventRecord(e0, nullptr));
-                        (void)trace(layouts[f], tracer, &o, &h);
+                        (void)trace(layouts[f], tracer, &o, &h, nt);
                         HIP_TRY(hipEventRecord(e1, nullptr));
                         HIP_TRY(hipEventSynchronize(e1));
                         float m = 0;
@@ -186,7 +215,7 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
                     if (!same) continue;
                     if (ms < best_ms) {
                         best_ms = ms;
-                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(n / (ms * 1e3)), 0, stack, chunks};
+                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(nt / (ms * 1e3)), 0, stack, chunks};
                     }
                 }
         }

@@ -80,15 +80,32 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     return so;
 }
 
-// Wave-aggregated append: every live lane gets a slot in [*count, *count + k).
-__device__ __forceinline__ uint32_t wave_append(bool live, uint32_t *count) {
+// Shading blocks are 1024 threads (16 wavefronts) so that appends to the
+// next queue cost one global atomic per block: atomics on one address
+// serialize (about 11 ns apart on MI355X), and one per wavefront over a 2M-ray
+// level costs more than the shading itself.
+constexpr int kShadeBlock = 1024;
+constexpr int kShadeWaves = kShadeBlock / 64;
+
+// Block-aggregated append: every live thread gets a distinct slot in
+// [*count, *count + live threads of the block).  Every thread of the block
+// must call it.  `sh` is kShadeWaves + 1 words of LDS.
+__device__ __forceinline__ uint32_t block_append(bool live, uint32_t *count, uint32_t *sh) {
     uint64_t mask = __ballot(live);
-    uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    if (mask && lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)(mask ? leader : 0u));
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (int w = 0; w < kShadeWaves; ++w) {
+            uint32_t c = sh[w];
+            sh[w] = sum;
+            sum += c;
+        }
+        sh[kShadeWaves] = sum ? atomicAdd(count, sum) : 0u;
+    }
+    __syncthreads();
+    return sh[kShadeWaves] + sh[wave] + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
 } // namespace
@@ -109,10 +126,11 @@ __global__ __launch_bounds__(256) void k_camera_rays(cam_frame f, ray_out out, u
 
 // Plain shading: slot i is path i; the next ray replaces this one in place.
 // counts[level + 1] receives the number of live next-level rays.
-__global__ __launch_bounds__(256) void k_shade_plain(pt_scene ps, pt_params pp, int level, const uint64_t *frame_seed,
-                                                    ray_out rays, hit_soa hits, float *thr, float *rad,
-                                                    uint32_t *counts, uint32_t n) {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+__global__ __launch_bounds__(kShadeBlock) void k_shade_plain(pt_scene ps, pt_params pp, int level,
+                                                            const uint64_t *frame_seed, ray_out rays, hit_soa hits,
+                                                            float *thr, float *rad, uint32_t *counts, uint32_t n) {
+    __shared__ uint32_t sh[kShadeWaves + 1];
+    uint32_t i = blockIdx.x * (uint32_t)kShadeBlock + threadIdx.x;
     bool live = false;
     if (i < n && rays.tmax[i] >= 0.0f) {
         vec3f in_dir{rays.dx[i], rays.dy[i], rays.dz[i]};
@@ -122,19 +140,20 @@ __global__ __launch_bounds__(256) void k_shade_plain(pt_scene ps, pt_params pp, 
         if (live) write_ray(rays, i, so.org, kRayEps, so.dir, FLT_MAX);
         else write_disabled(rays, i);
     }
-    if (level < pp.max_depth) wave_append(live, counts + level + 1);
+    if (level < pp.max_depth) block_append(live, counts + level + 1, sh);
 }
 
 // Compacting shading: reads queue slot i < counts[level] (path path_in[i]),
 // appends live paths to the next queue.
-__global__ __launch_bounds__(256) void k_shade_compact(pt_scene ps, pt_params pp, int level,
-                                                      const uint64_t *frame_seed, ray_out rays_in, hit_soa hits,
-                                                      const uint32_t *path_in, ray_out rays_out,
-                                                      uint32_t *path_out, float *thr, float *rad,
-                                                      uint32_t *counts) {
-    uint32_t i = blockIdx.x * 256u + threadIdx.x;
+__global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_params pp, int level,
+                                                              const uint64_t *frame_seed, ray_out rays_in,
+                                                              hit_soa hits, const uint32_t *path_in,
+                                                              ray_out rays_out, uint32_t *path_out, float *thr,
+                                                              float *rad, uint32_t *counts) {
+    __shared__ uint32_t sh[kShadeWaves + 1];
+    uint32_t i = blockIdx.x * (uint32_t)kShadeBlock + threadIdx.x;
     uint32_t n = counts[level];
-    if (blockIdx.x * 256u >= n) return;   // whole block past the queue: uniform exit
+    if (blockIdx.x * (uint32_t)kShadeBlock >= n) return;   // whole block past the queue: uniform exit
     bool live = false;
     shade_out so{};
     uint32_t path = 0;
@@ -146,7 +165,7 @@ __global__ __launch_bounds__(256) void k_shade_compact(pt_scene ps, pt_params pp
         live = so.live;
     }
     if (level >= pp.max_depth) return;
-    uint32_t slot = wave_append(live, counts + level + 1);
+    uint32_t slot = block_append(live, counts + level + 1, sh);
     if (live) {
         write_ray(rays_out, slot, so.org, kRayEps, so.dir, FLT_MAX);
         path_out[slot] = path;
@@ -206,8 +225,8 @@ hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *
 hipError_t launch_shade_plain(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
                               const ray_out &rays, const hit_soa &hits, float *thr, float *rad, uint32_t *counts,
                               uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade_plain, dim3((n + 255) / 256), dim3(256), 0, s, ps, pp, level, frame_seed, rays, hits,
-                       thr, rad, counts, n);
+    hipLaunchKernelGGL(k_shade_plain, dim3((n + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, ps, pp,
+                       level, frame_seed, rays, hits, thr, rad, counts, n);
     return hipGetLastError();
 }
 
@@ -215,8 +234,8 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 const ray_out &rays_in, const hit_soa &hits, const uint32_t *path_in,
                                 const ray_out &rays_out, uint32_t *path_out, float *thr, float *rad,
                                 uint32_t *counts, uint32_t n_max, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade_compact, dim3((n_max + 255) / 256), dim3(256), 0, s, ps, pp, level, frame_seed,
-                       rays_in, hits, path_in, rays_out, path_out, thr, rad, counts);
+    hipLaunchKernelGGL(k_shade_compact, dim3((n_max + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, ps,
+                       pp, level, frame_seed, rays_in, hits, path_in, rays_out, path_out, thr, rad, counts);
     return hipGetLastError();
 }
 

@@ -8,6 +8,8 @@
 // launched before the graph, so nothing host-side changes between replays.
 #include "capi_internal.h"
 
+#include <algorithm>
+
 #include "core/raygen.h"
 #include "device/kernels.h"
 #include "device/pt_kernels.h"
@@ -290,7 +292,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     HIP_TRY(p->work.alloc(rt::dev::kCounterBytes));
     p->pcfg.speculative = p->tracer >= HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE;
     p->pcfg.refill = p->tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
-    p->pcfg.refill_thr = 40;
+    p->pcfg.refill_thr = params->refill_threshold > 0 ? std::min(params->refill_threshold, 64) : 40;
     p->pcfg.blocks = 0;
     p->pcfg.stack = params->stack_config == 1 ? 1 : 0;
     p->pcfg.chunks = params->work_chunks == 8 ? 8 : 1;
@@ -298,7 +300,8 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
         // Resolve the persistent grid size outside the capture, then record
         // the frame once.  The graph never clears the accumulator: frame 0
         // after a reset is rendered eagerly with clear_radiance set.
-        HIP_TRY(rt::dev::persistent_grid(sc->desc, false, p->pcfg, &p->pcfg.blocks));
+        if (params->blocks > 0) p->pcfg.blocks = params->blocks;
+        else HIP_TRY(rt::dev::persistent_grid(sc->desc, false, p->pcfg, &p->pcfg.blocks));
         HIP_TRY(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
         hipError_t rec = p->record(p->cap, false);

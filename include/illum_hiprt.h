@@ -212,11 +212,13 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
 
 /* --------------------------------------------------------------- autotuner
  * Times candidate configurations on a sample of a scene's rays -- tracer
- * variant (persistent / speculative / dynamic), refill threshold, waves per
- * CU (grid size), stack configuration and BVH layout (binary vs collapsed
- * 4-wide) -- and returns
- * the fastest.  Every candidate's hits on the sample are compared with the
- * plain kernel's; a candidate that differs other than at equal t is
+ * variant (persistent / speculative / dynamic), refill threshold, stack
+ * configuration, work chunks and BVH layout (binary vs collapsed 4-wide), on
+ * the occupancy-derived grid -- and returns the fastest.  A sample of fewer
+ * than 2^20 rays is tiled up to that size before timing: a batch smaller than
+ * the lanes a persistent grid holds (~300k) measures tail latency and ranks
+ * configurations wrongly.  Every candidate's hits on the sample are compared
+ * with the plain kernel's; a candidate that differs other than at equal t is
  * rejected.  Results never depend on the choice.  The scene is committed in
  * the chosen layout into *tuned (caller destroys it). */
 typedef struct {
@@ -284,8 +286,10 @@ typedef struct {
     uint64_t seed;
     int32_t tracer;      /* tracer id; -1 = while-while (plain) / default (batched) */
     int32_t batched;
-    int32_t stack_config;   /* persistent tracer options (batched), as in hiprt_tracer_opts */
-    int32_t work_chunks;
+    int32_t stack_config;   /* persistent tracer options (batched), as in hiprt_tracer_opts; */
+    int32_t work_chunks;    /* 0 in refill_threshold / blocks selects the default */
+    int32_t refill_threshold;
+    int32_t blocks;
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;
