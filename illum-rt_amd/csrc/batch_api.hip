@@ -28,7 +28,7 @@ rt::dev::persistent_cfg cfg_for(int tracer, const hiprt_tracer_opts *opts) {
     c.refill_thr = (opts && opts->refill_threshold > 0) ? std::min(opts->refill_threshold, 64) : 40;
     c.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
     c.stats = nullptr;
-    c.stack = (opts && opts->stack_config == 1) ? 1 : 0;
+    c.stack = (opts && opts->stack_config > 0 && opts->stack_config < rt::dev::kStackConfigs) ? opts->stack_config : 0;
     c.chunks = (opts && opts->work_chunks == 8) ? 8 : 1;
     return c;
 }
@@ -177,7 +177,7 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
     float best_ms = 1e30f;
     int count = 0;
     for (int f = 0; f < 2; ++f)
-    for (int stack = 0; stack < 2; ++stack) {
+    for (int stack = 0; stack < rt::dev::kStackConfigs; ++stack) {
         if (layouts[f]->stack_need > rt::dev::persistent_stack_cap(f, stack)) continue;
         for (int tracer = HIPRT_TRACER_PERSISTENT_WHILE_WHILE; tracer < HIPRT_TRACER_COUNT; ++tracer) {
             std::vector<int> thresholds = tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE ? std::vector<int>{24, 40, 56}

@@ -536,7 +536,7 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     if (hits) h = rt::hit_soa{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
     hipStream_t s = static_cast<hipStream_t>(stream);
     int need = sc->stack_need;
-    int stack = (opts && opts->stack_config == 1) ? 1 : 0;
+    int stack = (opts && opts->stack_config > 0 && opts->stack_config < rt::dev::kStackConfigs) ? opts->stack_config : 0;
     bool persistent_fits = need <= rt::dev::persistent_stack_cap(sc->desc.bvh, stack);
     if (sc->desc.bvh == rt::BVH4 && (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits))
         return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH needs a persistent tracer with a stack of " +

@@ -22,6 +22,7 @@ constexpr int kLdsStack = 28;
 constexpr int kLdsStack4 = 44;
 constexpr int kPersistentStackCap = kRegStack + kLdsStack;
 constexpr int kPersistentWavesPerBlock = 4;
+constexpr int kStackConfigs = 4;
 // Work counters of the persistent kernels: one per chunk, each in its own
 // 256-byte line; kCounterBytes of device memory hold all of them.
 constexpr uint32_t kMaxChunks = 8;
@@ -54,7 +55,7 @@ struct persistent_cfg {
     int refill_thr;       // active-lane threshold for refill (out of 64)
     int blocks;           // grid size; 0 = occupancy-derived default
     unsigned long long *stats = nullptr;   // non-null: counting variant (inner nodes, triangle tests)
-    int stack = 0;        // stack configuration, see persistent_stack_cap
+    int stack = 0;        // stack configuration 0..kStackConfigs-1, see persistent_stack_cap
     int chunks = 1;       // 1: one work counter; 8: one contiguous chunk of rays per XCD
 };
 

@@ -352,13 +352,18 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t 
 
 namespace {
 
-// Stack configurations: (VGPR entries, LDS entries) per lane.  Config 0 keeps
-// more in LDS (fewer VGPRs, 28 KB of LDS per block for binary trees: 5
-// blocks per CU); config 1 keeps 8 entries in VGPRs and 16 in LDS (16 KB per
-// block: occupancy bound by VGPRs instead of LDS).
-constexpr int reg_for(int stk) { return stk == 0 ? kRegStack : 8; }
+// Stack configurations: (VGPR entries, LDS entries) per lane.  A push or pop
+// of the VGPR part shifts every register entry (static indices keep it out
+// of scratch), so VGPR entries cost VALU moves; LDS entries cost LDS space,
+// which bounds the blocks per CU.
+//   0: 4 VGPR + 28 LDS (44 for 4-wide)   28 KB of LDS per block (binary)
+//   1: 8 VGPR + 16 LDS (32)              16 KB
+//   2: 2 VGPR + 20 LDS (36)              20 KB
+//   3: 0 VGPR + 24 LDS (40)              24 KB
+constexpr int reg_for(int stk) { return stk == 0 ? kRegStack : stk == 1 ? 8 : stk == 2 ? 2 : 0; }
 constexpr int lds_for(int bvh, int stk) {
-    return bvh == 2 ? (stk == 0 ? kLdsStack : 16) : (stk == 0 ? kLdsStack4 : 32);
+    return bvh == 2 ? (stk == 0 ? kLdsStack : stk == 1 ? 16 : stk == 2 ? 20 : 24)
+                    : (stk == 0 ? kLdsStack4 : stk == 1 ? 32 : stk == 2 ? 36 : 40);
 }
 
 // Blocks of one instantiation that fit the device at once (occupancy x CUs),
@@ -435,8 +440,12 @@ hipError_t dispatch_flags(const launch_args &a, const persistent_cfg &cfg, hipSt
 
 template <int TRI, bool ANY, int BVH, bool MULTI>
 hipError_t dispatch_stack(const launch_args &a, const persistent_cfg &cfg, hipStream_t s, bool grid_only, int *g) {
-    if (cfg.stack == 1) return dispatch_flags<TRI, ANY, BVH, MULTI, 1>(a, cfg, s, grid_only, g);
-    return dispatch_flags<TRI, ANY, BVH, MULTI, 0>(a, cfg, s, grid_only, g);
+    switch (cfg.stack) {
+    case 1: return dispatch_flags<TRI, ANY, BVH, MULTI, 1>(a, cfg, s, grid_only, g);
+    case 2: return dispatch_flags<TRI, ANY, BVH, MULTI, 2>(a, cfg, s, grid_only, g);
+    case 3: return dispatch_flags<TRI, ANY, BVH, MULTI, 3>(a, cfg, s, grid_only, g);
+    default: return dispatch_flags<TRI, ANY, BVH, MULTI, 0>(a, cfg, s, grid_only, g);
+    }
 }
 
 template <bool MULTI>
@@ -460,7 +469,8 @@ hipError_t dispatch(int tri, bool any, int bvh, const launch_args &a, const pers
 
 int persistent_stack_cap(int bvh, int stack) {
     int b = bvh == BVH4 ? 4 : 2;
-    return stack == 1 ? reg_for(1) + lds_for(b, 1) : reg_for(0) + lds_for(b, 0);
+    int k = (stack >= 0 && stack < kStackConfigs) ? stack : 0;
+    return reg_for(k) + lds_for(b, k);
 }
 
 hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks) {

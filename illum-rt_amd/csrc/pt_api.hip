@@ -268,7 +268,8 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
         return fail(HIPRT_ERR_ARG, "the plain schedule uses the while-while tracer");
     if (!p->batched && sc->desc.bvh != rt::BVH2)
         return fail(HIPRT_ERR_UNSUPPORTED, "the plain schedule needs a binary BVH scene");
-    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh, params->stack_config == 1 ? 1 : 0)
+    const int stack = (params->stack_config > 0 && params->stack_config < rt::dev::kStackConfigs) ? params->stack_config : 0;
+    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh, stack)
                                      : rt::dev::kPlainStackCap - 1))
         return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     uint32_t n = p->n;
@@ -294,7 +295,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pcfg.refill = p->tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
     p->pcfg.refill_thr = params->refill_threshold > 0 ? std::min(params->refill_threshold, 64) : 40;
     p->pcfg.blocks = 0;
-    p->pcfg.stack = params->stack_config == 1 ? 1 : 0;
+    p->pcfg.stack = stack;
     p->pcfg.chunks = params->work_chunks == 8 ? 8 : 1;
     if (p->batched) {
         // Resolve the persistent grid size outside the capture, then record

@@ -174,8 +174,9 @@ typedef struct {
     uint64_t *d_fetch_stats;  /* non-NULL: run the counting variant, which adds the inner
                                  nodes it fetches to [0] and the triangle records it
                                  tests to [1] (device uint64[2]) */
-    int32_t stack_config;     /* traversal stack per lane: 0 = 4 VGPR + 28 LDS entries
-                                 (44 for 4-wide), 1 = 8 VGPR + 16 LDS (32 for 4-wide) */
+    int32_t stack_config;     /* traversal stack per lane (VGPR + LDS entries, binary /
+                                 4-wide): 0 = 4 + 28/44, 1 = 8 + 16/32, 2 = 2 + 20/36,
+                                 3 = 0 + 24/40 */
     int32_t work_chunks;      /* 1 = one work counter; 8 = one contiguous chunk of the
                                  rays per XCD (blocks start on their own XCD's chunk) */
 } hiprt_tracer_opts;
