@@ -159,6 +159,10 @@ def main():
                     help="skip the shadow-ray and 64-scene legs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
+    ap.add_argument("--kernel-only", choices=("primary", "diffuse"),
+                    help="only trace the hall's primary or first-bounce diffuse rays 3 times with the dynamic "
+                         "tracer (for counter collection) and print their fetch counts")
+    ap.add_argument("--stack-config", type=int, default=0, help="stack configuration for --kernel-only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays are independent, so "
@@ -194,6 +198,20 @@ def main():
     hiprt.bounce_cosine(ds, d_cam, cam_hits, d_diff, seed=12345, level=0, stream=stream.cuda_stream)
     torch.cuda.synchronize()
 
+    if args.kernel_only:
+        rays = d_cam if args.kernel_only == "primary" else d_diff
+        ms, _ = time_trace(torch, hiprt, ds, rays, "dynamic-while-while", stream, reps=3,
+                           stack_config=args.stack_config)
+        st = torch.zeros(2, dtype=torch.int64, device="cuda")
+        hiprt.trace_closest(ds, rays, hiprt.new_hits(rays.shape[1]), tracer="dynamic-while-while",
+                            stream=stream.cuda_stream, fetch_stats=st, stack_config=args.stack_config)
+        torch.cuda.synchronize()
+        print(json.dumps({"rays": int(rays.shape[1]), "kind": args.kernel_only, "ms": round(ms, 4),
+                          "inner_nodes": int(st[0]), "triangle_tests": int(st[1])}), flush=True)
+        ds.close()
+        ctx.close()
+        return
+
     # Autotune the persistent tracer on the frame's first-bounce diffuse rays
     # (tracer, refill threshold, stack configuration, work chunks, BVH
     # layout); the batched schedule runs with the chosen configuration.
@@ -206,20 +224,21 @@ def main():
 
     # The trace autotuner times one launch over rays already in pixel order
     # with the tree warm in L2; inside a frame every level follows a shading
-    # pass and reads a compacted queue.  So the frame's configuration is
-    # chosen among a few candidates by timing warm-up frames (never the
-    # timed ones): the autotuned one, its one-counter variant, and the fixed
-    # defaults (binary tree, dynamic tracer, stack 0, one counter).
+    # pass and reads a compacted queue, and there the configurations measure
+    # within noise of each other.  The timed frames use the library defaults
+    # (binary tree, dynamic tracer, stack 0, one work counter), so the
+    # dominant kernel is the same in every run; warm-up frames of the
+    # autotuned configurations are reported beside them.
     candidates = {"autotuned": (ds_tuned, pt_kw),
                   "autotuned, one work counter": (ds_tuned, dict(pt_kw, work_chunks=1)),
-                  "fixed defaults": (ds, {})}
+                  "library defaults": (ds, {})}
     frame_ms = {}
     for name, (dsc, kw) in candidates.items():
         pt = hiprt.PathTracer(dsc, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, **kw)
         frame_ms[name], _, _ = time_frames(torch, pt, 4, 1, stream)
         pt.close()
-    choice = min(frame_ms, key=frame_ms.get)
-    log(f"frame configuration trial (ms/frame): {frame_ms}; using {choice}")
+    choice = "library defaults"
+    log(f"frame configuration trial (ms/frame): {frame_ms}")
     ds_frame, frame_kw = candidates[choice]
 
     batched = hiprt.PathTracer(ds_frame, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True,
@@ -349,10 +368,11 @@ def main():
         "trace_kernels_mrays": kernels,
         "autotuned": tuned,
         "frame_config_trial_ms": {k: round(v, 4) for k, v in frame_ms.items()},
-        "frame_config": choice,
+
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_trace_persistent ({choice})",
+                     "kernel": "k_trace_persistent<0, false, true, true, 4, 28, false, 2, false> "
+                               "(dynamic-while-while, stack 0, binary tree)",
                      "avg_launch_ms": round(float(ms_lvl.sum()) / max(launches, 1), 4),
                      "bytes_per_launch": round(float(bytes_lvl.sum()) / max(launches, 1)),
                      "compulsory_GBs": round(compulsory, 1)},
