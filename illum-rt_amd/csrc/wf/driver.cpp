@@ -15,7 +15,7 @@
 // reference; "run" checks the same preconditions.
 //
 // Commands: at, look, up, fovy, resolution, sppx, load, generate, pointlight,
-// platform, raytracer, bvh, triangles, add-scene-step, commit, algo, run,
+// platform, raytracer, tracer-option, bvh, triangles, add-scene-step, commit, algo, run,
 // outfile, stats, echo, quit.  Images are written as PFM (float) or, for any
 // other extension, 8-bit PPM with a 2.2 gamma.
 #include "hip-platform.h"
@@ -193,7 +193,8 @@ void eval(const string &line) {
                 error("There is no platform called '" << name << "'");
             }
             uc.tracer_touched_at = uc.cmdid;
-        } else if (command == "raytracer" || command == "bvh" || command == "triangles") {
+        } else if (command == "raytracer" || command == "bvh" || command == "triangles" ||
+                   command == "tracer-option") {
             if (!rc->platform) error("No platform selected, this command needs the hip platform");
             if (!rc->platform->interprete(command, in)) error("unknown command");
             if (command == "raytracer") uc.tracer_touched_at = uc.cmdid;

@@ -126,7 +126,7 @@ void batch_rt::compute_closest_hit() {
     if (!pf->ds) throw std::logic_error("no committed scene");
     hiprt_rays r = rd->in();
     hiprt_hits h = rd->hit();
-    check(hiprt_trace_closest(pf->ds, &r, rd->n(), &h, tracer, nullptr, nullptr), "find closest hit");
+    check(hiprt_trace_closest(pf->ds, &r, rd->n(), &h, tracer, &pf->opts, nullptr), "find closest hit");
 }
 
 void batch_rt::compute_any_hit() {
@@ -134,7 +134,7 @@ void batch_rt::compute_any_hit() {
     if (!pf->ds) throw std::logic_error("no committed scene");
     hiprt_rays r = rd->in();
     hiprt_hits h = rd->hit();
-    check(hiprt_trace_any(pf->ds, &r, rd->n(), rd->occluded.as<uint8_t>(), &h, tracer, nullptr, nullptr),
+    check(hiprt_trace_any(pf->ds, &r, rd->n(), rd->occluded.as<uint8_t>(), &h, tracer, &pf->opts, nullptr),
           "find any hit");
 }
 
@@ -373,6 +373,30 @@ bool platform::interprete(const std::string &command, std::istringstream &in) {
             return true;
         }
         error("Unknown bvh subcommand " << sub);
+    }
+    if (command == "tracer-option") {
+        // No counterpart in the reference (its CUDA tracers are compiled with
+        // fixed settings): the persistent tracers' tunables.
+        std::string name;
+        int v = 0;
+        in >> name >> v;
+        check_in_complete("Syntax error, \"tracer-option\" requires a name and an integer value");
+        if (name == "refill") {
+            if (v < 1 || v > 64) error("Parameter error, refill threshold must be 1..64");
+            opts.refill_threshold = v;
+        } else if (name == "blocks") {
+            if (v < 0) error("Parameter error, blocks must be >= 0 (0 = occupancy default)");
+            opts.blocks = v;
+        } else if (name == "stack") {
+            if (v < 0 || v > 3) error("Parameter error, stack configuration must be 0..3");
+            opts.stack_config = v;
+        } else if (name == "chunks") {
+            if (v != 1 && v != 8) error("Parameter error, chunks must be 1 or 8");
+            opts.work_chunks = v;
+        } else {
+            error("Unknown tracer option " << name << " (refill, blocks, stack, chunks)");
+        }
+        return true;
     }
     if (command == "triangles") {
         std::string v;

@@ -33,7 +33,7 @@ def read_pfm(path):
 
 def test_driver_errors_follow_the_reference(tmp_path):
     r = run_script("echo hi\nplatform cuda\nraytracer while-while\nrun\nresolution 0 5\nsppx x\nbogus\n"
-                   "generate 9\nbvh type sah\n# comment\n// comment\n", tmp_path)
+                   "generate 9\nbvh type sah\ntracer-option stack 3\n# comment\n// comment\n", tmp_path)
     assert r.returncode == 0
     err = r.stderr.splitlines()
     assert "command (platform): There is no platform called 'cuda'" in err
@@ -42,8 +42,9 @@ def test_driver_errors_follow_the_reference(tmp_path):
     assert "command (resolution): Resolution must be positive" in err
     assert "command (sppx): Syntax error, requires exactly one positive integral value" in err
     assert "command (bogus): Unknown command" in err
+    assert any(e.startswith("command (tracer-option): No platform selected") for e in err)
     assert r.stdout.strip() == "hi"
-    assert len(err) == 8
+    assert len(err) == 9
 
 
 SCRIPT_HEAD = """platform hip
@@ -73,6 +74,23 @@ def test_primary_wf_image_matches_oracle(tmp_path, torch_cuda):
     want = np.where(t != np.float32(3.4028234663852886e38), np.float32(0.8), np.float32(0)).reshape(h, w)
     assert np.array_equal(img[..., 0], want) and np.array_equal(img[..., 2], want)
     assert "find closest hit" in r.stdout
+
+
+@pytest.mark.gpu
+def test_tracer_options_keep_the_image(tmp_path, torch_cuda):
+    print("case: rtgi-hip primary-wf with tracer options, 8k random triangles, 160x120", flush=True)
+    w, h = 160, 120
+    head = SCRIPT_HEAD.format(cfg=1, w=w, h=h, tracer="dynamic-while-while")
+    r0 = run_script(head + "algo primary-wf\noutfile a.pfm\nrun\n", tmp_path)
+    r1 = run_script(head + "tracer-option stack 3\ntracer-option chunks 8\ntracer-option refill 56\n"
+                    "tracer-option stack 9\ntracer-option speed 1\nalgo primary-wf\noutfile b.pfm\nrun\n", tmp_path)
+    assert r0.returncode == 0 and not r0.stderr.strip(), r0.stderr
+    err = r1.stderr.splitlines()
+    assert err == ["command (tracer-option): Parameter error, stack configuration must be 0..3",
+                   "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks)"], err
+    # primary-wf shades hits with the albedo, so the images agree unless two hits tie in t
+    a, b = read_pfm(tmp_path / "a.pfm"), read_pfm(tmp_path / "b.pfm")
+    assert np.array_equal(a, b)
 
 
 @pytest.mark.gpu
