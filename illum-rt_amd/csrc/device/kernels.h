@@ -23,6 +23,9 @@ constexpr int kLdsStack4 = 44;
 constexpr int kPersistentStackCap = kRegStack + kLdsStack;
 constexpr int kPersistentWavesPerBlock = 4;
 constexpr int kStackConfigs = 4;
+// Leaves a closest-hit lane of the speculative persistent tracers parks
+// before it stops descending (at most 4).
+constexpr int kParkClosest = 4;
 // Work counters of the persistent kernels: one per chunk, each in its own
 // 256-byte line; kCounterBytes of device memory hold all of them.
 constexpr uint32_t kMaxChunks = 8;

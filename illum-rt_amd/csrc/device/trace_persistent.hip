@@ -12,7 +12,11 @@
 //     column owned by the lane (level-major, lane-minor: conflict free);
 //   * leaf postponement (SPEC): a lane that reaches a leaf parks it and keeps
 //     descending until every lane still searching has a parked leaf, so the
-//     wave runs triangle tests together instead of one lane at a time;
+//     wave runs triangle tests together instead of one lane at a time.  A
+//     closest-hit lane parks up to kParkClosest leaves (any-hit: one), so
+//     lanes that find leaves early keep doing node work while the wave waits
+//     for its slowest lane's first leaf (node-loop lane occupancy on diffuse
+//     hall rays 33% with one parked leaf, 55% with four);
 //   * dynamic refill (REFILL): when fewer than `refill_thr` lanes still have a
 //     ray, the wave leaves the traversal loop and hands the idle lanes new
 //     rays, so long rays do not hold 63 idle lanes hostage;
@@ -152,7 +156,11 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
     lane_scene ls = scene_of(sc);
     uint32_t desc = 0;
     dray r{};
-    int32_t cur = kInvalidRef, leaf = kInvalidRef;
+    int32_t cur = kInvalidRef;
+    // parked leaves (SPEC): p0 is the oldest, pc how many are held
+    constexpr int PARK = ANY ? 1 : kParkClosest;
+    int32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    int pc = 0;
     uint32_t ray_id = 0;
     dhit h{};
     bool alive = false;
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                         r = load_ray(rays, idx);
                     }
                     cur = ls.root;
-                    leaf = kInvalidRef;
+                    pc = 0;
                     st.reset();
                     h.found = false;
                     h.t = ANY ? INFINITY : FLT_MAX;
@@ -270,23 +278,37 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                         }
                     }
                     if (SPEC) {
-                        if (cur < 0 && leaf >= 0) {
-                            leaf = cur;
+                        // park leaves (up to PARK), then keep descending
+                        while (cur < 0 && pc < PARK) {
+                            if (PARK > 3) p3 = pc == 3 ? cur : p3;
+                            if (PARK > 2) p2 = pc == 2 ? cur : p2;
+                            if (PARK > 1) p1 = pc == 1 ? cur : p1;
+                            p0 = pc == 0 ? cur : p0;
+                            ++pc;
                             cur = st.pop();
-                            searching = false;
                         }
+                        searching = pc == 0;
                         if (__ballot(searching) == 0) break;
                     }
                 }
-                // ---- leaves ----
-                // Without postponement (or when the root itself is a leaf)
-                // the leaf is still in cur.
-                if (leaf >= 0 && cur < 0) {
-                    leaf = cur;
-                    cur = st.pop();
-                }
-                while (leaf < 0) {
-                    uint32_t slot = leaf_slot(leaf), cnt = leaf_count(leaf);
+                // ---- leaves: the parked ones, oldest first, then a leaf left in
+                // cur and the leaves that follow it on the stack (the reference
+                // order when nothing is parked) ----
+                for (;;) {
+                    int32_t lf;
+                    if (SPEC && pc > 0) {
+                        lf = p0;
+                        if (PARK > 1) p0 = p1;
+                        if (PARK > 2) p1 = p2;
+                        if (PARK > 3) p2 = p3;
+                        --pc;
+                    } else if (cur < 0) {
+                        lf = cur;
+                        cur = st.pop();
+                    } else {
+                        break;
+                    }
+                    uint32_t slot = leaf_slot(lf), cnt = leaf_count(lf);
                     for (uint32_t k = 0; k < cnt; ++k) {
                         float t, u, v;
                         if (COUNT) ++c_tris;
@@ -302,11 +324,9 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                     }
                     if (ANY && h.found) {
                         cur = kInvalidRef;
-                        leaf = kInvalidRef;
+                        pc = 0;
                         break;
                     }
-                    leaf = cur;
-                    if (cur < 0) cur = st.pop();
                 }
                 if (cur == kInvalidRef) break;
                 if (REFILL && __popcll(__ballot(1)) < refill_thr) break;

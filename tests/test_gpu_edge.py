@@ -195,9 +195,10 @@ def test_fetch_counts_match_the_oracle(torch_cuda, c1s):
     """Inner-node fetches and triangle tests counted by the kernels against the
     oracle's counting traversal over the same tree and rays: equal for the
     kernels that keep the reference's order (plain, persistent); the
-    speculative ones postpone leaves, so the best t arrives later and culls
-    less -- measured +1.8% nodes and +4.0% triangle tests on these rays --
-    and are held to 2% and 5%."""
+    speculative ones postpone up to four leaves per lane, so the best t
+    arrives later and culls less: +7.8% inner nodes and +13.8% triangle tests
+    on these rays (+4% and +5% on the hall's diffuse rays), held to 10% and
+    20%."""
     print("case: C1 fetch counters vs oracle counts", flush=True)
     torch = torch_cuda
     cam = hiprt.camera_rays(c1s["sc"].cam)
@@ -213,7 +214,7 @@ def test_fetch_counts_match_the_oracle(torch_cuda, c1s):
         if tracer in ("while-while", "persistent-while-while"):
             assert np.array_equal(got, want)
         else:
-            assert 1.0 <= got[0] / want[0] <= 1.02 and 1.0 <= got[1] / want[1] <= 1.05, (tracer, got, want)
+            assert 1.0 <= got[0] / want[0] <= 1.10 and 1.0 <= got[1] / want[1] <= 1.20, (tracer, got, want)
 
 
 def test_c2_shadow_rays_any_hit(torch_cuda, hip_ctx):
