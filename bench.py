@@ -118,6 +118,30 @@ def bench_shadow(torch, hiprt, ctx, stream):
     return out
 
 
+def bench_device_build(torch, hiprt, ctx, stream, sc, d_diff, host_build_s):
+    """Linear BVH of the hall built on the device from device-resident
+    triangles, and the diffuse trace over it (the tree is not the SAH one)."""
+    pos, idx, mat = sc.arrays()
+    d_pos = torch.from_numpy(pos).cuda()
+    d_idx = torch.from_numpy(idx.astype("int32")).cuda()
+    d_mat = torch.from_numpy(mat.astype("int32")).cuda()
+    table = sc.materials()
+    runs = []
+    for _ in range(3):
+        t0 = time.time()
+        ds, ms = hiprt.build_device_scene(ctx, d_pos, d_idx, materials=d_mat, material_table=table)
+        runs.append((ms, time.time() - t0))
+        if len(runs) < 3:
+            ds.close()
+    ms_trace, _ = time_trace(torch, hiprt, ds, d_diff, "dynamic-while-while", stream)
+    out = {"triangles": int(len(idx)), "device_ms": round(min(r[0] for r in runs), 3),
+           "call_ms": round(1e3 * min(r[1] for r in runs), 1), "host_sah_esc_s": round(host_build_s, 3),
+           "inner_nodes": int(ds.n_nodes), "depth": int(ds.depth),
+           "diffuse_mrays": round(d_diff.shape[1] / ms_trace / 1e3, 1)}
+    ds.close()
+    return out
+
+
 def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
     """64 random-triangle scenes of log-uniform size with their own rays: one
     descriptor-table launch against one launch per scene, both with the
@@ -130,6 +154,14 @@ def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
     entries, scenes, layouts = [], [], []
     total = 0
     n_tris = sum(sc.n_tris for sc, _, _ in work)
+    # the same 64 trees built on the device instead (linear BVHs)
+    dev_ms = 0.0
+    for sc, _, _ in work:
+        pos, idx, _ = sc.arrays()
+        dsd, ms = hiprt.build_device_scene(ctx, torch.from_numpy(pos).cuda(),
+                                           torch.from_numpy(idx.astype("int32")).cuda())
+        dev_ms += ms
+        dsd.close()
     for sc, bvh, rays in work:
         d_rays = torch.from_numpy(rays).cuda()
         sample = d_rays[:, ::max(1, rays.shape[1] // 262144)].contiguous()
@@ -140,7 +172,7 @@ def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
         scenes.append(ds)
         total += rays.shape[1]
     del work
-    tune_s = time.time() - t0 - build_s
+    tune_s = time.time() - t0 - build_s   # includes the device builds above
 
     def separate():
         for ds, r, h, _ in entries:
@@ -155,7 +187,7 @@ def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
            "batched_mrays": round(total / ms_bat / 1e3, 1), "separate_mrays": round(total / ms_sep / 1e3, 1),
            "batched_ms": round(ms_bat, 3), "separate_ms": round(ms_sep, 3),
            "bvh4_scenes": sum(1 for l in layouts if l == "bvh4"),
-           "host_build_s": round(build_s, 1), "autotune_s": round(tune_s, 1)}
+           "host_build_s": round(build_s, 1), "device_build_ms": round(dev_ms, 1), "autotune_s": round(tune_s, 1)}
     for ds in scenes:
         ds.close()
     return out
@@ -194,7 +226,9 @@ def main():
     ctx = hiprt.Context(0)
     sc = hiprt.HostScene.generate(3)
     pos, idx, _ = sc.arrays()
+    t_build = time.time()
     bvh = hiprt.Bvh(sc)
+    host_build_s = time.time() - t_build
     ds = hiprt.DeviceScene(ctx, sc, bvh)
     cam = sc.cam
     w, h = int(cam[10]), int(cam[11])
@@ -320,6 +354,8 @@ def main():
 
     extra = {}
     if not args.no_extra:
+        extra["device_build"] = bench_device_build(torch, hiprt, ctx, stream, sc, d_diff, host_build_s)
+        log(f"device build: {extra['device_build']}")
         extra["shadow_any_hit_mrays"] = bench_shadow(torch, hiprt, ctx, stream)
         log(f"shadow rays: {extra['shadow_any_hit_mrays']}")
         extra["multi_scene"] = bench_multi_scene(torch, hiprt, ctx, stream)

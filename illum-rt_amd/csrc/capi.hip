@@ -141,6 +141,18 @@ int hiprt_scene_info(hiprt_host_scene s, uint32_t *n_vert, uint32_t *n_tris) {
     return ok();
 }
 
+int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_materials) {
+    if (!s) return fail(HIPRT_ERR_ARG, "null scene");
+    if (n_materials) *n_materials = (uint32_t)s->sc.mats.size();
+    if (table)
+        for (size_t m = 0; m < s->sc.mats.size(); ++m) {
+            const rt::material &mt = s->sc.mats[m];
+            float v[8] = {mt.albedo.x, mt.albedo.y, mt.albedo.z, 0.0f, mt.emissive.x, mt.emissive.y, mt.emissive.z, 0.0f};
+            std::memcpy(table + 8 * m, v, sizeof v);
+        }
+    return ok();
+}
+
 int hiprt_scene_copy(hiprt_host_scene s, float *pos, uint32_t *idx, uint32_t *mat) {
     if (!s) return fail(HIPRT_ERR_ARG, "null scene");
     if (pos)
@@ -230,7 +242,7 @@ int hiprt_destroy(hiprt_context ctx) {
 
 } // extern "C"
 
-namespace {
+namespace hiprt_detail {
 
 // Checks a flattened tree and computes its depth.  Children must have larger
 // indices than their parent, which rules out cycles; leaf slot ranges must lie
@@ -293,23 +305,8 @@ std::string check_layout(rt::flat_scene &f, uint32_t &stack_need) {
     return "";
 }
 
-int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
-    uint32_t stack_need = 0;
-    std::string e = check_layout(f, stack_need);
-    if (!e.empty()) return fail(HIPRT_ERR_ARG, "invalid BVH layout: " + e);
-    HIP_TRY(hipSetDevice(ctx->device));
-    std::unique_ptr<hiprt_scene_s> s(new (std::nothrow) hiprt_scene_s);
-    if (!s) return fail(HIPRT_ERR_ARG, "out of host memory");
-    s->ctx = ctx;
-    s->host = std::move(f);
+void set_desc(hiprt_scene_s *s, uint32_t stack_need) {
     const rt::flat_scene &h = s->host;
-    if (h.bvh_kind == rt::BVH2) {
-        HIP_TRY(s->nodes.upload(h.nodes2.data(), h.nodes2.size() * sizeof(rt::node2)));
-    } else {
-        HIP_TRY(s->nodes.upload(h.nodes4.data(), h.nodes4.size() * sizeof(rt::node4)));
-    }
-    HIP_TRY(s->tris.upload(h.tris.data(), h.tris.size() * sizeof(rt::tri_rec)));
-    HIP_TRY(s->slot_ids.upload(h.slot_ids.data(), h.slot_ids.size() * 4));
     s->desc.nodes = s->nodes.p;
     s->desc.tris = static_cast<const rt::tri_rec *>(s->tris.p);
     s->desc.slot_ids = static_cast<const uint32_t *>(s->slot_ids.p);
@@ -329,6 +326,32 @@ int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
             if (k % row < boxes && std::isnan(fv[k])) nan_free = false;
         s->desc.nan_free = nan_free ? 1 : 0;
     }
+}
+
+} // namespace hiprt_detail
+
+namespace {
+
+using hiprt_detail::check_layout;
+
+int upload_flat(hiprt_context ctx, rt::flat_scene &&f, hiprt_scene *out) {
+    uint32_t stack_need = 0;
+    std::string e = check_layout(f, stack_need);
+    if (!e.empty()) return fail(HIPRT_ERR_ARG, "invalid BVH layout: " + e);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<hiprt_scene_s> s(new (std::nothrow) hiprt_scene_s);
+    if (!s) return fail(HIPRT_ERR_ARG, "out of host memory");
+    s->ctx = ctx;
+    s->host = std::move(f);
+    const rt::flat_scene &h = s->host;
+    if (h.bvh_kind == rt::BVH2) {
+        HIP_TRY(s->nodes.upload(h.nodes2.data(), h.nodes2.size() * sizeof(rt::node2)));
+    } else {
+        HIP_TRY(s->nodes.upload(h.nodes4.data(), h.nodes4.size() * sizeof(rt::node4)));
+    }
+    HIP_TRY(s->tris.upload(h.tris.data(), h.tris.size() * sizeof(rt::tri_rec)));
+    HIP_TRY(s->slot_ids.upload(h.slot_ids.data(), h.slot_ids.size() * 4));
+    hiprt_detail::set_desc(s.get(), stack_need);
     *out = s.release();
     return ok();
 }
@@ -422,11 +445,14 @@ int hiprt_scene_download(hiprt_scene sc, void *nodes, void *tri_records, uint32_
     if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
     HIP_TRY(hipSetDevice(sc->ctx->device));
     HIP_TRY(hipDeviceSynchronize());
-    if (nodes && sc->nodes.bytes) HIP_TRY(hipMemcpy(nodes, sc->nodes.p, sc->nodes.bytes, hipMemcpyDeviceToHost));
-    if (tri_records && sc->tris.bytes)
-        HIP_TRY(hipMemcpy(tri_records, sc->tris.p, sc->tris.bytes, hipMemcpyDeviceToHost));
-    if (slot_ids && sc->slot_ids.bytes)
-        HIP_TRY(hipMemcpy(slot_ids, sc->slot_ids.p, sc->slot_ids.bytes, hipMemcpyDeviceToHost));
+    // sizes of the layout (hiprt_scene_layout), not of the allocations, which
+    // a device build makes for its worst case
+    const rt::flat_scene &h = sc->host;
+    size_t node_bytes = h.bvh_kind == rt::BVH2 ? h.nodes2.size() * sizeof(rt::node2) : h.nodes4.size() * sizeof(rt::node4);
+    size_t rec_bytes = h.tris.size() * sizeof(rt::tri_rec), id_bytes = h.slot_ids.size() * 4;
+    if (nodes && node_bytes) HIP_TRY(hipMemcpy(nodes, sc->nodes.p, node_bytes, hipMemcpyDeviceToHost));
+    if (tri_records && rec_bytes) HIP_TRY(hipMemcpy(tri_records, sc->tris.p, rec_bytes, hipMemcpyDeviceToHost));
+    if (slot_ids && id_bytes) HIP_TRY(hipMemcpy(slot_ids, sc->slot_ids.p, id_bytes, hipMemcpyDeviceToHost));
     return ok();
 }
 

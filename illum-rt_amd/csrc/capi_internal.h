@@ -82,3 +82,15 @@ struct hiprt_scene_s {
     std::vector<uint32_t> refit_off;             // level boundaries in refit_list
     rt::scene_desc desc{};
 };
+
+namespace hiprt_detail {
+
+// Checks a flattened tree (children after their parent, leaf slots in range,
+// depth and node limits) and computes the most stack entries a traversal can
+// hold.  Returns "" when the tree is valid.
+std::string check_layout(rt::flat_scene &f, uint32_t &stack_need);
+
+// Fills the scene descriptor from the device buffers and the host copy.
+void set_desc(hiprt_scene_s *s, uint32_t stack_need);
+
+} // namespace hiprt_detail

@@ -91,6 +91,7 @@ def lib():
             "hiprt_scene_write_obj": (c_int, [c_void_p, c_char_p]),
             "hiprt_scene_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
             "hiprt_scene_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+            "hiprt_scene_materials": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_free": (None, [c_void_p]),
             "hiprt_bvh_build": (c_int, [c_void_p, POINTER(_BvhParams), POINTER(c_void_p)]),
             "hiprt_bvh_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
@@ -107,6 +108,9 @@ def lib():
                                            POINTER(c_uint32), POINTER(c_int32), POINTER(c_int32)]),
             "hiprt_scene_download": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
             "hiprt_scene_refit": (c_int, [c_void_p, c_void_p, c_void_p]),
+            "hiprt_scene_build": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p, c_uint32, c_void_p, c_void_p,
+                                          c_uint32, c_int32, c_int32, POINTER(c_void_p), POINTER(c_float),
+                                          c_void_p]),
             "hiprt_scene_destroy": (c_int, [c_void_p]),
             "hiprt_tracer_id": (c_int, [c_char_p]),
             "hiprt_tracer_name": (c_char_p, [c_int]),
@@ -207,9 +211,18 @@ class HostScene:
         check(lib().hiprt_scene_info(self.h, ctypes.byref(nv), ctypes.byref(nt)))
         pos = np.empty((nv.value, 3), np.float32)
         idx = np.empty((nt.value, 3), np.uint32)
-        mat = np.empty(nt.value, np.uint32)
+        mat = np.zeros(nt.value, np.uint32)
         check(lib().hiprt_scene_copy(self.h, _ptr(pos), _ptr(idx), _ptr(mat)))
         return pos, idx, mat
+
+    def materials(self):
+        """Material table (k, 8) float32: albedo rgb, 0, emission rgb, 0."""
+        k = c_uint32()
+        check(lib().hiprt_scene_materials(self.h, None, ctypes.byref(k)))
+        table = np.zeros((k.value, 8), np.float32)
+        if k.value:
+            check(lib().hiprt_scene_materials(self.h, _ptr(table), ctypes.byref(k)))
+        return table
 
     @property
     def n_tris(self):
@@ -313,6 +326,28 @@ class DeviceScene:
     def __del__(self):
         if _lib is not None:
             self.close()
+
+
+def build_device_scene(ctx, positions, indices, materials=None, material_table=None, max_leaf=4,
+                       tri_format=TRI_EXACT, stream=None):
+    """Linear BVH built on the device from CUDA tensors: positions (n_vert, 3)
+    float32, indices (n_tris, 3) int32, optional per-triangle material ids
+    (n_tris,) int32 and a host material table (k, 8) float32.  Returns the
+    DeviceScene and the build kernels' time in ms."""
+    assert positions.is_cuda and indices.is_cuda and positions.is_contiguous() and indices.is_contiguous()
+    assert positions.element_size() == 4 and indices.element_size() == 4
+    nv = positions.shape[0] if positions.numel() else 0
+    nt = indices.shape[0] if indices.numel() else 0
+    table = None if material_table is None else np.ascontiguousarray(material_table, dtype=np.float32).reshape(-1, 8)
+    h = c_void_p()
+    ms = c_float(0.0)
+    check(lib().hiprt_scene_build(ctx.h, c_void_p(positions.data_ptr()) if nv else None, nv,
+                                  c_void_p(indices.data_ptr()) if nt else None, nt,
+                                  c_void_p(materials.data_ptr()) if materials is not None else None,
+                                  table.ctypes.data if table is not None else None,
+                                  0 if table is None else table.shape[0], max_leaf, tri_format, ctypes.byref(h),
+                                  ctypes.byref(ms), _stream(stream)))
+    return DeviceScene(ctx, None, None, handle=h), ms.value
 
 
 def _rays_struct(rays):

@@ -70,6 +70,9 @@ int hiprt_scene_write_obj(hiprt_host_scene s, const char *path);
 int hiprt_scene_info(hiprt_host_scene s, uint32_t *n_vert, uint32_t *n_tris);
 /* Any output pointer may be NULL. */
 int hiprt_scene_copy(hiprt_host_scene s, float *pos, uint32_t *idx, uint32_t *mat);
+/* Material table: 8 floats per material (albedo rgb, 0, emission rgb, 0), the
+ * layout hiprt_scene_build takes; *n_materials = count (table may be NULL). */
+int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_materials);
 void hiprt_scene_free(hiprt_host_scene s);
 
 /* ------------------------------------------------------------------ host BVH
@@ -119,6 +122,22 @@ typedef struct hiprt_scene_s *hiprt_scene;
  * batch_rt::build + compact_bvh_node_builder (rt/cuda/base.cpp:88-117,157-178). */
 int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32_t bvh_format,
                        int32_t tri_format, hiprt_scene *out);
+/* Builds a linear BVH on the device (Morton codes, radix sort, binary radix
+ * tree, bottom-up boxes; leaves of at most max_leaf triangles) from
+ * triangles already in device memory -- positions (3 floats per vertex) and
+ * vertex indices (3 per triangle) -- and commits it with exact or Woop
+ * records, shading data (optional per-triangle material ids on the device,
+ * host material table of 8 floats: albedo rgb, 0, emission rgb, 0) and vertex
+ * indices, so tracing, path tracing, the adjoint and refit all work on it.
+ * The device counterpart of the host build + commit that every commit runs
+ * (rt/cuda/base.cpp:88-117).  This tree is not the reference's SAH tree:
+ * hits agree with brute force up to equal-t ties and grazing hits.
+ * *device_ms (optional) receives the build kernels' time on `stream`; the
+ * call returns after the scene is ready (it copies the flattened tree back
+ * to the host once to check it). */
+int hiprt_scene_build(hiprt_context ctx, const float *d_positions, uint32_t n_vert, const uint32_t *d_indices,
+                      uint32_t n_tris, const uint32_t *d_materials, const float *materials, uint32_t n_materials,
+                      int32_t max_leaf, int32_t tri_format, hiprt_scene *out, float *device_ms, void *stream);
 /* Uploads an already flattened scene (host arrays): n_nodes nodes of 64 B
  * (bvh_format 0) or 128 B (1); n_slots triangle records of 48 B; slot_ids[n_slots]
  * triangle id per slot; root_ref = index of the root inner node or the leaf
