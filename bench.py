@@ -107,9 +107,12 @@ def bench_shadow(torch, hiprt, ctx, stream):
     hiprt.shadow_rays(ds, prim, hits, shadow, cam[12:15], stream.cuda_stream)
     occ = torch.empty(n, dtype=torch.uint8, device="cuda")
     out = {"triangles": sc.n_tris, "rays": n}
-    for tr in ("while-while", "dynamic-while-while"):
-        ms = time_calls(torch, lambda: hiprt.trace_any(ds, shadow, occ, tracer=tr, stream=stream.cuda_stream), stream)
-        out[f"any/{tr}"] = round(n / ms / 1e3, 1)
+    for tr, kw in (("while-while", {}), ("dynamic-while-while", {}), ("dynamic-while-while", {"work_chunks": 1}),
+                   ("persistent-speculative-while-while", {})):
+        ms = time_calls(torch, lambda: hiprt.trace_any(ds, shadow, occ, tracer=tr, stream=stream.cuda_stream, **kw),
+                        stream)
+        tag = "".join(f",{k}={v}" for k, v in kw.items())
+        out[f"any/{tr}{tag}"] = round(n / ms / 1e3, 1)
     ms = time_calls(torch, lambda: hiprt.trace_closest(ds, prim, hits, tracer="dynamic-while-while",
                                                       stream=stream.cuda_stream), stream)
     out["primary/dynamic-while-while"] = round(n / ms / 1e3, 1)
@@ -271,19 +274,24 @@ def main():
     # The trace autotuner times one launch over rays already in pixel order
     # with the tree warm in L2; inside a frame every level follows a shading
     # pass and reads a compacted queue.  So the frame's configuration is
-    # chosen on warm-up frames (never the timed ones): the autotuned one (or
-    # its one-counter variant) when it beats the library defaults (binary
-    # tree, dynamic tracer, stack 0, one work counter) by more than 2%, the
-    # defaults otherwise, which keeps the choice stable between runs.
+    # chosen on warm-up frames (never the timed ones): the best of the
+    # autotuned one, its one-counter variant and a 4-wide tree, when it beats
+    # the library defaults (binary tree, dynamic tracer, stack 0, eight work
+    # chunks) by more than 2%, the defaults otherwise, which keeps the choice
+    # stable between runs.  The autotuner's sample is the first bounce, where
+    # the two layouts trace about equally fast; the deeper, less coherent
+    # bounces favour the 4-wide tree, so it is a candidate of its own.
+    ds4 = hiprt.DeviceScene(ctx, sc, bvh, hiprt.BVH4)
     candidates = {"autotuned": (ds_tuned, pt_kw),
                   "autotuned, one work counter": (ds_tuned, dict(pt_kw, work_chunks=1)),
+                  "4-wide tree, stack 3": (ds4, dict(tracer="dynamic-while-while", stack_config=3)),
                   "library defaults": (ds, {})}
     frame_ms = {}
     for name, (dsc, kw) in candidates.items():
         pt = hiprt.PathTracer(dsc, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, **kw)
         frame_ms[name], _, _ = time_frames(torch, pt, 4, 1, stream)
         pt.close()
-    best_tuned = min(("autotuned", "autotuned, one work counter"), key=frame_ms.get)
+    best_tuned = min(("autotuned", "autotuned, one work counter", "4-wide tree, stack 3"), key=frame_ms.get)
     choice = best_tuned if frame_ms[best_tuned] < 0.98 * frame_ms["library defaults"] else "library defaults"
     log(f"frame configuration trial (ms/frame): {frame_ms}; timed frames use {choice}")
     ds_frame, frame_kw = candidates[choice]
@@ -314,9 +322,8 @@ def main():
     kernels = {}
     variants = (("while-while", {}), ("dynamic-while-while", {}),
                 ("dynamic-while-while", {"stack_config": 1}), ("dynamic-while-while", {"stack_config": 2}),
-                ("dynamic-while-while", {"stack_config": 3}), ("dynamic-while-while", {"work_chunks": 8}),
-                ("dynamic-while-while", {"stack_config": 1, "work_chunks": 8}),
-                ("dynamic-while-while", {"stack_config": 2, "work_chunks": 8}))
+                ("dynamic-while-while", {"stack_config": 3}), ("dynamic-while-while", {"work_chunks": 1}),
+                ("dynamic-while-while", {"stack_config": 3, "work_chunks": 1}))
     for name, rays in (("primary", d_cam), ("diffuse", d_diff)):
         for tr, kw in variants:
             ms, _ = time_trace(torch, hiprt, ds, rays, tr, stream, **kw)
@@ -434,6 +441,7 @@ def main():
     print(json.dumps(out), flush=True)
     batched.close()
     plain.close()
+    ds4.close()
     ds_tuned.close()
     ds.close()
     ctx.close()

@@ -29,7 +29,7 @@ rt::dev::persistent_cfg cfg_for(int tracer, const hiprt_tracer_opts *opts) {
     c.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
     c.stats = nullptr;
     c.stack = (opts && opts->stack_config > 0 && opts->stack_config < rt::dev::kStackConfigs) ? opts->stack_config : 0;
-    c.chunks = (opts && opts->work_chunks == 8) ? 8 : 1;
+    c.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
     return c;
 }
 

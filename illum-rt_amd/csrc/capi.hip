@@ -583,7 +583,7 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.blocks = (opts && opts->blocks > 0) ? opts->blocks : 0;
     cfg.stats = opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats) : nullptr;
     cfg.stack = stack;
-    cfg.chunks = (opts && opts->work_chunks == 8) ? 8 : 1;
+    cfg.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
     HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
                                              static_cast<uint32_t *>(sc->ctx->counter.p), any, cfg, s));
     return ok();

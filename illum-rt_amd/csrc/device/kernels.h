@@ -59,7 +59,7 @@ struct persistent_cfg {
     int blocks;           // grid size; 0 = occupancy-derived default
     unsigned long long *stats = nullptr;   // non-null: counting variant (inner nodes, triangle tests)
     int stack = 0;        // stack configuration 0..kStackConfigs-1, see persistent_stack_cap
-    int chunks = 1;       // 1: one work counter; 8: one contiguous chunk of rays per XCD
+    int chunks = 8;       // 1: one work counter; 8: one contiguous chunk of rays per XCD
 };
 
 // `counter` points to kCounterBytes of device memory (cleared by the launch);

@@ -296,7 +296,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pcfg.refill_thr = params->refill_threshold > 0 ? std::min(params->refill_threshold, 64) : 40;
     p->pcfg.blocks = 0;
     p->pcfg.stack = stack;
-    p->pcfg.chunks = params->work_chunks == 8 ? 8 : 1;
+    p->pcfg.chunks = params->work_chunks == 1 ? 1 : 8;
     if (p->batched) {
         // Resolve the persistent grid size outside the capture, then record
         // the frame once.  The graph never clears the accumulator: frame 0

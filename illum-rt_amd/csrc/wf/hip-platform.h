@@ -93,7 +93,7 @@ public:
     hiprt_bvh_params bvh_params{2, 4, 16, 1};
     int tri_format = HIPRT_TRI_EXACT;
     // options of the persistent tracers, set by "tracer-option <name> <value>"
-    hiprt_tracer_opts opts{40, 0, nullptr, 0, 1};
+    hiprt_tracer_opts opts{40, 0, nullptr, 0, 8};
     hiprt_bvh bvh = nullptr;
     hiprt_scene ds = nullptr;          // device scene of the last commit
     wf::scene *sd = nullptr;

@@ -368,13 +368,13 @@ def tracer_id(name):
     return i
 
 
-def _opts(refill_threshold=0, blocks=0, fetch_stats=None, stack_config=0, work_chunks=1):
+def _opts(refill_threshold=0, blocks=0, fetch_stats=None, stack_config=0, work_chunks=0):
     return _TracerOpts(refill_threshold, blocks, fetch_stats.data_ptr() if fetch_stats is not None else None,
                        stack_config, work_chunks)
 
 
 def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
-                  fetch_stats=None, stack_config=0, work_chunks=1):
+                  fetch_stats=None, stack_config=0, work_chunks=0):
     """Closest hits of `rays` (8, n) into `hits`.  fetch_stats: optional int64
     CUDA tensor of 2 that receives inner-node fetches and triangle tests."""
     n = rays.shape[1]
@@ -386,7 +386,7 @@ def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_
 
 
 def trace_any(dscene, rays, occluded, hits=None, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
-              stack_config=0, work_chunks=1):
+              stack_config=0, work_chunks=0):
     n = rays.shape[1]
     r, o = _rays_struct(rays), _opts(refill_threshold, blocks, None, stack_config, work_chunks)
     hp = ctypes.byref(_hits_struct(hits)) if hits is not None else None
@@ -436,7 +436,7 @@ class PathTracer:
     """Wavefront diffuse path tracer over a device scene (plain or batched)."""
 
     def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None,
-                 stack_config=0, work_chunks=1, refill_threshold=0, blocks=0):
+                 stack_config=0, work_chunks=0, refill_threshold=0, blocks=0):
         self.dscene = dscene
         c = np.ascontiguousarray(cam[:12], dtype=np.float32)
         self.w, self.h = int(c[10]), int(c[11])
@@ -511,7 +511,7 @@ def adjoint_closest(dscene, rays, hits, grad_t, grad_u, grad_v, grad_vertices=No
 
 
 def trace_batch(entries, any=False, tracer="dynamic-while-while", stream=None, refill_threshold=0, blocks=0,
-                stack_config=0, work_chunks=1):
+                stack_config=0, work_chunks=0):
     """entries: list of (dscene, rays (8,n), hits dict or None, occluded tensor or None)."""
     arr = (_BatchEntry * len(entries))()
     for k, (ds, rays, hits, occ) in enumerate(entries):

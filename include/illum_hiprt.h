@@ -196,8 +196,10 @@ typedef struct {
     int32_t stack_config;     /* traversal stack per lane (VGPR + LDS entries, binary /
                                  4-wide): 0 = 4 + 28/44, 1 = 8 + 16/32, 2 = 2 + 20/36,
                                  3 = 0 + 24/40 */
-    int32_t work_chunks;      /* 1 = one work counter; 8 = one contiguous chunk of the
-                                 rays per XCD (blocks start on their own XCD's chunk) */
+    int32_t work_chunks;      /* 1 = one work counter; 8 (and 0, the default) = one
+                                 contiguous chunk of the rays per XCD, blocks start on
+                                 their own XCD's chunk (fetch atomics spread over 8
+                                 counters: +57% on short any-hit rays) */
 } hiprt_tracer_opts;
 
 /* compute_closest_hit of batch_ray_tracer (libgi/wavefront-rt.h:145-149,

@@ -85,14 +85,14 @@ def test_other_stack_configs_match_up_to_ties(torch_cuda, c1, tracer, stack):
     assert same_t.all()
 
 
-@pytest.mark.parametrize("blocks", [0, 3])
+@pytest.mark.parametrize("chunks,blocks", [(8, 0), (8, 3), (1, 0), (1, 3)])
 @pytest.mark.parametrize("tracer", TRACERS[1:])
-def test_xcd_work_chunks_match_up_to_ties(torch_cuda, c1, tracer, blocks):
-    """Eight work chunks, one per XCD, also with fewer blocks than chunks
-    (every block then serves several chunks): every ray traced once, same
-    hits up to ties."""
-    print(f"case: C1 camera+random rays, {tracer}, 8 work chunks, blocks {blocks}", flush=True)
-    got = _run_closest(torch_cuda, c1["exact"], c1["rays"], tracer, work_chunks=8, blocks=blocks)
+def test_work_chunks_match_up_to_ties(torch_cuda, c1, tracer, chunks, blocks):
+    """One work counter, or eight chunks (one per XCD, the default) also with
+    fewer blocks than chunks (every block then serves several chunks): every
+    ray traced once, same hits up to ties."""
+    print(f"case: C1 camera+random rays, {tracer}, {chunks} work chunks, blocks {blocks}", flush=True)
+    got = _run_closest(torch_cuda, c1["exact"], c1["rays"], tracer, work_chunks=chunks, blocks=blocks)
     assert (got[0].view(np.uint32) != np.float32(-7).view(np.uint32)).all()   # every hit slot written
     rep = classify(orc, c1["pos"], c1["idx"], c1["rays"], got, c1["ref"])
     assert rep["unexplained"] == 0, rep
