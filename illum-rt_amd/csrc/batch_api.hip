@@ -80,7 +80,9 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
     HIP_TRY(hipStreamSynchronize(s));
     size_t bytes = 0;
     for (auto &g : groups) bytes += g.second.size() * sizeof(rt::dev::batch_desc);
-    dbuf &table = ctx->batch_table;
+    hiprt_stream_state *st = ctx->state(stream);
+    if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
+    dbuf &table = st->batch_table;
     if (table.bytes < bytes) HIP_TRY(table.alloc(bytes));
     size_t off = 0;
     for (auto &g : groups) {
@@ -90,7 +92,7 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
         auto *dst = reinterpret_cast<rt::dev::batch_desc *>(static_cast<char *>(table.p) + off);
         HIP_TRY(hipMemcpy(dst, g.second.data(), g.second.size() * sizeof(rt::dev::batch_desc), hipMemcpyHostToDevice));
         HIP_TRY(rt::dev::launch_trace_batch(dst, (uint32_t)g.second.size(), (uint32_t)totals[g.first], tri, bvh,
-                                            any != 0, ctx->counter.as<uint32_t>(), cfg_for(tracer, opts), s));
+                                            any != 0, st->counter.as<uint32_t>(), cfg_for(tracer, opts), s));
         off += g.second.size() * sizeof(rt::dev::batch_desc);
     }
     return ok();

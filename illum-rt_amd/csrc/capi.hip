@@ -34,6 +34,20 @@ using hiprt_detail::dbuf;
 using hiprt_detail::fail;
 using hiprt_detail::ok;
 
+hiprt_stream_state *hiprt_context_s::state(void *stream) {
+    std::lock_guard<std::mutex> g(lock);
+    auto it = streams.find(stream);
+    if (it != streams.end()) return it->second.get();
+    std::unique_ptr<hiprt_stream_state> st(new (std::nothrow) hiprt_stream_state);
+    if (!st) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    if (st->counter.alloc(rt::dev::kCounterBytes) != hipSuccess) return nullptr;
+    if (hipMemset(st->counter.p, 0, rt::dev::kCounterBytes) != hipSuccess) return nullptr;
+    hiprt_stream_state *p = st.get();
+    streams.emplace(stream, std::move(st));
+    return p;
+}
+
 extern "C" {
 
 const char *hiprt_last_error(void) { return g_err.c_str(); }
@@ -225,8 +239,7 @@ int hiprt_create(int32_t device, hiprt_context *out) {
     std::unique_ptr<hiprt_context_s> c(new (std::nothrow) hiprt_context_s);
     if (!c) return fail(HIPRT_ERR_ARG, "out of host memory");
     c->device = device;
-    HIP_TRY(c->counter.alloc(rt::dev::kCounterBytes));
-    HIP_TRY(hipMemset(c->counter.p, 0, rt::dev::kCounterBytes));
+    if (!c->state(nullptr)) return fail(HIPRT_ERR_HIP, "cannot allocate the work counters");
     *out = c.release();
     return ok();
 }
@@ -584,8 +597,10 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.stats = opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats) : nullptr;
     cfg.stack = stack;
     cfg.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
+    hiprt_stream_state *st = sc->ctx->state(stream);
+    if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
     HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
-                                             static_cast<uint32_t *>(sc->ctx->counter.p), any, cfg, s));
+                                             st->counter.as<uint32_t>(), any, cfg, s));
     return ok();
 }
 

@@ -11,6 +11,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -64,10 +67,21 @@ struct hiprt_bvh_s {
     rt::bvh_stats st;
 };
 
+// Per-stream state of a context: launches on one stream are ordered, so
+// they can share a work-counter block and a descriptor table; launches on
+// different streams get their own.
+struct hiprt_stream_state {
+    hiprt_detail::dbuf counter;       // work counters of the persistent kernels
+    hiprt_detail::dbuf batch_table;   // descriptor table of the last batched trace on the stream
+};
+
 struct hiprt_context_s {
     int device = 0;
-    hiprt_detail::dbuf counter;       // work counter of the persistent kernels
-    hiprt_detail::dbuf batch_table;   // descriptor table of the last batched trace
+    std::mutex lock;                  // guards `streams`
+    std::map<void *, std::unique_ptr<hiprt_stream_state>> streams;
+    // The state of `stream` (created on first use, counters zeroed); null on
+    // allocation failure.
+    hiprt_stream_state *state(void *stream);
 };
 
 struct hiprt_scene_s {

@@ -251,3 +251,36 @@ def test_c2_shadow_rays_any_hit(torch_cuda, hip_ctx):
         got = _any(torch, ds, srays, tracer)
         assert np.array_equal(got, want), tracer
     ds.close()
+
+
+def test_two_streams_share_a_context(torch_cuda, hip_ctx, c1s):
+    """Persistent launches on two streams of one context at the same time:
+    each stream has its own work counters and descriptor table."""
+    print("case: C1 + C2, persistent traces and a batched trace on two concurrent streams", flush=True)
+    torch = torch_cuda
+    sc2 = hiprt.HostScene.generate(2)
+    ds2 = hiprt.DeviceScene(hip_ctx, sc2, hiprt.Bvh(sc2))
+    r1 = torch.from_numpy(uniform_sphere_rays(300_000, 41)).cuda()
+    r2 = torch.from_numpy(uniform_sphere_rays(300_000, 42, hi=15.0)).cuda()
+    want = []
+    for ds, r in ((c1s["ds"], r1), (ds2, r2)):
+        h = hiprt.new_hits(r.shape[1])
+        hiprt.trace_closest(ds, r, h, tracer="while-while")
+        want.append(h)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for tracer in TRACERS[1:]:
+        got = [hiprt.new_hits(r1.shape[1]), hiprt.new_hits(r2.shape[1])]
+        batch = hiprt.new_hits(r1.shape[1])
+        torch.cuda.synchronize()
+        for _ in range(3):
+            hiprt.trace_closest(c1s["ds"], r1, got[0], tracer=tracer, stream=s1.cuda_stream)
+            hiprt.trace_closest(ds2, r2, got[1], tracer=tracer, stream=s2.cuda_stream)
+            hiprt.trace_batch([(c1s["ds"], r1, batch, None)], tracer=tracer, stream=s1.cuda_stream)
+        torch.cuda.synchronize()
+        for g, w in list(zip(got, want)) + [(batch, want[0])]:
+            # same t everywhere (ties aside the ids agree too)
+            assert torch.equal(g["t"], w["t"]), tracer
+            diff = g["id"] != w["id"]
+            assert int(diff.sum()) <= int((diff & (g["t"] == w["t"])).sum())
+    ds2.close()
