@@ -248,6 +248,49 @@ bool write_obj(const scene &s, const std::string &path) {
     return std::fclose(f) == 0;
 }
 
+namespace {
+
+float luma(vec3f c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }
+
+// Material of an MTL "newmtl" block as the reference's scene::add derives it
+// (libgi/scene.cpp:317-332): albedo = Kd if its luma exceeds 1e-4, else Ks,
+// raised to the power 2.2; emission = Ke.
+struct mtl_entry {
+    vec3f kd{0, 0, 0}, ks{0, 0, 0}, ke{0, 0, 0};
+    material resolve() const {
+        vec3f a = luma(kd) > 1e-4f ? kd : ks;
+        return material{vec3f{std::pow(a.x, 2.2f), std::pow(a.y, 2.2f), std::pow(a.z, 2.2f)}, ke};
+    }
+};
+
+bool read_mtl(const std::string &path, std::vector<std::string> &names, std::vector<mtl_entry> &mats,
+              std::string &err) {
+    std::ifstream in(path);
+    if (!in) {
+        err = "cannot open material library " + path;
+        return false;
+    }
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string tag;
+        if (!(ls >> tag)) continue;
+        if (tag == "newmtl") {
+            std::string name;
+            ls >> name;
+            names.push_back(name);
+            mats.emplace_back();
+        } else if (!mats.empty() && (tag == "Kd" || tag == "Ks" || tag == "Ke")) {
+            vec3f c{0, 0, 0};
+            ls >> c.x >> c.y >> c.z;
+            (tag == "Kd" ? mats.back().kd : tag == "Ks" ? mats.back().ks : mats.back().ke) = c;
+        }
+    }
+    return true;
+}
+
+} // namespace
+
 bool read_obj(const std::string &path, scene &out, std::string &err) {
     std::ifstream in(path);
     if (!in) {
@@ -255,45 +298,98 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
         return false;
     }
     out = scene{};
-    uint32_t cur_mat = 0;
+    std::vector<vec3f> normals;
+    std::vector<std::string> mtl_names;
+    std::vector<mtl_entry> mtl;
+    bool legacy = false;        // "usemtl m<id>" without a material library (write_obj's files)
+    uint32_t cur_mat = 0;       // index into out.mats once resolved; see below
+    int32_t cur_named = -1;     // current MTL material, -1 = none
+    std::vector<int32_t> tri_named;
+    std::string dir;
+    if (size_t slash = path.find_last_of('/'); slash != std::string::npos) dir = path.substr(0, slash + 1);
     std::string line;
     size_t lineno = 0;
+    auto where = [&]() { return path + ":" + std::to_string(lineno) + ": "; };
     while (std::getline(in, line)) {
         ++lineno;
         std::istringstream ls(line);
         std::string tag;
         if (!(ls >> tag)) continue;
-        if (tag == "v") {
+        if (tag == "v" || tag == "vn") {
             float x, y, z;
             if (!(ls >> x >> y >> z)) {
-                err = path + ":" + std::to_string(lineno) + ": malformed vertex";
+                err = where() + "malformed " + tag;
                 return false;
             }
-            out.pos.push_back(vec3f{x, y, z});
+            (tag == "v" ? out.pos : normals).push_back(vec3f{x, y, z});
         } else if (tag == "f") {
-            std::vector<long> ids;
+            std::vector<long> ids, nids;
             std::string tok;
             while (ls >> tok) {
-                long v = std::strtol(tok.c_str(), nullptr, 10);   // "a", "a/b", "a//c", "a/b/c"
+                // "a", "a/b", "a//c", "a/b/c" (1-based, negative = relative)
+                long v = std::strtol(tok.c_str(), nullptr, 10), n = 0;
+                size_t s1 = tok.find('/');
+                if (s1 != std::string::npos) {
+                    size_t s2 = tok.find('/', s1 + 1);
+                    if (s2 != std::string::npos && s2 + 1 < tok.size()) n = std::strtol(tok.c_str() + s2 + 1, nullptr, 10);
+                }
                 if (v < 0) v = (long)out.pos.size() + v + 1;
+                if (n < 0) n = (long)normals.size() + n + 1;
                 if (v <= 0 || (size_t)v > out.pos.size()) {
-                    err = path + ":" + std::to_string(lineno) + ": face index out of range";
+                    err = where() + "face index out of range";
+                    return false;
+                }
+                if (n < 0 || (size_t)n > normals.size()) {
+                    err = where() + "normal index out of range";
                     return false;
                 }
                 ids.push_back(v - 1);
+                nids.push_back(n - 1);   // -1: no normal
             }
+            // fan triangulation; winding follows the shading normals when
+            // every corner has one (libgi/scene.cpp:264-277)
             for (size_t k = 2; k < ids.size(); ++k) {
-                out.idx.push_back((uint32_t)ids[0]);
-                out.idx.push_back((uint32_t)ids[k - 1]);
-                out.idx.push_back((uint32_t)ids[k]);
+                uint32_t a = (uint32_t)ids[0], b = (uint32_t)ids[k - 1], c = (uint32_t)ids[k];
+                if (nids[0] >= 0 && nids[k - 1] >= 0 && nids[k] >= 0) {
+                    vec3f pa = out.pos[a], pb = out.pos[b], pc = out.pos[c];
+                    vec3f g = cross(pb - pa, pc - pa);
+                    vec3f m = (normals[nids[0]] + normals[nids[k - 1]] + normals[nids[k]]) * 0.333f;
+                    if (!(dot(g, m) > 0)) std::swap(b, c);
+                }
+                out.idx.push_back(a);
+                out.idx.push_back(b);
+                out.idx.push_back(c);
                 out.mat.push_back(cur_mat);
+                tri_named.push_back(cur_named);
             }
+        } else if (tag == "mtllib") {
+            std::string name;
+            while (ls >> name)
+                if (!read_mtl(dir + name, mtl_names, mtl, err)) return false;
         } else if (tag == "usemtl") {
             std::string name;
             ls >> name;
-            cur_mat = (name.size() > 1 && name[0] == 'm') ? (uint32_t)std::strtoul(name.c_str() + 1, nullptr, 10) : 0;
+            cur_named = -1;
+            for (size_t k = 0; k < mtl_names.size(); ++k)
+                if (mtl_names[k] == name) cur_named = (int32_t)k;
+            if (cur_named < 0 && name.size() > 1 && name[0] == 'm' &&
+                name.find_first_not_of("0123456789", 1) == std::string::npos) {
+                legacy = true;
+                cur_mat = (uint32_t)std::strtoul(name.c_str() + 1, nullptr, 10);
+            } else {
+                cur_mat = 0;
+            }
         }
     }
+    if (!mtl.empty()) {
+        // material 0: faces without a known material (default albedo 0.8);
+        // 1 + k: the k-th material of the libraries, in definition order
+        out.mats.assign(1, material{v3(0.8f), v3(0.0f)});
+        for (const mtl_entry &e : mtl) out.mats.push_back(e.resolve());
+        for (size_t t = 0; t < out.mat.size(); ++t) out.mat[t] = tri_named[t] >= 0 ? (uint32_t)tri_named[t] + 1 : 0u;
+        return true;
+    }
+    (void)legacy;
     uint32_t max_mat = 0;
     for (uint32_t m : out.mat) max_mat = m > max_mat ? m : max_mat;
     out.mats.assign(max_mat + 1, material{v3(0.8f), v3(0.0f)});

@@ -64,10 +64,16 @@ workload make_workload(int config);
 // Number of debris triangles that brings the dense hall to exactly 2^18.
 uint32_t hall_debris_for_config3();
 
-// OBJ: "v x y z" with %.9g (round-trips every float) and "f a b c" (1-based),
-// one object, no normals, materials as "usemtl m<id>" groups written in
-// triangle order.  The reader accepts v/f (triangulating polygons as fans)
-// and usemtl, ignores everything else, and keeps face order.
+// OBJ: write_obj writes "v x y z" with %.9g (round-trips every float) and
+// "f a b c" (1-based), one object, no normals, materials as "usemtl m<id>"
+// groups in triangle order.  read_obj keeps face order (the triangle order of
+// RTGI's assimp-based loader for files whose meshes follow the file), reads
+// v, vn and f ("a", "a/b", "a//c", "a/b/c", negative = relative), fan-
+// triangulates polygons, swaps b and c of a triangle whose geometric normal
+// disagrees with the mean of its corners' normals (libgi/scene.cpp:271-277),
+// and takes materials from "mtllib" libraries (newmtl/Kd/Ks/Ke, resolved as
+// libgi/scene.cpp:317-332 does: material 0 is the default, 1 + k the k-th
+// library material) or, without a library, from "usemtl m<id>" names.
 bool write_obj(const scene &s, const std::string &path);
 bool read_obj(const std::string &path, scene &out, std::string &err);
 

@@ -1,6 +1,6 @@
 """GPU parity on the cases the reference's tracer handles by construction and
 a careless kernel would not: rays with zero, infinite or NaN components,
-empty and inverted t intervals, a root that is a leaf (SURVEY Q9), exact ties
+empty and inverted t intervals, a root that is a leaf, exact ties
 from duplicated triangles, degenerate triangles, empty batches; plus the
 kernels' fetch counters against the oracle's counting traversal and the C2
 any-hit shadow-ray workload."""

@@ -78,6 +78,48 @@ def test_obj_reader_fans_polygons(tmp_path):
     assert idx.tolist() == [[0, 1, 2], [0, 2, 3]]
 
 
+def test_obj_materials_from_a_library(tmp_path):
+    """mtllib materials resolved as RTGI's loader does: albedo = Kd^2.2, or
+    Ks^2.2 when Kd's luma is ~0; emission = Ke; material 0 for faces with no
+    (or an unknown) material."""
+    (tmp_path / "lib.mtl").write_text("newmtl red\nKd 0.8 0.1 0.1\n\nnewmtl lamp\nKd 0 0 0\nKs 0.5 0.25 0.5\n"
+                                      "Ke 4 3 2\n")
+    (tmp_path / "m.obj").write_text(
+        "mtllib lib.mtl\n" + "".join(f"v {i} {i % 2} {i % 3}\n" for i in range(7)) +
+        "usemtl red\nf 1 2 3\nusemtl lamp\nf 4 5 6\nusemtl nothing\nf 1 3 5\nf 2 4 6 7\n")
+    sc = hiprt.HostScene.load_obj(tmp_path / "m.obj")
+    _, idx, mat = sc.arrays()
+    table = sc.materials()
+    assert mat.tolist() == [1, 2, 0, 0, 0]
+    assert table.shape == (3, 8)
+    f32 = np.float32
+    want_red = np.array([0.8, 0.1, 0.1], f32) ** f32(2.2)
+    want_lamp = np.array([0.5, 0.25, 0.5], f32) ** f32(2.2)
+    assert np.allclose(table[1, :3], want_red, rtol=1e-6) and (table[1, 4:7] == 0).all()
+    assert np.allclose(table[2, :3], want_lamp, rtol=1e-6) and table[2, 4:7].tolist() == [4, 3, 2]
+    assert np.allclose(table[0, :3], 0.8) and (table[0, 4:7] == 0).all()
+
+
+def test_obj_winding_follows_shading_normals(tmp_path):
+    """A triangle whose geometric normal points away from the mean of its
+    corner normals has b and c swapped (libgi/scene.cpp:271-277); one whose
+    normals agree, or that has no normals, keeps its order."""
+    (tmp_path / "n.obj").write_text(
+        "v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nvn 0 0 -1\n"
+        "f 1//1 2//1 3//1\n"            # geometric normal +z, shading +z: kept
+        "f 1//2 2//2 3//2\n"            # shading -z: swapped
+        "f -3//-1 -2//-1 -1//-1\n"      # relative indices, shading -z: swapped
+        "f 1 2 3\n")                    # no normals: kept
+    _, idx, _ = hiprt.HostScene.load_obj(tmp_path / "n.obj").arrays()
+    assert idx.tolist() == [[0, 1, 2], [0, 2, 1], [0, 2, 1], [0, 1, 2]]
+
+
+def test_obj_rejects_bad_normal_index(tmp_path):
+    (tmp_path / "b.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//2\n")
+    with pytest.raises(hiprt.HiprtError, match="normal index"):
+        hiprt.HostScene.load_obj(tmp_path / "b.obj")
+
+
 # ---------------------------------------------------------------- BVH builders
 
 @pytest.mark.parametrize("split", [hiprt.SPLIT_OM, hiprt.SPLIT_SM, hiprt.SPLIT_SAH])
