@@ -104,15 +104,22 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
     if (!full_rays(*sample)) return fail(HIPRT_ERR_ARG, "null ray plane");
     if (n == 0) return fail(HIPRT_ERR_ARG, "empty sample");
     HIP_TRY(hipSetDevice(ctx->device));
-    hiprt_scene layouts[2] = {nullptr, nullptr};
-    auto cleanup = [&]() {
-        for (auto &l : layouts)
-            if (l) hiprt_scene_destroy(l);
-    };
+    // the two committed layouts and the timing events are released on every
+    // return path; the winner's layout is handed to the caller
+    struct owned {
+        hiprt_scene layouts[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~owned() {
+            for (hiprt_scene l : layouts)
+                if (l) hiprt_scene_destroy(l);
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } own;
+    hiprt_scene *layouts = own.layouts;
     for (int f = 0; f < 2; ++f) {
         int st = hiprt_scene_commit(ctx, s, b, f, HIPRT_TRI_EXACT, &layouts[f]);
         if (st != HIPRT_OK) {
-            cleanup();
             return st;
         }
     }
@@ -124,7 +131,6 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
     const uint32_t reps = n >= kMinTuneRays ? 1u : (kMinTuneRays + n - 1) / n;
     const uint64_t nt64 = (uint64_t)n * reps;
     if (nt64 > 0xF0000000ull) {
-        cleanup();
         return fail(HIPRT_ERR_ARG, "sample too large");
     }
     const uint32_t nt = (uint32_t)nt64;
@@ -132,7 +138,6 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
     hiprt_rays tr = *sample;
     if (reps > 1) {
         if (tiled.alloc(32ull * nt) != hipSuccess) {
-            cleanup();
             return fail(HIPRT_ERR_HIP, "cannot allocate the tiled sample");
         }
         const float *src[8] = {sample->d_ox, sample->d_oy, sample->d_oz, sample->d_tmin,
@@ -149,7 +154,6 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
     // reference hits of the sample: plain kernel, binary layout
     dbuf hb, ref_hb, occ;
     if (hb.alloc(16ull * nt) != hipSuccess || ref_hb.alloc(16ull * n) != hipSuccess || occ.alloc(nt) != hipSuccess) {
-        cleanup();
         return fail(HIPRT_ERR_HIP, "cannot allocate the sample's hit buffers");
     }
     auto hits_of = [](dbuf &d, uint32_t m) {
@@ -161,20 +165,16 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
         return any ? hiprt_trace_any(sc, &tr, m, occ.as<uint8_t>(), hh, tracer, o, nullptr)
                    : hiprt_trace_closest(sc, &tr, m, hh, tracer, o, nullptr);
     };
-    if (int st = trace(layouts[0], HIPRT_TRACER_WHILE_WHILE, nullptr, &ref_h, n)) {
-        cleanup();
-        return st;
-    }
+    if (int st = trace(layouts[0], HIPRT_TRACER_WHILE_WHILE, nullptr, &ref_h, n)) return st;
     std::vector<float> ref_t(n), got_t(n);
     std::vector<uint32_t> ref_id(n), got_id(n);
     if (hipMemcpy(ref_t.data(), ref_h.d_t, 4ull * n, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(ref_id.data(), ref_h.d_id, 4ull * n, hipMemcpyDeviceToHost) != hipSuccess) {
-        cleanup();
         return fail(HIPRT_ERR_HIP, "cannot read the reference hits");
     }
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventCreate(&own.ev[0]));
+    HIP_TRY(hipEventCreate(&own.ev[1]));
+    hipEvent_t e0 = own.ev[0], e1 = own.ev[1];
     hiprt_tune_result best{};
     float best_ms = 1e30f;
     int count = 0;
@@ -191,12 +191,7 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
             for (int thr : thresholds)
                 for (int chunks : {1, 8}) {
                     hiprt_tracer_opts o{thr, full, nullptr, stack, chunks};
-                    if (int st = trace(layouts[f], tracer, &o, &h, nt)) {
-                        (void)hipEventDestroy(e0);
-                        (void)hipEventDestroy(e1);
-                        cleanup();
-                        return st;
-                    }
+                    if (int st = trace(layouts[f], tracer, &o, &h, nt)) return st;
                     float ms = 1e30f;
                     for (int rep = 0; rep < 2; ++rep) {
                         HIP_TRY(hipEventRecord(e0, nullptr));
@@ -222,17 +217,11 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
                 }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (best_ms >= 1e30f) {
-        cleanup();
-        return fail(HIPRT_ERR_UNSUPPORTED, "no configuration reproduced the plain kernel's hits");
-    }
+    if (best_ms >= 1e30f) return fail(HIPRT_ERR_UNSUPPORTED, "no configuration reproduced the plain kernel's hits");
     best.candidates = count;
     *result = best;
     *tuned = layouts[best.bvh_format];
     layouts[best.bvh_format] = nullptr;
-    cleanup();
     return ok();
 }
 
