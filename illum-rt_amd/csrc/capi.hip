@@ -527,6 +527,12 @@ int hiprt_scene_refit(hiprt_scene sc, const float *d_positions, void *stream) {
     return ok();
 }
 
+int hiprt_scene_stack_need(hiprt_scene sc, uint32_t *stack_need) {
+    if (!sc || !stack_need) return fail(HIPRT_ERR_ARG, "null argument");
+    *stack_need = (uint32_t)sc->stack_need;
+    return ok();
+}
+
 int hiprt_scene_destroy(hiprt_scene sc) {
     if (!sc) return fail(HIPRT_ERR_ARG, "null scene");
     (void)hipSetDevice(sc->ctx->device);

@@ -112,6 +112,7 @@ def lib():
                                           c_uint32, c_int32, c_int32, POINTER(c_void_p), POINTER(c_float),
                                           c_void_p]),
             "hiprt_scene_destroy": (c_int, [c_void_p]),
+            "hiprt_scene_stack_need": (c_int, [c_void_p, POINTER(c_uint32)]),
             "hiprt_tracer_id": (c_int, [c_char_p]),
             "hiprt_tracer_name": (c_char_p, [c_int]),
             "hiprt_trace_closest": (c_int, [c_void_p, POINTER(_Rays), c_uint32, POINTER(_Hits), c_int32,
@@ -299,6 +300,9 @@ class DeviceScene:
         check(lib().hiprt_scene_layout(self.h, ctypes.byref(nn), ctypes.byref(ns), ctypes.byref(root),
                                        ctypes.byref(depth), ctypes.byref(bf), ctypes.byref(tf)))
         self.n_nodes, self.n_slots, self.root_ref, self.depth = nn.value, ns.value, root.value, depth.value
+        need = c_uint32()
+        check(lib().hiprt_scene_stack_need(self.h, ctypes.byref(need)))
+        self.stack_need = need.value
         self.bvh_format, self.tri_format = bf.value, tf.value
 
     def download(self):

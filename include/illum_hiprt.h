@@ -162,6 +162,9 @@ int hiprt_scene_download(hiprt_scene sc, void *nodes, void *tri_records, uint32_
  * stream-ordered, no host synchronisation. */
 int hiprt_scene_refit(hiprt_scene sc, const float *d_positions, void *stream);
 int hiprt_scene_destroy(hiprt_scene sc);
+/* Most traversal-stack entries any ray can need in this tree (the persistent
+ * tracers need it to fit their stack configuration, the plain one 63). */
+int hiprt_scene_stack_need(hiprt_scene sc, uint32_t *stack_need);
 
 /* --------------------------------------------------------------------- trace */
 typedef struct {
