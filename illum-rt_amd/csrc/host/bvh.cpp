@@ -196,6 +196,10 @@ void early_split_clipping(const scene &s, std::vector<prim> &prims, std::vector<
     size_t rank = 9 * areas.size() / 10;
     std::nth_element(areas.begin(), areas.begin() + rank, areas.end());
     float thres = areas[rank];
+    // With most boxes flat (zero area) the threshold is 0 and no fragment of
+    // a triangle with area could ever get below it: the reference recurses
+    // without end there.  Such scenes keep whole-triangle boxes.
+    if (!(thres > 0.0f)) return;
     uint32_t n = (uint32_t)prims.size();
     std::vector<vec3f> poly(3);
     for (uint32_t i = 0; i < n; ++i) {
@@ -214,10 +218,33 @@ void early_split_clipping(const scene &s, std::vector<prim> &prims, std::vector<
 
 } // namespace
 
-std::vector<aabb> esc_split(const std::vector<vec3f> &poly, float threshold) {
+namespace {
+
+bool same_box(const aabb &x, const aabb &y) {
+    return x.min.x == y.min.x && x.min.y == y.min.y && x.min.z == y.min.z && x.max.x == y.max.x &&
+           x.max.y == y.max.y && x.max.z == y.max.z;
+}
+
+aabb poly_box(const std::vector<vec3f> &poly) {
     aabb b = empty_box();
     for (const vec3f &v : poly) grow(b, v);
-    if (box_area(b) <= threshold) return {b};
+    return b;
+}
+
+// Splits one polygon; `budget` counts the splits this triangle may still
+// make.  Two guards end recursions the reference would not end: a split
+// whose halves are not smaller than the polygon (rounding at large
+// coordinates) and a triangle that would need more than kEscMaxSplits
+// splits (a threshold far below its area).  The workloads' triangles need
+// at most 5 splits, so neither guard changes their fragments.
+constexpr int kEscMaxSplits = 1023;
+
+void esc_rec(const std::vector<vec3f> &poly, float threshold, int &budget, std::vector<aabb> &out) {
+    aabb b = poly_box(poly);
+    if (box_area(b) <= threshold || budget <= 0) {
+        out.push_back(b);
+        return;
+    }
     int axis = largest_axis(b.max - b.min);
     float center = (comp(b.max, axis) + comp(b.min, axis)) * 0.5f;
 
@@ -237,7 +264,10 @@ std::vector<aabb> esc_split(const std::vector<vec3f> &poly, float threshold) {
     if (cross_second == -1) cross_second = n - 1;
     // The reference asserts here; it can only happen when rounding puts the
     // box centre onto the box minimum, and the unsplit box is the safe answer.
-    if (cross_first == -1) return {b};
+    if (cross_first == -1) {
+        out.push_back(b);
+        return;
+    }
 
     auto crossing = [&](int a) {
         float delta = center - comp(poly[a], axis);
@@ -256,10 +286,22 @@ std::vector<aabb> esc_split(const std::vector<vec3f> &poly, float threshold) {
     p1.erase(std::unique(p1.begin(), p1.end()), p1.end());
     p2.erase(std::unique(p2.begin(), p2.end()), p2.end());
 
-    std::vector<aabb> res = esc_split(p1, threshold);
-    std::vector<aabb> r2 = esc_split(p2, threshold);
-    res.insert(res.end(), r2.begin(), r2.end());
-    return res;
+    if (same_box(poly_box(p1), b) || same_box(poly_box(p2), b)) {
+        out.push_back(b);
+        return;
+    }
+    --budget;
+    esc_rec(p1, threshold, budget, out);
+    esc_rec(p2, threshold, budget, out);
+}
+
+} // namespace
+
+std::vector<aabb> esc_split(const std::vector<vec3f> &poly, float threshold) {
+    std::vector<aabb> out;
+    int budget = kEscMaxSplits;
+    esc_rec(poly, threshold, budget, out);
+    return out;
 }
 
 bvh build_bvh(const scene &s, const bvh_params &p) {

@@ -258,3 +258,18 @@ def test_config_camera_rays_and_depth_fit_the_kernels():
         b = hiprt.Bvh(sc)
         # persistent kernels keep 32 stack entries; the plain kernel 63
         assert b.depth <= 32, (cfg, b.depth)
+
+
+def test_bvh_build_ends_on_large_coordinates():
+    """Triangles out to 2^49 along x: float rounding collapses most of them to
+    flat boxes (a zero split threshold) and stops midpoint splits from
+    shrinking the far ones.  The build must end and give a valid tree."""
+    n = 50
+    c = np.stack([2.0 ** np.arange(n), np.zeros(n), np.zeros(n)], 1)
+    pos = (c[:, None, :] + np.random.default_rng(1).normal(0, 0.05, (n, 3, 3))).reshape(-1, 3).astype(np.float32)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    sc = hiprt.HostScene.from_arrays(pos, idx)
+    for max_tris in (1, 4):
+        bvh = hiprt.Bvh(sc, max_tris=max_tris)
+        assert bvh.validate()[0] == 0
+        assert np.array_equal(np.unique(bvh.index), np.arange(n))
