@@ -72,7 +72,7 @@ def kernel_name(kw, binary):
     spec = "true" if tracer in ("persistent-speculative-while-while", "dynamic-while-while") else "false"
     refill = "true" if tracer == "dynamic-while-while" else "false"
     reg = {0: 4, 1: 8, 2: 2, 3: 0}[kw.get("stack_config", 0)]
-    lds = ({0: 28, 1: 16, 2: 20, 3: 24} if binary else {0: 44, 1: 32, 2: 36, 3: 40})[kw.get("stack_config", 0)]
+    lds = {0: 28, 1: 16, 2: 20, 3: 24}[kw.get("stack_config", 0)]
     return (f"k_trace_persistent<0, false, {spec}, {refill}, {reg}, {lds}, false, {2 if binary else 4}, false> "
             f"({tracer}, stack {kw.get('stack_config', 0)}, {'binary' if binary else '4-wide'} tree)")
 

@@ -87,12 +87,15 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
     size_t off = 0;
     for (auto &g : groups) {
         int bvh = g.first.first, tri = g.first.second;
-        if (need[g.first] > rt::dev::persistent_stack_cap(bvh, cfg_for(tracer, opts).stack))
-            return fail(HIPRT_ERR_UNSUPPORTED, "a scene's BVH is deeper than the persistent traversal stack");
+        rt::dev::persistent_cfg cfg = cfg_for(tracer, opts);
+        rt::scene_desc fmt{};
+        fmt.bvh = bvh;
+        fmt.tri = tri;
+        HIP_TRY(hiprt_detail::attach_spill(st, s, fmt, any != 0, true, need[g.first], cfg));
         auto *dst = reinterpret_cast<rt::dev::batch_desc *>(static_cast<char *>(table.p) + off);
         HIP_TRY(hipMemcpy(dst, g.second.data(), g.second.size() * sizeof(rt::dev::batch_desc), hipMemcpyHostToDevice));
         HIP_TRY(rt::dev::launch_trace_batch(dst, (uint32_t)g.second.size(), (uint32_t)totals[g.first], tri, bvh,
-                                            any != 0, st->counter.as<uint32_t>(), cfg_for(tracer, opts), s));
+                                            any != 0, st->counter.as<uint32_t>(), cfg, s));
         off += g.second.size() * sizeof(rt::dev::batch_desc);
     }
     return ok();
@@ -180,7 +183,6 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
     int count = 0;
     for (int f = 0; f < 2; ++f)
     for (int stack = 0; stack < rt::dev::kStackConfigs; ++stack) {
-        if (layouts[f]->stack_need > rt::dev::persistent_stack_cap(f, stack)) continue;
         for (int tracer = HIPRT_TRACER_PERSISTENT_WHILE_WHILE; tracer < HIPRT_TRACER_COUNT; ++tracer) {
             std::vector<int> thresholds = tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE ? std::vector<int>{24, 40, 56}
                                                                                      : std::vector<int>{40};

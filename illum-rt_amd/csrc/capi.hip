@@ -28,6 +28,30 @@ int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+hipError_t attach_spill(hiprt_stream_state *st, hipStream_t s, const rt::scene_desc &d, bool any, bool multi,
+                        int need, rt::dev::persistent_cfg &cfg) {
+    cfg.spill = nullptr;
+    cfg.spill_cap = rt::dev::persistent_spill_entries(d.bvh, cfg.stack, need);
+    cfg.spill_bytes = 0;
+    if (cfg.spill_cap == 0) return hipSuccess;
+    int blocks = 0;
+    hipError_t e = rt::dev::persistent_grid(d, any, cfg, &blocks, multi);
+    if (e != hipSuccess) return e;
+    size_t bytes = rt::dev::persistent_spill_bytes(blocks, cfg.spill_cap);
+    if (st->spill.bytes < bytes) {
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        e = st->spill.alloc(bytes);
+        if (e != hipSuccess) {
+            st->spill.release();
+            return e;
+        }
+    }
+    cfg.spill = st->spill.as<int32_t>();
+    cfg.spill_bytes = st->spill.bytes;
+    return hipSuccess;
+}
+
 } // namespace hiprt_detail
 
 using hiprt_detail::dbuf;
@@ -582,11 +606,9 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     hipStream_t s = static_cast<hipStream_t>(stream);
     int need = sc->stack_need;
     int stack = (opts && opts->stack_config > 0 && opts->stack_config < rt::dev::kStackConfigs) ? opts->stack_config : 0;
-    bool persistent_fits = need <= rt::dev::persistent_stack_cap(sc->desc.bvh, stack);
-    if (sc->desc.bvh == rt::BVH4 && (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits))
-        return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH needs a persistent tracer with a stack of " +
-                                               std::to_string(need) + " entries");
-    if (tracer == HIPRT_TRACER_WHILE_WHILE || !persistent_fits) {
+    if (sc->desc.bvh == rt::BVH4 && tracer == HIPRT_TRACER_WHILE_WHILE)
+        return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH needs a persistent tracer");
+    if (tracer == HIPRT_TRACER_WHILE_WHILE) {
         if (need > rt::dev::kPlainStackCap - 1)
             return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack (" + std::to_string(need) + " entries)");
         HIP_TRY(rt::dev::launch_trace_plain(sc->desc, r, h, occluded, n, any, s,
@@ -605,6 +627,7 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
     hiprt_stream_state *st = sc->ctx->state(stream);
     if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
+    HIP_TRY(hiprt_detail::attach_spill(st, s, sc->desc, any, false, need, cfg));
     HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
                                              st->counter.as<uint32_t>(), any, cfg, s));
     return ok();

@@ -5,6 +5,7 @@
 #include "../../include/illum_hiprt.h"
 
 #include "core/layout.h"
+#include "device/kernels.h"
 #include "host/bvh.h"
 #include "host/flatten.h"
 #include "host/scene.h"
@@ -73,6 +74,7 @@ struct hiprt_bvh_s {
 struct hiprt_stream_state {
     hiprt_detail::dbuf counter;       // work counters of the persistent kernels
     hiprt_detail::dbuf batch_table;   // descriptor table of the last batched trace on the stream
+    hiprt_detail::dbuf spill;         // stack spill buffer of the persistent kernels (grown on demand)
 };
 
 struct hiprt_context_s {
@@ -106,5 +108,13 @@ std::string check_layout(rt::flat_scene &f, uint32_t &stack_need);
 
 // Fills the scene descriptor from the device buffers and the host copy.
 void set_desc(hiprt_scene_s *s, uint32_t stack_need);
+
+// Gives `cfg` the spill buffer a persistent launch on stream `s` needs for
+// trees whose traversal can hold `need` stack entries (none when the
+// configuration's VGPR and LDS entries hold them): the stream state's
+// buffer, grown once the stream's earlier launches, which may still use the
+// old one, have finished.  `multi` selects the descriptor-table kernel.
+hipError_t attach_spill(hiprt_stream_state *st, hipStream_t s, const rt::scene_desc &d, bool any, bool multi,
+                        int need, rt::dev::persistent_cfg &cfg);
 
 } // namespace hiprt_detail

@@ -15,12 +15,11 @@ namespace dev {
 // than kPlainStackCap - 1 inner levels for them.
 constexpr int kPlainStackCap = 64;
 
-// Persistent kernels: per-lane stack of kRegStack entries in VGPRs backed by
-// kLdsStack (binary BVH) or kLdsStack4 (4-wide BVH) entries per lane in LDS.
+// Persistent kernels, stack configuration 0: per-lane stack of kRegStack
+// entries in VGPRs backed by a ring of kLdsStack entries per lane in LDS and,
+// for deeper traversals, a per-lane column of a global spill buffer.
 constexpr int kRegStack = 4;
 constexpr int kLdsStack = 28;
-constexpr int kLdsStack4 = 44;
-constexpr int kPersistentStackCap = kRegStack + kLdsStack;
 constexpr int kPersistentWavesPerBlock = 4;
 constexpr int kStackConfigs = 4;
 // Leaves a closest-hit lane of the speculative persistent tracers parks
@@ -32,9 +31,16 @@ constexpr uint32_t kMaxChunks = 8;
 constexpr uint32_t kCounterStride = 64;
 constexpr size_t kCounterBytes = kMaxChunks * kCounterStride * 4;
 
-// Stack entries the persistent kernels hold for a BVH of the given format
-// and stack configuration (0: 4 VGPR + 28/44 LDS entries, 1: 8 VGPR + 16/32).
+// Stack entries the persistent kernels hold in VGPRs and LDS for a BVH of
+// the given format and stack configuration (0: 4 VGPR + 28 LDS entries,
+// 1: 8 + 16, 2: 2 + 20, 3: 0 + 24).
 int persistent_stack_cap(int bvh, int stack = 0);
+
+// Spill entries per lane a persistent launch needs for a tree whose
+// traversal can hold `need` stack entries (0 when VGPRs and LDS hold them),
+// and the bytes of spill buffer a grid of `blocks` needs for that many.
+uint32_t persistent_spill_entries(int bvh, int stack, int need);
+size_t persistent_spill_bytes(int blocks, uint32_t entries);
 
 // One entry of a multi-scene batch: a scene and its own ray and hit buffers;
 // its rays occupy [first, first + n) of the launch's index space.
@@ -60,12 +66,18 @@ struct persistent_cfg {
     unsigned long long *stats = nullptr;   // non-null: counting variant (inner nodes, triangle tests)
     int stack = 0;        // stack configuration 0..kStackConfigs-1, see persistent_stack_cap
     int chunks = 8;       // 1: one work counter; 8: one contiguous chunk of rays per XCD
+    // Global spill buffer: spill_cap entries per lane of the grid
+    // (persistent_spill_bytes); spill_cap 0 when the tree fits VGPRs + LDS.
+    int32_t *spill = nullptr;
+    uint32_t spill_cap = 0;
+    size_t spill_bytes = 0;
 };
 
 // `counter` points to kCounterBytes of device memory (cleared by the launch);
-// `count` is the
-// number of rays, read from device memory when count_ptr is non-null (so a
-// captured graph can trace a batch whose size an earlier kernel produced).
+// `n` is the number of rays, read from device memory when count_ptr is
+// non-null (so a captured graph can trace a batch whose size an earlier
+// kernel produced).  The launch fails with hipErrorInvalidValue when
+// cfg.spill_cap is set and the spill buffer is smaller than the grid needs.
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
                                    uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,
                                    bool any, const persistent_cfg &cfg, hipStream_t stream);
@@ -77,8 +89,11 @@ hipError_t launch_trace_batch(const batch_desc *d_table, uint32_t n_desc, uint32
                               uint32_t *counter, const persistent_cfg &cfg, hipStream_t stream);
 
 // Grid size (blocks) of the persistent kernel instantiation for these
-// settings: blocks resident per CU times the CU count.
-hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks);
+// settings (`multi`: the descriptor-table variant; only the scene's
+// triangle and BVH formats are read): cfg.blocks when positive, else blocks
+// resident per CU times the CU count.
+hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks,
+                           bool multi = false);
 
 // Fills `n` uint32 words with `value` (used instead of hipMemsetAsync so the
 // clearing can be captured into graphs).

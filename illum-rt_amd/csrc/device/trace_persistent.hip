@@ -8,8 +8,10 @@
 // "<" on the best t -- and adds:
 //
 //   * a stack whose newest REG entries live in VGPRs (a shift register with
-//     static indices, so no scratch) and whose older entries spill to an LDS
-//     column owned by the lane (level-major, lane-minor: conflict free);
+//     static indices, so no scratch), the next LDS entries in a ring in an
+//     LDS column owned by the lane (level-major, lane-minor: conflict free),
+//     and the oldest, when a deep traversal outgrows both, in a per-lane
+//     column of a global spill buffer sized by the host for the tree;
 //   * leaf postponement (SPEC): a lane that reaches a leaf parks it and keeps
 //     descending until every lane still searching has a parked leaf, so the
 //     wave runs triangle tests together instead of one lane at a time.  A
@@ -44,27 +46,66 @@ namespace dev {
 
 namespace {
 
+// Spill column of one lane: `cap` stack entries `stride` words apart in a
+// global buffer (entry k of every lane of the grid is contiguous).
+struct spill_col {
+    int32_t *p;
+    uint32_t stride, cap;
+};
+
 template <int REG, int LDS>
 struct lane_stack {
+    static_assert(LDS > 0, "the LDS ring needs at least one entry");
     int32_t s[REG > 0 ? REG : 1];
     int rs;         // entries held in registers
-    int ls;         // entries held in LDS
+    int ls;         // entries held in the LDS ring
+    int ts;         // ring slot the next LDS entry goes to
+    uint32_t gs;    // entries spilled to the global column (the oldest ones)
     int32_t *col;   // this lane's LDS column, entries 64 words apart
+    spill_col g;
 
     __device__ __forceinline__ void reset() {
         rs = 0;
         ls = 0;
+        ts = 0;
+        gs = 0;
+    }
+    // Below the registers: an LDS ring of LDS entries.  When it is full, its
+    // oldest entry (the one in slot ts) moves to the spill column, so the
+    // order of the whole stack is kept: registers newest, then the ring, then
+    // the spill column.  The host sizes the column for the tree's stack
+    // need; entries beyond it would be dropped, never written out of bounds.
+    __device__ __forceinline__ void push_lds(int32_t x) {
+        if (ls == LDS) {
+            if (gs < g.cap) g.p[(size_t)gs * g.stride] = col[64 * ts];
+            ++gs;
+        } else {
+            ++ls;
+        }
+        col[64 * ts] = x;
+        if constexpr ((LDS & (LDS - 1)) == 0) ts = (ts + 1) & (LDS - 1);
+        else ts = ts + 1 == LDS ? 0 : ts + 1;
+    }
+    __device__ __forceinline__ int32_t pop_lds() {
+        if (ls > 0) {
+            --ls;
+            if constexpr ((LDS & (LDS - 1)) == 0) ts = (ts - 1) & (LDS - 1);
+            else ts = ts == 0 ? LDS - 1 : ts - 1;
+            return col[64 * ts];
+        }
+        if (gs > 0) {
+            --gs;
+            return gs < g.cap ? g.p[(size_t)gs * g.stride] : kInvalidRef;
+        }
+        return kInvalidRef;
     }
     __device__ __forceinline__ void push(int32_t x) {
         if (REG == 0) {
-            if (ls < LDS) col[64 * ls++] = x;
+            push_lds(x);
             return;
         }
-        if (rs == REG) {
-            if (ls < LDS) col[64 * ls++] = s[REG - 1];
-        } else {
-            ++rs;
-        }
+        if (rs == REG) push_lds(s[REG - 1]);
+        else ++rs;
 #pragma unroll
         for (int k = REG - 1; k > 0; --k) s[k] = s[k - 1];
         s[0] = x;
@@ -77,8 +118,7 @@ struct lane_stack {
             --rs;
             return x;
         }
-        if (ls > 0) return col[64 * --ls];
-        return kInvalidRef;
+        return pop_lds();
     }
 };
 
@@ -141,7 +181,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                                                           const uint32_t *count_ptr, uint32_t *counter,
                                                           int refill_thr, unsigned long long *stats,
                                                           const batch_desc *table, uint32_t n_desc,
-                                                          uint32_t n_chunks) {
+                                                          uint32_t n_chunks, int32_t *spill, uint32_t spill_cap) {
     __shared__ int32_t lds_stack[kPersistentWavesPerBlock * LDS * 64];
     const uint32_t n = count_ptr ? *count_ptr : n_static;
     uint32_t chunk = blockIdx.x % n_chunks;   // wave-uniform: the chunk this wave takes rays from
@@ -151,6 +191,8 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
 
     lane_stack<REG, LDS> st;
     st.col = lds_stack + wave * LDS * 64 + lane;
+    st.g = spill ? spill_col{spill + (size_t)blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u, spill_cap}
+                 : spill_col{nullptr, 0u, 0u};
     st.reset();
 
     lane_scene ls = scene_of(sc);
@@ -372,18 +414,20 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t value, uint32_t n, hipStream_t 
 
 namespace {
 
-// Stack configurations: (VGPR entries, LDS entries) per lane.  A push or pop
-// of the VGPR part shifts every register entry (static indices keep it out
-// of scratch), so VGPR entries cost VALU moves; LDS entries cost LDS space,
-// which bounds the blocks per CU.
-//   0: 4 VGPR + 28 LDS (44 for 4-wide)   28 KB of LDS per block (binary)
-//   1: 8 VGPR + 16 LDS (32)              16 KB
-//   2: 2 VGPR + 20 LDS (36)              20 KB
-//   3: 0 VGPR + 24 LDS (40)              24 KB
+// Stack configurations: (VGPR entries, LDS ring entries) per lane, for both
+// tree widths.  A push or pop of the VGPR part shifts every register entry
+// (static indices keep it out of scratch), so VGPR entries cost VALU moves;
+// LDS entries cost LDS space, which bounds the blocks per CU.  Entries beyond
+// both go to the global spill column, so a configuration never limits the
+// tree depth; it only decides how often a deep traversal spills.
+//   0: 4 VGPR + 28 LDS   28 KB of LDS per block
+//   1: 8 VGPR + 16 LDS   16 KB
+//   2: 2 VGPR + 20 LDS   20 KB
+//   3: 0 VGPR + 24 LDS   24 KB
 constexpr int reg_for(int stk) { return stk == 0 ? kRegStack : stk == 1 ? 8 : stk == 2 ? 2 : 0; }
 constexpr int lds_for(int bvh, int stk) {
-    return bvh == 2 ? (stk == 0 ? kLdsStack : stk == 1 ? 16 : stk == 2 ? 20 : 24)
-                    : (stk == 0 ? kLdsStack4 : stk == 1 ? 32 : stk == 2 ? 36 : 40);
+    (void)bvh;
+    return stk == 0 ? kLdsStack : stk == 1 ? 16 : stk == 2 ? 20 : 24;
 }
 
 // Blocks of one instantiation that fit the device at once (occupancy x CUs),
@@ -433,18 +477,22 @@ hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipSt
     }
     if (grid_out) *grid_out = blocks;
     if (grid_only) return hipSuccess;
+    // the spill column of every lane of this grid must fit the buffer
+    if (cfg.spill_cap && (!cfg.spill || cfg.spill_bytes < persistent_spill_bytes(blocks, cfg.spill_cap)))
+        return hipErrorInvalidValue;
     constexpr int R = reg_for(STK), L = lds_for(BVH, STK);
     if constexpr (!MULTI) {
         if (cfg.stats) {
             hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, true, BVH, MULTI>), dim3(blocks),
                                dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
-                               cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks));
+                               cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill,
+                               cfg.spill_cap);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, false, BVH, MULTI>), dim3(blocks), dim3(256),
                        0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter, cfg.refill_thr,
-                       cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks));
+                       cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill, cfg.spill_cap);
     return hipGetLastError();
 }
 
@@ -493,8 +541,18 @@ int persistent_stack_cap(int bvh, int stack) {
     return reg_for(k) + lds_for(b, k);
 }
 
-hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks) {
+uint32_t persistent_spill_entries(int bvh, int stack, int need) {
+    int cap = persistent_stack_cap(bvh, stack);
+    return need > cap ? (uint32_t)(need - cap) : 0u;
+}
+
+size_t persistent_spill_bytes(int blocks, uint32_t entries) {
+    return (size_t)(blocks > 0 ? blocks : 0) * 256u * entries * 4u;
+}
+
+hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg &cfg, int *blocks, bool multi) {
     launch_args a{};
+    if (multi) return dispatch<true>(sc.tri, any, sc.bvh, a, cfg, nullptr, true, blocks);
     return dispatch<false>(sc.tri, any, sc.bvh, a, cfg, nullptr, true, blocks);
 }
 

@@ -57,7 +57,7 @@ struct hiprt_pt_s {
     bool batched = false;
     rt::dev::persistent_cfg pcfg{};
     ray_planes rays[2];
-    dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work;
+    dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work, spill;
     rt::hit_soa h{};
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -269,8 +269,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     if (!p->batched && sc->desc.bvh != rt::BVH2)
         return fail(HIPRT_ERR_UNSUPPORTED, "the plain schedule needs a binary BVH scene");
     const int stack = (params->stack_config > 0 && params->stack_config < rt::dev::kStackConfigs) ? params->stack_config : 0;
-    if (sc->stack_need > (p->batched ? rt::dev::persistent_stack_cap(sc->desc.bvh, stack)
-                                     : rt::dev::kPlainStackCap - 1))
+    if (!p->batched && sc->stack_need > rt::dev::kPlainStackCap - 1)
         return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     uint32_t n = p->n;
     HIP_TRY(p->rays[0].alloc(n));
@@ -303,6 +302,13 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
         // after a reset is rendered eagerly with clear_radiance set.
         if (params->blocks > 0) p->pcfg.blocks = params->blocks;
         else HIP_TRY(rt::dev::persistent_grid(sc->desc, false, p->pcfg, &p->pcfg.blocks));
+        // the graph keeps the spill buffer's address: allocate it for the grid now
+        p->pcfg.spill_cap = rt::dev::persistent_spill_entries(sc->desc.bvh, stack, sc->stack_need);
+        if (p->pcfg.spill_cap) {
+            HIP_TRY(p->spill.alloc(rt::dev::persistent_spill_bytes(p->pcfg.blocks, p->pcfg.spill_cap)));
+            p->pcfg.spill = p->spill.as<int32_t>();
+            p->pcfg.spill_bytes = p->spill.bytes;
+        }
         HIP_TRY(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
         hipError_t rec = p->record(p->cap, false);
