@@ -65,16 +65,19 @@ def time_trace(torch, hiprt, ds, rays, tracer="dynamic-while-while", stream=None
     return e0.elapsed_time(e1) / reps, hits
 
 
-def kernel_name(kw, binary):
+def kernel_name(kw, ds, binary):
     """Template instantiation of the persistent closest-hit kernel that a
-    path-tracer configuration runs (exact records)."""
+    path-tracer configuration runs on scene `ds` (exact records)."""
     tracer = kw.get("tracer", "dynamic-while-while")
     spec = "true" if tracer in ("persistent-speculative-while-while", "dynamic-while-while") else "false"
     refill = "true" if tracer == "dynamic-while-while" else "false"
     reg = {0: 4, 1: 8, 2: 2, 3: 0}[kw.get("stack_config", 0)]
     lds = {0: 28, 1: 16, 2: 20, 3: 24}[kw.get("stack_config", 0)]
-    return (f"k_trace_persistent<0, false, {spec}, {refill}, {reg}, {lds}, false, {2 if binary else 4}, false> "
-            f"({tracer}, stack {kw.get('stack_config', 0)}, {'binary' if binary else '4-wide'} tree)")
+    # the LDS ring with a spill column when the tree outgrows the
+    # configuration, the plain LDS stack otherwise
+    ring = "true" if ds.stack_need > reg + lds else "false"
+    return (f"k_trace_persistent<0, false, {spec}, {refill}, {reg}, {lds}, false, {2 if binary else 4}, false, "
+            f"{ring}> ({tracer}, stack {kw.get('stack_config', 0)}, {'binary' if binary else '4-wide'} tree)")
 
 
 def time_calls(torch, fn, stream, reps=5):
@@ -429,7 +432,7 @@ def main():
 
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": kernel_name(frame_kw, ds_frame.bvh_format == hiprt.BVH2),
+                     "kernel": kernel_name(frame_kw, ds_frame, ds_frame.bvh_format == hiprt.BVH2),
                      "avg_launch_ms": round(float(ms_lvl.sum()) / max(launches, 1), 4),
                      "bytes_per_launch": round(float(bytes_lvl.sum()) / max(launches, 1)),
                      "compulsory_GBs": round(compulsory, 1)},

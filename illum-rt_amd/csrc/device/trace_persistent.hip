@@ -53,9 +53,12 @@ struct spill_col {
     uint32_t stride, cap;
 };
 
-template <int REG, int LDS>
+// RING selects the LDS part: a ring that spills its oldest entries to the
+// global column (any tree depth), or a plain LDS array (trees whose stack
+// need fits REG + LDS; two fewer VALU operations per push and pop).
+template <int REG, int LDS, bool RING>
 struct lane_stack {
-    static_assert(LDS > 0, "the LDS ring needs at least one entry");
+    static_assert(LDS > 0, "the LDS part needs at least one entry");
     int32_t s[REG > 0 ? REG : 1];
     int rs;         // entries held in registers
     int ls;         // entries held in the LDS ring
@@ -76,6 +79,11 @@ struct lane_stack {
     // the spill column.  The host sizes the column for the tree's stack
     // need; entries beyond it would be dropped, never written out of bounds.
     __device__ __forceinline__ void push_lds(int32_t x) {
+        if (!RING) {
+            // the host picks this form only when the tree's need fits
+            if (ls < LDS) col[64 * ls++] = x;
+            return;
+        }
         if (ls == LDS) {
             if (gs < g.cap) g.p[(size_t)gs * g.stride] = col[64 * ts];
             ++gs;
@@ -87,6 +95,7 @@ struct lane_stack {
         else ts = ts + 1 == LDS ? 0 : ts + 1;
     }
     __device__ __forceinline__ int32_t pop_lds() {
+        if (!RING) return ls > 0 ? col[64 * --ls] : kInvalidRef;
         if (ls > 0) {
             --ls;
             if constexpr ((LDS & (LDS - 1)) == 0) ts = (ts - 1) & (LDS - 1);
@@ -175,7 +184,7 @@ __device__ __forceinline__ uint32_t find_desc(const batch_desc *table, uint32_t 
 
 } // namespace
 
-template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, int LDS, bool COUNT, int BVH, bool MULTI>
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, int LDS, bool COUNT, int BVH, bool MULTI, bool RING>
 __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits,
                                                           uint8_t *occluded, uint32_t n_static,
                                                           const uint32_t *count_ptr, uint32_t *counter,
@@ -189,7 +198,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 
-    lane_stack<REG, LDS> st;
+    lane_stack<REG, LDS, RING> st;
     st.col = lds_stack + wave * LDS * 64 + lane;
     st.g = spill ? spill_col{spill + (size_t)blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u, spill_cap}
                  : spill_col{nullptr, 0u, 0u};
@@ -432,7 +441,7 @@ constexpr int lds_for(int bvh, int stk) {
 
 // Blocks of one instantiation that fit the device at once (occupancy x CUs),
 // asked once per device.
-template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK>
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK, bool RING>
 hipError_t grid_of(int *blocks) {
     static int cached[64] = {0};
     int dev = 0;
@@ -445,7 +454,8 @@ hipError_t grid_of(int *blocks) {
         if (e != hipSuccess) return e;
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu,
-            k_trace_persistent<TRI, ANY, SPEC, REFILL, reg_for(STK), lds_for(BVH, STK), false, BVH, MULTI>, 256, 0);
+            k_trace_persistent<TRI, ANY, SPEC, REFILL, reg_for(STK), lds_for(BVH, STK), false, BVH, MULTI, RING>, 256,
+            0);
         if (e != hipSuccess) return e;
         cached[dev] = cus * (per_cu > 0 ? per_cu : 1);
     }
@@ -467,12 +477,12 @@ struct launch_args {
 
 __host__ __device__ constexpr uint32_t chunks_of(int c) { return c == 8 ? 8u : 1u; }
 
-template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK>
-hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipStream_t stream, bool grid_only,
-                          int *grid_out) {
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK, bool RING>
+hipError_t launch_ring(const launch_args &a, const persistent_cfg &cfg, hipStream_t stream, bool grid_only,
+                       int *grid_out) {
     int blocks = cfg.blocks;
     if (blocks <= 0) {
-        hipError_t e = grid_of<TRI, ANY, SPEC, REFILL, BVH, MULTI, STK>(&blocks);
+        hipError_t e = grid_of<TRI, ANY, SPEC, REFILL, BVH, MULTI, STK, RING>(&blocks);
         if (e != hipSuccess) return e;
     }
     if (grid_out) *grid_out = blocks;
@@ -481,19 +491,32 @@ hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipSt
     if (cfg.spill_cap && (!cfg.spill || cfg.spill_bytes < persistent_spill_bytes(blocks, cfg.spill_cap)))
         return hipErrorInvalidValue;
     constexpr int R = reg_for(STK), L = lds_for(BVH, STK);
-    if constexpr (!MULTI) {
+    if constexpr (!MULTI && RING) {
         if (cfg.stats) {
-            hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, true, BVH, MULTI>), dim3(blocks),
-                               dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
-                               cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill,
-                               cfg.spill_cap);
+            hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, true, BVH, MULTI, true>),
+                               dim3(blocks), dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr,
+                               a.counter, cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks),
+                               cfg.spill, cfg.spill_cap);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, false, BVH, MULTI>), dim3(blocks), dim3(256),
-                       0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter, cfg.refill_thr,
-                       cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill, cfg.spill_cap);
+    hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, false, BVH, MULTI, RING>), dim3(blocks),
+                       dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
+                       cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill, cfg.spill_cap);
     return hipGetLastError();
+}
+
+// The plain LDS stack runs single-scene launches whose tree fits the
+// configuration; descriptor-table launches, counting launches and trees
+// that spill use the ring.
+template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK>
+hipError_t launch_variant(const launch_args &a, const persistent_cfg &cfg, hipStream_t stream, bool grid_only,
+                          int *grid_out) {
+    if constexpr (!MULTI) {
+        if (!cfg.stats && cfg.spill_cap == 0)
+            return launch_ring<TRI, ANY, SPEC, REFILL, BVH, MULTI, STK, false>(a, cfg, stream, grid_only, grid_out);
+    }
+    return launch_ring<TRI, ANY, SPEC, REFILL, BVH, MULTI, STK, true>(a, cfg, stream, grid_only, grid_out);
 }
 
 template <int TRI, bool ANY, int BVH, bool MULTI, int STK>
