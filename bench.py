@@ -276,25 +276,27 @@ def main():
 
     # The trace autotuner times one launch over rays already in pixel order
     # with the tree warm in L2; inside a frame every level follows a shading
-    # pass and reads a compacted queue.  So the frame's configuration is
-    # chosen on warm-up frames (never the timed ones): the best of the
-    # autotuned one, its one-counter variant and a 4-wide tree, when it beats
-    # the library defaults (binary tree, dynamic tracer, stack 0, eight work
-    # chunks) by more than 2%, the defaults otherwise, which keeps the choice
-    # stable between runs.  The autotuner's sample is the first bounce, where
-    # the two layouts trace about equally fast; the deeper, less coherent
-    # bounces favour the 4-wide tree, so it is a candidate of its own.
+    # pass and reads a sorted queue.  So the frame's configuration is chosen
+    # on warm-up frames (never the timed ones): the best of the autotuned
+    # one, its one-counter variant, the 4-wide tree and the binary tree with
+    # stack configuration 3, when it beats the library defaults (binary tree,
+    # dynamic tracer, stack 0, eight work chunks) by more than 2%, the
+    # defaults otherwise, which keeps the choice stable between runs.  The
+    # autotuner's sample is the first bounce; the deeper bounces weigh the
+    # stack configurations and layouts differently, so the fixed candidates
+    # are tried on whole frames too.
     ds4 = hiprt.DeviceScene(ctx, sc, bvh, hiprt.BVH4)
     candidates = {"autotuned": (ds_tuned, pt_kw),
                   "autotuned, one work counter": (ds_tuned, dict(pt_kw, work_chunks=1)),
                   "4-wide tree, stack 3": (ds4, dict(tracer="dynamic-while-while", stack_config=3)),
+                  "binary tree, stack 3": (ds, dict(tracer="dynamic-while-while", stack_config=3)),
                   "library defaults": (ds, {})}
     frame_ms = {}
     for name, (dsc, kw) in candidates.items():
         pt = hiprt.PathTracer(dsc, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, **kw)
         frame_ms[name], _, _ = time_frames(torch, pt, 4, 1, stream)
         pt.close()
-    best_tuned = min(("autotuned", "autotuned, one work counter", "4-wide tree, stack 3"), key=frame_ms.get)
+    best_tuned = min((k for k in candidates if k != "library defaults"), key=frame_ms.get)
     choice = best_tuned if frame_ms[best_tuned] < 0.98 * frame_ms["library defaults"] else "library defaults"
     log(f"frame configuration trial (ms/frame): {frame_ms}; timed frames use {choice}")
     ds_frame, frame_kw = candidates[choice]

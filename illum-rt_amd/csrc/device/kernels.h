@@ -57,6 +57,14 @@ struct batch_desc {
 hipError_t launch_trace_plain(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
                               uint32_t n, bool any, hipStream_t stream, unsigned long long *stats = nullptr);
 
+// Rays of a single-scene persistent launch read as 32-byte records
+// (ox, oy, oz, tmin | dx, dy, dz, tmax) instead of planes: ray idx is record
+// perm[idx] (record idx when perm is null); hits are still written at idx.
+struct ray_gather {
+    const float4 *aos = nullptr;
+    const uint32_t *perm = nullptr;
+};
+
 // Tunables of the persistent kernel.
 struct persistent_cfg {
     int speculative;      // leaf postponement
@@ -80,7 +88,8 @@ struct persistent_cfg {
 // cfg.spill_cap is set and the spill buffer is smaller than the grid needs.
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
                                    uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,
-                                   bool any, const persistent_cfg &cfg, hipStream_t stream);
+                                   bool any, const persistent_cfg &cfg, hipStream_t stream,
+                                   ray_gather gather = ray_gather{});
 
 // One persistent launch over a descriptor table (device memory, sorted by
 // `first`, first entry at 0, `total` rays in all).  All scenes must share the

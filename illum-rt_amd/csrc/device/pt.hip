@@ -41,7 +41,10 @@ __device__ __forceinline__ void write_disabled(const ray_out &o, uint32_t i) {
 
 __device__ __forceinline__ vec3f f3(float4 v) { return vec3f{v.x, v.y, v.z}; }
 
-// Shading of one hit.  Returns true and the next ray when the path goes on.
+// Shading of one hit.  Returns true and the next ray when the path goes on;
+// T is the path's throughput, updated when it does.  Emission is added to
+// the path's radiance only where the material emits (adding a zero product
+// leaves the sum unchanged, so every schedule accumulates the same bits).
 struct shade_out {
     bool live;
     vec3f org, dir;
@@ -49,7 +52,7 @@ struct shade_out {
 
 __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_params &pp, uint32_t path,
                                                int level, uint64_t frame_seed, vec3f in_dir, float t, float beta,
-                                               float gamma, uint32_t tri, float *thr, float *rad) {
+                                               float gamma, uint32_t tri, vec3f &T, float *rad) {
     shade_out so{false, v3(0.0f), v3(0.0f)};
     if (t == FLT_MAX) return so;
     const float4 *g = ps.tri_geo + 3 * (size_t)tri;
@@ -57,10 +60,11 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     vec3f A = f3(g0), B = f3(g1), C = f3(g2);
     uint32_t m = __float_as_uint(g0.w);
     float4 alb = ps.mats[2 * m], emi = ps.mats[2 * m + 1];
-    vec3f T{thr[3 * path], thr[3 * path + 1], thr[3 * path + 2]};
-    rad[3 * path + 0] += T.x * emi.x;
-    rad[3 * path + 1] += T.y * emi.y;
-    rad[3 * path + 2] += T.z * emi.z;
+    if (emi.x != 0.0f || emi.y != 0.0f || emi.z != 0.0f) {
+        rad[3 * path + 0] += T.x * emi.x;
+        rad[3 * path + 1] += T.y * emi.y;
+        rad[3 * path + 2] += T.z * emi.z;
+    }
     if (level >= pp.max_depth) return so;
     pcg32 rng(pcg_stream_id(frame_seed, path, (uint64_t)level));
     float xi0 = rng.next_float(), xi1 = rng.next_float();
@@ -73,11 +77,16 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     vec3f n = geom_normal(A, B, C);
     so.dir = cosine_bounce(n, in_dir, xi0, xi1);
     so.org = hit_point(A, B, C, beta, gamma);
-    thr[3 * path + 0] = T.x * alb.x * scale;
-    thr[3 * path + 1] = T.y * alb.y * scale;
-    thr[3 * path + 2] = T.z * alb.z * scale;
+    T = vec3f{T.x * alb.x * scale, T.y * alb.y * scale, T.z * alb.z * scale};
     so.live = true;
     return so;
+}
+
+__device__ __forceinline__ vec3f load3(const float *p, uint32_t i) { return vec3f{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ void store3(float *p, uint32_t i, vec3f v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
 }
 
 // Shading blocks are 1024 threads (16 wavefronts) so that appends to the
@@ -134,9 +143,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_plain(pt_scene ps, pt_par
     bool live = false;
     if (i < n && rays.tmax[i] >= 0.0f) {
         vec3f in_dir{rays.dx[i], rays.dy[i], rays.dz[i]};
+        vec3f T = load3(thr, i);
         shade_out so = shade_one(ps, pp, i, level, *frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i],
-                                 hits.id[i], thr, rad);
+                                 hits.id[i], T, rad);
         live = so.live;
+        if (live) store3(thr, i, T);
         if (live) write_ray(rays, i, so.org, kRayEps, so.dir, FLT_MAX);
         else write_disabled(rays, i);
     }
@@ -160,15 +171,106 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
     if (i < n) {
         path = path_in[i];
         vec3f in_dir{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
-        so = shade_one(ps, pp, path, level, *frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], thr,
+        vec3f T = load3(thr, path);
+        so = shade_one(ps, pp, path, level, *frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
                        rad);
         live = so.live;
+        if (live) store3(thr, path, T);
     }
     if (level >= pp.max_depth) return;
     uint32_t slot = block_append(live, counts + level + 1, sh);
     if (live) {
         write_ray(rays_out, slot, so.org, kRayEps, so.dir, FLT_MAX);
         path_out[slot] = path;
+    }
+}
+
+// ---- sorted bounce queues (the batched schedule's default) ----
+//
+// A queue written in shading order is, after a bounce or two, incoherent:
+// neighbouring slots start anywhere in the scene and point anywhere, so the
+// rays a wavefront traces together share few nodes.  In the sorting form the
+// shade step writes each next-level ray as a 32-byte record, its (path id,
+// throughput) as a 16-byte record and a 15-bit key (the octant of the
+// direction, then the Morton code of the origin's cell in a 16 x 16 x 16
+// grid over the scene box), all in append order; a stable radix sort of the
+// keys (device/ray_sort.hip) gives the permutation the trace kernel reads
+// the ray records through, and the next shade step reads directions and
+// path records through the same permutation.  Sorted rays fill the
+// persistent kernel's work chunks key range by key range, so each XCD
+// traces rays of one direction octant and one part of the scene.  The
+// throughput travels with the ray because a sorted queue visits pixels in
+// no useful order: a per-pixel throughput array would be read and written
+// at random addresses (that alone cost 110 us per level at 2M rays).
+// Paths keep their own random streams, so the image does not depend on the
+// queue order.
+
+__device__ __forceinline__ uint32_t spread4(uint32_t v) {   // 4 bits -> bits 0, 3, 6, 9
+    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6);
+}
+
+__device__ __forceinline__ uint32_t cell_of(float x, float lo, float scale) {
+    float q = (x - lo) * scale;
+    q = fminf(fmaxf(q, 0.0f), 15.0f);   // fmaxf maps NaN to 0
+    return (uint32_t)q;
+}
+
+__device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d) {
+    uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
+    uint32_t m = (spread4(cell_of(o.x, g.lo[0], g.scale[0])) << 2) |
+                 (spread4(cell_of(o.y, g.lo[1], g.scale[1])) << 1) | spread4(cell_of(o.z, g.lo[2], g.scale[2]));
+    return (oct << 12) | m;
+}
+
+// Shade step of the sorting form.  Queue slot i < counts[level] holds the
+// ray of record src = perm[i] (src = i without a permutation).  At the camera
+// level (in_rec null) the direction comes from plane i of rays_in and the
+// path and throughput from path_in[i] and thr; later, the direction from
+// record src of in_rec and (path id bits, throughput) from in_path[src].  The
+// hit is slot i.  Live next-level rays are appended as records to out_rec,
+// with their (path, throughput) to out_path and their keys to keys: the
+// throughput travels with the ray, so no step reads or writes per-pixel
+// throughput at a random address.
+__global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_params pp, int level,
+                                                            const uint64_t *frame_seed, ray_out rays_in,
+                                                            const uint32_t *path_in, const float *thr,
+                                                            const float4 *in_rec, const float4 *in_path,
+                                                            const uint32_t *perm, hit_soa hits, float4 *out_rec,
+                                                            float4 *out_path, uint32_t *keys, float *rad,
+                                                            uint32_t *counts, sort_grid grid) {
+    __shared__ uint32_t sh[kShadeWaves + 1];
+    uint32_t i = blockIdx.x * (uint32_t)kShadeBlock + threadIdx.x;
+    uint32_t n = counts[level];
+    if (blockIdx.x * (uint32_t)kShadeBlock >= n) return;   // whole block past the queue: uniform exit
+    bool live = false;
+    shade_out so{};
+    uint32_t path = 0;
+    vec3f T{};
+    if (i < n) {
+        vec3f in_dir;
+        if (in_rec) {
+            uint32_t src = perm ? perm[i] : i;
+            float4 d = in_rec[2ull * src + 1];
+            float4 pr = in_path[src];
+            in_dir = vec3f{d.x, d.y, d.z};
+            path = __float_as_uint(pr.x);
+            T = vec3f{pr.y, pr.z, pr.w};
+        } else {
+            path = path_in[i];
+            in_dir = vec3f{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
+            T = load3(thr, path);
+        }
+        so = shade_one(ps, pp, path, level, *frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
+                       rad);
+        live = so.live;
+    }
+    if (level >= pp.max_depth) return;
+    uint32_t slot = block_append(live, counts + level + 1, sh);
+    if (live) {
+        out_rec[2ull * slot] = make_float4(so.org.x, so.org.y, so.org.z, kRayEps);
+        out_rec[2ull * slot + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, FLT_MAX);
+        out_path[slot] = make_float4(__uint_as_float(path), T.x, T.y, T.z);
+        keys[slot] = ray_key(grid, so.org, so.dir);
     }
 }
 
@@ -236,6 +338,17 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 uint32_t *counts, uint32_t n_max, hipStream_t s) {
     hipLaunchKernelGGL(k_shade_compact, dim3((n_max + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, ps,
                        pp, level, frame_seed, rays_in, hits, path_in, rays_out, path_out, thr, rad, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
+                              const ray_out &rays_in, const uint32_t *path_in, const float *thr,
+                              const float4 *in_rec, const float4 *in_path, const uint32_t *perm,
+                              const hit_soa &hits, float4 *out_rec, float4 *out_path, uint32_t *keys, float *rad,
+                              uint32_t *counts, const sort_grid &grid, uint32_t n_max, hipStream_t s) {
+    hipLaunchKernelGGL(k_shade_stage, dim3((n_max + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, ps, pp,
+                       level, frame_seed, rays_in, path_in, thr, in_rec, in_path, perm, hits, out_rec, out_path, keys,
+                       rad, counts, grid);
     return hipGetLastError();
 }
 

@@ -40,6 +40,25 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 const ray_out &rays_in, const hit_soa &hits, const uint32_t *path_in,
                                 const ray_out &rays_out, uint32_t *path_out, float *thr, float *rad,
                                 uint32_t *counts, uint32_t n_max, hipStream_t s);
+// Sorted bounce queues: ray keys are the direction octant (3 bits) and the
+// Morton code of the origin's cell in a 16^3 grid over the scene box.
+constexpr int kRayKeyBits = 15;
+struct sort_grid {
+    float lo[3];      // scene box minimum
+    float scale[3];   // 16 / extent per axis (0 for a flat axis)
+};
+// Shade step of the sorting form: slot i < counts[level] is the ray of
+// record src = perm[i] (or i when perm is null).  Camera level (in_rec
+// null): direction from the planes of rays_in, path from path_in[i],
+// throughput from thr.  Later levels: direction from record src of in_rec
+// (two float4 per ray), (path id bits, throughput) from in_path[src].  Live
+// next-level rays are appended to out_rec, out_path and keys and counted in
+// counts[level + 1].
+hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
+                              const ray_out &rays_in, const uint32_t *path_in, const float *thr,
+                              const float4 *in_rec, const float4 *in_path, const uint32_t *perm,
+                              const hit_soa &hits, float4 *out_rec, float4 *out_path, uint32_t *keys, float *rad,
+                              uint32_t *counts, const sort_grid &grid, uint32_t n_max, hipStream_t s);
 hipError_t launch_set_u64(uint64_t *p, uint64_t v, hipStream_t s);
 // totals[i] += counts[i] for i < levels.
 hipError_t launch_add_counts(unsigned long long *totals, const uint32_t *counts, int levels, hipStream_t s);

@@ -1,0 +1,29 @@
+// Stable radix sort of 32-bit keys with their slot index (device/ray_sort.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rt {
+namespace dev {
+
+constexpr uint32_t kSortTile = 4096;   // items per block and pass
+
+// Device buffers of one sort: keys_in, keys_tmp, vals_tmp and vals_out hold
+// n_max words each, counts 256 * radix_tiles(n_max), totals 256.  keys_in is
+// overwritten; vals_out receives, for sorted position j < *count_ptr, the
+// slot of keys_in that sorts there (ties keep slot order).
+struct radix_sort_buffers {
+    uint32_t *keys_in, *keys_tmp, *vals_tmp, *vals_out, *counts, *totals;
+};
+
+uint32_t radix_tiles(uint32_t n_max);
+
+// Sorts the first *count_ptr (<= n_max) keys by their low `bits` bits
+// (1..32, eight per pass).  Asynchronous on `s`, capturable into a graph.
+hipError_t launch_radix_sort(const radix_sort_buffers &b, uint32_t n_max, const uint32_t *count_ptr, int bits,
+                             hipStream_t s);
+
+} // namespace dev
+} // namespace rt

@@ -41,6 +41,24 @@ __device__ __forceinline__ dray load_ray(const ray_soa &r, uint32_t i) {
     return q;
 }
 
+// The same ray from a 32-byte record (ox, oy, oz, tmin | dx, dy, dz, tmax).
+__device__ __forceinline__ dray load_ray_aos(const float4 *p, uint32_t i) {
+    float4 a = p[2ull * i], b = p[2ull * i + 1];
+    dray q;
+    q.ox = a.x;
+    q.oy = a.y;
+    q.oz = a.z;
+    q.tmin = a.w;
+    q.dx = b.x;
+    q.dy = b.y;
+    q.dz = b.z;
+    q.tmax = b.w;
+    q.ix = 1.0f / q.dx;
+    q.iy = 1.0f / q.dy;
+    q.iz = 1.0f / q.dz;
+    return q;
+}
+
 // One slab: the two ternaries of intersect4 (t1 = min, t2 = max as selects).
 __device__ __forceinline__ void slab(float lo, float hi, float o, float id, float &a, float &b) {
     float t1 = (lo - o) * id;

@@ -190,7 +190,8 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                                                           const uint32_t *count_ptr, uint32_t *counter,
                                                           int refill_thr, unsigned long long *stats,
                                                           const batch_desc *table, uint32_t n_desc,
-                                                          uint32_t n_chunks, int32_t *spill, uint32_t spill_cap) {
+                                                          uint32_t n_chunks, int32_t *spill, uint32_t spill_cap,
+                                                          ray_gather gather) {
     __shared__ int32_t lds_stack[kPersistentWavesPerBlock * LDS * 64];
     const uint32_t n = count_ptr ? *count_ptr : n_static;
     uint32_t chunk = blockIdx.x % n_chunks;   // wave-uniform: the chunk this wave takes rays from
@@ -251,7 +252,8 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                         r = load_ray(bd.rays, ray_id);
                     } else {
                         ray_id = idx;
-                        r = load_ray(rays, idx);
+                        if (gather.aos) r = load_ray_aos(gather.aos, gather.perm ? gather.perm[idx] : idx);
+                        else r = load_ray(rays, idx);
                     }
                     cur = ls.root;
                     pc = 0;
@@ -466,6 +468,7 @@ hipError_t grid_of(int *blocks) {
 struct launch_args {
     scene_desc sc;
     ray_soa rays;
+    ray_gather gather;
     hit_soa hits;
     uint8_t *occluded;
     uint32_t n;
@@ -496,13 +499,14 @@ hipError_t launch_ring(const launch_args &a, const persistent_cfg &cfg, hipStrea
             hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, true, BVH, MULTI, true>),
                                dim3(blocks), dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr,
                                a.counter, cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks),
-                               cfg.spill, cfg.spill_cap);
+                               cfg.spill, cfg.spill_cap, a.gather);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, false, BVH, MULTI, RING>), dim3(blocks),
                        dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
-                       cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill, cfg.spill_cap);
+                       cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill, cfg.spill_cap,
+                       a.gather);
     return hipGetLastError();
 }
 
@@ -581,10 +585,10 @@ hipError_t persistent_grid(const scene_desc &sc, bool any, const persistent_cfg 
 
 hipError_t launch_trace_persistent(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits,
                                    uint8_t *occluded, uint32_t n, const uint32_t *count_ptr, uint32_t *counter,
-                                   bool any, const persistent_cfg &cfg, hipStream_t stream) {
+                                   bool any, const persistent_cfg &cfg, hipStream_t stream, ray_gather gather) {
     hipError_t e = launch_fill_u32(counter, 0u, kMaxChunks * kCounterStride, stream);
     if (e != hipSuccess) return e;
-    launch_args a{sc, rays, hits, occluded, n, count_ptr, counter, nullptr, 0};
+    launch_args a{sc, rays, gather, hits, occluded, n, count_ptr, counter, nullptr, 0};
     return dispatch<false>(sc.tri, any, sc.bvh, a, cfg, stream, false, nullptr);
 }
 

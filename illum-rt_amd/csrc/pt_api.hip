@@ -13,9 +13,11 @@
 #include "core/raygen.h"
 #include "device/kernels.h"
 #include "device/pt_kernels.h"
+#include "device/ray_sort.h"
 #include "device/steps_kernels.h"
 #include "device/adjoint_kernels.h"
 
+#include <cmath>
 #include <memory>
 #include <new>
 #include <vector>
@@ -25,6 +27,38 @@ using hiprt_detail::fail;
 using hiprt_detail::ok;
 
 namespace {
+
+// Sort grid over the scene box: the union of the root's child boxes (a
+// one-leaf or empty tree sorts by direction octant only).
+rt::dev::sort_grid grid_of(const rt::flat_scene &h) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto grow = [&](float xl, float xh, float yl, float yh, float zl, float zh) {
+        lo[0] = std::min(lo[0], xl);
+        lo[1] = std::min(lo[1], yl);
+        lo[2] = std::min(lo[2], zl);
+        hi[0] = std::max(hi[0], xh);
+        hi[1] = std::max(hi[1], yh);
+        hi[2] = std::max(hi[2], zh);
+    };
+    if (h.root_ref >= 0 && h.bvh_kind == rt::BVH2 && (size_t)h.root_ref < h.nodes2.size()) {
+        const rt::node2 &r = h.nodes2[h.root_ref];
+        grow(r.lxlo, r.lxhi, r.lylo, r.lyhi, r.lzlo, r.lzhi);
+        grow(r.rxlo, r.rxhi, r.rylo, r.ryhi, r.rzlo, r.rzhi);
+    } else if (h.root_ref >= 0 && h.bvh_kind == rt::BVH4 && (size_t)h.root_ref < h.nodes4.size()) {
+        const rt::node4 &r = h.nodes4[h.root_ref];
+        for (int k = 0; k < 4; ++k)
+            if (r.ref[k] != rt::kInvalidRef) grow(r.lox[k], r.hix[k], r.loy[k], r.hiy[k], r.loz[k], r.hiz[k]);
+    }
+    rt::dev::sort_grid g{};
+    for (int a = 0; a < 3; ++a) {
+        float ext = hi[a] - lo[a];
+        bool ok_axis = std::isfinite(lo[a]) && std::isfinite(ext) && ext > 0.0f;
+        g.lo[a] = ok_axis ? lo[a] : 0.0f;
+        g.scale[a] = ok_axis ? 16.0f / ext : 0.0f;
+        if (!std::isfinite(g.scale[a])) g.scale[a] = 0.0f;
+    }
+    return g;
+}
 
 rt::cam_frame frame_from(const float *cam) {
     return rt::make_cam_frame(rt::vec3f{cam[0], cam[1], cam[2]}, rt::vec3f{cam[3], cam[4], cam[5]},
@@ -58,6 +92,11 @@ struct hiprt_pt_s {
     rt::dev::persistent_cfg pcfg{};
     ray_planes rays[2];
     dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work, spill;
+    // sorted bounce queues: ray records and (path id, throughput) records per
+    // level parity, keys, the permutation, and the radix sort's scratch
+    bool sorted = false;
+    dbuf rec[2], rpath[2], keys, keys_tmp, vals_tmp, perm, sort_counts, sort_totals;
+    rt::dev::sort_grid grid{};
     rt::hit_soa h{};
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -105,19 +144,48 @@ struct hiprt_pt_s {
                 e = launch_shade_plain(ps(), pp, L, fs, rays[0].out, h, thr.as<float>(), rad.as<float>(), cnt, n, s);
             } else {
                 // persistent tracers read the queue length from counts[L]
-                const ray_planes &in = rays[L & 1], &outq = rays[(L + 1) & 1];
                 persistent_cfg cfg = pcfg;
                 cfg.stats = lstats;
-                e = launch_trace_persistent(sc->desc, in.out.as_in(), h, nullptr, n, cnt + L, work.as<uint32_t>(), false,
-                                            cfg, s);
-                if (e != hipSuccess) return e;
-                if (ev) {
-                    e = hipEventRecord(ev[2 * L + 1], s);
+                if (!sorted) {
+                    const ray_planes &in = rays[L & 1], &outq = rays[(L + 1) & 1];
+                    e = launch_trace_persistent(sc->desc, in.out.as_in(), h, nullptr, n, cnt + L,
+                                                work.as<uint32_t>(), false, cfg, s);
                     if (e != hipSuccess) return e;
+                    if (ev) {
+                        e = hipEventRecord(ev[2 * L + 1], s);
+                        if (e != hipSuccess) return e;
+                    }
+                    e = launch_shade_compact(ps(), pp, L, fs, in.out, h, paths[L & 1].as<const uint32_t>(),
+                                             outq.out, paths[(L + 1) & 1].as<uint32_t>(), thr.as<float>(),
+                                             rad.as<float>(), cnt, n, s);
+                } else {
+                    // level 0: camera ray planes in pixel order; later levels:
+                    // the previous shade step's records through the sorted
+                    // permutation
+                    const float4 *in_rec = L > 0 ? rec[(L - 1) & 1].as<const float4>() : nullptr;
+                    const float4 *in_path = L > 0 ? rpath[(L - 1) & 1].as<const float4>() : nullptr;
+                    const uint32_t *pm = L > 0 ? perm.as<const uint32_t>() : nullptr;
+                    ray_gather g;
+                    g.aos = in_rec;
+                    g.perm = pm;
+                    e = launch_trace_persistent(sc->desc, rays[0].out.as_in(), h, nullptr, n, cnt + L,
+                                                work.as<uint32_t>(), false, cfg, s, g);
+                    if (e != hipSuccess) return e;
+                    if (ev) {
+                        e = hipEventRecord(ev[2 * L + 1], s);
+                        if (e != hipSuccess) return e;
+                    }
+                    e = launch_shade_stage(ps(), pp, L, fs, rays[0].out, paths[0].as<const uint32_t>(),
+                                           thr.as<const float>(), in_rec, in_path, pm, h, rec[L & 1].as<float4>(),
+                                           rpath[L & 1].as<float4>(), keys.as<uint32_t>(), rad.as<float>(), cnt,
+                                           grid, n, s);
+                    if (e == hipSuccess && L < pp.max_depth) {
+                        radix_sort_buffers rb{keys.as<uint32_t>(),      keys_tmp.as<uint32_t>(),
+                                              vals_tmp.as<uint32_t>(),  perm.as<uint32_t>(),
+                                              sort_counts.as<uint32_t>(), sort_totals.as<uint32_t>()};
+                        e = launch_radix_sort(rb, n, cnt + L + 1, kRayKeyBits, s);
+                    }
                 }
-                e = launch_shade_compact(ps(), pp, L, fs, in.out, h, paths[L & 1].as<const uint32_t>(), outq.out,
-                                         paths[(L + 1) & 1].as<uint32_t>(), thr.as<float>(), rad.as<float>(), cnt, n,
-                                         s);
             }
             if (e != hipSuccess) return e;
         }
@@ -272,11 +340,14 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     if (!p->batched && sc->stack_need > rt::dev::kPlainStackCap - 1)
         return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     uint32_t n = p->n;
+    p->sorted = p->batched && params->ray_order == 0 && p->pp.max_depth > 0;
     HIP_TRY(p->rays[0].alloc(n));
     if (p->batched) {
-        HIP_TRY(p->rays[1].alloc(n));
         HIP_TRY(p->paths[0].alloc(4ull * n));
-        HIP_TRY(p->paths[1].alloc(4ull * n));
+        if (!p->sorted) {
+            HIP_TRY(p->rays[1].alloc(n));
+            HIP_TRY(p->paths[1].alloc(4ull * n));
+        }
     }
     HIP_TRY(p->hits.alloc(16ull * n));
     float *hb = p->hits.as<float>();
@@ -296,6 +367,19 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pcfg.blocks = 0;
     p->pcfg.stack = stack;
     p->pcfg.chunks = params->work_chunks == 1 ? 1 : 8;
+    if (p->sorted) {
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(p->rec[k].alloc(32ull * n));
+            HIP_TRY(p->rpath[k].alloc(16ull * n));
+        }
+        HIP_TRY(p->keys.alloc(4ull * n));
+        HIP_TRY(p->keys_tmp.alloc(4ull * n));
+        HIP_TRY(p->vals_tmp.alloc(4ull * n));
+        HIP_TRY(p->perm.alloc(4ull * n));
+        HIP_TRY(p->sort_counts.alloc(4ull * 256 * rt::dev::radix_tiles(n)));
+        HIP_TRY(p->sort_totals.alloc(4ull * 256));
+        p->grid = grid_of(sc->host);
+    }
     if (p->batched) {
         // Resolve the persistent grid size outside the capture, then record
         // the frame once.  The graph never clears the accumulator: frame 0

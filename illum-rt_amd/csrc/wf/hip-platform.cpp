@@ -481,7 +481,7 @@ public:
         rc->camera(cam);
         int tracer = pf->rt ? pf->rt->tracer : -1;
         if (tracer == HIPRT_TRACER_WHILE_WHILE) tracer = -1;   // batched needs a persistent tracer
-        hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, 0, 8, 0, 0};
+        hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, 0, 8, 0, 0, 0};
         check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
         frames = 0;
     }

@@ -303,7 +303,12 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
  *                kept as disabled rays (the reference's wavefront form);
  *   batched = 1: live paths compacted into a queue after every level, traced
  *                by a persistent kernel that reads the queue length on the
- *                device, the whole frame captured once in a hipGraph. */
+ *                device, the whole frame captured once in a hipGraph.
+ * In the batched schedule the queue of every bounce is, by default
+ * (ray_order = 0), sorted by direction octant and origin cell (a 16^3 grid
+ * over the scene box) so that wavefronts trace neighbouring rays; 1 keeps
+ * the order the shading step appends them in.  Paths keep their own random
+ * streams, so the image does not depend on the order. */
 typedef struct {
     int32_t max_depth;   /* last trace level; 4 = primary + 4 bounces */
     int32_t rr_start;    /* first level whose ray is subject to roulette (3) */
@@ -315,6 +320,7 @@ typedef struct {
     int32_t work_chunks;    /* 0 in refill_threshold / blocks selects the default */
     int32_t refill_threshold;
     int32_t blocks;
+    int32_t ray_order;      /* batched: 0 = sorted bounce queues, 1 = shading order */
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;

@@ -8,7 +8,9 @@
 * the diffuse rays of the hall trace bit-identically on GPU and CPU;
 * the batched (compacted, persistent, graph-replayed) path tracer produces
   the same per-level ray counts and the same image as the plain per-bounce
-  one, and its level counts follow the Russian-roulette law.
+  one, with bounce queues sorted by ray key or in shading order (a ray
+  scattered to the wrong slot would continue another pixel's path), and its
+  level counts follow the Russian-roulette law.
 """
 import numpy as np
 import pytest
@@ -75,13 +77,17 @@ def test_bounce_rays_match_oracle_and_trace_exactly(torch_cuda, hall):
     assert bit_exact(_hits_np(hits2), ref)
 
 
-@pytest.mark.parametrize("res,stack,chunks", [((320, 180), 0, 1), ((320, 180), 1, 8)])
-def test_batched_equals_plain(torch_cuda, hall, res, stack, chunks):
-    print(f"case: hall path tracer {res}, plain vs batched (stack {stack}, chunks {chunks}), 2 frames", flush=True)
+@pytest.mark.parametrize("res,stack,chunks,sort", [((320, 180), 0, 1, True), ((320, 180), 1, 8, True),
+                                                   ((320, 180), 0, 8, False), ((333, 97), 3, 8, True),
+                                                   ((333, 97), 3, 1, False)])
+def test_batched_equals_plain(torch_cuda, hall, res, stack, chunks, sort):
+    print(f"case: hall path tracer {res}, plain vs batched (stack {stack}, chunks {chunks}, sorted queues {sort}), "
+          "2 frames", flush=True)
     cam = hall["cam"].copy()
     cam[10], cam[11] = res
     plain = hiprt.PathTracer(hall["ds"], cam, batched=False, seed=5)
-    batched = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=5, stack_config=stack, work_chunks=chunks)
+    batched = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=5, stack_config=stack, work_chunks=chunks,
+                               sort_rays=sort)
     counts = []
     for pt in (plain, batched):
         pt.render(0)

@@ -143,10 +143,10 @@ def test_spilled_path_tracer(torch_cuda, deep):
     # camera near the apex of the nesting, looking at it
     cam = np.array([0.02, 0.02, 0.02, -1, -1, -1, 0, 1, 0, 60, 160, 120], np.float32)
     imgs = {}
-    for name, ds, batched, stack in (("plain", ds2, False, 0), ("batched bvh2", ds2, True, 3),
-                                     ("batched bvh4", ds4, True, 3)):
+    for name, ds, batched, stack, srt in (("plain", ds2, False, 0, False), ("batched bvh2", ds2, True, 3, False),
+                                          ("batched bvh4", ds4, True, 3, True)):
         print(f"case: nested scene path tracer, {name}", flush=True)
-        pt = hiprt.PathTracer(ds, cam, batched=batched, seed=5, stack_config=stack)
+        pt = hiprt.PathTracer(ds, cam, batched=batched, seed=5, stack_config=stack, sort_rays=srt)
         pt.render(0)
         imgs[name] = pt.image()
         pt.close()
