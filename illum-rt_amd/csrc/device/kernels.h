@@ -57,12 +57,14 @@ struct batch_desc {
 hipError_t launch_trace_plain(const scene_desc &sc, const ray_soa &rays, const hit_soa &hits, uint8_t *occluded,
                               uint32_t n, bool any, hipStream_t stream, unsigned long long *stats = nullptr);
 
-// Rays of a single-scene persistent launch read as 32-byte records
-// (ox, oy, oz, tmin | dx, dy, dz, tmax) instead of planes: ray idx is record
-// perm[idx] (record idx when perm is null); hits are still written at idx.
+// Rays of a single-scene persistent launch read from records of `stride`
+// float4 (at least two) that begin (ox, oy, oz, tmin | dx, dy, dz, tmax)
+// instead of planes: ray idx is record perm[idx] (record idx when perm is
+// null); hits are still written at idx.
 struct ray_gather {
     const float4 *aos = nullptr;
     const uint32_t *perm = nullptr;
+    uint32_t stride = 2;
 };
 
 // Tunables of the persistent kernel.

@@ -196,7 +196,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
 // grid over the scene box), all in append order; a stable radix sort of the
 // keys (device/ray_sort.hip) gives the permutation the trace kernel reads
 // the ray records through, and the next shade step reads directions and
-// path records through the same permutation.  Sorted rays fill the
+// path records through the same permutation.  A ray and its path share one
+// 48-byte record (origin, t_min | direction, t_max | path id, throughput):
+// the trace kernel reads its first 32 bytes, the shade step its last 32.  Sorted rays fill the
 // persistent kernel's work chunks key range by key range, so each XCD
 // traces rays of one direction octant and one part of the scene.  The
 // throughput travels with the ray because a sorted queue visits pixels in
@@ -225,19 +227,17 @@ __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d
 // Shade step of the sorting form.  Queue slot i < counts[level] holds the
 // ray of record src = perm[i] (src = i without a permutation).  At the camera
 // level (in_rec null) the direction comes from plane i of rays_in and the
-// path and throughput from path_in[i] and thr; later, the direction from
-// record src of in_rec and (path id bits, throughput) from in_path[src].  The
-// hit is slot i.  Live next-level rays are appended as records to out_rec,
-// with their (path, throughput) to out_path and their keys to keys: the
+// path and throughput from path_in[i] and thr; later, direction and (path id
+// bits, throughput) from record src of in_rec.  The hit is slot i.  Live
+// next-level rays are appended as records to out_rec with their keys: the
 // throughput travels with the ray, so no step reads or writes per-pixel
 // throughput at a random address.
 __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_params pp, int level,
                                                             const uint64_t *frame_seed, ray_out rays_in,
                                                             const uint32_t *path_in, const float *thr,
-                                                            const float4 *in_rec, const float4 *in_path,
-                                                            const uint32_t *perm, hit_soa hits, float4 *out_rec,
-                                                            float4 *out_path, uint32_t *keys, float *rad,
-                                                            uint32_t *counts, sort_grid grid) {
+                                                            const float4 *in_rec, const uint32_t *perm,
+                                                            hit_soa hits, float4 *out_rec, uint32_t *keys,
+                                                            float *rad, uint32_t *counts, sort_grid grid) {
     __shared__ uint32_t sh[kShadeWaves + 1];
     uint32_t i = blockIdx.x * (uint32_t)kShadeBlock + threadIdx.x;
     uint32_t n = counts[level];
@@ -250,8 +250,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
         vec3f in_dir;
         if (in_rec) {
             uint32_t src = perm ? perm[i] : i;
-            float4 d = in_rec[2ull * src + 1];
-            float4 pr = in_path[src];
+            float4 d = in_rec[3ull * src + 1];
+            float4 pr = in_rec[3ull * src + 2];
             in_dir = vec3f{d.x, d.y, d.z};
             path = __float_as_uint(pr.x);
             T = vec3f{pr.y, pr.z, pr.w};
@@ -267,9 +267,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
     if (level >= pp.max_depth) return;
     uint32_t slot = block_append(live, counts + level + 1, sh);
     if (live) {
-        out_rec[2ull * slot] = make_float4(so.org.x, so.org.y, so.org.z, kRayEps);
-        out_rec[2ull * slot + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, FLT_MAX);
-        out_path[slot] = make_float4(__uint_as_float(path), T.x, T.y, T.z);
+        out_rec[3ull * slot] = make_float4(so.org.x, so.org.y, so.org.z, kRayEps);
+        out_rec[3ull * slot + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, FLT_MAX);
+        out_rec[3ull * slot + 2] = make_float4(__uint_as_float(path), T.x, T.y, T.z);
         keys[slot] = ray_key(grid, so.org, so.dir);
     }
 }
@@ -343,12 +343,12 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
 
 hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
                               const ray_out &rays_in, const uint32_t *path_in, const float *thr,
-                              const float4 *in_rec, const float4 *in_path, const uint32_t *perm,
-                              const hit_soa &hits, float4 *out_rec, float4 *out_path, uint32_t *keys, float *rad,
-                              uint32_t *counts, const sort_grid &grid, uint32_t n_max, hipStream_t s) {
+                              const float4 *in_rec, const uint32_t *perm, const hit_soa &hits, float4 *out_rec,
+                              uint32_t *keys, float *rad, uint32_t *counts, const sort_grid &grid, uint32_t n_max,
+                              hipStream_t s) {
     hipLaunchKernelGGL(k_shade_stage, dim3((n_max + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, ps, pp,
-                       level, frame_seed, rays_in, path_in, thr, in_rec, in_path, perm, hits, out_rec, out_path, keys,
-                       rad, counts, grid);
+                       level, frame_seed, rays_in, path_in, thr, in_rec, perm, hits, out_rec, keys, rad, counts,
+                       grid);
     return hipGetLastError();
 }
 

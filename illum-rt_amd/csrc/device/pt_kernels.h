@@ -47,18 +47,20 @@ struct sort_grid {
     float lo[3];      // scene box minimum
     float scale[3];   // 16 / extent per axis (0 for a flat axis)
 };
+// Ray records of the sorting form: kRayRecord float4 per ray,
+// (ox, oy, oz, t_min | dx, dy, dz, t_max | path id bits, throughput).
+constexpr uint32_t kRayRecord = 3;
 // Shade step of the sorting form: slot i < counts[level] is the ray of
 // record src = perm[i] (or i when perm is null).  Camera level (in_rec
 // null): direction from the planes of rays_in, path from path_in[i],
-// throughput from thr.  Later levels: direction from record src of in_rec
-// (two float4 per ray), (path id bits, throughput) from in_path[src].  Live
-// next-level rays are appended to out_rec, out_path and keys and counted in
-// counts[level + 1].
+// throughput from thr.  Later levels: direction, path and throughput from
+// record src of in_rec.  Live next-level rays are appended to out_rec and
+// keys and counted in counts[level + 1].
 hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
                               const ray_out &rays_in, const uint32_t *path_in, const float *thr,
-                              const float4 *in_rec, const float4 *in_path, const uint32_t *perm,
-                              const hit_soa &hits, float4 *out_rec, float4 *out_path, uint32_t *keys, float *rad,
-                              uint32_t *counts, const sort_grid &grid, uint32_t n_max, hipStream_t s);
+                              const float4 *in_rec, const uint32_t *perm, const hit_soa &hits, float4 *out_rec,
+                              uint32_t *keys, float *rad, uint32_t *counts, const sort_grid &grid, uint32_t n_max,
+                              hipStream_t s);
 hipError_t launch_set_u64(uint64_t *p, uint64_t v, hipStream_t s);
 // totals[i] += counts[i] for i < levels.
 hipError_t launch_add_counts(unsigned long long *totals, const uint32_t *counts, int levels, hipStream_t s);

@@ -1,7 +1,7 @@
 // Stable LSD radix sort of 32-bit keys with their slot index, eight key bits
 // per pass, for the path tracer's bounce queues.
 //
-// Per pass, over tiles of kSortTile items (256 threads x 16 items):
+// Per pass, over tiles of kSortTile items (256 threads x kSortTile/256 items):
 //   k_radix_count   digit histogram of each tile (LDS), written digit-major
 //                   (counts[digit * n_tiles + tile]);
 //   k_radix_rows    one block per digit: exclusive scan of its row over the

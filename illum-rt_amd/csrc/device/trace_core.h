@@ -41,9 +41,10 @@ __device__ __forceinline__ dray load_ray(const ray_soa &r, uint32_t i) {
     return q;
 }
 
-// The same ray from a 32-byte record (ox, oy, oz, tmin | dx, dy, dz, tmax).
-__device__ __forceinline__ dray load_ray_aos(const float4 *p, uint32_t i) {
-    float4 a = p[2ull * i], b = p[2ull * i + 1];
+// The same ray from a record of `stride` float4 that begins
+// (ox, oy, oz, tmin | dx, dy, dz, tmax).
+__device__ __forceinline__ dray load_ray_aos(const float4 *p, uint32_t i, uint32_t stride) {
+    float4 a = p[(size_t)stride * i], b = p[(size_t)stride * i + 1];
     dray q;
     q.ox = a.x;
     q.oy = a.y;

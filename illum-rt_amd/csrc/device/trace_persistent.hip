@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                         r = load_ray(bd.rays, ray_id);
                     } else {
                         ray_id = idx;
-                        if (gather.aos) r = load_ray_aos(gather.aos, gather.perm ? gather.perm[idx] : idx);
+                        if (gather.aos) r = load_ray_aos(gather.aos, gather.perm ? gather.perm[idx] : idx, gather.stride);
                         else r = load_ray(rays, idx);
                     }
                     cur = ls.root;

@@ -92,10 +92,10 @@ struct hiprt_pt_s {
     rt::dev::persistent_cfg pcfg{};
     ray_planes rays[2];
     dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work, spill;
-    // sorted bounce queues: ray records and (path id, throughput) records per
-    // level parity, keys, the permutation, and the radix sort's scratch
+    // sorted bounce queues: ray records (ray, path id, throughput) per level
+    // parity, keys, the permutation, and the radix sort's scratch
     bool sorted = false;
-    dbuf rec[2], rpath[2], keys, keys_tmp, vals_tmp, perm, sort_counts, sort_totals;
+    dbuf rec[2], keys, keys_tmp, vals_tmp, perm, sort_counts, sort_totals;
     rt::dev::sort_grid grid{};
     rt::hit_soa h{};
     hipStream_t cap = nullptr;
@@ -163,11 +163,11 @@ struct hiprt_pt_s {
                     // the previous shade step's records through the sorted
                     // permutation
                     const float4 *in_rec = L > 0 ? rec[(L - 1) & 1].as<const float4>() : nullptr;
-                    const float4 *in_path = L > 0 ? rpath[(L - 1) & 1].as<const float4>() : nullptr;
                     const uint32_t *pm = L > 0 ? perm.as<const uint32_t>() : nullptr;
                     ray_gather g;
                     g.aos = in_rec;
                     g.perm = pm;
+                    g.stride = kRayRecord;
                     e = launch_trace_persistent(sc->desc, rays[0].out.as_in(), h, nullptr, n, cnt + L,
                                                 work.as<uint32_t>(), false, cfg, s, g);
                     if (e != hipSuccess) return e;
@@ -176,9 +176,8 @@ struct hiprt_pt_s {
                         if (e != hipSuccess) return e;
                     }
                     e = launch_shade_stage(ps(), pp, L, fs, rays[0].out, paths[0].as<const uint32_t>(),
-                                           thr.as<const float>(), in_rec, in_path, pm, h, rec[L & 1].as<float4>(),
-                                           rpath[L & 1].as<float4>(), keys.as<uint32_t>(), rad.as<float>(), cnt,
-                                           grid, n, s);
+                                           thr.as<const float>(), in_rec, pm, h, rec[L & 1].as<float4>(),
+                                           keys.as<uint32_t>(), rad.as<float>(), cnt, grid, n, s);
                     if (e == hipSuccess && L < pp.max_depth) {
                         radix_sort_buffers rb{keys.as<uint32_t>(),      keys_tmp.as<uint32_t>(),
                                               vals_tmp.as<uint32_t>(),  perm.as<uint32_t>(),
@@ -369,8 +368,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pcfg.chunks = params->work_chunks == 1 ? 1 : 8;
     if (p->sorted) {
         for (int k = 0; k < 2; ++k) {
-            HIP_TRY(p->rec[k].alloc(32ull * n));
-            HIP_TRY(p->rpath[k].alloc(16ull * n));
+            HIP_TRY(p->rec[k].alloc(16ull * rt::dev::kRayRecord * n));
         }
         HIP_TRY(p->keys.alloc(4ull * n));
         HIP_TRY(p->keys_tmp.alloc(4ull * n));
