@@ -399,10 +399,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # HBM bytes per launch of the frame's trace kernel, from the committed
+    # FETCH_SIZE / WRITE_SIZE counter summary (null when it holds no entry
+    # for this instantiation)
+    kname = kernel_name(frame_kw, ds_frame, ds_frame.bvh_format == hiprt.BVH2)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "trace_hall_traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        entry = json.load(open(tpath)).get("kernels", {}).get(kname.split(" (")[0])
+        traffic = entry["bytes_per_launch"] if entry else None
 
     out = {
         "metric": METRIC,
@@ -434,7 +439,7 @@ def main():
 
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": kernel_name(frame_kw, ds_frame, ds_frame.bvh_format == hiprt.BVH2),
+                     "kernel": kname,
                      "avg_launch_ms": round(float(ms_lvl.sum()) / max(launches, 1), 4),
                      "bytes_per_launch": round(float(bytes_lvl.sum()) / max(launches, 1)),
                      "compulsory_GBs": round(compulsory, 1)},
