@@ -40,6 +40,8 @@ extern "C" {
 int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int32_t any, int32_t tracer,
                       const hiprt_tracer_opts *opts, void *stream) {
     if (!entries && n_entries) return fail(HIPRT_ERR_ARG, "null entries");
+    const bool alpha = (tracer & HIPRT_TRACER_ALPHA) != 0;
+    tracer &= ~HIPRT_TRACER_ALPHA;
     if (tracer < HIPRT_TRACER_PERSISTENT_WHILE_WHILE || tracer >= HIPRT_TRACER_COUNT)
         return fail(HIPRT_ERR_ARG, "the batched trace needs a persistent tracer");
     if (n_entries == 0) return ok();
@@ -60,7 +62,7 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
             return fail(HIPRT_ERR_ARG, "entry " + std::to_string(k) + ": any-hit needs occlusion or hit planes");
         auto key = std::make_pair((int)e.scene->desc.bvh, (int)e.scene->desc.tri);
         rt::dev::batch_desc d{};
-        d.scene = e.scene->desc;
+        d.scene = hiprt_detail::desc_for(e.scene, alpha);
         d.rays = rt::ray_soa{e.rays.d_ox, e.rays.d_oy, e.rays.d_oz, e.rays.d_tmin,
                              e.rays.d_dx, e.rays.d_dy, e.rays.d_dz, e.rays.d_tmax};
         d.hits = full_hits(e.hits) ? rt::hit_soa{e.hits.d_t, e.hits.d_u, e.hits.d_v, e.hits.d_id}

@@ -28,6 +28,12 @@ int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+rt::scene_desc desc_for(const hiprt_scene_s *sc, bool alpha) {
+    rt::scene_desc d = sc->desc;
+    d.alpha = (alpha && d.alpha_tri) ? 1 : 0;
+    return d;
+}
+
 hipError_t attach_spill(hiprt_stream_state *st, hipStream_t s, const rt::scene_desc &d, bool any, bool multi,
                         int need, rt::dev::persistent_cfg &cfg) {
     cfg.spill = nullptr;
@@ -188,6 +194,35 @@ int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_material
             float v[8] = {mt.albedo.x, mt.albedo.y, mt.albedo.z, 0.0f, mt.emissive.x, mt.emissive.y, mt.emissive.z, 0.0f};
             std::memcpy(table + 8 * m, v, sizeof v);
         }
+    return ok();
+}
+
+int hiprt_scene_set_texcoords(hiprt_host_scene s, const float *tc, uint32_t n_vert) {
+    if (!s || (n_vert && !tc)) return fail(HIPRT_ERR_ARG, "null argument");
+    if (n_vert != s->sc.pos.size()) return fail(HIPRT_ERR_ARG, "texture coordinate count differs from the vertex count");
+    s->sc.tc.assign(tc, tc + 2ull * n_vert);
+    return ok();
+}
+
+int hiprt_scene_set_texture(hiprt_host_scene s, uint32_t material, const float *rgba, uint32_t w, uint32_t h) {
+    if (!s) return fail(HIPRT_ERR_ARG, "null scene");
+    if (material >= s->sc.mats.size()) return fail(HIPRT_ERR_ARG, "material id out of range");
+    rt::material &m = s->sc.mats[material];
+    if (!rgba) {
+        m.tex = -1;
+        return ok();
+    }
+    if (w == 0 || h == 0 || (uint64_t)w * h > (1ull << 28)) return fail(HIPRT_ERR_ARG, "texture size must be 1..2^28 texels");
+    rt::texture t;
+    t.w = w;
+    t.h = h;
+    t.rgba.assign(rgba, rgba + 4ull * w * h);
+    if (m.tex >= 0) {
+        s->sc.texs[(size_t)m.tex] = std::move(t);
+    } else {
+        m.tex = (int32_t)s->sc.texs.size();
+        s->sc.texs.push_back(std::move(t));
+    }
     return ok();
 }
 
@@ -435,6 +470,42 @@ int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32
     HIP_TRY(ds->tri_geo.upload(geo.data(), geo.size() * 4));
     HIP_TRY(ds->mats.upload(mats.data(), mats.size() * 4));
     HIP_TRY(ds->tri_vidx.upload(sc.idx.data(), sc.idx.size() * 4));
+    // Opacity data for the alpha-aware tracers, when a material has a texture.
+    bool textured = false;
+    for (const rt::material &m : sc.mats) textured = textured || (m.tex >= 0 && (size_t)m.tex < sc.texs.size());
+    if (textured && nt) {
+        std::vector<float> atri(8ull * nt, 0.0f);
+        for (uint32_t t = 0; t < nt; ++t) {
+            uint32_t m = sc.mat.empty() ? 0u : sc.mat[t];
+            int32_t tex = m < sc.mats.size() ? sc.mats[m].tex : -1;
+            uint32_t texb = tex >= 0 ? (uint32_t)tex : 0xffffffffu;
+            float *o = &atri[8ull * t];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t v = sc.idx[3ull * t + k];
+                bool have = sc.tc.size() >= 2ull * (v + 1);
+                float uu = have ? sc.tc[2ull * v] : 0.0f, vv = have ? sc.tc[2ull * v + 1] : 0.0f;
+                o[2 * k] = uu;   // a.u a.v b.u b.v | c.u c.v
+                o[2 * k + 1] = vv;
+            }
+            std::memcpy(&o[6], &texb, 4);
+        }
+        std::vector<uint32_t> atex(4 * sc.texs.size(), 0u);
+        std::vector<uint8_t> mask;
+        for (size_t k = 0; k < sc.texs.size(); ++k) {
+            const rt::texture &tx = sc.texs[k];
+            atex[4 * k] = (uint32_t)mask.size();
+            atex[4 * k + 1] = tx.w;
+            atex[4 * k + 2] = tx.h;
+            for (size_t i = 0; i < (size_t)tx.w * tx.h; ++i) mask.push_back(tx.rgba[4 * i + 3] > 0.5f ? 1 : 0);
+        }
+        if (mask.size() > 0xffffffffull) return fail(HIPRT_ERR_ARG, "textures too large");
+        HIP_TRY(ds->alpha_tri.upload(atri.data(), atri.size() * 4));
+        HIP_TRY(ds->alpha_tex.upload(atex.data(), atex.size() * 4));
+        HIP_TRY(ds->alpha_mask.upload(mask.data(), mask.size()));
+        ds->desc.alpha_tri = ds->alpha_tri.p;
+        ds->desc.alpha_tex = ds->alpha_tex.p;
+        ds->desc.alpha_mask = ds->alpha_mask.as<const uint8_t>();
+    }
     ds->n_tris = nt;
     ds->n_vert = (uint32_t)sc.pos.size();
     *out = guard.release();
@@ -566,20 +637,26 @@ int hiprt_scene_destroy(hiprt_scene sc) {
 
 // ----------------------------------------------------------------------- trace
 
-static const char *const kTracerNames[HIPRT_TRACER_COUNT] = {
-    "while-while", "persistent-while-while", "persistent-speculative-while-while", "dynamic-while-while"};
+static const char *const kTracerNames[2][HIPRT_TRACER_COUNT] = {
+    {"while-while", "persistent-while-while", "persistent-speculative-while-while", "dynamic-while-while"},
+    {"while-while-alpha", "persistent-while-while-alpha", "persistent-speculative-while-while-alpha",
+     "dynamic-while-while-alpha"}};
 
 int hiprt_tracer_id(const char *name) {
     if (!name) return -1;
     if (std::strcmp(name, "default") == 0) return HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
-    for (int i = 0; i < HIPRT_TRACER_COUNT; ++i)
-        if (std::strcmp(name, kTracerNames[i]) == 0) return i;
+    if (std::strcmp(name, "default-alpha") == 0) return HIPRT_TRACER_DYNAMIC_WHILE_WHILE | HIPRT_TRACER_ALPHA;
+    for (int a = 0; a < 2; ++a)
+        for (int i = 0; i < HIPRT_TRACER_COUNT; ++i)
+            if (std::strcmp(name, kTracerNames[a][i]) == 0) return i | (a ? HIPRT_TRACER_ALPHA : 0);
     return -1;
 }
 
 const char *hiprt_tracer_name(int tracer) {
-    if (tracer < 0 || tracer >= HIPRT_TRACER_COUNT) return "";
-    return kTracerNames[tracer];
+    int a = (tracer & HIPRT_TRACER_ALPHA) ? 1 : 0;
+    int t = tracer & ~HIPRT_TRACER_ALPHA;
+    if (t < 0 || t >= HIPRT_TRACER_COUNT) return "";
+    return kTracerNames[a][t];
 }
 
 namespace {
@@ -587,6 +664,8 @@ namespace {
 int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *occluded, const hiprt_hits *hits,
                  bool any, int32_t tracer, const hiprt_tracer_opts *opts, void *stream) {
     if (!sc || !rays) return fail(HIPRT_ERR_ARG, "null argument");
+    const bool alpha = (tracer & HIPRT_TRACER_ALPHA) != 0;
+    tracer &= ~HIPRT_TRACER_ALPHA;
     if (tracer < 0 || tracer >= HIPRT_TRACER_COUNT) return fail(HIPRT_ERR_ARG, "unknown tracer id");
     if (n > 0xF0000000u) return fail(HIPRT_ERR_ARG, "too many rays in one batch");
     if (n == 0) return ok();
@@ -604,14 +683,15 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     rt::hit_soa h{nullptr, nullptr, nullptr, nullptr};
     if (hits) h = rt::hit_soa{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const rt::scene_desc desc = hiprt_detail::desc_for(sc, alpha);
     int need = sc->stack_need;
     int stack = (opts && opts->stack_config > 0 && opts->stack_config < rt::dev::kStackConfigs) ? opts->stack_config : 0;
-    if (sc->desc.bvh == rt::BVH4 && tracer == HIPRT_TRACER_WHILE_WHILE)
+    if (desc.bvh == rt::BVH4 && tracer == HIPRT_TRACER_WHILE_WHILE)
         return fail(HIPRT_ERR_UNSUPPORTED, "4-wide BVH needs a persistent tracer");
     if (tracer == HIPRT_TRACER_WHILE_WHILE) {
         if (need > rt::dev::kPlainStackCap - 1)
             return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack (" + std::to_string(need) + " entries)");
-        HIP_TRY(rt::dev::launch_trace_plain(sc->desc, r, h, occluded, n, any, s,
+        HIP_TRY(rt::dev::launch_trace_plain(desc, r, h, occluded, n, any, s,
                                             opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats)
                                                  : nullptr));
         return ok();
@@ -627,8 +707,8 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
     hiprt_stream_state *st = sc->ctx->state(stream);
     if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
-    HIP_TRY(hiprt_detail::attach_spill(st, s, sc->desc, any, false, need, cfg));
-    HIP_TRY(rt::dev::launch_trace_persistent(sc->desc, r, h, occluded, n, nullptr,
+    HIP_TRY(hiprt_detail::attach_spill(st, s, desc, any, false, need, cfg));
+    HIP_TRY(rt::dev::launch_trace_persistent(desc, r, h, occluded, n, nullptr,
                                              st->counter.as<uint32_t>(), any, cfg, s));
     return ok();
 }

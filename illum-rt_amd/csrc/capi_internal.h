@@ -92,6 +92,7 @@ struct hiprt_scene_s {
     hiprt_detail::dbuf nodes, tris, slot_ids;    // traversal data
     hiprt_detail::dbuf tri_geo, mats;            // shading data (scenes committed from a host scene)
     hiprt_detail::dbuf tri_vidx;                 // vertex indices per triangle (adjoint scatter)
+    hiprt_detail::dbuf alpha_tri, alpha_tex, alpha_mask;   // opacity data (scenes with textures)
     uint32_t n_tris = 0, n_vert = 0;
     int stack_need = 0;                          // most stack entries any traversal can hold
     hiprt_detail::dbuf refit_list;               // inner nodes by level, deepest first (refit)
@@ -108,6 +109,10 @@ std::string check_layout(rt::flat_scene &f, uint32_t &stack_need);
 
 // Fills the scene descriptor from the device buffers and the host copy.
 void set_desc(hiprt_scene_s *s, uint32_t stack_need);
+
+// The scene's descriptor for one trace: the opacity test on when `alpha`
+// is asked for and the scene has opacity data.
+rt::scene_desc desc_for(const hiprt_scene_s *sc, bool alpha);
 
 // Gives `cfg` the spill buffer a persistent launch on stream `s` needs for
 // trees whose traversal can hold `need` stack entries (none when the

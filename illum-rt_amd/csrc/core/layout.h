@@ -87,6 +87,16 @@ struct scene_desc {
     int32_t tri;              // tri_format
     int32_t depth;            // longest root-to-leaf path, in inner nodes
     int32_t nan_free;         // no node box bound is NaN (kernels may use the min/max box test)
+    // Alpha-aware tracing (the reference's *-alpha tracers): when `alpha` is
+    // set, a triangle hit counts only where the opacity of its material's
+    // texture at the hit exceeds 0.5 (alpha_opaque in device/trace_core.h).
+    // Per triangle two float4 (a.u, a.v, b.u, b.v | c.u, c.v, texture index
+    // bits or ~0u, 0); per texture (first texel, w, h, 0); one byte per texel,
+    // 1 where its alpha > 0.5.  Null when the scene has no textures.
+    const void *alpha_tri;    // float4 pairs
+    const void *alpha_tex;    // uint4
+    const uint8_t *alpha_mask;
+    int32_t alpha;
 };
 
 } // namespace rt

@@ -184,6 +184,40 @@ __device__ __forceinline__ bool tri_woop(const float4 &m0, const float4 &m1, con
 
 __device__ __forceinline__ uint32_t as_u32(float f) { return __float_as_uint(f); }
 
+// Texel of texture2d::sample (libgi/scene.h:28-36): wrap by subtracting the
+// floor, nearest texel by (int)(u * w + 0.5), index w (a coordinate that
+// rounds up to 1) wraps to 0.  Non-finite coordinates, for which the
+// reference indexes outside the texture, take texel 0.
+__host__ __device__ __forceinline__ uint32_t texel_index(float u, float v, uint32_t w, uint32_t h) {
+    u = u - floorf(u);
+    v = v - floorf(v);
+    int x = (int)(u * (float)w + 0.5f);
+    int y = (int)(v * (float)h + 0.5f);
+    if (x == (int)w) x = 0;
+    if (y == (int)h) y = 0;
+    if (!(x >= 0 && x < (int)w)) x = 0;
+    if (!(y >= 0 && y < (int)h)) y = 0;
+    return (uint32_t)y * w + (uint32_t)x;
+}
+
+// Opacity test of the reference's alpha-aware tracers (rt/cpu/bvh2.cpp:
+// 109-119 and 171-176): a hit counts when diff_geom::opacity() > 0.5
+// (libgi/rt.cpp:32-35) -- 1 without an albedo texture, else the alpha of the
+// texel at tc = (1 - beta - gamma) tc_a + beta tc_b + gamma tc_c
+// (libgi/rt.cpp:11), products rounded separately and summed left to right.
+// The mask byte holds (alpha > 0.5) for every texel.
+__device__ __forceinline__ bool alpha_opaque(const scene_desc &sc, uint32_t tri, float beta, float gamma) {
+    const float4 *at = static_cast<const float4 *>(sc.alpha_tri) + 2ull * tri;
+    float4 p = at[0], q = at[1];
+    uint32_t tex = __float_as_uint(q.z);
+    if (tex == 0xffffffffu) return true;
+    uint4 t = static_cast<const uint4 *>(sc.alpha_tex)[tex];
+    float w0 = 1.0f - beta - gamma;
+    float u = w0 * p.x + beta * p.z + gamma * q.x;
+    float v = w0 * p.y + beta * p.w + gamma * q.y;
+    return sc.alpha_mask[t.x + texel_index(u, v, t.y, t.z)] != 0;
+}
+
 template <int TRI>
 __device__ __forceinline__ bool test_slot(const scene_desc &sc, uint32_t slot, const dray &r, float best, float &t,
                                           float &u, float &v) {

@@ -219,6 +219,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
     bool exhausted = false;   // wave-uniform: every chunk's counter has run past its end
     bool finite = true;       // this lane's ray has only finite components
     bool exact = !sc.nan_free;   // wave-uniform: use the select-based box test
+    bool alpha_on = !MULTI && sc.alpha;   // opacity test on hits (batches: the lane's descriptor)
     uint32_t c_nodes = 0, c_tris = 0;
 
     for (;;) {
@@ -248,6 +249,7 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                         desc = find_desc(table, n_desc, idx);
                         const batch_desc &bd = table[desc];
                         ls = scene_of(bd.scene);
+                        alpha_on = bd.scene.alpha != 0;
                         ray_id = idx - bd.first;
                         r = load_ray(bd.rays, ray_id);
                     } else {
@@ -365,7 +367,9 @@ __global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa
                     for (uint32_t k = 0; k < cnt; ++k) {
                         float t, u, v;
                         if (COUNT) ++c_tris;
-                        if (test_rec<TRI>(ls, slot + k, r, h.t, t, u, v)) {
+                        if (test_rec<TRI>(ls, slot + k, r, h.t, t, u, v) &&
+                            (!alpha_on || alpha_opaque(MULTI ? table[desc].scene : sc,
+                                                       rec_tri_id<TRI>(ls, slot + k), u, v))) {
                             h.found = true;
                             h.slot = slot + k;
                             if (!ANY) {

@@ -51,7 +51,8 @@ __device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray 
         for (uint32_t k = 0; k < cnt; ++k) {
             float t, u, v;
             if (COUNT) ++n_tris;
-            if (test_slot<TRI>(sc, slot + k, r, h.t, t, u, v)) {
+            if (test_slot<TRI>(sc, slot + k, r, h.t, t, u, v) &&
+                (!sc.alpha || alpha_opaque(sc, slot_tri_id<TRI>(sc, slot + k), u, v))) {
                 h.found = true;
                 h.slot = slot + k;
                 if (ANY) return;

@@ -18,6 +18,15 @@ namespace rt {
 struct material {
     vec3f albedo;
     vec3f emissive;
+    int32_t tex = -1;   // albedo texture (index into scene::texs), -1: none
+};
+
+// An RGBA texture as libgi/scene.h:20-56 keeps it: w x h texels, row y at
+// texel[y * w], nearest-texel lookup with wrap.  Only the alpha channel
+// (opacity) takes part in tracing.
+struct texture {
+    uint32_t w = 0, h = 0;
+    std::vector<float> rgba;   // 4 floats per texel
 };
 
 struct scene {
@@ -25,6 +34,8 @@ struct scene {
     std::vector<uint32_t> idx;     // 3 per triangle
     std::vector<uint32_t> mat;     // material id per triangle
     std::vector<material> mats;
+    std::vector<float> tc;         // per vertex (u, v), empty when the scene has no texture coordinates
+    std::vector<texture> texs;
 
     uint32_t n_tris() const { return (uint32_t)(idx.size() / 3); }
     vec3f a(uint32_t t) const { return pos[idx[3 * t + 0]]; }

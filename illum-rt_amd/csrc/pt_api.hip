@@ -88,6 +88,7 @@ struct hiprt_pt_s {
     rt::dev::pt_params pp{};
     uint64_t seed = 0;
     int tracer = 0;
+    bool alpha = false;   // alpha-aware tracing (tracer id | HIPRT_TRACER_ALPHA)
     bool batched = false;
     rt::dev::persistent_cfg pcfg{};
     ray_planes rays[2];
@@ -135,7 +136,8 @@ struct hiprt_pt_s {
             }
             unsigned long long *lstats = stats ? stats + 2 * L : nullptr;
             if (!batched) {
-                e = launch_trace_plain(sc->desc, rays[0].out.as_in(), h, nullptr, n, false, s, lstats);
+                e = launch_trace_plain(hiprt_detail::desc_for(sc, alpha), rays[0].out.as_in(), h, nullptr, n, false, s,
+                                       lstats);
                 if (e != hipSuccess) return e;
                 if (ev) {
                     e = hipEventRecord(ev[2 * L + 1], s);
@@ -148,7 +150,8 @@ struct hiprt_pt_s {
                 cfg.stats = lstats;
                 if (!sorted) {
                     const ray_planes &in = rays[L & 1], &outq = rays[(L + 1) & 1];
-                    e = launch_trace_persistent(sc->desc, in.out.as_in(), h, nullptr, n, cnt + L,
+                    e = launch_trace_persistent(hiprt_detail::desc_for(sc, alpha), in.out.as_in(), h, nullptr, n,
+                                                cnt + L,
                                                 work.as<uint32_t>(), false, cfg, s);
                     if (e != hipSuccess) return e;
                     if (ev) {
@@ -168,7 +171,8 @@ struct hiprt_pt_s {
                     g.aos = in_rec;
                     g.perm = pm;
                     g.stride = kRayRecord;
-                    e = launch_trace_persistent(sc->desc, rays[0].out.as_in(), h, nullptr, n, cnt + L,
+                    e = launch_trace_persistent(hiprt_detail::desc_for(sc, alpha), rays[0].out.as_in(), h, nullptr,
+                                                n, cnt + L,
                                                 work.as<uint32_t>(), false, cfg, s, g);
                     if (e != hipSuccess) return e;
                     if (ev) {
@@ -326,8 +330,10 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive};
     p->seed = params->seed;
     p->batched = params->batched != 0;
-    p->tracer = params->tracer >= 0 ? params->tracer
-                                    : (p->batched ? HIPRT_TRACER_DYNAMIC_WHILE_WHILE : HIPRT_TRACER_WHILE_WHILE);
+    int tracer = params->tracer;
+    p->alpha = tracer >= 0 && (tracer & HIPRT_TRACER_ALPHA) != 0;
+    if (tracer >= 0) tracer &= ~HIPRT_TRACER_ALPHA;
+    p->tracer = tracer >= 0 ? tracer : (p->batched ? HIPRT_TRACER_DYNAMIC_WHILE_WHILE : HIPRT_TRACER_WHILE_WHILE);
     if (p->tracer >= HIPRT_TRACER_COUNT) return fail(HIPRT_ERR_ARG, "unknown tracer id");
     if (p->batched && p->tracer == HIPRT_TRACER_WHILE_WHILE)
         return fail(HIPRT_ERR_ARG, "the batched schedule needs a persistent tracer");

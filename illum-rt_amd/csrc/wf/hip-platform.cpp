@@ -276,10 +276,13 @@ platform::platform(const std::vector<std::string> &args) : wf::platform("hip") {
     for (auto &a : args) std::cerr << "Platform hip does not support the argument " << a << std::endl;
     check(hiprt_create(0, &ctx), "platform hip");
     pf = this;
-    for (int t = 0; t < HIPRT_TRACER_COUNT; ++t) {
-        std::string n = hiprt_tracer_name(t);
-        tracers[n] = [t]() -> wf::batch_ray_tracer * { return new batch_rt(t); };
-    }
+    // every tracer and its alpha-aware "-alpha" twin (rt/cuda/platform.cpp:107-122)
+    for (int a : {0, (int)HIPRT_TRACER_ALPHA})
+        for (int t = 0; t < HIPRT_TRACER_COUNT; ++t) {
+            int id = t | a;
+            std::string n = hiprt_tracer_name(id);
+            tracers[n] = [id]() -> wf::batch_ray_tracer * { return new batch_rt(id); };
+        }
     link_tracer(hiprt_tracer_name(hiprt_tracer_id("default")), "default");
     register_wf_step_by_id(, initialize_framebuffer);
     register_wf_step_by_id(, sample_camera_rays);
@@ -393,8 +396,11 @@ bool platform::interprete(const std::string &command, std::istringstream &in) {
         } else if (name == "chunks") {
             if (v != 1 && v != 8) error("Parameter error, chunks must be 1 or 8");
             opts.work_chunks = v;
+        } else if (name == "ray-order") {
+            if (v != 0 && v != 1) error("Parameter error, ray-order must be 0 (sorted) or 1 (shading order)");
+            ray_order = v;
         } else {
-            error("Unknown tracer option " << name << " (refill, blocks, stack, chunks)");
+            error("Unknown tracer option " << name << " (refill, blocks, stack, chunks, ray-order)");
         }
         return true;
     }
@@ -480,8 +486,12 @@ public:
         float cam[12];
         rc->camera(cam);
         int tracer = pf->rt ? pf->rt->tracer : -1;
-        if (tracer == HIPRT_TRACER_WHILE_WHILE) tracer = -1;   // batched needs a persistent tracer
-        hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, 0, 8, 0, 0, 0};
+        // batched needs a persistent tracer: the plain one becomes the default (keeping "-alpha")
+        if (tracer >= 0 && (tracer & ~HIPRT_TRACER_ALPHA) == HIPRT_TRACER_WHILE_WHILE)
+            tracer = HIPRT_TRACER_DYNAMIC_WHILE_WHILE | (tracer & HIPRT_TRACER_ALPHA);
+        const hiprt_tracer_opts &o = pf->opts;
+        hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, o.stack_config, o.work_chunks, o.refill_threshold, o.blocks,
+                          pf->ray_order};
         check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
         frames = 0;
     }

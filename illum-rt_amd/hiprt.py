@@ -25,6 +25,8 @@ SPLIT_OM, SPLIT_SM, SPLIT_SAH = 0, 1, 2
 
 TRACERS = ("while-while", "persistent-while-while", "persistent-speculative-while-while",
            "dynamic-while-while")
+# "<tracer>-alpha": the alpha-aware variant (hit only where texture alpha > 0.5)
+TRACER_ALPHA = 0x100
 
 # numpy view of the reference's 56-byte bbvh_node (rt/cpu/bvh.h:42-52)
 BBVH_NODE = np.dtype([("box_l", np.float32, (2, 3)), ("box_r", np.float32, (2, 3)),
@@ -92,6 +94,8 @@ def lib():
             "hiprt_scene_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
             "hiprt_scene_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
             "hiprt_scene_materials": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
+            "hiprt_scene_set_texcoords": (c_int, [c_void_p, c_void_p, c_uint32]),
+            "hiprt_scene_set_texture": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_uint32]),
             "hiprt_scene_free": (None, [c_void_p]),
             "hiprt_bvh_build": (c_int, [c_void_p, POINTER(_BvhParams), POINTER(c_void_p)]),
             "hiprt_bvh_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_uint32),
@@ -215,6 +219,22 @@ class HostScene:
         mat = np.zeros(nt.value, np.uint32)
         check(lib().hiprt_scene_copy(self.h, _ptr(pos), _ptr(idx), _ptr(mat)))
         return pos, idx, mat
+
+    def set_texcoords(self, tc):
+        """Per-vertex texture coordinates (n_vert, 2) float32 (alpha-aware tracing)."""
+        tc = np.ascontiguousarray(tc, dtype=np.float32).reshape(-1, 2)
+        check(lib().hiprt_scene_set_texcoords(self.h, _ptr(tc), len(tc)))
+
+    def set_texture(self, material, rgba):
+        """Attach an RGBA texture (h, w, 4) float32 as the albedo texture of a
+        material (None detaches it); its alpha is the opacity alpha-aware
+        tracers test."""
+        if rgba is None:
+            check(lib().hiprt_scene_set_texture(self.h, material, None, 0, 0))
+            return
+        rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+        assert rgba.ndim == 3 and rgba.shape[2] == 4, rgba.shape
+        check(lib().hiprt_scene_set_texture(self.h, material, _ptr(rgba), rgba.shape[1], rgba.shape[0]))
 
     def materials(self):
         """Material table (k, 8) float32: albedo rgb, 0, emission rgb, 0."""

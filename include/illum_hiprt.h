@@ -73,6 +73,18 @@ int hiprt_scene_copy(hiprt_host_scene s, float *pos, uint32_t *idx, uint32_t *ma
 /* Material table: 8 floats per material (albedo rgb, 0, emission rgb, 0), the
  * layout hiprt_scene_build takes; *n_materials = count (table may be NULL). */
 int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_materials);
+/* Per-vertex texture coordinates, n_vert (u, v) pairs: vertex::tc of
+ * libgi/rt.h, which the reference fills from assimp (libgi/scene.cpp:253-262).
+ * n_vert must be the scene's vertex count.  Only alpha-aware tracing reads
+ * them.  Returns HIPRT_ERR_ARG on a count mismatch. */
+int hiprt_scene_set_texcoords(hiprt_host_scene s, const float *tc, uint32_t n_vert);
+/* Attaches an RGBA texture (w*h texels of 4 floats, texel (x, y) at
+ * 4*(y*w + x)) to material `material` as its albedo texture, the
+ * material::albedo_tex that libgi/scene.cpp:340-349 loads; rgba NULL detaches
+ * it.  Alpha-aware tracers test its alpha channel (texture2d::sample's
+ * nearest texel with wrap, libgi/scene.h:28-36).  Returns HIPRT_ERR_ARG for
+ * a material id past the scene's table or an empty texture. */
+int hiprt_scene_set_texture(hiprt_host_scene s, uint32_t material, const float *rgba, uint32_t w, uint32_t h);
 void hiprt_scene_free(hiprt_host_scene s);
 
 /* ------------------------------------------------------------------ host BVH
@@ -184,8 +196,16 @@ enum {
     HIPRT_TRACER_PERSISTENT_WHILE_WHILE = 1,             /* persistent waves, one atomic per fetch */
     HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE = 2, /* + leaf postponement */
     HIPRT_TRACER_DYNAMIC_WHILE_WHILE = 3,                /* + leaf postponement + dynamic refill */
-    HIPRT_TRACER_COUNT = 4
+    HIPRT_TRACER_COUNT = 4,
+    /* OR'ed into a tracer id: its alpha-aware variant, the reference's
+     * "<tracer>-alpha" (rt/cuda/platform.cpp:107-122; CPU semantics of
+     * rt/cpu/bvh2.cpp:109-119,171-176): a triangle hit counts only where the
+     * alpha of its material's albedo texture at the hit's texture coordinate
+     * exceeds 0.5 (materials without a texture are opaque).  Scenes committed
+     * without textures trace as with the plain id. */
+    HIPRT_TRACER_ALPHA = 0x100
 };
+/* Names map to ids and back, "<name>-alpha" to id | HIPRT_TRACER_ALPHA. */
 int hiprt_tracer_id(const char *name);
 const char *hiprt_tracer_name(int tracer);
 
@@ -314,7 +334,7 @@ typedef struct {
     int32_t rr_start;    /* first level whose ray is subject to roulette (3) */
     float rr_survive;    /* survival probability (0.8) */
     uint64_t seed;
-    int32_t tracer;      /* tracer id; -1 = while-while (plain) / default (batched) */
+    int32_t tracer;      /* tracer id, may carry HIPRT_TRACER_ALPHA; -1 = while-while (plain) / default (batched) */
     int32_t batched;
     int32_t stack_config;   /* persistent tracer options (batched), as in hiprt_tracer_opts; */
     int32_t work_chunks;    /* 0 in refill_threshold / blocks selects the default */

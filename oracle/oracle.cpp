@@ -7,7 +7,8 @@
 //   ray construction         libgi/rt.h:24-31        (id = 1/d, t_min 1e-4)
 //   Shirley triangle test    libgi/intersect.h:19-70
 //   slab test intersect4     libgi/intersect.h:230-263
-//   BVH closest/any hit      rt/cpu/bvh2.cpp:72-189  (binary_bvh_tracer)
+//   BVH closest/any hit      rt/cpu/bvh2.cpp:72-189  (binary_bvh_tracer, also alpha_aware)
+//   opacity                  libgi/rt.cpp:11,32-35, libgi/scene.h:28-36
 //   brute force              rt/cpu/seq.cpp:9-30     (seq_tri_is)
 //   batch adapter            rt/cpu/wavefront.cpp:53-70 (OpenMP over rows)
 //   camera rays              libgi/camera.h:18-24, libgi/camera.cpp:44-61
@@ -117,6 +118,44 @@ bool intersect_tri(const scene_view &s, uint32_t t, const ray &ray, hit &info) {
     return false;
 }
 
+// Opacity of the alpha-aware tracers (rt/cpu/bvh2.cpp:109-119, 171-176,
+// ALPHA_THRESHOLD 0.5 of rt/cpu/bvh.h:56): diff_geom::opacity
+// (libgi/rt.cpp:32-35) is 1 without an albedo texture, else the alpha of
+// texture2d::sample (libgi/scene.h:28-36) at the interpolated texture
+// coordinate of libgi/rt.cpp:11.  Textures are RGBA float texels.
+struct alpha_view {
+    const float *tc;          // per vertex (u, v)
+    const uint32_t *mat;      // per triangle material id (null: all 0)
+    const int32_t *mat_tex;   // per material texture index, -1 none
+    uint32_t n_mats;
+    const uint32_t *tex;      // per texture (first texel, w, h)
+    const float *texels;      // 4 floats per texel
+};
+
+float opacity(const alpha_view &a, const scene_view &s, uint32_t tri, float beta, float gamma) {
+    uint32_t m = a.mat ? a.mat[tri] : 0u;
+    int32_t t = m < a.n_mats ? a.mat_tex[m] : -1;
+    if (t < 0) return 1.0f;
+    const uint32_t *ia = s.idx + 3ull * tri;
+    float w0 = 1.0f - beta - gamma;
+    float u = w0 * a.tc[2 * ia[0]] + beta * a.tc[2 * ia[1]] + gamma * a.tc[2 * ia[2]];
+    float v = w0 * a.tc[2 * ia[0] + 1] + beta * a.tc[2 * ia[1] + 1] + gamma * a.tc[2 * ia[2] + 1];
+    uint32_t first = a.tex[3 * t], w = a.tex[3 * t + 1], h = a.tex[3 * t + 2];
+    u = u - floor(u);
+    v = v - floor(v);
+    int x = (int)(u * w + 0.5f);
+    int y = (int)(v * h + 0.5f);
+    if (x == (int)w) x = 0;
+    if (y == (int)h) y = 0;
+    // Non-finite coordinates: the reference reads outside the texture; the
+    // oracle and the kernels both take texel 0.
+    if (!(x >= 0 && x < (int)w)) x = 0;
+    if (!(y >= 0 && y < (int)h)) y = 0;
+    return a.texels[4ull * (first + (uint64_t)y * w + x) + 3];
+}
+
+constexpr float kAlphaThreshold = 0.5f;
+
 // libgi/intersect.h:230-263
 bool intersect4(const box &bx, const ray &ray, float &is) {
     float idx = ray.id.x, idy = ray.id.y, idz = ray.id.z;
@@ -151,7 +190,8 @@ struct tree {
 
 // binary_bvh_tracer::closest_hit, rt/cpu/bvh2.cpp:72-137.  counts[0] += inner
 // nodes visited, counts[1] += triangles tested (when counts is non-null).
-hit closest_hit(const scene_view &s, const tree &t, const ray &ray, uint32_t *counts) {
+hit closest_hit(const scene_view &s, const tree &t, const ray &ray, uint32_t *counts,
+                const alpha_view *av = nullptr) {
     hit closest{FLT_MAX, 0, 0, 0}, intersection{FLT_MAX, 0, 0, 0};
     if (t.n_nodes == 0) return closest;
     std::vector<uint32_t> stack;
@@ -184,9 +224,13 @@ hit closest_hit(const scene_view &s, const tree &t, const ray &ray, uint32_t *co
             for (int32_t i = 0; i < cnt; ++i) {
                 uint32_t tri = t.index[off + i];
                 ++n_tri;
-                if (intersect_tri(s, tri, ray, intersection) && intersection.t < closest.t) {
-                    closest = intersection;
-                    closest.ref = tri;
+                if (intersect_tri(s, tri, ray, intersection)) {
+                    if (av && !(opacity(*av, s, tri, intersection.beta, intersection.gamma) > kAlphaThreshold))
+                        continue;
+                    if (intersection.t < closest.t) {
+                        closest = intersection;
+                        closest.ref = tri;
+                    }
                 }
             }
         }
@@ -199,7 +243,7 @@ hit closest_hit(const scene_view &s, const tree &t, const ray &ray, uint32_t *co
 }
 
 // binary_bvh_tracer::any_hit, rt/cpu/bvh2.cpp:140-189
-bool any_hit(const scene_view &s, const tree &t, const ray &ray) {
+bool any_hit(const scene_view &s, const tree &t, const ray &ray, const alpha_view *av = nullptr) {
     hit intersection{FLT_MAX, 0, 0, 0};
     if (t.n_nodes == 0) return false;
     std::vector<uint32_t> stack;
@@ -227,8 +271,12 @@ bool any_hit(const scene_view &s, const tree &t, const ray &ray) {
             }
         } else {
             int32_t off = -nd.link_l, cnt = -nd.link_r;
-            for (int32_t i = 0; i < cnt; ++i)
-                if (intersect_tri(s, t.index[off + i], ray, intersection)) return true;
+            for (int32_t i = 0; i < cnt; ++i) {
+                uint32_t tri = t.index[off + i];
+                if (intersect_tri(s, tri, ray, intersection) &&
+                    (!av || opacity(*av, s, tri, intersection.beta, intersection.gamma) > kAlphaThreshold))
+                    return true;
+            }
         }
     }
     return false;
@@ -308,6 +356,51 @@ void orc_trace_any(const float *pos, const uint32_t *idx, uint32_t n_tris, const
     for (int k = 0; k < 8; ++k) R.p[k] = ray_planes[k];
 #pragma omp parallel for schedule(static) num_threads(set_threads(threads))
     for (int64_t i = 0; i < (int64_t)n; ++i) occluded[i] = any_hit(s, tr, R.get((uint32_t)i)) ? 1 : 0;
+}
+
+// Alpha-aware batch traces (binary_bvh_tracer<..., alpha_aware = true>):
+// tc per vertex, mat per triangle (may be null), mat_tex per material (-1:
+// untextured), tex = (first texel, w, h) per texture, texels RGBA floats.
+void orc_trace_closest_alpha(const float *pos, const uint32_t *idx, uint32_t n_tris, const void *nodes,
+                             uint32_t n_nodes, uint32_t root, const uint32_t *index, const float *tc,
+                             const uint32_t *mat, const int32_t *mat_tex, uint32_t n_mats, const uint32_t *tex,
+                             const float *texels, const float *const *ray_planes, uint32_t n, float *t, float *u,
+                             float *v, uint32_t *id, int threads) {
+    scene_view s{pos, idx, n_tris};
+    tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
+    alpha_view av{tc, mat, mat_tex, n_mats, tex, texels};
+    rays_in R;
+    for (int k = 0; k < 8; ++k) R.p[k] = ray_planes[k];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(set_threads(threads))
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        hit h = closest_hit(s, tr, R.get((uint32_t)i), nullptr, &av);
+        t[i] = h.t;
+        u[i] = h.t != FLT_MAX ? h.beta : 0.0f;
+        v[i] = h.t != FLT_MAX ? h.gamma : 0.0f;
+        id[i] = h.ref;
+    }
+}
+
+void orc_trace_any_alpha(const float *pos, const uint32_t *idx, uint32_t n_tris, const void *nodes, uint32_t n_nodes,
+                         uint32_t root, const uint32_t *index, const float *tc, const uint32_t *mat,
+                         const int32_t *mat_tex, uint32_t n_mats, const uint32_t *tex, const float *texels,
+                         const float *const *ray_planes, uint32_t n, uint8_t *occluded, int threads) {
+    scene_view s{pos, idx, n_tris};
+    tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
+    alpha_view av{tc, mat, mat_tex, n_mats, tex, texels};
+    rays_in R;
+    for (int k = 0; k < 8; ++k) R.p[k] = ray_planes[k];
+#pragma omp parallel for schedule(static) num_threads(set_threads(threads))
+    for (int64_t i = 0; i < (int64_t)n; ++i) occluded[i] = any_hit(s, tr, R.get((uint32_t)i), &av) ? 1 : 0;
+}
+
+// Opacity of triangle tri[i] at barycentrics (beta[i], gamma[i]).
+void orc_opacity(const float *pos, const uint32_t *idx, uint32_t n_tris, const float *tc, const uint32_t *mat,
+                 const int32_t *mat_tex, uint32_t n_mats, const uint32_t *tex, const float *texels,
+                 const uint32_t *tri, const float *beta, const float *gamma, uint32_t n, float *out) {
+    scene_view s{pos, idx, n_tris};
+    alpha_view av{tc, mat, mat_tex, n_mats, tex, texels};
+    for (uint32_t i = 0; i < n; ++i) out[i] = opacity(av, s, tri[i], beta[i], gamma[i]);
 }
 
 // Brute force over all triangles.

@@ -80,6 +80,72 @@ def trace_any(pos, idx, nodes, root, index, rays, threads=0):
     return occ
 
 
+class Alpha:
+    """Opacity inputs of the alpha-aware tracers: per-vertex texture
+    coordinates (n_vert, 2), per-triangle material ids (or None), and a dict
+    material -> RGBA texture (h, w, 4)."""
+
+    def __init__(self, tc, mat, textures, n_mats):
+        self.tc = np.ascontiguousarray(tc, dtype=np.float32).reshape(-1, 2)
+        self.mat = None if mat is None else np.ascontiguousarray(mat, dtype=np.uint32)
+        self.mat_tex = np.full(max(n_mats, 1), -1, np.int32)
+        tex, texels, first = [], [], 0
+        for k, (m, img) in enumerate(sorted(textures.items())):
+            img = np.ascontiguousarray(img, dtype=np.float32)
+            h, w = img.shape[:2]
+            self.mat_tex[m] = k
+            tex.append((first, w, h))
+            texels.append(img.reshape(-1, 4))
+            first += w * h
+        self.tex = np.array(tex if tex else [(0, 1, 1)], np.uint32).reshape(-1, 3)
+        self.texels = np.concatenate(texels) if texels else np.zeros((1, 4), np.float32)
+        self.n_mats = n_mats
+
+    def args(self):
+        return (_p(self.tc), _p(self.mat), _p(self.mat_tex), c_uint32(self.n_mats), _p(self.tex), _p(self.texels))
+
+
+def trace_closest_alpha(pos, idx, nodes, root, index, rays, alpha, threads=0):
+    """binary_bvh_tracer<..., alpha_aware>::closest_hit. Returns t, u, v, id."""
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    t = np.empty(n, np.float32)
+    u = np.empty(n, np.float32)
+    v = np.empty(n, np.float32)
+    i = np.empty(n, np.uint32)
+    nodes = np.ascontiguousarray(nodes)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    lib().orc_trace_closest_alpha(_p(pos), _p(idx), c_uint32(len(idx)), _p(nodes), c_uint32(len(nodes)),
+                                  c_uint32(root), _p(index), *alpha.args(), planes, c_uint32(n), _p(t), _p(u), _p(v),
+                                  _p(i), c_int(threads))
+    return t, u, v, i
+
+
+def trace_any_alpha(pos, idx, nodes, root, index, rays, alpha, threads=0):
+    pos, idx = _scene(pos, idx)
+    rays, planes = _planes(rays)
+    n = rays.shape[1]
+    occ = np.empty(n, np.uint8)
+    nodes = np.ascontiguousarray(nodes)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    lib().orc_trace_any_alpha(_p(pos), _p(idx), c_uint32(len(idx)), _p(nodes), c_uint32(len(nodes)), c_uint32(root),
+                              _p(index), *alpha.args(), planes, c_uint32(n), _p(occ), c_int(threads))
+    return occ
+
+
+def opacity(pos, idx, alpha, tri, beta, gamma):
+    """diff_geom::opacity of triangle tri[i] at (beta[i], gamma[i])."""
+    pos, idx = _scene(pos, idx)
+    tri = np.ascontiguousarray(tri, dtype=np.uint32)
+    beta = np.ascontiguousarray(beta, dtype=np.float32)
+    gamma = np.ascontiguousarray(gamma, dtype=np.float32)
+    out = np.empty(len(tri), np.float32)
+    lib().orc_opacity(_p(pos), _p(idx), c_uint32(len(idx)), *alpha.args(), _p(tri), _p(beta), _p(gamma),
+                      c_uint32(len(tri)), _p(out))
+    return out
+
+
 def seq_closest(pos, idx, rays, threads=0):
     pos, idx = _scene(pos, idx)
     rays, planes = _planes(rays)

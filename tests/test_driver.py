@@ -87,7 +87,7 @@ def test_tracer_options_keep_the_image(tmp_path, torch_cuda):
     assert r0.returncode == 0 and not r0.stderr.strip(), r0.stderr
     err = r1.stderr.splitlines()
     assert err == ["command (tracer-option): Parameter error, stack configuration must be 0..3",
-                   "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks)"], err
+                   "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks, ray-order)"], err
     # primary-wf shades hits with the albedo, so the images agree unless two hits tie in t
     a, b = read_pfm(tmp_path / "a.pfm"), read_pfm(tmp_path / "b.pfm")
     assert np.array_equal(a, b)
@@ -132,10 +132,11 @@ def test_direct_wf_shadows_match_oracle(tmp_path, torch_cuda):
 
 
 @pytest.mark.gpu
-def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda):
-    print("case: rtgi-hip pt-wf, hall 128x72, 2 spp vs hiprt.PathTracer", flush=True)
+@pytest.mark.parametrize("options", ["", "tracer-option ray-order 1\ntracer-option stack 3\n"])
+def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options):
+    print(f"case: rtgi-hip pt-wf, hall 128x72, 2 spp vs hiprt.PathTracer, options {options!r}", flush=True)
     w, h = 128, 72
-    r = run_script(SCRIPT_HEAD.format(cfg=3, w=w, h=h, tracer="dynamic-while-while") +
+    r = run_script(SCRIPT_HEAD.format(cfg=3, w=w, h=h, tracer="dynamic-while-while") + options +
                    "sppx 2\nalgo pt-wf\noutfile pt.pfm\nrun\n", tmp_path, timeout=300)
     assert r.returncode == 0 and not r.stderr.strip(), r.stderr
     img = read_pfm(tmp_path / "pt.pfm")
