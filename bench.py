@@ -148,6 +148,30 @@ def bench_device_build(torch, hiprt, ctx, stream, sc, d_diff, host_build_s):
     return out
 
 
+def bench_alpha(torch, hiprt, ctx, stream, d_cam, d_diff):
+    """Alpha-aware tracing on the hall: every material gets a 16 x 16 texture
+    whose alpha alternates 0.2 / 0.9 (a cut-out checkerboard), texture
+    coordinates from the vertex position (x / 2, z / 2).  The same rays are
+    traced with the plain and the alpha-aware dynamic tracer."""
+    import numpy as np
+    sc = hiprt.HostScene.generate(3)
+    pos, _, _ = sc.arrays()
+    sc.set_texcoords(np.stack([pos[:, 0] * 0.5, pos[:, 2] * 0.5], 1))
+    y, x = np.mgrid[0:16, 0:16]
+    img = np.ones((16, 16, 4), np.float32)
+    img[..., 3] = np.where((x + y) % 2 == 0, 0.2, 0.9)
+    for m in range(len(sc.materials())):
+        sc.set_texture(m, img)
+    ds = hiprt.DeviceScene(ctx, sc, hiprt.Bvh(sc))
+    out = {"texture": "16x16 checkerboard alpha 0.2/0.9 on every material"}
+    for name, rays in (("primary", d_cam), ("diffuse", d_diff)):
+        for tracer in ("dynamic-while-while", "dynamic-while-while-alpha"):
+            ms, _ = time_trace(torch, hiprt, ds, rays, tracer, stream)
+            out[f"{name}/{tracer}"] = round(rays.shape[1] / ms / 1e3, 1)
+    ds.close()
+    return out
+
+
 def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
     """64 random-triangle scenes of log-uniform size with their own rays: one
     descriptor-table launch against one launch per scene, both with the
@@ -206,7 +230,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the shadow-ray and 64-scene legs")
+                    help="skip the device-build, shadow-ray, 64-scene and alpha legs")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     ap.add_argument("--kernel-only", choices=("primary", "diffuse"),
@@ -372,6 +396,8 @@ def main():
         log(f"shadow rays: {extra['shadow_any_hit_mrays']}")
         extra["multi_scene"] = bench_multi_scene(torch, hiprt, ctx, stream)
         log(f"multi-scene: {extra['multi_scene']}")
+        extra["alpha"] = bench_alpha(torch, hiprt, ctx, stream, d_cam, d_diff)
+        log(f"alpha-aware tracing: {extra['alpha']}")
 
     cpu = None
     if not args.no_cpu:
