@@ -204,6 +204,16 @@ int hiprt_scene_set_texcoords(hiprt_host_scene s, const float *tc, uint32_t n_ve
     return ok();
 }
 
+int hiprt_scene_get_texcoords(hiprt_host_scene s, float *tc, uint32_t *n_vert) {
+    if (!s) return fail(HIPRT_ERR_ARG, "null scene");
+    const size_t nv = s->sc.pos.size();
+    if (n_vert) *n_vert = (uint32_t)nv;
+    if (tc) {
+        for (size_t i = 0; i < 2 * nv; ++i) tc[i] = i < s->sc.tc.size() ? s->sc.tc[i] : 0.0f;
+    }
+    return ok();
+}
+
 int hiprt_scene_set_texture(hiprt_host_scene s, uint32_t material, const float *rgba, uint32_t w, uint32_t h) {
     if (!s) return fail(HIPRT_ERR_ARG, "null scene");
     if (material >= s->sc.mats.size()) return fail(HIPRT_ERR_ARG, "material id out of range");

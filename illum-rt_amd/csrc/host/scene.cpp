@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <map>
 #include <sstream>
 
 namespace rt {
@@ -299,6 +300,8 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
     }
     out = scene{};
     std::vector<vec3f> normals;
+    std::vector<float> uvs;                 // "vt" entries, (u, v) pairs
+    std::vector<long> corner_vt;            // per triangle corner: vt index, -1 none
     std::vector<std::string> mtl_names;
     std::vector<mtl_entry> mtl;
     bool legacy = false;        // "usemtl m<id>" without a material library (write_obj's files)
@@ -322,19 +325,36 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
                 return false;
             }
             (tag == "v" ? out.pos : normals).push_back(vec3f{x, y, z});
+        } else if (tag == "vt") {
+            float u, v = 0.0f;
+            if (!(ls >> u)) {
+                err = where() + "malformed vt";
+                return false;
+            }
+            ls >> v;   // a 1-component coordinate has v = 0
+            uvs.push_back(u);
+            uvs.push_back(v);
         } else if (tag == "f") {
-            std::vector<long> ids, nids;
+            std::vector<long> ids, nids, tids;
             std::string tok;
             while (ls >> tok) {
                 // "a", "a/b", "a//c", "a/b/c" (1-based, negative = relative)
-                long v = std::strtol(tok.c_str(), nullptr, 10), n = 0;
+                long v = std::strtol(tok.c_str(), nullptr, 10), n = 0, tx = 0;
                 size_t s1 = tok.find('/');
                 if (s1 != std::string::npos) {
                     size_t s2 = tok.find('/', s1 + 1);
                     if (s2 != std::string::npos && s2 + 1 < tok.size()) n = std::strtol(tok.c_str() + s2 + 1, nullptr, 10);
+                    if (s1 + 1 < tok.size() && tok[s1 + 1] != '/') tx = std::strtol(tok.c_str() + s1 + 1, nullptr, 10);
                 }
                 if (v < 0) v = (long)out.pos.size() + v + 1;
                 if (n < 0) n = (long)normals.size() + n + 1;
+                if (uvs.empty()) tx = 0;   // a file without "vt" lines: corner coordinate indices are ignored
+                if (tx < 0) tx = (long)(uvs.size() / 2) + tx + 1;
+                if (tx < 0 || (size_t)tx > uvs.size() / 2) {
+                    err = where() + "texture coordinate index out of range";
+                    return false;
+                }
+                tids.push_back(tx - 1);   // -1: no texture coordinate
                 if (v <= 0 || (size_t)v > out.pos.size()) {
                     err = where() + "face index out of range";
                     return false;
@@ -350,15 +370,22 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
             // every corner has one (libgi/scene.cpp:264-277)
             for (size_t k = 2; k < ids.size(); ++k) {
                 uint32_t a = (uint32_t)ids[0], b = (uint32_t)ids[k - 1], c = (uint32_t)ids[k];
+                long ta = tids[0], tb = tids[k - 1], tcn = tids[k];
                 if (nids[0] >= 0 && nids[k - 1] >= 0 && nids[k] >= 0) {
                     vec3f pa = out.pos[a], pb = out.pos[b], pc = out.pos[c];
                     vec3f g = cross(pb - pa, pc - pa);
                     vec3f m = (normals[nids[0]] + normals[nids[k - 1]] + normals[nids[k]]) * 0.333f;
-                    if (!(dot(g, m) > 0)) std::swap(b, c);
+                    if (!(dot(g, m) > 0)) {
+                        std::swap(b, c);
+                        std::swap(tb, tcn);
+                    }
                 }
                 out.idx.push_back(a);
                 out.idx.push_back(b);
                 out.idx.push_back(c);
+                corner_vt.push_back(ta);
+                corner_vt.push_back(tb);
+                corner_vt.push_back(tcn);
                 out.mat.push_back(cur_mat);
                 tri_named.push_back(cur_named);
             }
@@ -379,6 +406,29 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
             } else {
                 cur_mat = 0;
             }
+        }
+    }
+    if (!uvs.empty()) {
+        // Texture coordinates belong to face corners (assimp gives every
+        // corner its own vertex, libgi/scene.cpp:251-262): each distinct
+        // (position, vt) pair becomes a vertex of its own after the "v"
+        // vertices, which keep (0, 0) for corners without a coordinate.
+        const size_t nv = out.pos.size();
+        out.tc.assign(2 * nv, 0.0f);
+        std::map<std::pair<uint32_t, long>, uint32_t> split;
+        for (size_t k = 0; k < out.idx.size(); ++k) {
+            long t = corner_vt[k];
+            if (t < 0) continue;
+            auto key = std::make_pair(out.idx[k], t);
+            auto it = split.find(key);
+            if (it == split.end()) {
+                uint32_t id = (uint32_t)out.pos.size();
+                out.pos.push_back(out.pos[out.idx[k]]);
+                out.tc.push_back(uvs[2 * t]);
+                out.tc.push_back(uvs[2 * t + 1]);
+                it = split.emplace(key, id).first;
+            }
+            out.idx[k] = it->second;
         }
     }
     if (!mtl.empty()) {

@@ -95,6 +95,7 @@ def lib():
             "hiprt_scene_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
             "hiprt_scene_materials": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_set_texcoords": (c_int, [c_void_p, c_void_p, c_uint32]),
+            "hiprt_scene_get_texcoords": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_set_texture": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_uint32]),
             "hiprt_scene_free": (None, [c_void_p]),
             "hiprt_bvh_build": (c_int, [c_void_p, POINTER(_BvhParams), POINTER(c_void_p)]),
@@ -224,6 +225,14 @@ class HostScene:
         """Per-vertex texture coordinates (n_vert, 2) float32 (alpha-aware tracing)."""
         tc = np.ascontiguousarray(tc, dtype=np.float32).reshape(-1, 2)
         check(lib().hiprt_scene_set_texcoords(self.h, _ptr(tc), len(tc)))
+
+    def texcoords(self):
+        """Per-vertex texture coordinates (n_vert, 2) float32 (zeros when none)."""
+        nv = c_uint32()
+        check(lib().hiprt_scene_get_texcoords(self.h, None, ctypes.byref(nv)))
+        tc = np.zeros((nv.value, 2), np.float32)
+        check(lib().hiprt_scene_get_texcoords(self.h, _ptr(tc), ctypes.byref(nv)))
+        return tc
 
     def set_texture(self, material, rgba):
         """Attach an RGBA texture (h, w, 4) float32 as the albedo texture of a

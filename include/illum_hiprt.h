@@ -78,6 +78,11 @@ int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_material
  * n_vert must be the scene's vertex count.  Only alpha-aware tracing reads
  * them.  Returns HIPRT_ERR_ARG on a count mismatch. */
 int hiprt_scene_set_texcoords(hiprt_host_scene s, const float *tc, uint32_t n_vert);
+/* The per-vertex texture coordinates (2 floats per vertex; zeros for a scene
+ * without them); *n_vert = the vertex count; tc may be NULL.  An OBJ file's
+ * "vt" coordinates arrive here: every distinct (position, vt) corner pair is
+ * a vertex of its own, after the file's "v" vertices. */
+int hiprt_scene_get_texcoords(hiprt_host_scene s, float *tc, uint32_t *n_vert);
 /* Attaches an RGBA texture (w*h texels of 4 floats, texel (x, y) at
  * 4*(y*w + x)) to material `material` as its albedo texture, the
  * material::albedo_tex that libgi/scene.cpp:340-349 loads; rgba NULL detaches

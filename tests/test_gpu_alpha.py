@@ -146,3 +146,35 @@ def test_path_tracer_alpha(torch_cuda, alpha_scene):
         pt.close()
     assert counts["plain"] == counts["batched"], counts
     assert counts["batched"] != counts["batched, no alpha"], counts
+
+
+def test_obj_texture_coordinates_drive_alpha(torch_cuda, hip_ctx, tmp_path):
+    """A textured OBJ (per-corner "vt") traced alpha-aware equals the oracle
+    fed with the coordinates the reader assigned."""
+    torch = torch_cuda
+    base = hiprt.HostScene.random(3000, 0.03, 31)
+    pos, idx, _ = base.arrays()
+    rng = np.random.default_rng(31)
+    uv = rng.uniform(-1.0, 2.0, (3 * len(idx), 2)).astype(np.float32)
+    lines = [f"v {x!r} {y!r} {z!r}" for x, y, z in pos.tolist()]
+    lines += [f"vt {u!r} {v!r}" for u, v in uv.tolist()]
+    lines += [f"f {a + 1}/{3 * t + 1} {b + 1}/{3 * t + 2} {c + 1}/{3 * t + 3}" for t, (a, b, c) in enumerate(idx.tolist())]
+    (tmp_path / "tex.obj").write_text("\n".join(lines) + "\n")
+    sc = hiprt.HostScene.load_obj(tmp_path / "tex.obj")
+    p2, i2, m2 = sc.arrays()
+    tc = sc.texcoords()
+    assert len(p2) == len(pos) + 3 * len(idx)
+    assert np.array_equal(tc[i2.reshape(-1)], uv)   # every corner carries its own coordinate
+    tex = _textures()
+    sc.set_texture(0, tex[0])
+    bvh = hiprt.Bvh(sc)
+    ds = hiprt.DeviceScene(hip_ctx, sc, bvh)
+    rays = uniform_sphere_rays(50_000, 32)
+    a = orc.Alpha(tc, m2, {0: tex[0]}, 1)
+    want = orc.trace_closest_alpha(p2, i2, bvh.nodes, bvh.root, bvh.index, rays, a)
+    print("case: textured OBJ, persistent-while-while-alpha", flush=True)
+    got = _closest(torch, ds, rays, "persistent-while-while-alpha")
+    assert bit_exact(got, want)
+    plain = orc.trace_closest(p2, i2, bvh.nodes, bvh.root, bvh.index, rays)
+    assert (want[3] != plain[3]).mean() > 0.05
+    ds.close()

@@ -116,6 +116,25 @@ def test_batched_equals_plain(torch_cuda, hall, res, stack, chunks, sort):
     assert np.array_equal(img_p.view(np.uint32), img_b.view(np.uint32))
 
 
+@pytest.mark.parametrize("res,depth", [((13, 7), 4), ((64, 40), 0), ((64, 40), 1), ((47, 45), 2)])
+def test_small_frames_and_depths(torch_cuda, hall, res, depth):
+    """Queues shorter than one sort tile, no bounce at all, and a single bounce:
+    the sorted batched schedule still equals the plain one."""
+    print(f"case: hall path tracer {res}, max_depth {depth}, plain vs sorted batched", flush=True)
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = res
+    imgs, counts = [], []
+    for batched in (False, True):
+        pt = hiprt.PathTracer(hall["ds"], cam, max_depth=depth, batched=batched, seed=11)
+        pt.render(0)
+        pt.render(1)
+        imgs.append(pt.image())
+        counts.append(pt.ray_counts().tolist())
+        pt.close()
+    assert counts[0] == counts[1]
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+
+
 def test_graph_replays_are_stable(torch_cuda, hall):
     print("case: hall path tracer 256x144, batched graph replayed 4x vs eager", flush=True)
     cam = hall["cam"].copy()

@@ -114,6 +114,41 @@ def test_obj_winding_follows_shading_normals(tmp_path):
     assert idx.tolist() == [[0, 1, 2], [0, 2, 1], [0, 2, 1], [0, 1, 2]]
 
 
+def test_obj_texture_coordinates_per_corner(tmp_path):
+    """"vt" coordinates belong to face corners: each distinct (position, vt)
+    pair becomes its own vertex after the file's vertices (positions copied),
+    corners without one keep the shared vertex with (0, 0); relative vt
+    indices count back from the last "vt"; the winding swap moves the
+    coordinates with their corners."""
+    (tmp_path / "t.obj").write_text(
+        "v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\n"
+        "vt 0.25 0.5\nvt 0.75 0.5\nvt 0.5 1.5\nvn 0 0 -1\n"
+        "f 1/1 2/2 3/3\n"            # three new vertices 4, 5, 6
+        "f 2/2 4 3/-1\n"             # (2, vt2) reused, 4 plain, (3, vt3) reused
+        "f 1/1/1 2/2/1 3/3/1\n")     # shading -z: b and c swap, with their coordinates
+    sc = hiprt.HostScene.load_obj(tmp_path / "t.obj")
+    pos, idx, _ = sc.arrays()
+    tc = sc.texcoords()
+    assert len(pos) == 7 and idx.tolist() == [[4, 5, 6], [5, 3, 6], [4, 6, 5]]
+    assert np.array_equal(pos[4:], pos[[0, 1, 2]])
+    assert tc.tolist() == [[0, 0], [0, 0], [0, 0], [0, 0], [0.25, 0.5], [0.75, 0.5], [0.5, 1.5]]
+    (tmp_path / "b.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nf 1/1 2/2 3/1\n")
+    with pytest.raises(hiprt.HiprtError, match="texture coordinate index"):
+        hiprt.HostScene.load_obj(tmp_path / "b.obj")
+
+
+def test_texture_api_checks_its_arguments():
+    sc = hiprt.HostScene.random(10, 0.1, 1)
+    with pytest.raises(hiprt.HiprtError, match="vertex count"):
+        sc.set_texcoords(np.zeros((7, 2)))
+    with pytest.raises(hiprt.HiprtError, match="material id"):
+        sc.set_texture(5, np.ones((2, 2, 4)))
+    sc.set_texcoords(np.full((30, 2), 0.5))
+    assert (sc.texcoords() == 0.5).all()
+    sc.set_texture(0, np.ones((2, 3, 4)))
+    sc.set_texture(0, None)
+
+
 def test_obj_rejects_bad_normal_index(tmp_path):
     (tmp_path / "b.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//2\n")
     with pytest.raises(hiprt.HiprtError, match="normal index"):
