@@ -303,9 +303,12 @@ def main():
     # pass and reads a sorted queue.  So the frame's configuration is chosen
     # on warm-up frames (never the timed ones): the best of the autotuned
     # one, its one-counter variant, the 4-wide tree and the binary tree with
-    # stack configuration 3, when it beats the library defaults (binary tree,
-    # dynamic tracer, stack 0, eight work chunks) by more than 2%, the
-    # defaults otherwise, which keeps the choice stable between runs.  The
+    # stack configuration 3, and the defaults with 2, 4 or 8 samples per pixel in
+    # flight (consecutive frames' samples traced together, so every launch
+    # holds 2-8x the rays), when it beats the library defaults (binary tree,
+    # dynamic tracer, stack 0, eight work chunks, one sample) by more than 2%
+    # in rays per second, the defaults otherwise, which keeps the choice
+    # stable between runs.  The
     # autotuner's sample is the first bounce; the deeper bounces weigh the
     # stack configurations and layouts differently, so the fixed candidates
     # are tried on whole frames too.
@@ -314,16 +317,24 @@ def main():
                   "autotuned, one work counter": (ds_tuned, dict(pt_kw, work_chunks=1)),
                   "4-wide tree, stack 3": (ds4, dict(tracer="dynamic-while-while", stack_config=3)),
                   "binary tree, stack 3": (ds, dict(tracer="dynamic-while-while", stack_config=3)),
+                  "2 samples in flight": (ds, dict(samples=2)),
+                  "4 samples in flight": (ds, dict(samples=4)),
+                  "8 samples in flight": (ds, dict(samples=8)),
                   "library defaults": (ds, {})}
-    frame_ms = {}
+    # Rates, not frame times: a render call with k samples in flight traces k
+    # samples per pixel.
+    trial = {}
     for name, (dsc, kw) in candidates.items():
         pt = hiprt.PathTracer(dsc, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, **kw)
-        frame_ms[name], _, _ = time_frames(torch, pt, 4, 1, stream)
+        ms, rays, _ = time_frames(torch, pt, 4, 1, stream)
+        trial[name] = rays / ms / 1e3
         pt.close()
-    best_tuned = min((k for k in candidates if k != "library defaults"), key=frame_ms.get)
-    choice = best_tuned if frame_ms[best_tuned] < 0.98 * frame_ms["library defaults"] else "library defaults"
-    log(f"frame configuration trial (ms/frame): {frame_ms}; timed frames use {choice}")
+    best_tuned = max((k for k in candidates if k != "library defaults"), key=trial.get)
+    choice = best_tuned if trial[best_tuned] > 1.02 * trial["library defaults"] else "library defaults"
+    one_sample = max(v for k, v in trial.items() if "samples" not in candidates[k][1])
+    log(f"frame configuration trial (Mrays/s): {trial}; timed frames use {choice}")
     ds_frame, frame_kw = candidates[choice]
+    samples = frame_kw.get("samples", 1)
 
     batched = hiprt.PathTracer(ds_frame, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True,
                                **frame_kw)
@@ -449,17 +460,20 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "hall 262,144 triangles, 1920x1080, 4-bounce cosine path tracing, "
-                               "RR p=0.8 at bounces 3-4, batched persistent+graph",
+                               "RR p=0.8 at bounces 3-4, batched persistent+graph, "
+                               f"{samples} sample(s) per pixel per step",
                    "triangles": int(len(idx)), "width": w, "height": h, "max_depth": 4,
                    "bvh": "SAH 16 planes, 4 tris/leaf, ESC",
                    "layout": "bvh2" if ds_frame.bvh_format == hiprt.BVH2 else "bvh4",
-                   "rays_per_frame": int(round(rays_b))},
+                   "samples_per_step": samples,
+                   "rays_per_step": int(round(rays_b))},
         "plain_value": round(plain_value, 1),
         "plain_ms_per_step": round(ms_p, 4),
         "batched_over_plain": round(value / plain_value, 3),
         "trace_kernels_mrays": kernels,
         "autotuned": tuned,
-        "frame_config_trial_ms": {k: round(v, 4) for k, v in frame_ms.items()},
+        "frame_config_trial_mrays": {k: round(v, 1) for k, v in trial.items()},
+        "one_sample_in_flight_mrays": round(one_sample, 1),
         "frame_config": dict(frame_kw, layout="bvh2" if ds_frame.bvh_format == hiprt.BVH2 else "bvh4")
                         if frame_kw else "library defaults",
 

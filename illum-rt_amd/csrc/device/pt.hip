@@ -51,7 +51,7 @@ struct shade_out {
 };
 
 __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_params &pp, uint32_t path,
-                                               int level, uint64_t frame_seed, vec3f in_dir, float t, float beta,
+                                               int level, const uint64_t *frame_seed, vec3f in_dir, float t, float beta,
                                                float gamma, uint32_t tri, vec3f &T, float *rad) {
     shade_out so{false, v3(0.0f), v3(0.0f)};
     if (t == FLT_MAX) return so;
@@ -66,7 +66,8 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
         rad[3 * path + 2] += T.z * emi.z;
     }
     if (level >= pp.max_depth) return so;
-    pcg32 rng(pcg_stream_id(frame_seed, path, (uint64_t)level));
+    const uint32_t sample = path / pp.n_px, pixel = path - sample * pp.n_px;
+    pcg32 rng(pcg_stream_id(frame_seed[sample], pixel, (uint64_t)level));
     float xi0 = rng.next_float(), xi1 = rng.next_float();
     float scale = 1.0f;
     if (level + 1 >= pp.rr_start) {
@@ -119,18 +120,43 @@ __device__ __forceinline__ uint32_t block_append(bool live, uint32_t *count, uin
 
 } // namespace
 
-// Camera rays for every pixel; path i = pixel i.  Also resets throughput (when
-// thr is non-null), the accumulator (when `clear_radiance`) and the per-path
-// id table (when path_ids is non-null).
+// Camera rays for n paths; path i looks through pixel i % n_px.  Also resets
+// throughput (when thr is non-null), the per-pixel accumulator (when
+// `clear_radiance`), the per-path radiance (when rad_paths is non-null) and
+// the per-path id table (when path_ids is non-null).
 __global__ __launch_bounds__(256) void k_camera_rays(cam_frame f, ray_out out, uint32_t *path_ids, float *thr,
-                                                    float *rad, int clear_radiance, uint32_t n) {
+                                                    float *rad, int clear_radiance, uint32_t n, uint32_t n_px,
+                                                    float *rad_paths) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    int x = (int)(i % (uint32_t)f.w), y = (int)(i / (uint32_t)f.w);
+    uint32_t px = i % n_px;
+    int x = (int)(px % (uint32_t)f.w), y = (int)(px / (uint32_t)f.w);
     write_ray(out, i, f.pos, kRayEps, cam_dir(f, x, y), FLT_MAX);
     if (path_ids) path_ids[i] = i;
-    if (thr) thr[3 * i] = thr[3 * i + 1] = thr[3 * i + 2] = 1.0f;
-    if (clear_radiance) rad[3 * i] = rad[3 * i + 1] = rad[3 * i + 2] = 0.0f;
+    if (thr) thr[3ull * i] = thr[3ull * i + 1] = thr[3ull * i + 2] = 1.0f;
+    if (rad_paths) rad_paths[3ull * i] = rad_paths[3ull * i + 1] = rad_paths[3ull * i + 2] = 0.0f;
+    if (clear_radiance && i < n_px) rad[3ull * i] = rad[3ull * i + 1] = rad[3ull * i + 2] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_add_samples(float *image, const float *rad_paths, uint32_t n_px,
+                                                    uint32_t samples) {
+    uint32_t px = blockIdx.x * 256u + threadIdx.x;
+    if (px >= n_px) return;
+    float r = image[3ull * px], g = image[3ull * px + 1], b = image[3ull * px + 2];
+    for (uint32_t k = 0; k < samples; ++k) {
+        const float *q = rad_paths + 3ull * ((uint64_t)k * n_px + px);
+        r += q[0];
+        g += q[1];
+        b += q[2];
+    }
+    image[3ull * px] = r;
+    image[3ull * px + 1] = g;
+    image[3ull * px + 2] = b;
+}
+
+__global__ void k_set_seeds(uint64_t *p, uint64_t base, uint32_t n) {
+    uint32_t k = blockIdx.x * 64u + threadIdx.x;
+    if (k < n) p[k] = base + k;
 }
 
 // Plain shading: slot i is path i; the next ray replaces this one in place.
@@ -144,7 +170,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_plain(pt_scene ps, pt_par
     if (i < n && rays.tmax[i] >= 0.0f) {
         vec3f in_dir{rays.dx[i], rays.dy[i], rays.dz[i]};
         vec3f T = load3(thr, i);
-        shade_out so = shade_one(ps, pp, i, level, *frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i],
+        shade_out so = shade_one(ps, pp, i, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i],
                                  hits.id[i], T, rad);
         live = so.live;
         if (live) store3(thr, i, T);
@@ -172,7 +198,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
         path = path_in[i];
         vec3f in_dir{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
         vec3f T = load3(thr, path);
-        so = shade_one(ps, pp, path, level, *frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
+        so = shade_one(ps, pp, path, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
                        rad);
         live = so.live;
         if (live) store3(thr, path, T);
@@ -260,7 +286,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
             in_dir = vec3f{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
             T = load3(thr, path);
         }
-        so = shade_one(ps, pp, path, level, *frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
+        so = shade_one(ps, pp, path, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
                        rad);
         live = so.live;
     }
@@ -317,10 +343,21 @@ __global__ void k_set_u64(uint64_t *p, uint64_t v) {
 }
 
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
-                              bool clear_radiance, hipStream_t s) {
-    uint32_t n = (uint32_t)f.w * (uint32_t)f.h;
+                              bool clear_radiance, hipStream_t s, uint32_t n_paths, float *rad_paths) {
+    uint32_t n_px = (uint32_t)f.w * (uint32_t)f.h;
+    uint32_t n = n_paths ? n_paths : n_px;
     hipLaunchKernelGGL(k_camera_rays, dim3((n + 255) / 256), dim3(256), 0, s, f, out, path_ids, thr, rad,
-                       clear_radiance ? 1 : 0, n);
+                       clear_radiance ? 1 : 0, n, n_px, rad_paths);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_samples, dim3((n_px + 255) / 256), dim3(256), 0, s, image, rad_paths, n_px, samples);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_seeds(uint64_t *p, uint64_t base, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_seeds, dim3((n + 63) / 64), dim3(64), 0, s, p, base, n);
     return hipGetLastError();
 }
 

@@ -29,10 +29,21 @@ struct pt_params {
     int max_depth;      // last trace level (primary = 0)
     int rr_start;       // first level whose ray is subject to Russian roulette
     float rr_survive;   // survival probability
+    // Pixels per sample: path p is pixel p % n_px of sample p / n_px, whose
+    // frame seed is frame_seed[p / n_px] (several samples in flight at once).
+    uint32_t n_px;
 };
 
+// Camera rays for n_paths paths (0: one per pixel), path i looking through
+// pixel i % (w*h); resets path ids and throughput (when non-null), clears
+// the image accumulator `rad` (per pixel, when clear_radiance) and the
+// per-path radiance `rad_paths` (when non-null).
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
-                              bool clear_radiance, hipStream_t s);
+                              bool clear_radiance, hipStream_t s, uint32_t n_paths = 0, float *rad_paths = nullptr);
+// image[pixel] += rad_paths[k * n_px + pixel] for k = 0 .. samples-1 in order.
+hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s);
+// p[k] = base + k for k < n (the frame seeds of consecutive samples).
+hipError_t launch_set_seeds(uint64_t *p, uint64_t base, uint32_t n, hipStream_t s);
 hipError_t launch_shade_plain(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,
                               const ray_out &rays, const hit_soa &hits, float *thr, float *rad, uint32_t *counts,
                               uint32_t n, hipStream_t s);

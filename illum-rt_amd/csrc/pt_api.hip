@@ -93,6 +93,11 @@ struct hiprt_pt_s {
     rt::dev::persistent_cfg pcfg{};
     ray_planes rays[2];
     dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work, spill;
+    // samples in flight: n = samples * n_px paths; with more than one the
+    // paths accumulate radiance in rad_paths, added to `rad` (per pixel) at the
+    // end of the frame in sample order
+    uint32_t samples = 1, n_px = 0;
+    dbuf rad_paths;
     // sorted bounce queues: ray records (ray, path id, throughput) per level
     // parity, keys, the permutation, and the radix sort's scratch
     bool sorted = false;
@@ -125,8 +130,9 @@ struct hiprt_pt_s {
         if (e != hipSuccess) return e;
         e = launch_fill_u32(cnt, n, 1u, s);
         if (e != hipSuccess) return e;
+        float *prad = samples > 1 ? rad_paths.as<float>() : rad.as<float>();   // radiance per path
         e = launch_camera_rays(f, rays[0].out, batched ? paths[0].as<uint32_t>() : nullptr, thr.as<float>(),
-                               rad.as<float>(), clear_radiance, s);
+                               rad.as<float>(), clear_radiance, s, n, samples > 1 ? prad : nullptr);
         if (e != hipSuccess) return e;
         const uint64_t *fs = frame_seed.as<const uint64_t>();
         for (int L = 0; L < levels; ++L) {
@@ -143,7 +149,7 @@ struct hiprt_pt_s {
                     e = hipEventRecord(ev[2 * L + 1], s);
                     if (e != hipSuccess) return e;
                 }
-                e = launch_shade_plain(ps(), pp, L, fs, rays[0].out, h, thr.as<float>(), rad.as<float>(), cnt, n, s);
+                e = launch_shade_plain(ps(), pp, L, fs, rays[0].out, h, thr.as<float>(), prad, cnt, n, s);
             } else {
                 // persistent tracers read the queue length from counts[L]
                 persistent_cfg cfg = pcfg;
@@ -159,8 +165,8 @@ struct hiprt_pt_s {
                         if (e != hipSuccess) return e;
                     }
                     e = launch_shade_compact(ps(), pp, L, fs, in.out, h, paths[L & 1].as<const uint32_t>(),
-                                             outq.out, paths[(L + 1) & 1].as<uint32_t>(), thr.as<float>(),
-                                             rad.as<float>(), cnt, n, s);
+                                             outq.out, paths[(L + 1) & 1].as<uint32_t>(), thr.as<float>(), prad,
+                                             cnt, n, s);
                 } else {
                     // level 0: camera ray planes in pixel order; later levels:
                     // the previous shade step's records through the sorted
@@ -181,7 +187,7 @@ struct hiprt_pt_s {
                     }
                     e = launch_shade_stage(ps(), pp, L, fs, rays[0].out, paths[0].as<const uint32_t>(),
                                            thr.as<const float>(), in_rec, pm, h, rec[L & 1].as<float4>(),
-                                           keys.as<uint32_t>(), rad.as<float>(), cnt, grid, n, s);
+                                           keys.as<uint32_t>(), prad, cnt, grid, n, s);
                     if (e == hipSuccess && L < pp.max_depth) {
                         radix_sort_buffers rb{keys.as<uint32_t>(),      keys_tmp.as<uint32_t>(),
                                               vals_tmp.as<uint32_t>(),  perm.as<uint32_t>(),
@@ -190,6 +196,10 @@ struct hiprt_pt_s {
                     }
                 }
             }
+            if (e != hipSuccess) return e;
+        }
+        if (samples > 1) {
+            e = launch_add_samples(rad.as<float>(), rad_paths.as<const float>(), n_px, samples, s);
             if (e != hipSuccess) return e;
         }
         return launch_add_counts(totals.as<unsigned long long>(), cnt, levels, s);
@@ -326,8 +336,13 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     if (!p) return fail(HIPRT_ERR_ARG, "out of host memory");
     p->sc = sc;
     p->f = frame_from(cam);
-    p->n = (uint32_t)npix;
-    p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive};
+    const uint32_t samples = params->samples > 1 ? (uint32_t)params->samples : 1u;
+    if (samples > 1 && !params->batched) return fail(HIPRT_ERR_ARG, "samples in flight need the batched schedule");
+    if (npix * samples > 0x40000000) return fail(HIPRT_ERR_ARG, "image too large for that many samples in flight");
+    p->n_px = (uint32_t)npix;
+    p->samples = samples;
+    p->n = (uint32_t)npix * samples;
+    p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive, (uint32_t)npix};
     p->seed = params->seed;
     p->batched = params->batched != 0;
     int tracer = params->tracer;
@@ -358,13 +373,14 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     float *hb = p->hits.as<float>();
     p->h = rt::hit_soa{hb, hb + n, hb + 2ull * n, reinterpret_cast<uint32_t *>(hb + 3ull * n)};
     HIP_TRY(p->thr.alloc(12ull * n));
-    HIP_TRY(p->rad.alloc(12ull * n));
-    HIP_TRY(hipMemset(p->rad.p, 0, 12ull * n));
+    HIP_TRY(p->rad.alloc(12ull * p->n_px));
+    HIP_TRY(hipMemset(p->rad.p, 0, 12ull * p->n_px));
+    if (samples > 1) HIP_TRY(p->rad_paths.alloc(12ull * n));
     HIP_TRY(p->counts.alloc(4ull * (p->pp.max_depth + 2)));
     HIP_TRY(hipMemset(p->counts.p, 0, 4ull * (p->pp.max_depth + 2)));
     HIP_TRY(p->totals.alloc(8ull * (p->pp.max_depth + 1)));
     HIP_TRY(hipMemset(p->totals.p, 0, 8ull * (p->pp.max_depth + 1)));
-    HIP_TRY(p->frame_seed.alloc(8));
+    HIP_TRY(p->frame_seed.alloc(8ull * samples));
     HIP_TRY(p->work.alloc(rt::dev::kCounterBytes));
     p->pcfg.speculative = p->tracer >= HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE;
     p->pcfg.refill = p->tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
@@ -419,8 +435,9 @@ int hiprt_pt_render(hiprt_pt pt, uint32_t frame, void *stream) {
     if (!pt) return fail(HIPRT_ERR_ARG, "null path tracer");
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIP_TRY(hipSetDevice(pt->sc->ctx->device));
-    HIP_TRY(rt::dev::launch_set_u64(pt->frame_seed.as<uint64_t>(),
-                                    pt->seed * 0x100000001B3ULL + (uint64_t)frame, s));
+    // render call `frame` traces the samples of frames frame*samples .. +samples-1
+    HIP_TRY(rt::dev::launch_set_seeds(pt->frame_seed.as<uint64_t>(),
+                                      pt->seed * 0x100000001B3ULL + (uint64_t)frame * pt->samples, pt->samples, s));
     if (pt->batched && !pt->clear_next) {
         HIP_TRY(hipGraphLaunch(pt->exec, s));
     } else {
@@ -464,8 +481,8 @@ int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_le
         }
     } guard{ev};
     for (hipEvent_t &e : ev) HIP_TRY(hipEventCreate(&e));
-    uint64_t seed = pt->seed * 0x100000001B3ULL + (uint64_t)frame;
-    HIP_TRY(rt::dev::launch_set_u64(pt->frame_seed.as<uint64_t>(), seed, s));
+    uint64_t seed = pt->seed * 0x100000001B3ULL + (uint64_t)frame * pt->samples;
+    HIP_TRY(rt::dev::launch_set_seeds(pt->frame_seed.as<uint64_t>(), seed, pt->samples, s));
     HIP_TRY(pt->record(s, pt->clear_next, ev.data(), nullptr));
     pt->clear_next = false;
     HIP_TRY(hipStreamSynchronize(s));
@@ -476,7 +493,7 @@ int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_le
         dbuf st;
         HIP_TRY(st.alloc(16ull * levels));
         HIP_TRY(hipMemset(st.p, 0, 16ull * levels));
-        HIP_TRY(rt::dev::launch_set_u64(pt->frame_seed.as<uint64_t>(), seed, s));
+        HIP_TRY(rt::dev::launch_set_seeds(pt->frame_seed.as<uint64_t>(), seed, pt->samples, s));
         HIP_TRY(pt->record(s, false, nullptr, st.as<unsigned long long>()));
         HIP_TRY(hipStreamSynchronize(s));
         HIP_TRY(hipMemcpy(fetch_stats, st.p, 16ull * levels, hipMemcpyDeviceToHost));
@@ -487,7 +504,7 @@ int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_le
 int hiprt_pt_image(hiprt_pt pt, float *rgb) {
     if (!pt || !rgb) return fail(HIPRT_ERR_ARG, "null argument");
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(rgb, pt->rad.p, 12ull * pt->n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rgb, pt->rad.p, 12ull * pt->n_px, hipMemcpyDeviceToHost));
     return ok();
 }
 

@@ -491,7 +491,7 @@ public:
             tracer = HIPRT_TRACER_DYNAMIC_WHILE_WHILE | (tracer & HIPRT_TRACER_ALPHA);
         const hiprt_tracer_opts &o = pf->opts;
         hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, o.stack_config, o.work_chunks, o.refill_threshold, o.blocks,
-                          pf->ray_order};
+                          pf->ray_order, 1};
         check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
         frames = 0;
     }

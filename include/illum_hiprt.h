@@ -333,7 +333,11 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
  * (ray_order = 0), sorted by direction octant and origin cell (a 16^3 grid
  * over the scene box) so that wavefronts trace neighbouring rays; 1 keeps
  * the order the shading step appends them in.  Paths keep their own random
- * streams, so the image does not depend on the order. */
+ * streams, so the image does not depend on the order.  samples > 1 traces
+ * that many samples per pixel in one frame graph -- render call k traces the
+ * samples of frames k*samples .. k*samples + samples - 1, with their own
+ * random streams -- so every launch holds samples times more rays; radiance
+ * is summed per sample and added to the image in sample order. */
 typedef struct {
     int32_t max_depth;   /* last trace level; 4 = primary + 4 bounces */
     int32_t rr_start;    /* first level whose ray is subject to roulette (3) */
@@ -346,6 +350,7 @@ typedef struct {
     int32_t refill_threshold;
     int32_t blocks;
     int32_t ray_order;      /* batched: 0 = sorted bounce queues, 1 = shading order */
+    int32_t samples;        /* batched: samples per pixel traced together (0 or 1: one) */
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;

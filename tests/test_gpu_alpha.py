@@ -178,3 +178,20 @@ def test_obj_texture_coordinates_drive_alpha(torch_cuda, hip_ctx, tmp_path):
     plain = orc.trace_closest(p2, i2, bvh.nodes, bvh.root, bvh.index, rays)
     assert (want[3] != plain[3]).mean() > 0.05
     ds.close()
+
+
+def test_woop_records_alpha(torch_cuda, hip_ctx, alpha_scene):
+    """Woop records keep triangle ids in a side array; the opacity test reads
+    them there.  Woop barycentrics differ from the exact test's in the last
+    bits, so lookups may land on a neighbouring texel for a few rays."""
+    torch = torch_cuda
+    s = alpha_scene
+    ds = hiprt.DeviceScene(hip_ctx, s["sc"], s["bvh"], hiprt.BVH2, hiprt.TRI_WOOP)
+    rays = uniform_sphere_rays(50_000, 33)
+    want = orc.trace_closest_alpha(s["pos"], s["idx"], s["bvh"].nodes, s["bvh"].root, s["bvh"].index, rays,
+                                   s["alpha"])
+    for tracer in ("while-while-alpha", "dynamic-while-while-alpha"):
+        print(f"case: alpha scene, woop records, {tracer}", flush=True)
+        got = _closest(torch, ds, rays, tracer)
+        assert (got[3] == want[3]).mean() >= 0.999, tracer
+    ds.close()

@@ -135,6 +135,29 @@ def test_small_frames_and_depths(torch_cuda, hall, res, depth):
     assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
 
 
+@pytest.mark.parametrize("samples,res", [(3, (320, 180)), (2, (61, 33))])
+def test_samples_in_flight(torch_cuda, hall, samples, res):
+    """samples > 1: one render call traces the samples of `samples`
+    consecutive frames together.  Per level the same rays as rendering those
+    frames one by one (ray totals equal), and the same image up to the order
+    in which radiance is summed (per sample first, then into the image)."""
+    print(f"case: hall path tracer {res}, {samples} samples in flight vs one at a time", flush=True)
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = res
+    one = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=13)
+    for f in range(2 * samples):
+        one.render(f)
+    many = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=13, samples=samples)
+    many.render(0)
+    many.render(1)
+    assert one.ray_totals().tolist() == many.ray_totals().tolist()
+    a, b = one.image(), many.image()
+    one.close()
+    many.close()
+    assert a.sum() > 0
+    assert np.allclose(a, b, rtol=2e-6, atol=1e-6 * float(np.abs(a).max()))
+
+
 def test_graph_replays_are_stable(torch_cuda, hall):
     print("case: hall path tracer 256x144, batched graph replayed 4x vs eager", flush=True)
     cam = hall["cam"].copy()
