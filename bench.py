@@ -301,17 +301,17 @@ def main():
     # The trace autotuner times one launch over rays already in pixel order
     # with the tree warm in L2; inside a frame every level follows a shading
     # pass and reads a sorted queue.  So the frame's configuration is chosen
-    # on warm-up frames (never the timed ones): the best of the autotuned
-    # one, its one-counter variant, the 4-wide tree and the binary tree with
-    # stack configuration 3, and the defaults with 2, 4 or 8 samples per pixel in
-    # flight (consecutive frames' samples traced together, so every launch
-    # holds 2-8x the rays), when it beats the library defaults (binary tree,
-    # dynamic tracer, stack 0, eight work chunks, one sample) by more than 2%
-    # in rays per second, the defaults otherwise, which keeps the choice
-    # stable between runs.  The
-    # autotuner's sample is the first bounce; the deeper bounces weigh the
-    # stack configurations and layouts differently, so the fixed candidates
-    # are tried on whole frames too.
+    # on warm-up frames (never the timed ones), in rays per second: the best
+    # of the autotuned one, its one-counter variant, the 4-wide and the binary
+    # tree with stack configuration 3, the defaults with 2, 4 or 8 samples per
+    # pixel in flight (consecutive frames' samples traced together, so every
+    # launch holds 2-8x the rays) and the binary stack-3 tree with 8 (the
+    # 4-wide one with 8 measured the same), when it beats
+    # the library defaults (binary tree, dynamic tracer, stack 0, eight work
+    # chunks, one sample) by more than 2%, the defaults otherwise, which keeps
+    # the choice stable between runs.  The autotuner's sample is the first
+    # bounce; the deeper bounces weigh the stack configurations and layouts
+    # differently, so the fixed candidates are tried on whole frames too.
     ds4 = hiprt.DeviceScene(ctx, sc, bvh, hiprt.BVH4)
     candidates = {"autotuned": (ds_tuned, pt_kw),
                   "autotuned, one work counter": (ds_tuned, dict(pt_kw, work_chunks=1)),
@@ -320,6 +320,8 @@ def main():
                   "2 samples in flight": (ds, dict(samples=2)),
                   "4 samples in flight": (ds, dict(samples=4)),
                   "8 samples in flight": (ds, dict(samples=8)),
+                  "8 samples in flight, stack 3": (ds, dict(tracer="dynamic-while-while", stack_config=3,
+                                                            samples=8)),
                   "library defaults": (ds, {})}
     # Rates, not frame times: a render call with k samples in flight traces k
     # samples per pixel.
