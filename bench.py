@@ -71,8 +71,8 @@ def kernel_name(kw, ds, binary):
     tracer = kw.get("tracer", "dynamic-while-while")
     spec = "true" if tracer in ("persistent-speculative-while-while", "dynamic-while-while") else "false"
     refill = "true" if tracer == "dynamic-while-while" else "false"
-    reg = {0: 4, 1: 8, 2: 2, 3: 0}[kw.get("stack_config", 0)]
-    lds = {0: 28, 1: 16, 2: 20, 3: 24}[kw.get("stack_config", 0)]
+    reg = {0: 4, 1: 8, 2: 2, 3: 0, 4: 0}[kw.get("stack_config", 0)]
+    lds = {0: 28, 1: 16, 2: 20, 3: 24, 4: 20}[kw.get("stack_config", 0)]
     # the LDS ring with a spill column when the tree outgrows the
     # configuration, the plain LDS stack otherwise
     ring = "true" if ds.stack_need > reg + lds else "false"
@@ -305,8 +305,9 @@ def main():
     # of the autotuned one, its one-counter variant, the 4-wide and the binary
     # tree with stack configuration 3, the defaults with 2, 4 or 8 samples per
     # pixel in flight (consecutive frames' samples traced together, so every
-    # launch holds 2-8x the rays) and the binary stack-3 tree with 8 (the
-    # 4-wide one with 8 measured the same), when it beats
+    # launch holds 2-8x the rays) and the binary tree with stack configuration
+    # 4 (a 20-entry LDS stack with registers capped for 8 waves per SIMD) and 8
+    # or 16 samples in flight, when it beats
     # the library defaults (binary tree, dynamic tracer, stack 0, eight work
     # chunks, one sample) by more than 2%, the defaults otherwise, which keeps
     # the choice stable between runs.  The autotuner's sample is the first
@@ -320,8 +321,10 @@ def main():
                   "2 samples in flight": (ds, dict(samples=2)),
                   "4 samples in flight": (ds, dict(samples=4)),
                   "8 samples in flight": (ds, dict(samples=8)),
-                  "8 samples in flight, stack 3": (ds, dict(tracer="dynamic-while-while", stack_config=3,
+                  "8 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
                                                             samples=8)),
+                  "16 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
+                                                             samples=16)),
                   "library defaults": (ds, {})}
     # Rates, not frame times: a render call with k samples in flight traces k
     # samples per pixel.

@@ -21,7 +21,7 @@ constexpr int kPlainStackCap = 64;
 constexpr int kRegStack = 4;
 constexpr int kLdsStack = 28;
 constexpr int kPersistentWavesPerBlock = 4;
-constexpr int kStackConfigs = 4;
+constexpr int kStackConfigs = 5;
 // Leaves a closest-hit lane of the speculative persistent tracers parks
 // before it stops descending (at most 4).
 constexpr int kParkClosest = 4;

@@ -184,8 +184,18 @@ __device__ __forceinline__ uint32_t find_desc(const batch_desc *table, uint32_t 
 
 } // namespace
 
+// Waves per SIMD the register allocation must leave room for: 8 for
+// single-scene binary-tree launches with the 20-entry LDS stack and no VGPR
+// entries (20 KB per block lets 8 blocks share a CU's LDS; the register cap
+// costs SGPR spills to VGPR lanes only), no bound otherwise (4-wide and
+// descriptor-table kernels would spill VGPRs to scratch under it).
+constexpr int waves_hint(int reg, int lds, int bvh, bool multi) {
+    return reg == 0 && lds == 20 && bvh == 2 && !multi ? 8 : 1;
+}
+
 template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, int LDS, bool COUNT, int BVH, bool MULTI, bool RING>
-__global__ __launch_bounds__(256) void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint(REG, LDS, BVH, MULTI))))
+void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits,
                                                           uint8_t *occluded, uint32_t n_static,
                                                           const uint32_t *count_ptr, uint32_t *counter,
                                                           int refill_thr, unsigned long long *stats,
@@ -439,10 +449,11 @@ namespace {
 //   1: 8 VGPR + 16 LDS   16 KB
 //   2: 2 VGPR + 20 LDS   20 KB
 //   3: 0 VGPR + 24 LDS   24 KB
+//   4: 0 VGPR + 20 LDS   20 KB, registers capped for 8 waves per SIMD
 constexpr int reg_for(int stk) { return stk == 0 ? kRegStack : stk == 1 ? 8 : stk == 2 ? 2 : 0; }
 constexpr int lds_for(int bvh, int stk) {
     (void)bvh;
-    return stk == 0 ? kLdsStack : stk == 1 ? 16 : stk == 2 ? 20 : 24;
+    return stk == 0 ? kLdsStack : stk == 1 ? 16 : stk == 2 ? 20 : stk == 3 ? 24 : 20;
 }
 
 // Blocks of one instantiation that fit the device at once (occupancy x CUs),
@@ -543,6 +554,7 @@ hipError_t dispatch_stack(const launch_args &a, const persistent_cfg &cfg, hipSt
     case 1: return dispatch_flags<TRI, ANY, BVH, MULTI, 1>(a, cfg, s, grid_only, g);
     case 2: return dispatch_flags<TRI, ANY, BVH, MULTI, 2>(a, cfg, s, grid_only, g);
     case 3: return dispatch_flags<TRI, ANY, BVH, MULTI, 3>(a, cfg, s, grid_only, g);
+    case 4: return dispatch_flags<TRI, ANY, BVH, MULTI, 4>(a, cfg, s, grid_only, g);
     default: return dispatch_flags<TRI, ANY, BVH, MULTI, 0>(a, cfg, s, grid_only, g);
     }
 }

@@ -391,7 +391,7 @@ bool platform::interprete(const std::string &command, std::istringstream &in) {
             if (v < 0) error("Parameter error, blocks must be >= 0 (0 = occupancy default)");
             opts.blocks = v;
         } else if (name == "stack") {
-            if (v < 0 || v > 3) error("Parameter error, stack configuration must be 0..3");
+            if (v < 0 || v > 4) error("Parameter error, stack configuration must be 0..4");
             opts.stack_config = v;
         } else if (name == "chunks") {
             if (v != 1 && v != 8) error("Parameter error, chunks must be 1 or 8");

@@ -221,9 +221,11 @@ typedef struct {
     uint64_t *d_fetch_stats;  /* non-NULL: run the counting variant, which adds the inner
                                  nodes it fetches to [0] and the triangle records it
                                  tests to [1] (device uint64[2]) */
-    int32_t stack_config;     /* traversal stack per lane (VGPR + LDS entries, binary /
-                                 4-wide): 0 = 4 + 28/44, 1 = 8 + 16/32, 2 = 2 + 20/36,
-                                 3 = 0 + 24/40 */
+    int32_t stack_config;     /* traversal stack per lane (VGPR + LDS entries, both tree
+                                 widths): 0 = 4 + 28, 1 = 8 + 16, 2 = 2 + 20, 3 = 0 + 24,
+                                 4 = 0 + 20 with registers capped for 8 waves per SIMD
+                                 (binary trees, single scene); deeper traversals spill
+                                 to a global column per lane */
     int32_t work_chunks;      /* 1 = one work counter; 8 (and 0, the default) = one
                                  contiguous chunk of the rays per XCD, blocks start on
                                  their own XCD's chunk (fetch atomics spread over 8

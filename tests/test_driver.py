@@ -82,11 +82,11 @@ def test_tracer_options_keep_the_image(tmp_path, torch_cuda):
     w, h = 160, 120
     head = SCRIPT_HEAD.format(cfg=1, w=w, h=h, tracer="dynamic-while-while")
     r0 = run_script(head + "algo primary-wf\noutfile a.pfm\nrun\n", tmp_path)
-    r1 = run_script(head + "tracer-option stack 3\ntracer-option chunks 8\ntracer-option refill 56\n"
+    r1 = run_script(head + "tracer-option stack 4\ntracer-option chunks 8\ntracer-option refill 56\n"
                     "tracer-option stack 9\ntracer-option speed 1\nalgo primary-wf\noutfile b.pfm\nrun\n", tmp_path)
     assert r0.returncode == 0 and not r0.stderr.strip(), r0.stderr
     err = r1.stderr.splitlines()
-    assert err == ["command (tracer-option): Parameter error, stack configuration must be 0..3",
+    assert err == ["command (tracer-option): Parameter error, stack configuration must be 0..4",
                    "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks, ray-order)"], err
     # primary-wf shades hits with the albedo, so the images agree unless two hits tie in t
     a, b = read_pfm(tmp_path / "a.pfm"), read_pfm(tmp_path / "b.pfm")

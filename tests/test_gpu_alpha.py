@@ -92,7 +92,7 @@ def test_alpha_hits_match_oracle(torch_cuda, alpha_scene, layout):
     for tracer in TRACERS:
         if layout == "bvh4" and tracer == "while-while":
             continue
-        for stack in (0, 3):
+        for stack in (0, 3, 4):
             print(f"case: alpha scene, {layout}, {tracer}-alpha, stack {stack}", flush=True)
             got = _closest(torch, ds, rays, tracer + "-alpha", stack_config=stack)
             if layout == "bvh2" and tracer in ("while-while", "persistent-while-while"):

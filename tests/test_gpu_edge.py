@@ -102,7 +102,7 @@ def test_special_rays_every_tracer(torch_cuda, c1s):
     assert bit_exact(got, want)
     assert np.array_equal(_any(torch, c1s["ds"], rays, "while-while"), want_any)
     for tracer in TRACERS[1:]:
-        for stack in (0, 1, 2, 3):
+        for stack in (0, 1, 2, 3, 4):
             for ds in (c1s["ds"], c1s["ds4"]):
                 got = _closest(torch, ds, rays, tracer, stack_config=stack)
                 rep = classify(orc, c1s["pos"], c1s["idx"], rays, got, want)

@@ -79,7 +79,7 @@ def test_bounce_rays_match_oracle_and_trace_exactly(torch_cuda, hall):
 
 @pytest.mark.parametrize("res,stack,chunks,sort", [((320, 180), 0, 1, True), ((320, 180), 1, 8, True),
                                                    ((320, 180), 0, 8, False), ((333, 97), 3, 8, True),
-                                                   ((333, 97), 3, 1, False)])
+                                                   ((333, 97), 3, 1, False), ((333, 97), 4, 8, True)])
 def test_batched_equals_plain(torch_cuda, hall, res, stack, chunks, sort):
     print(f"case: hall path tracer {res}, plain vs batched (stack {stack}, chunks {chunks}, sorted queues {sort}), "
           "2 frames", flush=True)

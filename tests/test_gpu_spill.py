@@ -98,7 +98,7 @@ def test_spilled_traversal_matches_oracle(torch_cuda, deep, tracer, layout):
     want_any = orc.seq_any(pos, idx, rays)
     same_tree = orc.trace_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, rays)
     assert (want[0] < FLT_MAX).mean() > 0.3
-    for stack in range(4):
+    for stack in range(5):
         print(f"case: nested scene, {layout}, {tracer}, stack {stack}", flush=True)
         got = _closest(torch, ds, rays, tracer, stack_config=stack)
         if layout == "bvh2" and tracer == "persistent-while-while":
@@ -118,7 +118,7 @@ def test_spilled_batch_and_work_chunks(torch_cuda, hip_ctx, deep):
     pos_o, idx_o, _ = other.arrays()
     from parity import uniform_sphere_rays
     rays_o = uniform_sphere_rays(25_000, 13)
-    for stack in (1, 3):
+    for stack in (1, 3, 4):
         for chunks in (1, 8):
             print(f"case: nested + C1 batch, stack {stack}, chunks {chunks}", flush=True)
             d_r, d_o = torch.from_numpy(rays).cuda(), torch.from_numpy(rays_o).cuda()
@@ -144,6 +144,7 @@ def test_spilled_path_tracer(torch_cuda, deep):
     cam = np.array([0.02, 0.02, 0.02, -1, -1, -1, 0, 1, 0, 60, 160, 120], np.float32)
     imgs = {}
     for name, ds, batched, stack, srt in (("plain", ds2, False, 0, False), ("batched bvh2", ds2, True, 3, False),
+                                          ("batched bvh2 stack 4", ds2, True, 4, True),
                                           ("batched bvh4", ds4, True, 3, True)):
         print(f"case: nested scene path tracer, {name}", flush=True)
         pt = hiprt.PathTracer(ds, cam, batched=batched, seed=5, stack_config=stack, sort_rays=srt)
@@ -151,5 +152,6 @@ def test_spilled_path_tracer(torch_cuda, deep):
         imgs[name] = pt.image()
         pt.close()
     assert np.array_equal(imgs["plain"].view(np.uint32), imgs["batched bvh2"].view(np.uint32))
+    assert np.array_equal(imgs["plain"].view(np.uint32), imgs["batched bvh2 stack 4"].view(np.uint32))
     same = (imgs["plain"] == imgs["batched bvh4"]).all(axis=-1)
     assert same.mean() >= 0.999, same.mean()

@@ -71,11 +71,12 @@ def test_persistent_exact_matches_up_to_ties(torch_cuda, c1, tracer):
         assert np.array_equal(a[same].view(np.uint32), b[same].view(np.uint32))
 
 
-@pytest.mark.parametrize("stack", [1, 2, 3])
+@pytest.mark.parametrize("stack", [1, 2, 3, 4])
 @pytest.mark.parametrize("tracer", TRACERS[1:])
 def test_other_stack_configs_match_up_to_ties(torch_cuda, c1, tracer, stack):
-    """Stack configurations 1-3 (register / LDS splits of the per-lane
-    stack) change only where entries live, never the traversal order."""
+    """Stack configurations 1-4 (register / LDS splits of the per-lane
+    stack, and configuration 4's register cap) change only where entries
+    live, never the traversal order."""
     print(f"case: C1 camera+random rays, {tracer}, exact records, stack config {stack}", flush=True)
     got0 = _run_closest(torch_cuda, c1["exact"], c1["rays"], tracer)
     got1 = _run_closest(torch_cuda, c1["exact"], c1["rays"], tracer, stack_config=stack)
