@@ -233,21 +233,24 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
 // Paths keep their own random streams, so the image does not depend on the
 // queue order.
 
-__device__ __forceinline__ uint32_t spread4(uint32_t v) {   // 4 bits -> bits 0, 3, 6, 9
-    return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4) | ((v & 8u) << 6);
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {   // bit b -> bit 3b
+    uint32_t r = 0;
+#pragma unroll
+    for (int b = 0; b < kSortCellBits; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
 }
 
 __device__ __forceinline__ uint32_t cell_of(float x, float lo, float scale) {
     float q = (x - lo) * scale;
-    q = fminf(fmaxf(q, 0.0f), 15.0f);   // fmaxf maps NaN to 0
+    q = fminf(fmaxf(q, 0.0f), (float)((1u << kSortCellBits) - 1));   // fmaxf maps NaN to 0
     return (uint32_t)q;
 }
 
 __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d) {
     uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
-    uint32_t m = (spread4(cell_of(o.x, g.lo[0], g.scale[0])) << 2) |
-                 (spread4(cell_of(o.y, g.lo[1], g.scale[1])) << 1) | spread4(cell_of(o.z, g.lo[2], g.scale[2]));
-    return (oct << 12) | m;
+    uint32_t m = (spread3(cell_of(o.x, g.lo[0], g.scale[0])) << 2) |
+                 (spread3(cell_of(o.y, g.lo[1], g.scale[1])) << 1) | spread3(cell_of(o.z, g.lo[2], g.scale[2]));
+    return (oct << (3 * kSortCellBits)) | m;
 }
 
 // Shade step of the sorting form.  Queue slot i < counts[level] holds the

@@ -52,11 +52,13 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 const ray_out &rays_out, uint32_t *path_out, float *thr, float *rad,
                                 uint32_t *counts, uint32_t n_max, hipStream_t s);
 // Sorted bounce queues: ray keys are the direction octant (3 bits) and the
-// Morton code of the origin's cell in a 16^3 grid over the scene box.
-constexpr int kRayKeyBits = 15;
+// Morton code of the origin's cell in a grid of 2^kSortCellBits cells per
+// axis over the scene box.
+constexpr int kSortCellBits = 4;
+constexpr int kRayKeyBits = 3 + 3 * kSortCellBits;
 struct sort_grid {
     float lo[3];      // scene box minimum
-    float scale[3];   // 16 / extent per axis (0 for a flat axis)
+    float scale[3];   // cells per axis / extent (0 for a flat axis)
 };
 // Ray records of the sorting form: kRayRecord float4 per ray,
 // (ox, oy, oz, t_min | dx, dy, dz, t_max | path id bits, throughput).

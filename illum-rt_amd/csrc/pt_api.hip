@@ -54,7 +54,7 @@ rt::dev::sort_grid grid_of(const rt::flat_scene &h) {
         float ext = hi[a] - lo[a];
         bool ok_axis = std::isfinite(lo[a]) && std::isfinite(ext) && ext > 0.0f;
         g.lo[a] = ok_axis ? lo[a] : 0.0f;
-        g.scale[a] = ok_axis ? 16.0f / ext : 0.0f;
+        g.scale[a] = ok_axis ? (float)(1u << rt::dev::kSortCellBits) / ext : 0.0f;
         if (!std::isfinite(g.scale[a])) g.scale[a] = 0.0f;
     }
     return g;

@@ -200,7 +200,10 @@ enum {
     HIPRT_TRACER_WHILE_WHILE = 0,                        /* plain: one ray per lane, grid = batch */
     HIPRT_TRACER_PERSISTENT_WHILE_WHILE = 1,             /* persistent waves, one atomic per fetch */
     HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE = 2, /* + leaf postponement */
-    HIPRT_TRACER_DYNAMIC_WHILE_WHILE = 3,                /* + leaf postponement + dynamic refill */
+    HIPRT_TRACER_DYNAMIC_WHILE_WHILE = 3,                /* + leaf postponement + dynamic refill;
+                                                            2 and 3 may return either triangle
+                                                            of an equal-t tie, and which one
+                                                            can vary between runs */
     HIPRT_TRACER_COUNT = 4,
     /* OR'ed into a tracer id: its alpha-aware variant, the reference's
      * "<tracer>-alpha" (rt/cuda/platform.cpp:107-122; CPU semantics of
