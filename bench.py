@@ -367,8 +367,9 @@ def main():
     kernels = {}
     variants = (("while-while", {}), ("dynamic-while-while", {}),
                 ("dynamic-while-while", {"stack_config": 1}), ("dynamic-while-while", {"stack_config": 2}),
-                ("dynamic-while-while", {"stack_config": 3}), ("dynamic-while-while", {"work_chunks": 1}),
-                ("dynamic-while-while", {"stack_config": 3, "work_chunks": 1}))
+                ("dynamic-while-while", {"stack_config": 3}), ("dynamic-while-while", {"stack_config": 4}),
+                ("dynamic-while-while", {"work_chunks": 1}),
+                ("dynamic-while-while", {"stack_config": 4, "work_chunks": 1}))
     for name, rays in (("primary", d_cam), ("diffuse", d_diff)):
         for tr, kw in variants:
             ms, _ = time_trace(torch, hiprt, ds, rays, tr, stream, **kw)
