@@ -399,8 +399,11 @@ bool platform::interprete(const std::string &command, std::istringstream &in) {
         } else if (name == "ray-order") {
             if (v != 0 && v != 1) error("Parameter error, ray-order must be 0 (sorted) or 1 (shading order)");
             ray_order = v;
+        } else if (name == "samples") {
+            if (v < 1 || v > 64) error("Parameter error, samples must be 1..64");
+            samples = v;
         } else {
-            error("Unknown tracer option " << name << " (refill, blocks, stack, chunks, ray-order)");
+            error("Unknown tracer option " << name << " (refill, blocks, stack, chunks, ray-order, samples)");
         }
         return true;
     }
@@ -470,10 +473,14 @@ public:
 };
 
 // pt-wf: the batched wavefront path tracer of the library, one frame per
-// sample, the accumulator averaged into the framebuffer at the end.
+// sample, the accumulator averaged into the framebuffer at the end.  With
+// "tracer-option samples k" each render call traces k samples per pixel
+// together (k must divide sppx), so sppx / k calls render the same sppx
+// frames.
 class pt_wf : public simple_algorithm {
     hiprt_pt pt = nullptr;
-    int frames = 0;
+    int frames = 0;    // render calls so far
+    int samples = 1;   // samples per render call
 
 public:
     ~pt_wf() override {
@@ -490,14 +497,18 @@ public:
         if (tracer >= 0 && (tracer & ~HIPRT_TRACER_ALPHA) == HIPRT_TRACER_WHILE_WHILE)
             tracer = HIPRT_TRACER_DYNAMIC_WHILE_WHILE | (tracer & HIPRT_TRACER_ALPHA);
         const hiprt_tracer_opts &o = pf->opts;
+        samples = pf->samples;
+        if (rc->sppx % samples != 0)
+            throw std::runtime_error("pt-wf: sppx " + std::to_string(rc->sppx) +
+                                     " is not a multiple of tracer-option samples " + std::to_string(samples));
         hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, o.stack_config, o.work_chunks, o.refill_threshold, o.blocks,
-                          pf->ray_order, 1};
+                          pf->ray_order, samples};
         check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
         frames = 0;
     }
     bool compute_sample(int sppx, wf::timer *t) override {
         if (current_sample_index >= sppx) return false;
-        ++current_sample_index;
+        current_sample_index += samples;
         t->start("path trace frame");
         check(hiprt_pt_render(pt, (uint32_t)frames++, nullptr), "pt-wf");
         t->stop("path trace frame");
@@ -510,7 +521,8 @@ public:
         check(hiprt_pt_image(pt, rgb.data()), "pt-wf");
         rc->framebuffer.assign(4 * n, 1.0f);
         for (size_t i = 0; i < n; ++i)
-            for (int k = 0; k < 3; ++k) rc->framebuffer[4 * i + k] = rgb[3 * i + k] / (float)(frames ? frames : 1);
+            for (int k = 0; k < 3; ++k)
+                rc->framebuffer[4 * i + k] = rgb[3 * i + k] / (float)(frames ? frames * samples : 1);
         if (t) t->synchronize();
     }
 };

@@ -95,6 +95,7 @@ public:
     // options of the persistent tracers, set by "tracer-option <name> <value>"
     hiprt_tracer_opts opts{40, 0, nullptr, 0, 8};
     int ray_order = 0;   // path tracer bounce queues: 0 sorted, 1 shading order
+    int samples = 1;     // path tracer: samples per pixel traced together (divides sppx)
     hiprt_bvh bvh = nullptr;
     hiprt_scene ds = nullptr;          // device scene of the last commit
     wf::scene *sd = nullptr;

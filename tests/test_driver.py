@@ -87,7 +87,7 @@ def test_tracer_options_keep_the_image(tmp_path, torch_cuda):
     assert r0.returncode == 0 and not r0.stderr.strip(), r0.stderr
     err = r1.stderr.splitlines()
     assert err == ["command (tracer-option): Parameter error, stack configuration must be 0..4",
-                   "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks, ray-order)"], err
+                   "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks, ray-order, samples)"], err
     # primary-wf shades hits with the albedo, so the images agree unless two hits tie in t
     a, b = read_pfm(tmp_path / "a.pfm"), read_pfm(tmp_path / "b.pfm")
     assert np.array_equal(a, b)
@@ -132,8 +132,9 @@ def test_direct_wf_shadows_match_oracle(tmp_path, torch_cuda):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("options", ["", "tracer-option ray-order 1\ntracer-option stack 3\n"])
-def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options):
+@pytest.mark.parametrize("options,samples", [("", 1), ("tracer-option ray-order 1\ntracer-option stack 3\n", 1),
+                                             ("tracer-option stack 4\ntracer-option samples 2\n", 2)])
+def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options, samples):
     print(f"case: rtgi-hip pt-wf, hall 128x72, 2 spp vs hiprt.PathTracer, options {options!r}", flush=True)
     w, h = 128, 72
     r = run_script(SCRIPT_HEAD.format(cfg=3, w=w, h=h, tracer="dynamic-while-while") + options +
@@ -146,14 +147,26 @@ def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options):
     ds = hiprt.DeviceScene(ctx, sc, bvh)
     cam = sc.cam.copy()
     cam[10], cam[11] = w, h
-    pt = hiprt.PathTracer(ds, cam, seed=1, batched=True)
-    pt.render(0)
-    pt.render(1)
+    # with k samples per render call, calls 0 .. 2/k - 1 trace frames 0 and 1
+    pt = hiprt.PathTracer(ds, cam, seed=1, batched=True, samples=samples)
+    for f in range(2 // samples):
+        pt.render(f)
     want = pt.image() / np.float32(2)
     pt.close()
     ds.close()
     ctx.close()
     assert np.array_equal(img.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_pt_wf_samples_must_divide_sppx(tmp_path, torch_cuda):
+    print("case: rtgi-hip pt-wf, sppx 3 with tracer-option samples 2", flush=True)
+    r = run_script(SCRIPT_HEAD.format(cfg=1, w=32, h=24, tracer="dynamic-while-while") +
+                   "tracer-option samples 2\ntracer-option samples 65\nsppx 3\nalgo pt-wf\noutfile pt.pfm\nrun\n",
+                   tmp_path)
+    err = r.stderr.splitlines()
+    assert "command (tracer-option): Parameter error, samples must be 1..64" in err
+    assert "command (run): pt-wf: sppx 3 is not a multiple of tracer-option samples 2" in err
 
 
 @pytest.mark.gpu
