@@ -652,6 +652,19 @@ static const char *const kTracerNames[2][HIPRT_TRACER_COUNT] = {
     {"while-while-alpha", "persistent-while-while-alpha", "persistent-speculative-while-while-alpha",
      "dynamic-while-while-alpha"}};
 
+// The reference's other CUDA variants (rt/cuda/platform.cpp:107-122) are
+// loop structures of the same near-first traversal: each name selects the
+// kernel here that runs that traversal.
+static const struct {
+    const char *name;
+    int id;
+} kTracerAliases[] = {
+    {"simple", HIPRT_TRACER_WHILE_WHILE},
+    {"if-if", HIPRT_TRACER_WHILE_WHILE},
+    {"persistent-if-if", HIPRT_TRACER_PERSISTENT_WHILE_WHILE},
+    {"speculative-while-while", HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE},
+};
+
 int hiprt_tracer_id(const char *name) {
     if (!name) return -1;
     if (std::strcmp(name, "default") == 0) return HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
@@ -659,6 +672,13 @@ int hiprt_tracer_id(const char *name) {
     for (int a = 0; a < 2; ++a)
         for (int i = 0; i < HIPRT_TRACER_COUNT; ++i)
             if (std::strcmp(name, kTracerNames[a][i]) == 0) return i | (a ? HIPRT_TRACER_ALPHA : 0);
+    const size_t len = std::strlen(name);
+    const bool alpha = len > 6 && std::strcmp(name + len - 6, "-alpha") == 0;
+    for (const auto &al : kTracerAliases) {
+        const size_t n = std::strlen(al.name);
+        if ((alpha ? len == n + 6 : len == n) && std::strncmp(name, al.name, n) == 0)
+            return al.id | (alpha ? HIPRT_TRACER_ALPHA : 0);
+    }
     return -1;
 }
 

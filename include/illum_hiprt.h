@@ -195,7 +195,12 @@ typedef struct {
 
 /* Tracer variants, named as the reference registers its CUDA tracers
  * (rt/cuda/platform.cpp:34-56).  hiprt_tracer_id maps a name to an id, -1 if
- * unknown; "default" is "dynamic-while-while". */
+ * unknown; "default" is "dynamic-while-while".  The reference's other loop
+ * structures of the same traversal name the kernel that runs it here:
+ * "simple" and "if-if" -> while-while, "persistent-if-if" ->
+ * persistent-while-while, "speculative-while-while" ->
+ * persistent-speculative-while-while (each also with "-alpha");
+ * hiprt_tracer_name returns the canonical name. */
 enum {
     HIPRT_TRACER_WHILE_WHILE = 0,                        /* plain: one ray per lane, grid = batch */
     HIPRT_TRACER_PERSISTENT_WHILE_WHILE = 1,             /* persistent waves, one atomic per fetch */

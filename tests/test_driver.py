@@ -57,10 +57,11 @@ commit
 
 
 @pytest.mark.gpu
-def test_primary_wf_image_matches_oracle(tmp_path, torch_cuda):
-    print("case: rtgi-hip primary-wf, 8k random triangles, 160x120", flush=True)
+@pytest.mark.parametrize("tracer", ["while-while", "simple", "persistent-if-if"])
+def test_primary_wf_image_matches_oracle(tmp_path, torch_cuda, tracer):
+    print(f"case: rtgi-hip primary-wf, 8k random triangles, 160x120, raytracer {tracer}", flush=True)
     w, h = 160, 120
-    r = run_script(SCRIPT_HEAD.format(cfg=1, w=w, h=h, tracer="while-while") +
+    r = run_script(SCRIPT_HEAD.format(cfg=1, w=w, h=h, tracer=tracer) +
                    "algo primary-wf\noutfile p.pfm\nrun\nstats print\n", tmp_path)
     assert r.returncode == 0 and not r.stderr.strip(), r.stderr
     img = read_pfm(tmp_path / "p.pfm")
@@ -171,7 +172,7 @@ def test_pt_wf_samples_must_divide_sppx(tmp_path, torch_cuda):
 
 @pytest.mark.gpu
 def test_unknown_tracer_and_stale_bvh(tmp_path, torch_cuda):
-    r = run_script("platform hip\nraytracer if-if\ngenerate 1\nalgo primary-wf\nrun\n", tmp_path)
+    r = run_script("platform hip\nraytracer if-else\ngenerate 1\nalgo primary-wf\nrun\n", tmp_path)
     err = r.stderr.splitlines()
     assert "command (raytracer): There is no such hip ray tracer variant" in err
     assert any(e.startswith("command (run): We have to have a scene loaded") for e in err)

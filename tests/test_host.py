@@ -33,6 +33,16 @@ def test_errors_are_reported_not_thrown():
     assert hiprt.tracer_id("default") == 3
     with pytest.raises(hiprt.HiprtError):
         hiprt.tracer_id("no-such-tracer")
+    # the reference's other loop structures name the kernel of the same traversal
+    for alias, canonical in (("simple", "while-while"), ("if-if", "while-while"),
+                             ("persistent-if-if", "persistent-while-while"),
+                             ("speculative-while-while", "persistent-speculative-while-while")):
+        assert hiprt.tracer_id(alias) == hiprt.tracer_id(canonical)
+        assert hiprt.tracer_id(alias + "-alpha") == hiprt.tracer_id(canonical + "-alpha")
+        assert L.hiprt_tracer_name(hiprt.tracer_id(alias)).decode() == canonical
+    for bad_name in ("if-if-", "if-ifalpha", "-alpha", "simple-alpha-alpha", "if"):
+        with pytest.raises(hiprt.HiprtError):
+            hiprt.tracer_id(bad_name)
     bad = np.array([[0, 1, 5]], np.uint32)
     with pytest.raises(hiprt.HiprtError, match="out of range"):
         hiprt.HostScene.from_arrays(np.zeros((3, 3), np.float32), bad)
