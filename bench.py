@@ -111,7 +111,8 @@ def bench_shadow(torch, hiprt, ctx, stream):
     occ = torch.empty(n, dtype=torch.uint8, device="cuda")
     out = {"triangles": sc.n_tris, "rays": n}
     for tr, kw in (("while-while", {}), ("dynamic-while-while", {}), ("dynamic-while-while", {"work_chunks": 1}),
-                   ("persistent-speculative-while-while", {})):
+                   ("dynamic-while-while", {"stack_config": 4}), ("persistent-speculative-while-while", {}),
+                   ("persistent-speculative-while-while", {"stack_config": 4})):
         ms = time_calls(torch, lambda: hiprt.trace_any(ds, shadow, occ, tracer=tr, stream=stream.cuda_stream, **kw),
                         stream)
         tag = "".join(f",{k}={v}" for k, v in kw.items())
