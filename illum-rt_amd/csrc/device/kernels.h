@@ -33,7 +33,8 @@ constexpr size_t kCounterBytes = kMaxChunks * kCounterStride * 4;
 
 // Stack entries the persistent kernels hold in VGPRs and LDS for a BVH of
 // the given format and stack configuration (0: 4 VGPR + 28 LDS entries,
-// 1: 8 + 16, 2: 2 + 20, 3: 0 + 24).
+// 1: 8 + 16, 2: 2 + 20, 3: 0 + 24, 4: 0 + 20 with registers capped for 8
+// waves per SIMD on single-scene binary trees).
 int persistent_stack_cap(int bvh, int stack = 0);
 
 // Spill entries per lane a persistent launch needs for a tree whose

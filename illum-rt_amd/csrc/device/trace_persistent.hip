@@ -195,13 +195,10 @@ constexpr int waves_hint(int reg, int lds, int bvh, bool multi) {
 
 template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, int LDS, bool COUNT, int BVH, bool MULTI, bool RING>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint(REG, LDS, BVH, MULTI))))
-void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits,
-                                                          uint8_t *occluded, uint32_t n_static,
-                                                          const uint32_t *count_ptr, uint32_t *counter,
-                                                          int refill_thr, unsigned long long *stats,
-                                                          const batch_desc *table, uint32_t n_desc,
-                                                          uint32_t n_chunks, int32_t *spill, uint32_t spill_cap,
-                                                          ray_gather gather) {
+void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occluded, uint32_t n_static,
+                        const uint32_t *count_ptr, uint32_t *counter, int refill_thr, unsigned long long *stats,
+                        const batch_desc *table, uint32_t n_desc, uint32_t n_chunks, int32_t *spill,
+                        uint32_t spill_cap, ray_gather gather) {
     __shared__ int32_t lds_stack[kPersistentWavesPerBlock * LDS * 64];
     const uint32_t n = count_ptr ? *count_ptr : n_static;
     uint32_t chunk = blockIdx.x % n_chunks;   // wave-uniform: the chunk this wave takes rays from
