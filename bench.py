@@ -176,8 +176,8 @@ def bench_alpha(torch, hiprt, ctx, stream, d_cam, d_diff):
 def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
     """64 random-triangle scenes of log-uniform size with their own rays: one
     descriptor-table launch against one launch per scene, both with the
-    dynamic tracer; each scene's layout (binary or 4-wide) picked by the
-    autotuner on a sample of its rays."""
+    dynamic tracer and stack configuration 4; each scene's layout (binary or
+    4-wide) picked by the autotuner on a sample of its rays."""
     import workloads
     t0 = time.time()
     work = workloads.multi_scene(n_scenes, seed=4)
@@ -207,10 +207,10 @@ def bench_multi_scene(torch, hiprt, ctx, stream, n_scenes=64):
 
     def separate():
         for ds, r, h, _ in entries:
-            hiprt.trace_closest(ds, r, h, tracer="dynamic-while-while", stream=stream.cuda_stream)
+            hiprt.trace_closest(ds, r, h, tracer="dynamic-while-while", stack_config=4, stream=stream.cuda_stream)
 
     def batched():
-        hiprt.trace_batch(entries, tracer="dynamic-while-while", stream=stream.cuda_stream)
+        hiprt.trace_batch(entries, tracer="dynamic-while-while", stack_config=4, stream=stream.cuda_stream)
 
     ms_sep = time_calls(torch, separate, stream, reps=3)
     ms_bat = time_calls(torch, batched, stream, reps=3)
