@@ -167,8 +167,10 @@ def bench_alpha(torch, hiprt, ctx, stream, d_cam, d_diff):
     out = {"texture": "16x16 checkerboard alpha 0.2/0.9 on every material"}
     for name, rays in (("primary", d_cam), ("diffuse", d_diff)):
         for tracer in ("dynamic-while-while", "dynamic-while-while-alpha"):
-            ms, _ = time_trace(torch, hiprt, ds, rays, tracer, stream)
-            out[f"{name}/{tracer}"] = round(rays.shape[1] / ms / 1e3, 1)
+            for kw in ({}, {"stack_config": 4}):
+                ms, _ = time_trace(torch, hiprt, ds, rays, tracer, stream, **kw)
+                tag = "".join(f",{k}={v}" for k, v in kw.items())
+                out[f"{name}/{tracer}{tag}"] = round(rays.shape[1] / ms / 1e3, 1)
     ds.close()
     return out
 
