@@ -33,11 +33,16 @@
 //     of the XCD it runs on (the dispatcher places workgroup b on XCD b % 8)
 //     and moves on to the next chunk when its own runs dry.  Neighbouring
 //     rays -- neighbouring pixels, neighbouring path origins -- then share
-//     one XCD's L2 instead of being spread over all eight.
+//     one XCD's L2 instead of being spread over all eight;
+//   * a register cap for single-scene binary-tree kernels with a 20-entry
+//     LDS stack and no VGPR entries (waves_hint): 8 waves per SIMD instead
+//     of the 7 their SGPR count would allow.
 //
 // Postponement, refill and 4-wide nodes change the order in which boxes see
 // the running best t, so results can differ from the plain kernel only
-// between triangles with exactly equal t.
+// between triangles whose t agree within rounding (ties); which of them a
+// postponing kernel returns depends on when the lane tests its parked
+// leaves, a wave-wide vote, and so can vary between runs.
 #include "kernels.h"
 #include "trace_core.h"
 
