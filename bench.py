@@ -310,7 +310,7 @@ def main():
     # pixel in flight (consecutive frames' samples traced together, so every
     # launch holds 2-8x the rays) and the binary tree with stack configuration
     # 4 (a 20-entry LDS stack with registers capped for 8 waves per SIMD) and 8,
-    # 16 or 32 samples in flight, when it beats
+    # 16, 32 or 64 samples in flight, when it beats
     # the library defaults (binary tree, dynamic tracer, stack 0, eight work
     # chunks, one sample) by more than 2%, the defaults otherwise, which keeps
     # the choice stable between runs.  The autotuner's sample is the first
@@ -330,6 +330,8 @@ def main():
                                                              samples=16)),
                   "32 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
                                                              samples=32)),
+                  "64 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
+                                                             samples=64)),
                   "library defaults": (ds, {})}
     # Rates, not frame times: a render call with k samples in flight traces k
     # samples per pixel.
