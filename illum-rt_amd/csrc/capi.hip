@@ -663,12 +663,14 @@ static const struct {
     {"if-if", HIPRT_TRACER_WHILE_WHILE},
     {"persistent-if-if", HIPRT_TRACER_PERSISTENT_WHILE_WHILE},
     {"speculative-while-while", HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE},
+    // the reference's links (rt/cuda/platform.cpp:55-56): the deterministic
+    // while-while traversal
+    {"default", HIPRT_TRACER_WHILE_WHILE},
+    {"find closest hits", HIPRT_TRACER_WHILE_WHILE},
 };
 
 int hiprt_tracer_id(const char *name) {
     if (!name) return -1;
-    if (std::strcmp(name, "default") == 0) return HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
-    if (std::strcmp(name, "default-alpha") == 0) return HIPRT_TRACER_DYNAMIC_WHILE_WHILE | HIPRT_TRACER_ALPHA;
     for (int a = 0; a < 2; ++a)
         for (int i = 0; i < HIPRT_TRACER_COUNT; ++i)
             if (std::strcmp(name, kTracerNames[a][i]) == 0) return i | (a ? HIPRT_TRACER_ALPHA : 0);

@@ -8,11 +8,13 @@
 // (libgi/rt.cpp:7-15); cosine-weighted bounce about the normal flipped to the
 // ray side (rt/cpu/bounce.cpp:41-72); a disabled ray is o = 0, d = (1,1,1),
 // t_max = -FLT_MAX (bounce.cpp:50-51).  The path loop is simple_pt's
-// (gi/pt.cpp:57-119) with two fixed choices of this workload: emission is
-// added and the path continues (emitters also reflect), and Russian roulette
-// keeps a path with a fixed probability from level `rr_start` on.  Each
-// path/level draws from its own PCG32 stream, so both forms produce the same
-// paths.
+// (gi/pt.cpp:72-115): a miss ends the path (no sky light), an emitter adds
+// throughput x emission and ends it (unless emitters_reflect is set), a
+// zero throughput ends it; with two fixed choices of this workload: the
+// bounce is the cosine-weighted wavefront step above (throughput x albedo),
+// and Russian roulette keeps a path with a fixed probability from level
+// `rr_start` on.  Each path/level draws from its own PCG32 stream, so every
+// schedule produces the same paths.
 #include "kernels.h"
 #include "pt_kernels.h"
 
@@ -43,8 +45,8 @@ __device__ __forceinline__ vec3f f3(float4 v) { return vec3f{v.x, v.y, v.z}; }
 
 // Shading of one hit.  Returns true and the next ray when the path goes on;
 // T is the path's throughput, updated when it does.  Emission is added to
-// the path's radiance only where the material emits (adding a zero product
-// leaves the sum unchanged, so every schedule accumulates the same bits).
+// the path's radiance only where the material emits, and ends the path there
+// unless emitters reflect.
 struct shade_out {
     bool live;
     vec3f org, dir;
@@ -64,6 +66,7 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
         rad[3 * path + 0] += T.x * emi.x;
         rad[3 * path + 1] += T.y * emi.y;
         rad[3 * path + 2] += T.z * emi.z;
+        if (!pp.emitters_reflect) return so;
     }
     if (level >= pp.max_depth) return so;
     const uint32_t sample = path / pp.n_px, pixel = path - sample * pp.n_px;
@@ -79,7 +82,8 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     so.dir = cosine_bounce(n, in_dir, xi0, xi1);
     so.org = hit_point(A, B, C, beta, gamma);
     T = vec3f{T.x * alb.x * scale, T.y * alb.y * scale, T.z * alb.z * scale};
-    so.live = true;
+    // a path that can no longer carry light ends (gi/pt.cpp:110-111)
+    so.live = T.x != 0.0f || T.y != 0.0f || T.z != 0.0f;
     return so;
 }
 

@@ -32,6 +32,10 @@ struct pt_params {
     // Pixels per sample: path p is pixel p % n_px of sample p / n_px, whose
     // frame seed is frame_seed[p / n_px] (several samples in flight at once).
     uint32_t n_px;
+    // 0: a path that meets an emitter adds its emission and ends there, as
+    // simple_pt::path does (gi/pt.cpp:86-89); 1: it adds the emission and
+    // bounces on like any other surface.
+    int emitters_reflect;
 };
 
 // Camera rays for n_paths paths (0: one per pixel), path i looking through

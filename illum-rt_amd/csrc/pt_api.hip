@@ -342,7 +342,8 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->n_px = (uint32_t)npix;
     p->samples = samples;
     p->n = (uint32_t)npix * samples;
-    p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive, (uint32_t)npix};
+    p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive, (uint32_t)npix,
+                               params->emitters_reflect ? 1 : 0};
     p->seed = params->seed;
     p->batched = params->batched != 0;
     int tracer = params->tracer;

@@ -283,11 +283,13 @@ platform::platform(const std::vector<std::string> &args) : wf::platform("hip") {
             std::string n = hiprt_tracer_name(id);
             tracers[n] = [id]() -> wf::batch_ray_tracer * { return new batch_rt(id); };
         }
-    link_tracer(hiprt_tracer_name(hiprt_tracer_id("default")), "default");
-    // the reference's other CUDA loop structures of the same near-first
-    // traversal (rt/cuda/platform.cpp:107-122) select the wave64 kernel that
-    // runs it, so scripts naming them run unchanged (hiprt_tracer_id agrees)
-    for (const char *alias : {"simple", "if-if", "persistent-if-if", "speculative-while-while"})
+    // "default" and "find closest hits" link to while-while as in
+    // rt/cuda/platform.cpp:55-56; the reference's other CUDA loop structures
+    // of the same near-first traversal (rt/cuda/platform.cpp:107-122) select
+    // the wave64 kernel that runs it, so scripts naming them run unchanged
+    // (hiprt_tracer_id agrees)
+    link_tracer(hiprt_tracer_name(hiprt_tracer_id("find closest hits")), "find closest hits");
+    for (const char *alias : {"default", "simple", "if-if", "persistent-if-if", "speculative-while-while"})
         for (const char *suffix : {"", "-alpha"}) {
             std::string n = std::string(alias) + suffix;
             link_tracer(hiprt_tracer_name(hiprt_tracer_id(n.c_str())), n);

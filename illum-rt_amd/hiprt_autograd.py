@@ -58,5 +58,5 @@ class ClosestHit(torch.autograd.Function):
         return gv, go, gd, None, None
 
 
-def closest_hit(dscene, vertices, origins, directions, tracer="dynamic-while-while"):
+def closest_hit(dscene, vertices, origins, directions, tracer="persistent-while-while"):
     return ClosestHit.apply(vertices, origins, directions, dscene, tracer)

@@ -195,8 +195,9 @@ typedef struct {
 
 /* Tracer variants, named as the reference registers its CUDA tracers
  * (rt/cuda/platform.cpp:34-56).  hiprt_tracer_id maps a name to an id, -1 if
- * unknown; "default" is "dynamic-while-while".  The reference's other loop
- * structures of the same traversal name the kernel that runs it here:
+ * unknown; "default" and "find closest hits" are "while-while", the links of
+ * rt/cuda/platform.cpp:55-56.  The reference's other loop structures of the
+ * same traversal name the kernel that runs it here:
  * "simple" and "if-if" -> while-while, "persistent-if-if" ->
  * persistent-while-while, "speculative-while-while" ->
  * persistent-speculative-while-while (each also with "-alpha");
@@ -330,9 +331,12 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
                           const float *power, float *d_framebuffer, void *stream);
 
 /* ------------------------------------------------------- wavefront path tracer
- * A diffuse path tracer over one committed scene and camera: camera rays,
- * closest hit, shading (emission added, cosine-weighted bounce, throughput
- * times albedo), Russian roulette from level rr_start on.  Two schedules with
+ * A diffuse path tracer over one committed scene and camera, with the control
+ * flow of simple_pt::path (gi/pt.cpp:72-115): camera rays, closest hit; a
+ * miss ends the path, an emitter adds throughput x emission and ends it; else
+ * a cosine-weighted bounce (throughput times albedo; a zero throughput ends
+ * the path) and Russian roulette with a fixed survival probability from
+ * level rr_start on.  Two schedules with
  * identical per-path results:
  *   batched = 0: one launch per step per level over all w*h slots, dead paths
  *                kept as disabled rays (the reference's wavefront form);
@@ -361,6 +365,9 @@ typedef struct {
     int32_t blocks;
     int32_t ray_order;      /* batched: 0 = sorted bounce queues, 1 = shading order */
     int32_t samples;        /* batched: samples per pixel traced together (0 or 1: one) */
+    int32_t emitters_reflect; /* 0 (default): a path ends at the first emitter it meets, after
+                                 adding throughput x emission (gi/pt.cpp:86-89); 1: emitters
+                                 also reflect and the path goes on */
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;

@@ -188,19 +188,39 @@ struct tree {
     const uint32_t *index;
 };
 
+// Traversal stack: a fixed array, as the reference's uint32_t stack[25]
+// (rt/cpu/bvh2.cpp:76), continued on the heap past 64 entries, where the
+// reference's would already have overflowed.  No allocation per ray.
+struct trav_stack {
+    uint32_t fixed[64];
+    std::vector<uint32_t> more;
+    uint32_t n = 0;
+    bool empty() const { return n == 0; }
+    void push_back(uint32_t v) {
+        if (n < 64) fixed[n] = v;
+        else more.push_back(v);
+        ++n;
+    }
+    uint32_t pop() {
+        --n;
+        if (n < 64) return fixed[n];
+        uint32_t v = more.back();
+        more.pop_back();
+        return v;
+    }
+};
+
 // binary_bvh_tracer::closest_hit, rt/cpu/bvh2.cpp:72-137.  counts[0] += inner
 // nodes visited, counts[1] += triangles tested (when counts is non-null).
 hit closest_hit(const scene_view &s, const tree &t, const ray &ray, uint32_t *counts,
                 const alpha_view *av = nullptr) {
     hit closest{FLT_MAX, 0, 0, 0}, intersection{FLT_MAX, 0, 0, 0};
     if (t.n_nodes == 0) return closest;
-    std::vector<uint32_t> stack;
-    stack.reserve(64);
+    trav_stack stack;
     stack.push_back(t.root);
     uint32_t n_inner = 0, n_tri = 0;
     while (!stack.empty()) {
-        const node &nd = t.nodes[stack.back()];
-        stack.pop_back();
+        const node &nd = t.nodes[stack.pop()];
         if (nd.inner()) {
             ++n_inner;
             float dist_l, dist_r;
@@ -246,12 +266,10 @@ hit closest_hit(const scene_view &s, const tree &t, const ray &ray, uint32_t *co
 bool any_hit(const scene_view &s, const tree &t, const ray &ray, const alpha_view *av = nullptr) {
     hit intersection{FLT_MAX, 0, 0, 0};
     if (t.n_nodes == 0) return false;
-    std::vector<uint32_t> stack;
-    stack.reserve(64);
+    trav_stack stack;
     stack.push_back(t.root);
     while (!stack.empty()) {
-        const node &nd = t.nodes[stack.back()];
-        stack.pop_back();
+        const node &nd = t.nodes[stack.pop()];
         if (nd.inner()) {
             float dist_l, dist_r;
             bool hit_l = intersect4(nd.l, ray, dist_l);
@@ -654,6 +672,211 @@ void orc_cosine_rays(const float *pos, const uint32_t *idx, const float *const *
         out[6][i] = d[2];
         out[7][i] = tmax;
     }
+}
+
+} // extern "C"
+
+namespace {
+
+// ---- wavefront path tracer of the workload --------------------------------
+//
+// PCG32 of the reference's rng_pcg (libgi/random.h:59-96, PBRT-v3's PCG32):
+// state transition, XSH-RR output, stream selection, float mapping
+// min(1 - 2^-24, u * 2^-32).
+struct pcg32 {
+    uint64_t state, inc;
+    explicit pcg32(uint64_t initseq, uint64_t initstate = 0x853c49e6748fea9bULL) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next_u32();
+        state += initstate;
+        next_u32();
+    }
+    uint32_t next_u32() {
+        uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31));
+    }
+    float next_float() {
+        float f = (float)next_u32() * 0x1p-32f;
+        return f < 0x1.fffffep-1f ? f : 0x1.fffffep-1f;
+    }
+};
+
+// Stream of one (frame seed, path, bounce): SplitMix64's finaliser over
+// seed * golden + (path << 8) + bounce + 1.  This is the workload's stream
+// layout (the reference's wavefront steps draw from per-thread generators,
+// which no GPU schedule can reproduce); the device kernels use the same.
+uint64_t stream_of(uint64_t seed, uint64_t path, uint64_t bounce) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (path << 8) + bounce + 1;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// camera::update_frustum + cam_ray (libgi/camera.h:18-24, camera.cpp:44-61),
+// pixel-centre offset 0.
+struct pinhole {
+    v3 P, D, U, V;
+    float near_w, near_h;
+    int w, h;
+    pinhole(const float *cam) {
+        P = v3{cam[0], cam[1], cam[2]};
+        D = v3{cam[3], cam[4], cam[5]};
+        v3 Up{cam[6], cam[7], cam[8]};
+        w = (int)cam[10];
+        h = (int)cam[11];
+        float aspect = float(w) / h;
+        near_h = tanf(float(M_PI) * cam[9] * 0.5f / 180.0f);
+        near_w = aspect * near_h;
+        U = normalize(cross(D, Up));
+        V = normalize(cross(U, D));
+    }
+    ray at(int x, int y) const {
+        float uu = ((float)x + 0.5f + 0.0f) / (float)w * 2.0f - 1.0f;
+        float vv = ((float)y + 0.5f + 0.0f) / (float)h * 2.0f - 1.0f;
+        uu = near_w * uu;
+        vv = near_h * vv;
+        return make_ray(P, normalize(add(add(D, mul(U, uu)), mul(V, vv))), 1e-4f, FLT_MAX);
+    }
+};
+
+// Cosine-weighted direction about the geometric normal flipped to the side
+// the ray came from: sample_cos_weighted_dir (rt/cpu/bounce.cpp:41-72) with
+// flip_normals_to_ray (libgi/util.h:226-233), uniform_sample_disk +
+// cosine_sample_hemisphere (libgi/sampling.h:13-36) and align
+// (libgi/util.h:212-218).  ns is the shading normal that decides the flip.
+v3 cosine_dir(v3 ng, v3 ns, v3 in_d, float xi0, float xi1) {
+    const float pi = float(M_PI);
+    if (dot(in_d, ns) > 0) ng = v3{-ng.x, -ng.y, -ng.z};
+    float r = sqrtf(xi0);
+    float theta = 2 * pi * xi1;
+    float dx = r * cosf(theta), dy = r * sinf(theta);
+    float z = 1 - dx * dx - dy * dy;
+    v3 s{dx, dy, z > 0 ? sqrtf(z) : 0.f};
+    float sg = copysignf(1.f, ng.z);
+    v3 w{s.x, s.y, s.z * sg};
+    v3 hv{ng.x, ng.y, ng.z + sg};
+    float k = dot(w, hv) / (1.f + (ng.z < 0 ? -ng.z : ng.z));
+    return sub(smul(k, hv), w);
+}
+
+} // namespace
+
+extern "C" {
+
+// First n floats (next_float) of the PCG32 generator seeded with
+// (initstate, initseq), for known-answer checks; raw != 0 writes next_u32
+// values (as uint32 bits) instead.
+void orc_pcg_sequence(uint64_t initstate, uint64_t initseq, uint32_t n, int raw, uint32_t *out) {
+    pcg32 g(initseq, initstate);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (raw) {
+            out[i] = g.next_u32();
+        } else {
+            float f = g.next_float();
+            std::memcpy(&out[i], &f, 4);
+        }
+    }
+}
+
+// The workload's diffuse path tracer: the control flow of simple_pt::path
+// (gi/pt.cpp:72-115) with this workload's fixed choices.  Per frame seed s
+// and pixel p (path p of that frame), from the camera ray of p:
+//   level L = 0 .. max_depth: closest hit of the current ray (the BVH
+//   traversal above); a miss ends the path (no sky light, gi/pt.cpp:79-83);
+//   an emissive material adds throughput * emission and ends the path
+//   (gi/pt.cpp:86-89) -- or, with emitters_reflect, adds it and goes on;
+//   at L = max_depth the path ends; otherwise two uniforms from the PCG32
+//   stream (s, p, L) pick a cosine-weighted bounce (the wavefront step of
+//   rt/cpu/bounce.cpp:41-72: origin = hit point, t_min 1e-4); when
+//   L + 1 >= rr_start a third uniform xi keeps the path iff xi < rr_survive
+//   (fixed-probability roulette instead of the luma-based one of
+//   gi/pt.cpp:102-109) and scales by 1/rr_survive; the throughput is
+//   multiplied by the albedo (f * cos / pdf of the cosine-sampled
+//   Lambertian, albedo/pi * cos / (cos/pi)), and a zero throughput ends the
+//   path (gi/pt.cpp:110-111).
+// image (w*h*3) accumulates each pixel's path radiance frame by frame, in
+// the order of frame_seeds; level_rays[L] (max_depth + 1 entries, may be
+// null) accumulates the rays traced at level L; sig (w*h, may be null)
+// receives, per pixel, a hash of the last frame's hit triangle ids and
+// levels.  mats: 8 floats per material (albedo rgb, 0, emission rgb, 0);
+// mat: material per triangle.  vn (3 floats per vertex, may be null):
+// shading normals interpolated at the hit to decide the normal flip, as
+// flip_normals_to_ray does with diff_geom::ns; null uses the geometric normal.
+void orc_path_trace(const float *pos, const uint32_t *idx, uint32_t n_tris, const uint32_t *mat, const float *mats,
+                    const float *vn, const void *nodes, uint32_t n_nodes, uint32_t root, const uint32_t *index,
+                    const float *cam, int max_depth, int rr_start, float rr_survive, int emitters_reflect,
+                    const uint64_t *frame_seeds, uint32_t n_frames, float *image, uint64_t *level_rays,
+                    uint64_t *sig, int threads) {
+    scene_view s{pos, idx, n_tris};
+    tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
+    pinhole ph(cam);
+    const int64_t n_px = (int64_t)ph.w * ph.h;
+    const int levels = max_depth + 1;
+    std::vector<uint64_t> counts((size_t)levels, 0);
+#pragma omp parallel num_threads(set_threads(threads))
+    {
+        std::vector<uint64_t> mine((size_t)levels, 0);
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t p = 0; p < n_px; ++p) {
+            int x = (int)(p % ph.w), y = (int)(p / ph.w);
+            uint64_t h = 0;
+            for (uint32_t f = 0; f < n_frames; ++f) {
+                ray r = ph.at(x, y);
+                v3 T{1.0f, 1.0f, 1.0f};
+                h = 1469598103934665603ULL;
+                for (int L = 0; L <= max_depth; ++L) {
+                    ++mine[(size_t)L];
+                    hit c = closest_hit(s, tr, r, nullptr);
+                    h = (h ^ (c.t == FLT_MAX ? 0xFFFFFFFFull : (uint64_t)c.ref)) * 1099511628211ULL;
+                    h = (h ^ (uint64_t)L) * 1099511628211ULL;
+                    if (c.t == FLT_MAX) break;
+                    uint32_t tri = c.ref;
+                    const float *m = mats + 8 * (size_t)(mat ? mat[tri] : 0u);
+                    v3 alb{m[0], m[1], m[2]}, emi{m[4], m[5], m[6]};
+                    if (emi.x != 0.0f || emi.y != 0.0f || emi.z != 0.0f) {
+                        float *px = image + 3 * p;
+                        px[0] += T.x * emi.x;
+                        px[1] += T.y * emi.y;
+                        px[2] += T.z * emi.z;
+                        if (!emitters_reflect) break;
+                    }
+                    if (L >= max_depth) break;
+                    pcg32 g(stream_of(frame_seeds[f], (uint64_t)p, (uint64_t)L));
+                    float xi0 = g.next_float(), xi1 = g.next_float();
+                    float scale = 1.0f;
+                    if (L + 1 >= rr_start) {
+                        float xi = g.next_float();
+                        if (!(xi < rr_survive)) break;
+                        scale = 1.0f / rr_survive;
+                    }
+                    v3 A = s.p(idx[3 * tri]), B = s.p(idx[3 * tri + 1]), C = s.p(idx[3 * tri + 2]);
+                    float beta = c.beta, gamma = c.gamma;
+                    v3 xh = add(add(smul(1.0f - beta - gamma, A), smul(beta, B)), smul(gamma, C));
+                    v3 ng = normalize(cross(sub(B, A), sub(C, A)));
+                    v3 ns = ng;
+                    if (vn) {
+                        auto nv = [&](uint32_t v) { return v3{vn[3 * v], vn[3 * v + 1], vn[3 * v + 2]}; };
+                        ns = normalize(add(add(smul(1.0f - beta - gamma, nv(idx[3 * tri])),
+                                               smul(beta, nv(idx[3 * tri + 1]))),
+                                           smul(gamma, nv(idx[3 * tri + 2]))));
+                    }
+                    v3 d = cosine_dir(ng, ns, r.d, xi0, xi1);
+                    T = v3{T.x * alb.x * scale, T.y * alb.y * scale, T.z * alb.z * scale};
+                    if (T.x == 0.0f && T.y == 0.0f && T.z == 0.0f) break;
+                    r = make_ray(xh, d, 1e-4f, FLT_MAX);
+                }
+            }
+            if (sig) sig[p] = h;
+        }
+#pragma omp critical
+        for (int L = 0; L < levels; ++L) counts[(size_t)L] += mine[(size_t)L];
+    }
+    if (level_rays)
+        for (int L = 0; L < levels; ++L) level_rays[L] += counts[(size_t)L];
 }
 
 // Wall-clock seconds of one orc_trace_closest call (the CPU baseline leg).

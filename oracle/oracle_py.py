@@ -244,3 +244,45 @@ def adjoint_fp64(pos, idx, tri, rays, gt, gb, gg, fd_step=None):
         lib().orc_adjoint_fd(_p(pos), _p(idx), _p(tri), planes, c_uint32(n), _p(g[0]), _p(g[1]), _p(g[2]),
                              c_double(fd_step), _p(out))
     return out
+
+
+def pcg_sequence(initstate, initseq, n, raw=True):
+    """First n outputs of PCG32 seeded with (initstate, initseq): uint32 values
+    (raw) or next_float values."""
+    out = np.empty(n, np.uint32)
+    lib().orc_pcg_sequence(ctypes.c_uint64(initstate), ctypes.c_uint64(initseq), c_uint32(n), c_int(1 if raw else 0),
+                           _p(out))
+    return out if raw else out.view(np.float32)
+
+
+def frame_seeds(seed, first_frame, n_frames):
+    """Frame seeds of render frames first_frame .. first_frame + n_frames - 1:
+    seed * 0x100000001B3 + frame (mod 2^64)."""
+    base = (int(seed) * 0x100000001B3) % (1 << 64)
+    return np.array([(base + first_frame + k) % (1 << 64) for k in range(n_frames)], np.uint64)
+
+
+def path_trace(pos, idx, mat, materials, nodes, root, index, cam, seeds, max_depth=4, rr_start=3, rr_survive=0.8,
+               emitters_reflect=False, normals=None, threads=0):
+    """The workload's diffuse path tracer over the frame seeds `seeds`, in
+    order.  materials: (n_mat, 8) albedo rgb, 0, emission rgb, 0; mat: material
+    per triangle (None: material 0).  Returns image (h, w, 3) accumulated over
+    the frames, rays traced per level, and a per-pixel signature of the last
+    frame's hit sequence."""
+    pos, idx = _scene(pos, idx)
+    cam = np.ascontiguousarray(cam[:12], dtype=np.float32)
+    w, h = int(cam[10]), int(cam[11])
+    mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, 8)
+    mat = None if mat is None else np.ascontiguousarray(mat, dtype=np.uint32)
+    vn = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    nodes = np.ascontiguousarray(nodes)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    img = np.zeros((h, w, 3), np.float32)
+    levels = np.zeros(max_depth + 1, np.uint64)
+    sig = np.zeros(w * h, np.uint64)
+    lib().orc_path_trace(_p(pos), _p(idx), c_uint32(len(idx)), _p(mat), _p(mats), _p(vn), _p(nodes),
+                         c_uint32(len(nodes)), c_uint32(root), _p(index), _p(cam), c_int(max_depth), c_int(rr_start),
+                         c_float(rr_survive), c_int(1 if emitters_reflect else 0), _p(seeds), c_uint32(len(seeds)),
+                         _p(img), _p(levels), _p(sig), c_int(threads))
+    return img, levels, sig.reshape(h, w)

@@ -57,7 +57,7 @@ commit
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tracer", ["while-while", "simple", "persistent-if-if"])
+@pytest.mark.parametrize("tracer", ["while-while", "default", "simple", "persistent-if-if"])
 def test_primary_wf_image_matches_oracle(tmp_path, torch_cuda, tracer):
     print(f"case: rtgi-hip primary-wf, 8k random triangles, 160x120, raytracer {tracer}", flush=True)
     w, h = 160, 120
@@ -138,7 +138,7 @@ def test_direct_wf_shadows_match_oracle(tmp_path, torch_cuda):
 def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options, samples):
     print(f"case: rtgi-hip pt-wf, hall 128x72, 2 spp vs hiprt.PathTracer, options {options!r}", flush=True)
     w, h = 128, 72
-    r = run_script(SCRIPT_HEAD.format(cfg=3, w=w, h=h, tracer="dynamic-while-while") + options +
+    r = run_script(SCRIPT_HEAD.format(cfg=3, w=w, h=h, tracer="persistent-while-while") + options +
                    "sppx 2\nalgo pt-wf\noutfile pt.pfm\nrun\n", tmp_path, timeout=300)
     assert r.returncode == 0 and not r.stderr.strip(), r.stderr
     img = read_pfm(tmp_path / "pt.pfm")
@@ -149,7 +149,7 @@ def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options, sample
     cam = sc.cam.copy()
     cam[10], cam[11] = w, h
     # with k samples per render call, calls 0 .. 2/k - 1 trace frames 0 and 1
-    pt = hiprt.PathTracer(ds, cam, seed=1, batched=True, samples=samples)
+    pt = hiprt.PathTracer(ds, cam, seed=1, batched=True, samples=samples, tracer="persistent-while-while")
     for f in range(2 // samples):
         pt.render(f)
     want = pt.image() / np.float32(2)

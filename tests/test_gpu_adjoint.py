@@ -127,5 +127,5 @@ def test_autograd_function(c5):
     assert scale > 0 and diff <= 1e-4 * max(1.0, scale)
     # hits reported by the wrapper are the plain trace's
     h2 = hiprt.new_hits(t.shape[0])
-    hiprt.trace_closest(ds, rays.contiguous(), h2, tracer="dynamic-while-while")
+    hiprt.trace_closest(ds, rays.contiguous(), h2, tracer="persistent-while-while")
     assert torch.equal(h2["id"], tri) and torch.equal(h2["t"], t.detach())

@@ -137,8 +137,8 @@ def test_path_tracer_alpha(torch_cuda, alpha_scene):
     s = alpha_scene
     cam = np.array([0.5, 0.5, -1.0, 0, 0, 1, 0, 1, 0, 60, 200, 150], np.float32)
     counts = {}
-    for name, batched, tracer in (("plain", False, "while-while-alpha"), ("batched", True, "dynamic-while-while-alpha"),
-                                  ("batched, no alpha", True, "dynamic-while-while")):
+    for name, batched, tracer in (("plain", False, "while-while-alpha"), ("batched", True, "persistent-while-while-alpha"),
+                                  ("batched, no alpha", True, "persistent-while-while")):
         print(f"case: alpha scene path tracer, {name}", flush=True)
         pt = hiprt.PathTracer(s["ds2"], cam, batched=batched, seed=3, tracer=tracer)
         pt.render(0)

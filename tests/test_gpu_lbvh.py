@@ -186,7 +186,8 @@ def test_hall_path_tracer_and_refit(torch_cuda, hip_ctx):
     cam[10], cam[11] = 320, 180
     imgs = []
     for dsc, batched in ((ds, False), (ds, True), (ds_host, False)):
-        pt = hiprt.PathTracer(dsc, cam, batched=batched, seed=4)
+        pt = hiprt.PathTracer(dsc, cam, batched=batched, seed=4,
+                              tracer="persistent-while-while" if batched else None)
         pt.render(0)
         imgs.append(pt.image())
         pt.close()

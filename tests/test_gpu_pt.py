@@ -10,7 +10,11 @@
   the same per-level ray counts and the same image as the plain per-bounce
   one, with bounce queues sorted by ray key or in shading order (a ray
   scattered to the wrong slot would continue another pixel's path), and its
-  level counts follow the Russian-roulette law.
+  level counts follow the Russian-roulette law.  These bit-for-bit checks run
+  the persistent-while-while tracer, whose hits never depend on the schedule;
+  the speculative and dynamic tracers may pick the other triangle of an
+  equal-t tie and are checked against the oracle with its tolerance
+  (tests/test_gpu_pt_oracle.py).
 """
 import numpy as np
 import pytest
@@ -87,7 +91,7 @@ def test_batched_equals_plain(torch_cuda, hall, res, stack, chunks, sort):
     cam[10], cam[11] = res
     plain = hiprt.PathTracer(hall["ds"], cam, batched=False, seed=5)
     batched = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=5, stack_config=stack, work_chunks=chunks,
-                               sort_rays=sort)
+                               sort_rays=sort, tracer="persistent-while-while")
     counts = []
     for pt in (plain, batched):
         pt.render(0)
@@ -125,7 +129,8 @@ def test_small_frames_and_depths(torch_cuda, hall, res, depth):
     cam[10], cam[11] = res
     imgs, counts = [], []
     for batched in (False, True):
-        pt = hiprt.PathTracer(hall["ds"], cam, max_depth=depth, batched=batched, seed=11)
+        pt = hiprt.PathTracer(hall["ds"], cam, max_depth=depth, batched=batched, seed=11,
+                              tracer="persistent-while-while" if batched else None)
         pt.render(0)
         pt.render(1)
         imgs.append(pt.image())
@@ -139,15 +144,14 @@ def test_small_frames_and_depths(torch_cuda, hall, res, depth):
 def test_samples_in_flight(torch_cuda, hall, samples, res):
     """samples > 1: one render call traces the samples of `samples`
     consecutive frames together.  Per level the same rays as rendering those
-    frames one by one (ray totals equal), and the same image up to the order
-    in which radiance is summed (per sample first, then into the image)."""
+    frames one by one (ray totals equal), and the same image."""
     print(f"case: hall path tracer {res}, {samples} samples in flight vs one at a time", flush=True)
     cam = hall["cam"].copy()
     cam[10], cam[11] = res
-    one = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=13)
+    one = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=13, tracer="persistent-while-while")
     for f in range(2 * samples):
         one.render(f)
-    many = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=13, samples=samples)
+    many = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=13, samples=samples, tracer="persistent-while-while")
     many.render(0)
     many.render(1)
     assert one.ray_totals().tolist() == many.ray_totals().tolist()
@@ -155,14 +159,16 @@ def test_samples_in_flight(torch_cuda, hall, samples, res):
     one.close()
     many.close()
     assert a.sum() > 0
-    assert np.allclose(a, b, rtol=2e-6, atol=1e-6 * float(np.abs(a).max()))
+    # a path adds its radiance once (at the emitter that ends it), so summing
+    # per sample first changes no bits
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_graph_replays_are_stable(torch_cuda, hall):
     print("case: hall path tracer 256x144, batched graph replayed 4x vs eager", flush=True)
     cam = hall["cam"].copy()
     cam[10], cam[11] = 256, 144
-    a = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=9)
+    a = hiprt.PathTracer(hall["ds"], cam, batched=True, seed=9, tracer="persistent-while-while")
     b = hiprt.PathTracer(hall["ds"], cam, batched=False, seed=9)
     imgs = []
     for pt in (a, b):

@@ -1,0 +1,94 @@
+"""The GPU path tracer against the oracle's path loop (oracle.cpp:
+orc_path_trace, pinned in tests/test_oracle_pt.py): per-pixel radiance and
+rays per level on hall frames, same camera, same PCG32 streams.
+
+Both sides trace bit-identical hits for identical rays (tests/test_gpu_trace.py)
+and add one throughput x emission per path, so a pixel whose paths meet the
+same triangles has the same bits on both sides.  Bounce directions go through
+cosf/sinf, whose device (ocml) and host (glibc) results differ in the last
+bit for some arguments; a path whose direction differs that way can meet
+another triangle where it passes within about 1e-7 of an edge, or (for the
+speculative and dynamic tracers) another triangle of an equal-t tie, and from
+there its pixel differs.  The bar, per case:
+  * at most MAX_DIFF_PATHS of the paths traced may have taken another way:
+    the pixels whose bits differ are at most that fraction of all paths;
+  * the image sums agree within SUM_REL, and the rays traced per level
+    within COUNT_REL (a diverged path changes a level's count by at most one).
+"""
+import numpy as np
+import pytest
+
+import hiprt
+import oracle_py as orc
+
+pytestmark = pytest.mark.gpu
+
+MAX_DIFF_PATHS = 2e-4
+SUM_REL = 2e-3
+COUNT_REL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def hall(hip_ctx):
+    sc = hiprt.HostScene.generate(3)
+    pos, idx, mat = sc.arrays()
+    bvh = hiprt.Bvh(sc)
+    ds = hiprt.DeviceScene(hip_ctx, sc, bvh)
+    yield dict(sc=sc, pos=pos, idx=idx, mat=mat, mats=sc.materials(), bvh=bvh, ds=ds, cam=sc.cam)
+    ds.close()
+
+
+def _oracle(h, cam, seed, n_frames, **kw):
+    seeds = orc.frame_seeds(seed, 0, n_frames)
+    b = h["bvh"]
+    img, levels, _ = orc.path_trace(h["pos"], h["idx"], h["mat"], h["mats"], b.nodes, b.root, b.index, cam, seeds,
+                                    **kw)
+    return img, levels
+
+
+CASES = [
+    # name, PathTracer options, frame size, frames, render calls
+    ("plain", dict(batched=False), (320, 180), 2, 2),
+    ("batched, persistent-while-while, sorted queues, stack 4",
+     dict(batched=True, tracer="persistent-while-while", stack_config=4), (320, 180), 2, 2),
+    ("batched, persistent-while-while, shading order, one counter",
+     dict(batched=True, tracer="persistent-while-while", sort_rays=False, work_chunks=1), (333, 97), 2, 2),
+    ("batched, dynamic-while-while, sorted queues, stack 4, 64 samples in flight",
+     dict(batched=True, stack_config=4, samples=64), (80, 45), 64, 1),
+    ("batched, speculative, 4-wide tree, 8 samples in flight",
+     dict(batched=True, tracer="persistent-speculative-while-while", samples=8, bvh4=True), (160, 90), 8, 1),
+    ("plain, emitters reflect", dict(batched=False, emitters_reflect=True), (320, 180), 1, 1),
+]
+
+
+@pytest.mark.parametrize("name,opts,res,frames,calls", CASES, ids=[c[0] for c in CASES])
+def test_path_tracer_matches_oracle(torch_cuda, hip_ctx, hall, name, opts, res, frames, calls):
+    print(f"case: hall path tracer vs oracle, {name}, {res[0]}x{res[1]}, {frames} frames", flush=True)
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = res
+    opts = dict(opts)
+    ds = hall["ds"]
+    own = None
+    if opts.pop("bvh4", False):
+        own = ds = hiprt.DeviceScene(hip_ctx, hall["sc"], hall["bvh"], bvh_format=hiprt.BVH4)
+    reflect = opts.get("emitters_reflect", False)
+    pt = hiprt.PathTracer(ds, cam, seed=7, **opts)
+    for f in range(calls):
+        pt.render(f)
+    got, got_levels = pt.image(), pt.ray_totals()
+    pt.close()
+    if own is not None:
+        own.close()
+    want, want_levels = _oracle(hall, cam, 7, frames, emitters_reflect=reflect)
+    n_paths = res[0] * res[1] * frames
+    differ = (got.view(np.uint32) != want.view(np.uint32)).any(-1)
+    sum_rel = abs(float(got.sum(dtype=np.float64)) - float(want.sum(dtype=np.float64))) / float(
+        want.sum(dtype=np.float64))
+    level_diff = np.abs(got_levels.astype(np.int64) - want_levels.astype(np.int64))
+    print({"pixels_differ": int(differ.sum()), "paths": n_paths, "sum_rel": sum_rel,
+           "levels_gpu": got_levels.tolist(), "levels_oracle": want_levels.tolist()})
+    assert np.isfinite(got).all() and want.sum() > 0
+    assert want_levels[0] == n_paths and got_levels[0] == n_paths
+    assert differ.sum() <= MAX_DIFF_PATHS * n_paths
+    assert sum_rel <= SUM_REL
+    assert (level_diff <= COUNT_REL * want_levels + 1).all()

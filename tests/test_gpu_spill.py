@@ -147,7 +147,8 @@ def test_spilled_path_tracer(torch_cuda, deep):
                                           ("batched bvh2 stack 4", ds2, True, 4, True),
                                           ("batched bvh4", ds4, True, 3, True)):
         print(f"case: nested scene path tracer, {name}", flush=True)
-        pt = hiprt.PathTracer(ds, cam, batched=batched, seed=5, stack_config=stack, sort_rays=srt)
+        pt = hiprt.PathTracer(ds, cam, batched=batched, seed=5, stack_config=stack, sort_rays=srt,
+                              tracer="persistent-while-while" if batched else None)
         pt.render(0)
         imgs[name] = pt.image()
         pt.close()

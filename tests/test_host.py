@@ -30,7 +30,11 @@ def test_errors_are_reported_not_thrown():
     assert L.hiprt_scene_generate(9, ctypes.byref(h), None) == 1
     assert b"config" in L.hiprt_last_error()
     assert L.hiprt_scene_load_obj(b"/nonexistent.obj", ctypes.byref(h)) == 4
-    assert hiprt.tracer_id("default") == 3
+    # the reference links "default" and "find closest hits" to while-while
+    # (rt/cuda/platform.cpp:55-56)
+    assert hiprt.tracer_id("default") == hiprt.tracer_id("while-while") == 0
+    assert hiprt.tracer_id("find closest hits") == 0
+    assert hiprt.tracer_id("default-alpha") == hiprt.tracer_id("while-while-alpha")
     with pytest.raises(hiprt.HiprtError):
         hiprt.tracer_id("no-such-tracer")
     # the reference's other loop structures name the kernel of the same traversal
