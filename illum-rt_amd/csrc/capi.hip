@@ -21,7 +21,21 @@
 
 namespace {
 thread_local std::string g_err;
-}
+
+// Scratch of the allocation-free trace (hiprt_trace_closest_noalloc): the
+// persistent kernels' work counters followed by a traversal-stack spill
+// column.  One copy per device, held for the life of the process.  Host code
+// never uses the array's name as an address: hiprt_create resolves each
+// device's copy once with hipGetSymbolAddress and keeps it in
+// g_scratch_addr, and only that address reaches kernels.
+constexpr size_t kNoAllocSpillBytes = size_t(16) << 20;
+constexpr size_t kNoAllocScratchBytes = rt::dev::kCounterBytes + kNoAllocSpillBytes;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+void *g_scratch_addr[kMaxDevices] = {};
+} // namespace
+
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchBytes / 4];
 
 namespace hiprt_detail {
 int fail(int code, const std::string &msg) {
@@ -309,6 +323,17 @@ int hiprt_create(int32_t device, hiprt_context *out) {
     if (!c) return fail(HIPRT_ERR_ARG, "out of host memory");
     c->device = device;
     if (!c->state(nullptr)) return fail(HIPRT_ERR_HIP, "cannot allocate the work counters");
+    if (device >= kMaxDevices) return fail(HIPRT_ERR_UNSUPPORTED, "device index above 63");
+    {
+        // this device's copy of the allocation-free trace's scratch
+        std::lock_guard<std::mutex> g(g_scratch_lock);
+        if (!g_scratch_addr[device]) {
+            void *ptr = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            if (!ptr) return fail(HIPRT_ERR_HIP, "no device address for the scratch array");
+            g_scratch_addr[device] = ptr;
+        }
+    }
     *out = c.release();
     return ok();
 }
@@ -693,8 +718,10 @@ const char *hiprt_tracer_name(int tracer) {
 
 namespace {
 
+// noalloc: work counters and spill column from this device's static
+// scratch instead of the stream state's buffers (allocates nothing).
 int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *occluded, const hiprt_hits *hits,
-                 bool any, int32_t tracer, const hiprt_tracer_opts *opts, void *stream) {
+                 bool any, int32_t tracer, const hiprt_tracer_opts *opts, void *stream, bool noalloc = false) {
     if (!sc || !rays) return fail(HIPRT_ERR_ARG, "null argument");
     const bool alpha = (tracer & HIPRT_TRACER_ALPHA) != 0;
     tracer &= ~HIPRT_TRACER_ALPHA;
@@ -737,11 +764,37 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.stats = opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats) : nullptr;
     cfg.stack = stack;
     cfg.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
-    hiprt_stream_state *st = sc->ctx->state(stream);
-    if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
-    HIP_TRY(hiprt_detail::attach_spill(st, s, desc, any, false, need, cfg));
-    HIP_TRY(rt::dev::launch_trace_persistent(desc, r, h, occluded, n, nullptr,
-                                             st->counter.as<uint32_t>(), any, cfg, s));
+    uint32_t *counter = nullptr;
+    if (noalloc) {
+        char *base = nullptr;
+        {
+            std::lock_guard<std::mutex> g(g_scratch_lock);
+            base = static_cast<char *>(g_scratch_addr[sc->ctx->device]);
+        }
+        if (!base) return fail(HIPRT_ERR_HIP, "the device's scratch is not set up");
+        counter = reinterpret_cast<uint32_t *>(base);
+        cfg.spill_cap = rt::dev::persistent_spill_entries(desc.bvh, cfg.stack, need);
+        if (cfg.spill_cap) {
+            int blocks = 0;
+            HIP_TRY(rt::dev::persistent_grid(desc, any, cfg, &blocks));
+            const size_t bytes = rt::dev::persistent_spill_bytes(blocks, cfg.spill_cap);
+            if (bytes > kNoAllocSpillBytes)
+                return fail(HIPRT_ERR_UNSUPPORTED,
+                            "the traversal stack's spill column needs " + std::to_string(bytes) + " bytes (" +
+                                std::to_string(blocks) + " blocks x " + std::to_string(cfg.spill_cap) +
+                                " entries per lane); the allocation-free trace holds " +
+                                std::to_string(kNoAllocSpillBytes) +
+                                ": use fewer blocks, a stack configuration with more entries, or hiprt_trace_closest");
+            cfg.spill = reinterpret_cast<int32_t *>(base + rt::dev::kCounterBytes);
+            cfg.spill_bytes = kNoAllocSpillBytes;
+        }
+    } else {
+        hiprt_stream_state *st = sc->ctx->state(stream);
+        if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
+        HIP_TRY(hiprt_detail::attach_spill(st, s, desc, any, false, need, cfg));
+        counter = st->counter.as<uint32_t>();
+    }
+    HIP_TRY(rt::dev::launch_trace_persistent(desc, r, h, occluded, n, nullptr, counter, any, cfg, s));
     return ok();
 }
 
@@ -755,6 +808,11 @@ int hiprt_trace_closest(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, cons
 int hiprt_trace_any(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *d_occluded, const hiprt_hits *hits,
                     int32_t tracer, const hiprt_tracer_opts *opts, void *stream) {
     return trace_common(sc, rays, n, d_occluded, hits, true, tracer, opts, stream);
+}
+
+int hiprt_trace_closest_noalloc(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, const hiprt_hits *hits,
+                                int32_t tracer, const hiprt_tracer_opts *opts, void *stream) {
+    return trace_common(sc, rays, n, nullptr, hits, false, tracer, opts, stream, true);
 }
 
 } // extern "C"

@@ -125,6 +125,8 @@ def lib():
                                             POINTER(_TracerOpts), c_void_p]),
             "hiprt_trace_any": (c_int, [c_void_p, POINTER(_Rays), c_uint32, c_void_p, POINTER(_Hits), c_int32,
                                         POINTER(_TracerOpts), c_void_p]),
+            "hiprt_trace_closest_noalloc": (c_int, [c_void_p, POINTER(_Rays), c_uint32, POINTER(_Hits), c_int32,
+                                                    POINTER(_TracerOpts), c_void_p]),
             "hiprt_camera_rays": (c_int, [POINTER(c_float), POINTER(_Rays), c_void_p]),
             "hiprt_bounce_cosine": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), c_uint32, c_uint64, c_int32,
                                             POINTER(_Rays), c_void_p]),
@@ -417,6 +419,17 @@ def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_
     s = c_void_p(stream) if stream else None
     check(lib().hiprt_trace_closest(dscene.h, ctypes.byref(r), n, ctypes.byref(h), tracer_id(tracer),
                                     ctypes.byref(o), s))
+
+
+def trace_closest_noalloc(dscene, rays, hits, tracer="persistent-while-while", stream=None, refill_threshold=0,
+                          blocks=0, stack_config=0, work_chunks=0):
+    """hiprt_trace_closest_noalloc: closest hits with the library's static
+    scratch instead of any device allocation."""
+    r, h = _rays_struct(rays), _hits_struct(hits)
+    o = _opts(refill_threshold, blocks, None, stack_config, work_chunks)
+    s = c_void_p(stream) if stream else None
+    check(lib().hiprt_trace_closest_noalloc(dscene.h, ctypes.byref(r), rays.shape[1], ctypes.byref(h),
+                                            tracer_id(tracer), ctypes.byref(o), s))
 
 
 def trace_any(dscene, rays, occluded, hits=None, tracer="while-while", stream=None, refill_threshold=0, blocks=0,

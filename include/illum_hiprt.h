@@ -251,6 +251,23 @@ int hiprt_trace_closest(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, cons
  * t = ref = 0xFFFFFFFF (as float 4294967295) if occluded, reset() otherwise. */
 int hiprt_trace_any(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *d_occluded, const hiprt_hits *hits,
                     int32_t tracer, const hiprt_tracer_opts *opts, void *stream);
+/* hiprt_trace_closest without any device allocation by the library.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  The persistent tracers' work counters and their
+ * traversal-stack spill column live in a static per-device scratch of
+ * 2 KiB + 16 MiB (held until the process ends, its address resolved by
+ * hiprt_create, which must have run on the device); the caller passes rays
+ * and hits as for hiprt_trace_closest and gets the same results.  A launch
+ * whose spill column (grid blocks x 256 lanes x spill entries per lane x 4 B,
+ * entries = the tree's stack need minus the stack configuration's VGPR + LDS
+ * entries) exceeds 16 MiB is refused with HIPRT_ERR_UNSUPPORTED before any
+ * launch.  Every call on one device shares that scratch: calls that overlap
+ * in time on one device (concurrent streams, threads) race and must be
+ * serialized by the caller; consecutive calls on one stream are fine (the
+ * counters are cleared by a kernel at each launch). */
+int hiprt_trace_closest_noalloc(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, const hiprt_hits *hits,
+                                int32_t tracer, const hiprt_tracer_opts *opts, void *stream);
 
 /* ------------------------------------------------------------ batched trace
  * Many scenes, each with its own ray batch, traced by ONE persistent launch
