@@ -59,7 +59,10 @@ struct pcg32 {
 };
 
 // A 64-bit stream id for one (frame seed, path, bounce) triple.  The mixing is
-// SplitMix64's finaliser; distinct inputs give distinct PCG sequences.
+// SplitMix64's finaliser; distinct inputs give distinct PCG sequences.  The
+// path tracer's bounces use bounce = level (0..64); its sub-pixel camera
+// offsets use kCameraStream.
+constexpr uint64_t kCameraStream = 255;
 RNG_HD uint64_t pcg_stream_id(uint64_t seed, uint64_t path, uint64_t bounce) {
     uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (path << 8) + bounce + 1;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;

@@ -130,12 +130,14 @@ __device__ __forceinline__ uint32_t block_append(bool live, uint32_t *count, uin
 // the per-path id table (when path_ids is non-null).
 __global__ __launch_bounds__(256) void k_camera_rays(cam_frame f, ray_out out, uint32_t *path_ids, float *thr,
                                                     float *rad, int clear_radiance, uint32_t n, uint32_t n_px,
-                                                    float *rad_paths) {
+                                                    float *rad_paths, const uint64_t *jitter_seeds) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     uint32_t px = i % n_px;
     int x = (int)(px % (uint32_t)f.w), y = (int)(px / (uint32_t)f.w);
-    write_ray(out, i, f.pos, kRayEps, cam_dir(f, x, y), FLT_MAX);
+    float ox = 0.0f, oy = 0.0f;
+    if (jitter_seeds) pixel_jitter(jitter_seeds[i / n_px], px, ox, oy);
+    write_ray(out, i, f.pos, kRayEps, cam_dir(f, x, y, ox, oy), FLT_MAX);
     if (path_ids) path_ids[i] = i;
     if (thr) thr[3ull * i] = thr[3ull * i + 1] = thr[3ull * i + 2] = 1.0f;
     if (rad_paths) rad_paths[3ull * i] = rad_paths[3ull * i + 1] = rad_paths[3ull * i + 2] = 0.0f;
@@ -350,11 +352,12 @@ __global__ void k_set_u64(uint64_t *p, uint64_t v) {
 }
 
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
-                              bool clear_radiance, hipStream_t s, uint32_t n_paths, float *rad_paths) {
+                              bool clear_radiance, hipStream_t s, uint32_t n_paths, float *rad_paths,
+                              const uint64_t *jitter_seeds) {
     uint32_t n_px = (uint32_t)f.w * (uint32_t)f.h;
     uint32_t n = n_paths ? n_paths : n_px;
     hipLaunchKernelGGL(k_camera_rays, dim3((n + 255) / 256), dim3(256), 0, s, f, out, path_ids, thr, rad,
-                       clear_radiance ? 1 : 0, n, n_px, rad_paths);
+                       clear_radiance ? 1 : 0, n, n_px, rad_paths, jitter_seeds);
     return hipGetLastError();
 }
 

@@ -38,12 +38,25 @@ struct pt_params {
     int emitters_reflect;
 };
 
+// Sub-pixel offset (ox, oy) in [-0.5, 0.5)^2 of path `path`'s camera ray:
+// the first two uniforms of its PCG32 stream (frame seed, pixel,
+// kCameraStream), as simple_pt::sample_pixel offsets every sample
+// (gi/pt.cpp:36).
+RT_HD void pixel_jitter(uint64_t frame_seed, uint32_t pixel, float &ox, float &oy) {
+    pcg32 rng(pcg_stream_id(frame_seed, pixel, kCameraStream));
+    ox = rng.next_float() - 0.5f;
+    oy = rng.next_float() - 0.5f;
+}
+
 // Camera rays for n_paths paths (0: one per pixel), path i looking through
-// pixel i % (w*h); resets path ids and throughput (when non-null), clears
-// the image accumulator `rad` (per pixel, when clear_radiance) and the
-// per-path radiance `rad_paths` (when non-null).
+// pixel i % (w*h) -- at the pixel centre, or, when jitter_seeds is non-null,
+// at the offset pixel_jitter gives for frame seed jitter_seeds[i / (w*h)];
+// resets path ids and throughput (when non-null), clears the image
+// accumulator `rad` (per pixel, when clear_radiance) and the per-path
+// radiance `rad_paths` (when non-null).
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
-                              bool clear_radiance, hipStream_t s, uint32_t n_paths = 0, float *rad_paths = nullptr);
+                              bool clear_radiance, hipStream_t s, uint32_t n_paths = 0, float *rad_paths = nullptr,
+                              const uint64_t *jitter_seeds = nullptr);
 // image[pixel] += rad_paths[k * n_px + pixel] for k = 0 .. samples-1 in order.
 hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s);
 // p[k] = base + k for k < n (the frame seeds of consecutive samples).

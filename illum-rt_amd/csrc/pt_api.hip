@@ -90,6 +90,7 @@ struct hiprt_pt_s {
     int tracer = 0;
     bool alpha = false;   // alpha-aware tracing (tracer id | HIPRT_TRACER_ALPHA)
     bool batched = false;
+    bool jitter = true;   // camera rays through a random point of the pixel (else its centre)
     rt::dev::persistent_cfg pcfg{};
     ray_planes rays[2];
     dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work, spill;
@@ -131,10 +132,11 @@ struct hiprt_pt_s {
         e = launch_fill_u32(cnt, n, 1u, s);
         if (e != hipSuccess) return e;
         float *prad = samples > 1 ? rad_paths.as<float>() : rad.as<float>();   // radiance per path
-        e = launch_camera_rays(f, rays[0].out, batched ? paths[0].as<uint32_t>() : nullptr, thr.as<float>(),
-                               rad.as<float>(), clear_radiance, s, n, samples > 1 ? prad : nullptr);
-        if (e != hipSuccess) return e;
         const uint64_t *fs = frame_seed.as<const uint64_t>();
+        e = launch_camera_rays(f, rays[0].out, batched ? paths[0].as<uint32_t>() : nullptr, thr.as<float>(),
+                               rad.as<float>(), clear_radiance, s, n, samples > 1 ? prad : nullptr,
+                               jitter ? fs : nullptr);
+        if (e != hipSuccess) return e;
         for (int L = 0; L < levels; ++L) {
             if (ev) {
                 e = hipEventRecord(ev[2 * L], s);
@@ -346,6 +348,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
                                params->emitters_reflect ? 1 : 0};
     p->seed = params->seed;
     p->batched = params->batched != 0;
+    p->jitter = params->pixel_centre == 0;
     int tracer = params->tracer;
     p->alpha = tracer >= 0 && (tracer & HIPRT_TRACER_ALPHA) != 0;
     if (tracer >= 0) tracer &= ~HIPRT_TRACER_ALPHA;

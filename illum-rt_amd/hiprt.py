@@ -69,7 +69,7 @@ class _PtParams(ctypes.Structure):
     _fields_ = [("max_depth", c_int32), ("rr_start", c_int32), ("rr_survive", c_float), ("seed", c_uint64),
                 ("tracer", c_int32), ("batched", c_int32), ("stack_config", c_int32), ("work_chunks", c_int32),
                 ("refill_threshold", c_int32), ("blocks", c_int32), ("ray_order", c_int32), ("samples", c_int32),
-                ("emitters_reflect", c_int32)]
+                ("emitters_reflect", c_int32), ("pixel_centre", c_int32)]
 
 
 _lib = None
@@ -484,14 +484,15 @@ class PathTracer:
 
     def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None,
                  stack_config=0, work_chunks=0, refill_threshold=0, blocks=0, sort_rays=True, samples=1,
-                 emitters_reflect=False):
+                 emitters_reflect=False, jitter=True):
         self.dscene = dscene
         c = np.ascontiguousarray(cam[:12], dtype=np.float32)
         self.w, self.h = int(c[10]), int(c[11])
         self.levels = max_depth + 1
         tid = -1 if tracer is None else tracer_id(tracer)
         p = _PtParams(max_depth, rr_start, rr_survive, seed, tid, 1 if batched else 0, stack_config, work_chunks,
-                      refill_threshold, blocks, 0 if sort_rays else 1, samples, 1 if emitters_reflect else 0)
+                      refill_threshold, blocks, 0 if sort_rays else 1, samples, 1 if emitters_reflect else 0,
+                      0 if jitter else 1)
         self.h_ = c_void_p()
         check(lib().hiprt_pt_create(dscene.h, c.ctypes.data_as(POINTER(c_float)), ctypes.byref(p),
                                     ctypes.byref(self.h_)))

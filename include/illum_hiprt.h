@@ -349,7 +349,8 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
 
 /* ------------------------------------------------------- wavefront path tracer
  * A diffuse path tracer over one committed scene and camera, with the control
- * flow of simple_pt::path (gi/pt.cpp:72-115): camera rays, closest hit; a
+ * flow of simple_pt::path (gi/pt.cpp:72-115): camera rays (jittered within
+ * the pixel per sample), closest hit; a
  * miss ends the path, an emitter adds throughput x emission and ends it; else
  * a cosine-weighted bounce (throughput times albedo; a zero throughput ends
  * the path) and Russian roulette with a fixed survival probability from
@@ -385,6 +386,10 @@ typedef struct {
     int32_t emitters_reflect; /* 0 (default): a path ends at the first emitter it meets, after
                                  adding throughput x emission (gi/pt.cpp:86-89); 1: emitters
                                  also reflect and the path goes on */
+    int32_t pixel_centre;   /* 0 (default): each sample's camera ray passes through a random point
+                               of its pixel -- offsets in [-0.5, 0.5)^2 from the path's own PCG32
+                               stream, as simple_pt::sample_pixel jitters (gi/pt.cpp:36);
+                               1: through the pixel centre */
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;

@@ -594,10 +594,11 @@ void orc_adjoint_fd(const float *pos, const uint32_t *idx, const uint32_t *tri, 
     }
 }
 
-// Camera rays of libgi/camera.cpp:44-61 for a w x h image, pixel-centre
-// offset 0, t_min 1e-4, t_max FLT_MAX; out planes as ray_planes.
-void orc_camera_rays(const float *pos, const float *dir, const float *up, float fovy, int w, int h,
-                     float *const *out) {
+// Camera rays of libgi/camera.cpp:44-61 for a w x h image, t_min 1e-4,
+// t_max FLT_MAX; out planes as ray_planes.  Sub-pixel offsets ox[i], oy[i]
+// per pixel (both null: offset 0, the pixel centre).
+void orc_camera_rays(const float *pos, const float *dir, const float *up, float fovy, int w, int h, const float *ox,
+                     const float *oy, float *const *out) {
     v3 P{pos[0], pos[1], pos[2]}, D{dir[0], dir[1], dir[2]}, Up{up[0], up[1], up[2]};
     float aspect = float(w) / h;
     float near_h = tanf(float(M_PI) * fovy * 0.5f / 180.0f);
@@ -606,12 +607,13 @@ void orc_camera_rays(const float *pos, const float *dir, const float *up, float 
     v3 V = normalize(cross(U, D));
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {
-            float uu = ((float)x + 0.5f + 0.0f) / (float)w * 2.0f - 1.0f;
-            float vv = ((float)y + 0.5f + 0.0f) / (float)h * 2.0f - 1.0f;
+            size_t i = (size_t)y * w + x;
+            float offx = ox ? ox[i] : 0.0f, offy = oy ? oy[i] : 0.0f;
+            float uu = ((float)x + 0.5f + offx) / (float)w * 2.0f - 1.0f;
+            float vv = ((float)y + 0.5f + offy) / (float)h * 2.0f - 1.0f;
             uu = near_w * uu;
             vv = near_h * vv;
             v3 d = normalize(add(add(D, mul(U, uu)), mul(V, vv)));
-            size_t i = (size_t)y * w + x;
             out[0][i] = P.x;
             out[1][i] = P.y;
             out[2][i] = P.z;
@@ -716,8 +718,7 @@ uint64_t stream_of(uint64_t seed, uint64_t path, uint64_t bounce) {
     return z ^ (z >> 31);
 }
 
-// camera::update_frustum + cam_ray (libgi/camera.h:18-24, camera.cpp:44-61),
-// pixel-centre offset 0.
+// camera::update_frustum + cam_ray (libgi/camera.h:18-24, camera.cpp:44-61).
 struct pinhole {
     v3 P, D, U, V;
     float near_w, near_h;
@@ -734,9 +735,9 @@ struct pinhole {
         U = normalize(cross(D, Up));
         V = normalize(cross(U, D));
     }
-    ray at(int x, int y) const {
-        float uu = ((float)x + 0.5f + 0.0f) / (float)w * 2.0f - 1.0f;
-        float vv = ((float)y + 0.5f + 0.0f) / (float)h * 2.0f - 1.0f;
+    ray at(int x, int y, float ox, float oy) const {
+        float uu = ((float)x + 0.5f + ox) / (float)w * 2.0f - 1.0f;
+        float vv = ((float)y + 0.5f + oy) / (float)h * 2.0f - 1.0f;
         uu = near_w * uu;
         vv = near_h * vv;
         return make_ray(P, normalize(add(add(D, mul(U, uu)), mul(V, vv))), 1e-4f, FLT_MAX);
@@ -784,7 +785,10 @@ void orc_pcg_sequence(uint64_t initstate, uint64_t initseq, uint32_t n, int raw,
 
 // The workload's diffuse path tracer: the control flow of simple_pt::path
 // (gi/pt.cpp:72-115) with this workload's fixed choices.  Per frame seed s
-// and pixel p (path p of that frame), from the camera ray of p:
+// and pixel p (path p of that frame), from the camera ray of p -- through the
+// pixel centre, or with jitter through the offset (xi0 - 0.5, xi1 - 0.5) of
+// the first two uniforms of PCG32 stream (s, p, 255), as
+// simple_pt::sample_pixel offsets every sample (gi/pt.cpp:36):
 //   level L = 0 .. max_depth: closest hit of the current ray (the BVH
 //   traversal above); a miss ends the path (no sky light, gi/pt.cpp:79-83);
 //   an emissive material adds throughput * emission and ends the path
@@ -809,7 +813,7 @@ void orc_pcg_sequence(uint64_t initstate, uint64_t initseq, uint32_t n, int raw,
 void orc_path_trace(const float *pos, const uint32_t *idx, uint32_t n_tris, const uint32_t *mat, const float *mats,
                     const float *vn, const void *nodes, uint32_t n_nodes, uint32_t root, const uint32_t *index,
                     const float *cam, int max_depth, int rr_start, float rr_survive, int emitters_reflect,
-                    const uint64_t *frame_seeds, uint32_t n_frames, float *image, uint64_t *level_rays,
+                    int jitter, const uint64_t *frame_seeds, uint32_t n_frames, float *image, uint64_t *level_rays,
                     uint64_t *sig, int threads) {
     scene_view s{pos, idx, n_tris};
     tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
@@ -825,7 +829,13 @@ void orc_path_trace(const float *pos, const uint32_t *idx, uint32_t n_tris, cons
             int x = (int)(p % ph.w), y = (int)(p / ph.w);
             uint64_t h = 0;
             for (uint32_t f = 0; f < n_frames; ++f) {
-                ray r = ph.at(x, y);
+                float ox = 0.0f, oy = 0.0f;
+                if (jitter) {
+                    pcg32 g(stream_of(frame_seeds[f], (uint64_t)p, 255));
+                    ox = g.next_float() - 0.5f;
+                    oy = g.next_float() - 0.5f;
+                }
+                ray r = ph.at(x, y, ox, oy);
                 v3 T{1.0f, 1.0f, 1.0f};
                 h = 1469598103934665603ULL;
                 for (int L = 0; L <= max_depth; ++L) {

@@ -182,13 +182,16 @@ def fp64_intersect(pos, idx, tri, rays):
     return t, b, g, ins
 
 
-def camera_rays(pos, dir, up, fovy, w, h):
+def camera_rays(pos, dir, up, fovy, w, h, ox=None, oy=None):
+    """Camera rays (8, w*h); ox, oy: optional sub-pixel offsets per pixel."""
     out = np.empty((8, w * h), np.float32)
     arr = (c_void_p * 8)(*[out[k].ctypes.data for k in range(8)])
     P = np.asarray(pos, np.float32)
     D = np.asarray(dir, np.float32)
     U = np.asarray(up, np.float32)
-    lib().orc_camera_rays(_p(P), _p(D), _p(U), c_float(fovy), c_int(w), c_int(h), arr)
+    ox = None if ox is None else np.ascontiguousarray(ox, np.float32)
+    oy = None if oy is None else np.ascontiguousarray(oy, np.float32)
+    lib().orc_camera_rays(_p(P), _p(D), _p(U), c_float(fovy), c_int(w), c_int(h), _p(ox), _p(oy), arr)
     return out
 
 
@@ -263,7 +266,7 @@ def frame_seeds(seed, first_frame, n_frames):
 
 
 def path_trace(pos, idx, mat, materials, nodes, root, index, cam, seeds, max_depth=4, rr_start=3, rr_survive=0.8,
-               emitters_reflect=False, normals=None, threads=0):
+               emitters_reflect=False, normals=None, jitter=True, threads=0):
     """The workload's diffuse path tracer over the frame seeds `seeds`, in
     order.  materials: (n_mat, 8) albedo rgb, 0, emission rgb, 0; mat: material
     per triangle (None: material 0).  Returns image (h, w, 3) accumulated over
@@ -283,6 +286,7 @@ def path_trace(pos, idx, mat, materials, nodes, root, index, cam, seeds, max_dep
     sig = np.zeros(w * h, np.uint64)
     lib().orc_path_trace(_p(pos), _p(idx), c_uint32(len(idx)), _p(mat), _p(mats), _p(vn), _p(nodes),
                          c_uint32(len(nodes)), c_uint32(root), _p(index), _p(cam), c_int(max_depth), c_int(rr_start),
-                         c_float(rr_survive), c_int(1 if emitters_reflect else 0), _p(seeds), c_uint32(len(seeds)),
+                         c_float(rr_survive), c_int(1 if emitters_reflect else 0), c_int(1 if jitter else 0),
+                         _p(seeds), c_uint32(len(seeds)),
                          _p(img), _p(levels), _p(sig), c_int(threads))
     return img, levels, sig.reshape(h, w)

@@ -194,9 +194,13 @@ def test_fp64_closed_form():
 
 # ---------------------------------------------------------------- ray generation
 
-def test_camera_rays_closed_form():
+@pytest.mark.parametrize("jitter", [False, True])
+def test_camera_rays_closed_form(jitter):
     w, h, fovy = 64, 48, 60.0
-    rays = orc.camera_rays([0.5, 0.5, -1], [0, 0, 1], [0, 1, 0], fovy, w, h)
+    rng = np.random.default_rng(3)
+    ox = rng.uniform(-0.5, 0.5, w * h).astype(np.float32) if jitter else np.zeros(w * h, np.float32)
+    oy = rng.uniform(-0.5, 0.5, w * h).astype(np.float32) if jitter else np.zeros(w * h, np.float32)
+    rays = orc.camera_rays([0.5, 0.5, -1], [0, 0, 1], [0, 1, 0], fovy, w, h, *((ox, oy) if jitter else ()))
     d = rays[4:7].astype(np.float64)
     assert np.allclose(np.linalg.norm(d, axis=0), 1.0, atol=1e-6)
     assert np.all(rays[0] == np.float32(0.5)) and np.all(rays[2] == np.float32(-1))
@@ -205,16 +209,16 @@ def test_camera_rays_closed_form():
     nh = np.tan(np.pi * fovy / 360.0)
     nw = w / h * nh
     xs, ys = np.meshgrid(np.arange(w), np.arange(h))
-    u = ((xs + 0.5) / w * 2 - 1) * nw
-    v = ((ys + 0.5) / h * 2 - 1) * nh
+    u = ((xs + 0.5 + ox.reshape(h, w)) / w * 2 - 1) * nw
+    v = ((ys + 0.5 + oy.reshape(h, w)) / h * 2 - 1) * nh
     q = np.stack([-u, v, np.ones_like(u)]).reshape(3, -1)
     q /= np.linalg.norm(q, axis=0)
     assert np.max(np.abs(q - d)) < 2e-6
     # fovy is vertical and in degrees: d.y / d.z of pixel row y is
-    # ((y + 0.5) / h * 2 - 1) * tan(fovy / 2), independent of the column
+    # ((y + 0.5 + oy) / h * 2 - 1) * tan(fovy / 2), independent of the column
     ratio = (d[1] / d[2]).reshape(h, w)
-    want = ((np.arange(h) + 0.5) / h * 2 - 1) * np.tan(np.radians(fovy / 2))
-    assert np.max(np.abs(ratio - want[:, None])) < 1e-6
+    want = ((np.arange(h)[:, None] + 0.5 + oy.reshape(h, w)) / h * 2 - 1) * np.tan(np.radians(fovy / 2))
+    assert np.max(np.abs(ratio - want)) < 1e-6
 
 
 def test_product_camera_rays_equal_oracle():

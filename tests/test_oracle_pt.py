@@ -122,7 +122,7 @@ def test_roulette_counts_follow_the_streams():
     assert not img.any()
 
 
-def _composed(pos, idx, mat, mats, bvh, cam, seeds, max_depth=4, rr_start=3, p=0.8):
+def _composed(pos, idx, mat, mats, bvh, cam, seeds, max_depth=4, rr_start=3, p=0.8, jitter=True):
     """The path loop as a numpy loop over the oracle's separately pinned steps."""
     w, h = int(cam[10]), int(cam[11])
     n = w * h
@@ -130,8 +130,12 @@ def _composed(pos, idx, mat, mats, bvh, cam, seeds, max_depth=4, rr_start=3, p=0
     counts = np.zeros(max_depth + 1, np.uint64)
     scale_rr = np.float32(1.0) / np.float32(p)
     for seed in seeds:
-        rays = orc.camera_rays(cam[0:3], cam[3:6], cam[6:9], float(cam[9]), w, h)
         alive = np.arange(n, dtype=np.uint64)
+        off = ()
+        if jitter:   # sub-pixel offsets: first two uniforms of stream (seed, pixel, 255)
+            g = Pcg32(stream_id(int(seed), alive, 255))
+            off = (g.next_float() - np.float32(0.5), g.next_float() - np.float32(0.5))
+        rays = orc.camera_rays(cam[0:3], cam[3:6], cam[6:9], float(cam[9]), w, h, *off)
         T = np.ones((n, 3), np.float32)
         for L in range(max_depth + 1):
             counts[L] += len(alive)
@@ -159,8 +163,8 @@ def _composed(pos, idx, mat, mats, bvh, cam, seeds, max_depth=4, rr_start=3, p=0
     return img.reshape(h, w, 3), counts
 
 
-@pytest.mark.parametrize("max_depth,rr_start", [(4, 3), (2, 1)])
-def test_path_loop_equals_composed_steps(max_depth, rr_start):
+@pytest.mark.parametrize("max_depth,rr_start,jitter", [(4, 3, True), (2, 1, True), (4, 3, False)])
+def test_path_loop_equals_composed_steps(max_depth, rr_start, jitter):
     sc = hiprt.HostScene.random(3000, 0.03, seed=21)
     pos, idx, _ = sc.arrays()
     k = np.arange(len(idx))
@@ -171,8 +175,8 @@ def test_path_loop_equals_composed_steps(max_depth, rr_start):
     cam = np.array([0.5, 0.5, -1, 0, 0, 1, 0, 1, 0, 60, 48, 36], np.float32)
     seeds = orc.frame_seeds(5, 0, 3)
     img, levels, _ = orc.path_trace(pos, idx, mat, mats, bvh.nodes, bvh.root, bvh.index, cam, seeds,
-                                    max_depth=max_depth, rr_start=rr_start)
-    want_img, want_levels = _composed(pos, idx, mat, mats, bvh, cam, seeds, max_depth, rr_start)
+                                    max_depth=max_depth, rr_start=rr_start, jitter=jitter)
+    want_img, want_levels = _composed(pos, idx, mat, mats, bvh, cam, seeds, max_depth, rr_start, jitter=jitter)
     assert levels.tolist() == want_levels.tolist()
     assert levels[1] > 0 and img.sum() > 0
     assert np.array_equal(img.view(np.uint32), want_img.view(np.uint32))
