@@ -267,6 +267,32 @@ def bench_shadow(torch, hiprt, ctx, stream):
     return out
 
 
+def bench_adjoint(torch, hiprt, ds, d_diff, stream, reps=10):
+    """Closest-hit adjoint on 1M of the frame's first-bounce diffuse rays:
+    vertex, origin and direction gradients of sum(a t + b beta + c gamma)
+    with the hit set frozen; the vertex gradients are summed per hit
+    triangle after a sort."""
+    n_adj = 1_000_000
+    adj_rays = d_diff[:, :n_adj].contiguous()
+    adj_hits = hiprt.new_hits(n_adj)
+    hiprt.trace_closest(ds, adj_rays, adj_hits, tracer="persistent-while-while", stream=stream.cuda_stream)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    g_t, g_u, g_v = (torch.randn(n_adj, device="cuda", generator=gen) for _ in range(3))
+    g_vert = torch.zeros((hiprt.scene_counts(ds)[0], 3), dtype=torch.float32, device="cuda")
+    g_o = torch.empty((n_adj, 3), dtype=torch.float32, device="cuda")
+    g_d = torch.empty((n_adj, 3), dtype=torch.float32, device="cuda")
+    ms_adj = time_calls(torch, lambda: hiprt.adjoint_closest(ds, adj_rays, adj_hits, g_t, g_u, g_v, g_vert, g_o, g_d,
+                                                             stream=stream.cuda_stream), stream, reps=reps)
+    # algorithmic bytes per ray: the per-ray pass reads the ray (24), hit (8),
+    # upstream gradients (12) and triangle (48) and writes origin and
+    # direction gradients (24), a corner record (36) and a key (4); the sort
+    # moves key and index 3 times (3 x 16); the vertex pass reads key, index
+    # and corner record (44)
+    per_ray = 24 + 8 + 12 + 48 + 24 + 36 + 4 + 48 + 44
+    return {"rays": n_adj, "ms": round(ms_adj, 4), "mrays": round(n_adj / ms_adj / 1e3, 1),
+            "bytes_per_ray": per_ray, "achieved_GBs": round(n_adj * per_ray / (ms_adj * 1e-3) / 1e9, 1)}
+
+
 def bench_device_build(torch, hiprt, ctx, stream, sc, d_diff, host_build_s):
     """Linear BVH of the hall built on the device from device-resident
     triangles, and the diffuse trace over it (the tree is not the SAH one)."""
@@ -418,6 +444,8 @@ def main():
                     help="only trace the hall's primary or first-bounce diffuse rays 3 times with the dynamic "
                          "tracer (for counter collection) and print their fetch counts")
     ap.add_argument("--stack-config", type=int, default=0, help="stack configuration for --kernel-only")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time the closest-hit adjoint on 1M of the frame's diffuse rays (for profiling)")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (roofline traffic and VALU figures)")
     ap.add_argument("--counter-child", help=argparse.SUPPRESS)
@@ -460,6 +488,12 @@ def main():
     d_diff = torch.empty_like(d_cam)
     hiprt.bounce_cosine(ds, d_cam, cam_hits, d_diff, seed=12345, level=0, stream=stream.cuda_stream)
     torch.cuda.synchronize()
+
+    if args.adjoint_only:
+        print(json.dumps(bench_adjoint(torch, hiprt, ds, d_diff, stream, reps=20)), flush=True)
+        ds.close()
+        ctx.close()
+        return
 
     if args.kernel_only:
         rays = d_cam if args.kernel_only == "primary" else d_diff
@@ -584,20 +618,7 @@ def main():
         kernels[f"{name}/autotuned"] = round(rays.shape[1] / ms / 1e3, 1)
     log(f"trace kernels alone (Mrays/s): {kernels}")
 
-    # Closest-hit adjoint on 1M of the diffuse rays: vertex (atomic), origin
-    # and direction gradients of sum(a t + b beta + c gamma), hit set frozen.
-    n_adj = 1_000_000
-    adj_rays = d_diff[:, :n_adj].contiguous()
-    adj_hits = hiprt.new_hits(n_adj)
-    hiprt.trace_closest(ds, adj_rays, adj_hits, tracer="dynamic-while-while", stream=stream.cuda_stream)
-    gen = torch.Generator(device="cuda").manual_seed(5)
-    g_t, g_u, g_v = (torch.randn(n_adj, device="cuda", generator=gen) for _ in range(3))
-    g_vert = torch.zeros((hiprt.scene_counts(ds)[0], 3), dtype=torch.float32, device="cuda")
-    g_o = torch.empty((n_adj, 3), dtype=torch.float32, device="cuda")
-    g_d = torch.empty((n_adj, 3), dtype=torch.float32, device="cuda")
-    ms_adj = time_calls(torch, lambda: hiprt.adjoint_closest(ds, adj_rays, adj_hits, g_t, g_u, g_v, g_vert, g_o, g_d,
-                                                             stream=stream.cuda_stream), stream, reps=10)
-    adjoint = {"rays": n_adj, "ms": round(ms_adj, 4), "mrays": round(n_adj / ms_adj / 1e3, 1)}
+    adjoint = bench_adjoint(torch, hiprt, ds, d_diff, stream)
     log(f"adjoint: {adjoint}")
 
     # Device refit of the hall to its own vertices (records and boxes rebuilt;

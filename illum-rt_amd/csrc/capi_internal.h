@@ -75,6 +75,7 @@ struct hiprt_stream_state {
     hiprt_detail::dbuf counter;       // work counters of the persistent kernels
     hiprt_detail::dbuf batch_table;   // descriptor table of the last batched trace on the stream
     hiprt_detail::dbuf spill;         // stack spill buffer of the persistent kernels (grown on demand)
+    hiprt_detail::dbuf adjoint;       // sort scratch of the adjoint's vertex accumulation (grown on demand)
 };
 
 struct hiprt_context_s {
@@ -95,6 +96,9 @@ struct hiprt_scene_s {
     hiprt_detail::dbuf alpha_tri, alpha_tex, alpha_mask;   // opacity data (scenes with textures)
     uint32_t n_tris = 0, n_vert = 0;
     int stack_need = 0;                          // most stack entries any traversal can hold
+    hiprt_detail::dbuf light_rec, light_f, light_cdf;   // light list of the direct-lighting steps
+    uint32_t n_lights = 0;
+    float light_integral = 0.0f;
     hiprt_detail::dbuf refit_list;               // inner nodes by level, deepest first (refit)
     std::vector<uint32_t> refit_off;             // level boundaries in refit_list
     rt::scene_desc desc{};

@@ -15,7 +15,9 @@ constexpr uint32_t kSortTile = 2048;
 // Device buffers of one sort: keys_in, keys_tmp, vals_tmp and vals_out hold
 // n_max words each, counts 256 * radix_tiles(n_max), totals 256.  keys_in is
 // overwritten; vals_out receives, for sorted position j < *count_ptr, the
-// slot of keys_in that sorts there (ties keep slot order).
+// slot of keys_in that sorts there (ties keep slot order).  The sorted keys
+// end in keys_tmp after an odd number of passes (ceil(bits / 8)), in keys_in
+// after an even one.
 struct radix_sort_buffers {
     uint32_t *keys_in, *keys_tmp, *vals_tmp, *vals_out, *counts, *totals;
 };

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 
+#include "core/lights.h"
 #include "core/raygen.h"
 #include "device/kernels.h"
 #include "device/pt_kernels.h"
@@ -18,6 +19,7 @@
 #include "device/adjoint_kernels.h"
 
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <new>
 #include <vector>
@@ -280,8 +282,8 @@ int hiprt_shadow_rays(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *
 
 int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *hits,
                           const hiprt_rays *shadow_rays, const uint8_t *d_occluded, uint32_t n, const float *light,
-                          const float *power, float *d_framebuffer, void *stream) {
-    if (!sc || !full_rays(cam_rays) || !full_hits(hits) || !full_rays(shadow_rays) || !light || !power ||
+                          const float *col, float *d_framebuffer, void *stream) {
+    if (!sc || !full_rays(cam_rays) || !full_hits(hits) || !full_rays(shadow_rays) || !light || !col ||
         (n && (!d_occluded || !d_framebuffer)))
         return fail(HIPRT_ERR_ARG, "null argument");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
@@ -289,9 +291,119 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
     rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
     HIP_TRY(rt::dev::launch_integrate_light(ps, soa(cam_rays), hsoa(hits), soa(shadow_rays), d_occluded,
                                             rt::vec3f{light[0], light[1], light[2]},
-                                            rt::vec3f{power[0], power[1], power[2]},
+                                            rt::vec3f{col[0], col[1], col[2]},
                                             reinterpret_cast<float4 *>(d_framebuffer), n,
                                             static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_scene_set_lights(hiprt_scene sc, const float *point_lights, uint32_t n_points, uint32_t *n_lights) {
+    if (!sc || (n_points && !point_lights)) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    // the emissive triangles, in triangle order, from the committed shading data
+    std::vector<float> geo(12ull * sc->n_tris), mats(sc->mats.bytes / 4);
+    if (sc->n_tris) HIP_TRY(hipMemcpy(geo.data(), sc->tri_geo.p, geo.size() * 4, hipMemcpyDeviceToHost));
+    if (!mats.empty()) HIP_TRY(hipMemcpy(mats.data(), sc->mats.p, mats.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<rt::light_rec> recs;
+    std::vector<float> f;
+    auto bits = [](uint32_t u) {
+        float x;
+        std::memcpy(&x, &u, 4);
+        return x;
+    };
+    for (uint32_t t = 0; t < sc->n_tris; ++t) {
+        const float *g = &geo[12ull * t];
+        uint32_t m;
+        std::memcpy(&m, &g[3], 4);
+        if (8ull * m + 8 > mats.size()) return fail(HIPRT_ERR_ARG, "triangle material id out of range");
+        rt::vec3f e{mats[8ull * m + 4], mats[8ull * m + 5], mats[8ull * m + 6]};
+        if (e == rt::v3(0.0f)) continue;
+        rt::vec3f A{g[0], g[1], g[2]}, B{g[4], g[5], g[6]}, C{g[8], g[9], g[10]};
+        recs.push_back(rt::light_rec{{A.x, A.y, A.z, bits(rt::kLightTriangle), B.x, B.y, B.z, rt::tri_area(A, B, C),
+                                      C.x, C.y, C.z, 0.0f, e.x, e.y, e.z, 0.0f}});
+        f.push_back(rt::triangle_light_power(A, B, C, e));
+    }
+    for (uint32_t k = 0; k < n_points; ++k) {
+        const float *p = point_lights + 6ull * k;
+        rt::vec3f col{p[3], p[4], p[5]};
+        recs.push_back(rt::light_rec{{p[0], p[1], p[2], bits(rt::kLightPoint), 0, 0, 0, 0, 0, 0, 0, 0, col.x, col.y,
+                                      col.z, 0.0f}});
+        f.push_back(rt::point_light_power(col));
+    }
+    // distribution_1d::build_cdf (libgi/discrete_distributions.cpp:25-38)
+    const uint32_t n = (uint32_t)recs.size();
+    std::vector<float> cdf(n + 1, 0.0f);
+    for (uint32_t i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1];
+    const float integral = cdf[n];
+    for (uint32_t i = 1; i < n + 1; ++i) cdf[i] = integral > 0 ? cdf[i] / integral : (float)i / (float)n;
+    sc->light_rec.release();
+    sc->light_f.release();
+    sc->light_cdf.release();
+    sc->n_lights = 0;
+    if (n) {
+        HIP_TRY(sc->light_rec.upload(recs.data(), recs.size() * sizeof(rt::light_rec)));
+        HIP_TRY(sc->light_f.upload(f.data(), f.size() * 4));
+        HIP_TRY(sc->light_cdf.upload(cdf.data(), cdf.size() * 4));
+    }
+    sc->n_lights = n;
+    sc->light_integral = integral;
+    if (n_lights) *n_lights = n;
+    return ok();
+}
+
+int hiprt_sample_direct(hiprt_scene sc, int32_t mode, const hiprt_rays *cam_rays, const hiprt_hits *cam_hits,
+                        uint32_t n, uint64_t seed, const hiprt_ray_buffers *out, float *d_pdf, float *d_light_col,
+                        float *d_framebuffer, void *stream) {
+    if (!sc || !full_rays(cam_rays) || !full_hits(cam_hits) || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (mode < HIPRT_DIRECT_UNIFORM || mode > HIPRT_DIRECT_LIGHT) return fail(HIPRT_ERR_ARG, "unknown sampling mode");
+    if (!out->d_ox || !out->d_oy || !out->d_oz || !out->d_tmin || !out->d_dx || !out->d_dy || !out->d_dz ||
+        !out->d_tmax)
+        return fail(HIPRT_ERR_ARG, "null output plane");
+    if (n && (!d_pdf || !d_framebuffer)) return fail(HIPRT_ERR_ARG, "null pdf or framebuffer");
+    if (mode == HIPRT_DIRECT_LIGHT && n && !d_light_col) return fail(HIPRT_ERR_ARG, "light mode needs d_light_col");
+    if (mode == HIPRT_DIRECT_LIGHT && sc->n_lights == 0)
+        return fail(HIPRT_ERR_ARG, "light sampling needs lights (hiprt_scene_set_lights found none)");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::light_table lt{sc->light_rec.as<const float4>(), sc->light_f.as<const float>(),
+                            sc->light_cdf.as<const float>(), sc->n_lights, sc->light_integral};
+    rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
+    HIP_TRY(rt::dev::launch_sample_direct(ps, lt, mode, soa(cam_rays), hsoa(cam_hits), seed, o, d_pdf, d_light_col,
+                                          reinterpret_cast<float4 *>(d_framebuffer), n,
+                                          static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_integrate_dir_sample(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *cam_hits,
+                               const hiprt_rays *shadow_rays, const hiprt_hits *shadow_hits, const float *d_pdf,
+                               uint32_t n, float *d_framebuffer, void *stream) {
+    if (!sc || !full_rays(cam_rays) || !full_hits(cam_hits) || !full_rays(shadow_rays) || !full_hits(shadow_hits))
+        return fail(HIPRT_ERR_ARG, "null argument");
+    if (n && (!d_pdf || !d_framebuffer)) return fail(HIPRT_ERR_ARG, "null pdf or framebuffer");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    HIP_TRY(rt::dev::launch_integrate_dir(ps, soa(cam_rays), hsoa(cam_hits), soa(shadow_rays), hsoa(shadow_hits),
+                                          d_pdf, reinterpret_cast<float4 *>(d_framebuffer), n,
+                                          static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
+int hiprt_integrate_light_sample(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *cam_hits,
+                                 const hiprt_rays *shadow_rays, const uint8_t *d_occluded, const float *d_pdf,
+                                 const float *d_light_col, uint32_t n, float *d_framebuffer, void *stream) {
+    if (!sc || !full_rays(cam_rays) || !full_hits(cam_hits) || !full_rays(shadow_rays))
+        return fail(HIPRT_ERR_ARG, "null argument");
+    if (n && (!d_occluded || !d_pdf || !d_light_col || !d_framebuffer))
+        return fail(HIPRT_ERR_ARG, "null occlusion, pdf, light colour or framebuffer");
+    if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    HIP_TRY(rt::dev::launch_integrate_light_sample(ps, soa(cam_rays), hsoa(cam_hits), soa(shadow_rays), d_occluded,
+                                                   d_pdf, d_light_col, reinterpret_cast<float4 *>(d_framebuffer), n,
+                                                   static_cast<hipStream_t>(stream)));
     return ok();
 }
 
@@ -313,8 +425,26 @@ int hiprt_adjoint_closest(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hi
                             d_grad_vertices,
                             d_grad_origin,
                             d_grad_dir,
-                            d_grad_corners};
-    HIP_TRY(rt::dev::launch_adjoint_closest(a, n, static_cast<hipStream_t>(stream)));
+                            d_grad_corners,
+                            sc->n_tris};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void *scratch = nullptr;
+    if (d_grad_vertices && n) {
+        // the stream's sort scratch, grown once its earlier launches are done
+        hiprt_stream_state *st = sc->ctx->state(stream);
+        if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
+        const size_t bytes = rt::dev::adjoint_sort_bytes(n);
+        if (st->adjoint.bytes < bytes) {
+            HIP_TRY(hipStreamSynchronize(s));
+            hipError_t e = st->adjoint.alloc(bytes);
+            if (e != hipSuccess) {
+                st->adjoint.release();
+                return fail(HIPRT_ERR_HIP, std::string("adjoint sort scratch: ") + hipGetErrorString(e));
+            }
+        }
+        scratch = st->adjoint.p;
+    }
+    HIP_TRY(rt::dev::launch_adjoint_closest(a, n, scratch, s));
     return ok();
 }
 

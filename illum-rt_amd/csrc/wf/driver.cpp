@@ -16,7 +16,8 @@
 //
 // Commands: at, look, up, fovy, resolution, sppx, load, generate, pointlight,
 // platform, raytracer, tracer-option, bvh, triangles, add-scene-step, commit, algo, run,
-// outfile, stats, echo, quit.  Images are written as PFM (float) or, for any
+// outfile, stats, echo, quit, and the algorithm's own (direct-wf: is
+// uniform|cosine|light).  Images are written as PFM (float) or, for any
 // other extension, 8-bit PPM with a 2.2 gamma.
 #include "hip-platform.h"
 
@@ -245,6 +246,8 @@ void eval(const string &line) {
             cout << (text.empty() ? string(" ") : text.substr(text[0] == ' ' ? 1 : 0)) << endl;
         } else if (command == "quit" || command == "exit") {
             exit(0);
+        } else if (algo && algo->interprete(command, in)) {
+            // an algorithm's own command, e.g. direct-wf's "is uniform|cosine|light"
         } else {
             error("Unknown command");
         }

@@ -201,17 +201,72 @@ struct add_hitpoint_albedo : public rd_step {
     }
 };
 
-struct sample_light_dir : public wf::step {
-    static constexpr char id[] = "sample light dir";
+// Per-sample data of the direct-lighting algorithm (per_sample_data<float> /
+// <vec3> of gi/direct.cpp:478-481): the sample pdf and the light colour.
+struct direct_data {
+    device_buffer pdf, light_col;
+    void resize(uint32_t n) {
+        pdf.resize(4ull * n);
+        light_col.resize(12ull * n);
+    }
+};
+
+// compute_light_distribution (rt/cpu/preprocessing.cpp:9-20): the light list
+// of the committed scene -- its emissive triangles, then the point lights of
+// the script -- with the power distribution.
+struct compute_light_distribution : public wf::step {
+    static constexpr char id[] = "compute light distribution";
     std::string id_str = id;
-    raydata *cam = nullptr, *shadow = nullptr;
     void run() override {
         time_this_wf_step;
-        if (rc->scene.lights.empty()) throw std::logic_error("direct-wf needs a pointlight");
+        std::vector<float> pts;
+        for (const pointlight &l : rc->scene.lights) pts.insert(pts.end(), {l.pos[0], l.pos[1], l.pos[2], l.col[0], l.col[1], l.col[2]});
+        uint32_t n = 0;
+        check(hiprt_scene_set_lights(pf->ds, pts.data(), (uint32_t)(pts.size() / 6), &n), id);
+        if (n == 0) std::cerr << "WARNING: There is neither emissive geometry nor a light" << std::endl;
+    }
+};
+
+// sample_uniform_dir, sample_cos_weighted_dir, sample_light_dir
+// (rt/cpu/bounce.cpp:11-106): one shadow/direction ray per camera hit, its
+// pdf (and the light colour); camera hits on emitters add their emission to
+// the framebuffer.  Sample k of the step draws from seed k (1, 2, ...).
+template <int MODE, const char *ID> struct direct_sample_step : public wf::step {
+    static constexpr const char *id = ID;
+    std::string id_str = id;
+    raydata *cam = nullptr, *shadow = nullptr;
+    direct_data *dd = nullptr;
+    uint64_t seed = 1;
+    void run() override {
+        time_this_wf_step;
+        dd->resize(cam->n());
         hiprt_rays r = cam->in();
         hiprt_hits h = cam->hit();
         hiprt_ray_buffers o = shadow->out();
-        check(hiprt_shadow_rays(pf->ds, &r, &h, cam->n(), rc->scene.lights[0].pos, &o, nullptr), id);
+        check(hiprt_sample_direct(pf->ds, MODE, &r, &h, cam->n(), seed++, &o, dd->pdf.as<float>(),
+                                  dd->light_col.as<float>(), pf->framebuffer.as<float>(), nullptr),
+              id);
+    }
+};
+constexpr char kSampleUniform[] = "sample uniform dir";
+constexpr char kSampleCosine[] = "sample cos weighted dir";
+constexpr char kSampleLight[] = "sample light dir";
+using sample_uniform_dir = direct_sample_step<HIPRT_DIRECT_UNIFORM, kSampleUniform>;
+using sample_cos_weighted_dir = direct_sample_step<HIPRT_DIRECT_COSINE, kSampleCosine>;
+using sample_light_dir = direct_sample_step<HIPRT_DIRECT_LIGHT, kSampleLight>;
+
+struct integrate_dir_sample : public wf::step {
+    static constexpr char id[] = "integrate dir sample";
+    std::string id_str = id;
+    raydata *cam = nullptr, *shadow = nullptr;
+    direct_data *dd = nullptr;
+    void run() override {
+        time_this_wf_step;
+        hiprt_rays cr = cam->in(), sr = shadow->in();
+        hiprt_hits ch = cam->hit(), sh = shadow->hit();
+        check(hiprt_integrate_dir_sample(pf->ds, &cr, &ch, &sr, &sh, dd->pdf.as<float>(), cam->n(),
+                                         pf->framebuffer.as<float>(), nullptr),
+              id);
     }
 };
 
@@ -219,29 +274,15 @@ struct integrate_light_sample : public wf::step {
     static constexpr char id[] = "integrate light sample";
     std::string id_str = id;
     raydata *cam = nullptr, *shadow = nullptr;
+    direct_data *dd = nullptr;
     void run() override {
         time_this_wf_step;
-        const pointlight &l = rc->scene.lights[0];
         hiprt_rays cr = cam->in(), sr = shadow->in();
-        hiprt_hits h = cam->hit();
-        check(hiprt_integrate_light(pf->ds, &cr, &h, &sr, shadow->occluded.as<uint8_t>(), cam->n(), l.pos, l.col,
-                                    pf->framebuffer.as<float>(), nullptr),
+        hiprt_hits ch = cam->hit();
+        check(hiprt_integrate_light_sample(pf->ds, &cr, &ch, &sr, shadow->occluded.as<uint8_t>(),
+                                           dd->pdf.as<float>(), dd->light_col.as<float>(), cam->n(),
+                                           pf->framebuffer.as<float>(), nullptr),
               id);
-    }
-};
-
-struct sample_cos_weighted_dir : public wf::step {
-    static constexpr char id[] = "sample cos weighted dir";
-    std::string id_str = id;
-    raydata *cam = nullptr, *bounce = nullptr;
-    uint64_t seed = 1;
-    int level = 0;
-    void run() override {
-        time_this_wf_step;
-        hiprt_rays r = cam->in();
-        hiprt_hits h = cam->hit();
-        hiprt_ray_buffers o = bounce->out();
-        check(hiprt_bounce_cosine(pf->ds, &r, &h, cam->n(), seed++, level, &o, nullptr), id);
     }
 };
 
@@ -299,9 +340,12 @@ platform::platform(const std::vector<std::string> &args) : wf::platform("hip") {
     register_wf_step_by_id(, hip::find_closest_hits);
     register_wf_step_by_id(, hip::find_any_hits);
     register_wf_step_by_id(, add_hitpoint_albedo);
-    register_wf_step_by_id(, sample_light_dir);
-    register_wf_step_by_id(, integrate_light_sample);
+    register_wf_step_by_id(, compute_light_distribution);
+    register_wf_step_by_id(, sample_uniform_dir);
     register_wf_step_by_id(, sample_cos_weighted_dir);
+    register_wf_step_by_id(, sample_light_dir);
+    register_wf_step_by_id(, integrate_dir_sample);
+    register_wf_step_by_id(, integrate_light_sample);
     register_wf_step_by_id(, download_framebuffer);
     register_wf_step_by_id(, hip::build_accel_struct);
     timer = new event_timer;
@@ -455,30 +499,76 @@ public:
     }
 };
 
-// direct-wf with a point light: camera rays, closest hit, shadow rays, any
-// hit ("find occluders"), integrate (gi/direct.cpp:527-552).
+// direct-wf (wf::direct_light, gi/direct.cpp:470-585): camera rays and their
+// closest hits, then by the "is" mode
+//   uniform (the default) / cosine: sample a direction, trace its closest hit
+//     ("secondary hits"), integrate dir sample;
+//   light: compute light distribution (a data-reset step), sample light dir,
+//     any hit ("find occluders"), integrate light sample.
 class direct_wf : public simple_algorithm {
     std::unique_ptr<raydata> cam_rd, shadow_rd;
+    direct_data dd;
+    int mode = HIPRT_DIRECT_UNIFORM;
+
+    template <class S> void wire_sample(S *s) {
+        s->cam = cam_rd.get();
+        s->shadow = shadow_rd.get();
+        s->dd = &dd;
+    }
+    void regenerate_steps() {
+        data_reset_steps.clear();
+        frame_preparation_steps = {pf->step<initialize_framebuffer>()};
+        frame_finalization_steps = {pf->step<download_framebuffer>()};
+        auto *cam = pf->step<sample_camera_rays>("primary hits");
+        auto *hit = pf->step<hip::find_closest_hits>();
+        cam->use(cam_rd.get());
+        hit->use(cam_rd.get());
+        if (mode == HIPRT_DIRECT_LIGHT) {
+            data_reset_steps.push_back(pf->step<compute_light_distribution>());
+            auto *sample = pf->step<sample_light_dir>();
+            auto *occ = pf->step<hip::find_any_hits>("find occluders");
+            auto *integ = pf->step<integrate_light_sample>();
+            wire_sample(sample);
+            occ->use(shadow_rd.get());
+            wire_sample(integ);
+            sampling_steps = {cam, hit, sample, occ, integ};
+        } else {
+            wf::step *sample;
+            if (mode == HIPRT_DIRECT_UNIFORM) {
+                auto *u = pf->step<sample_uniform_dir>();
+                wire_sample(u);
+                sample = u;
+            } else {
+                auto *c = pf->step<sample_cos_weighted_dir>();
+                wire_sample(c);
+                sample = c;
+            }
+            auto *trace = pf->step<hip::find_closest_hits>("secondary hits");
+            auto *integ = pf->step<integrate_dir_sample>();
+            trace->use(shadow_rd.get());
+            wire_sample(integ);
+            sampling_steps = {cam, hit, sample, trace, integ};
+        }
+    }
 
 public:
     direct_wf() {
         cam_rd.reset(pf->allocate_raydata());
         shadow_rd.reset(pf->allocate_raydata());
-        auto *cam = pf->step<sample_camera_rays>();
-        auto *hit = pf->step<hip::find_closest_hits>();
-        auto *light = pf->step<sample_light_dir>();
-        auto *occ = pf->step<hip::find_any_hits>("find occluders");
-        auto *integ = pf->step<integrate_light_sample>();
-        cam->use(cam_rd.get());
-        hit->use(cam_rd.get());
-        light->cam = cam_rd.get();
-        light->shadow = shadow_rd.get();
-        occ->use(shadow_rd.get());
-        integ->cam = cam_rd.get();
-        integ->shadow = shadow_rd.get();
-        frame_preparation_steps.push_back(pf->step<initialize_framebuffer>());
-        sampling_steps = {cam, hit, light, occ, integ};
-        frame_finalization_steps.push_back(pf->step<download_framebuffer>());
+        regenerate_steps();
+    }
+    // "is uniform|cosine|light" (gi/direct.cpp:570-582)
+    bool interprete(const std::string &command, std::istringstream &in) override {
+        if (command != "is") return false;
+        std::string value;
+        in >> value;
+        if (value == "uniform") mode = HIPRT_DIRECT_UNIFORM;
+        else if (value == "cosine") mode = HIPRT_DIRECT_COSINE;
+        else if (value == "light") mode = HIPRT_DIRECT_LIGHT;
+        else if (value == "brdf") throw std::runtime_error("unsupported importance sampling method for wf/direct");
+        else std::cerr << "unknown sampling mode in interprete: " << value << std::endl;
+        regenerate_steps();
+        return true;
     }
 };
 

@@ -111,8 +111,8 @@ public:
 };
 
 // Algorithms (the gi/ step graphs): primary-wf (gi/primary-hit.cpp:85-117),
-// direct-wf with a point light (gi/direct.cpp:527-552) and pt-wf, the
-// wavefront path tracer of this build.
+// direct-wf with its uniform, cosine and light sampling modes
+// (gi/direct.cpp:470-585) and pt-wf, the wavefront path tracer of this build.
 std::unique_ptr<simple_algorithm> make_algorithm(const std::string &name);
 
 // Step times collected by the platform timer ("stats print" / "stats clear").

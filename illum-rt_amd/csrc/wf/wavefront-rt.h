@@ -179,6 +179,8 @@ public:
         for (auto *s : frame_finalization_steps) s->run();
         if (t) t->synchronize();
     }
+    // Algorithm-specific commands (algorithm::interprete, libgi/algorithm.h).
+    virtual bool interprete(const std::string &command, std::istringstream &in) { return false; }
 };
 
 } // namespace wf

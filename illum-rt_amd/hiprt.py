@@ -151,6 +151,13 @@ def lib():
                                           POINTER(_Rays), c_void_p]),
             "hiprt_integrate_light": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), POINTER(_Rays), c_void_p,
                                               c_uint32, POINTER(c_float), POINTER(c_float), c_void_p, c_void_p]),
+            "hiprt_scene_set_lights": (c_int, [c_void_p, c_void_p, c_uint32, POINTER(c_uint32)]),
+            "hiprt_sample_direct": (c_int, [c_void_p, c_int32, POINTER(_Rays), POINTER(_Hits), c_uint32, c_uint64,
+                                            POINTER(_Rays), c_void_p, c_void_p, c_void_p, c_void_p]),
+            "hiprt_integrate_dir_sample": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), POINTER(_Rays),
+                                                   POINTER(_Hits), c_void_p, c_uint32, c_void_p, c_void_p]),
+            "hiprt_integrate_light_sample": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), POINTER(_Rays),
+                                                     c_void_p, c_void_p, c_void_p, c_uint32, c_void_p, c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -477,6 +484,56 @@ def shadow_rays(dscene, rays, hits, out, light, stream=None):
     lp = np.ascontiguousarray(light[:3], dtype=np.float32)
     check(lib().hiprt_shadow_rays(dscene.h, ctypes.byref(r), ctypes.byref(h), n,
                                   lp.ctypes.data_as(POINTER(c_float)), ctypes.byref(o), _stream(stream)))
+
+
+DIRECT_MODES = {"uniform": 0, "cosine": 1, "light": 2}
+
+
+def set_lights(dscene, points=None):
+    """Light list of the direct-lighting steps: the scene's emissive triangles,
+    then point lights (k, 6) pos + col.  Returns the number of lights."""
+    pts = None if points is None else np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 6)
+    n = c_uint32()
+    check(lib().hiprt_scene_set_lights(dscene.h, _ptr(pts) if pts is not None else None,
+                                       0 if pts is None else len(pts), ctypes.byref(n)))
+    return n.value
+
+
+def sample_direct(dscene, mode, cam_rays, cam_hits, seed, out, pdf, fb, light_col=None, stream=None):
+    """hiprt_sample_direct: mode "uniform", "cosine" or "light"; out (8, n),
+    pdf (n,), fb (n, 4), light_col (n, 3) in light mode."""
+    check(lib().hiprt_sample_direct(dscene.h, DIRECT_MODES[mode], ctypes.byref(_rays_struct(cam_rays)),
+                                    ctypes.byref(_hits_struct(cam_hits)), cam_rays.shape[1], seed,
+                                    ctypes.byref(_rays_struct(out)), c_void_p(pdf.data_ptr()),
+                                    _dp(light_col), c_void_p(fb.data_ptr()), _stream(stream)))
+
+
+def integrate_light(dscene, cam_rays, cam_hits, shadow_rays, occluded, light, col, fb, stream=None):
+    """hiprt_integrate_light: one point light at `light` of colour `col`."""
+    lp = np.ascontiguousarray(light[:3], dtype=np.float32)
+    c = np.ascontiguousarray(col[:3], dtype=np.float32)
+    check(lib().hiprt_integrate_light(dscene.h, ctypes.byref(_rays_struct(cam_rays)),
+                                      ctypes.byref(_hits_struct(cam_hits)), ctypes.byref(_rays_struct(shadow_rays)),
+                                      c_void_p(occluded.data_ptr()), cam_rays.shape[1],
+                                      lp.ctypes.data_as(POINTER(c_float)), c.ctypes.data_as(POINTER(c_float)),
+                                      c_void_p(fb.data_ptr()), _stream(stream)))
+
+
+def integrate_dir_sample(dscene, cam_rays, cam_hits, shadow_rays, shadow_hits, pdf, fb, stream=None):
+    check(lib().hiprt_integrate_dir_sample(dscene.h, ctypes.byref(_rays_struct(cam_rays)),
+                                           ctypes.byref(_hits_struct(cam_hits)),
+                                           ctypes.byref(_rays_struct(shadow_rays)),
+                                           ctypes.byref(_hits_struct(shadow_hits)), c_void_p(pdf.data_ptr()),
+                                           cam_rays.shape[1], c_void_p(fb.data_ptr()), _stream(stream)))
+
+
+def integrate_light_sample(dscene, cam_rays, cam_hits, shadow_rays, occluded, pdf, light_col, fb, stream=None):
+    check(lib().hiprt_integrate_light_sample(dscene.h, ctypes.byref(_rays_struct(cam_rays)),
+                                             ctypes.byref(_hits_struct(cam_hits)),
+                                             ctypes.byref(_rays_struct(shadow_rays)),
+                                             c_void_p(occluded.data_ptr()), c_void_p(pdf.data_ptr()),
+                                             c_void_p(light_col.data_ptr()), cam_rays.shape[1],
+                                             c_void_p(fb.data_ptr()), _stream(stream)))
 
 
 class PathTracer:

@@ -340,12 +340,61 @@ int hiprt_add_hitpoint_albedo(hiprt_scene sc, const hiprt_hits *hits, uint32_t n
  * (rt/cpu/bounce.cpp:74-106). */
 int hiprt_shadow_rays(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, const float *light,
                       const hiprt_ray_buffers *out, void *stream);
-/* framebuffer[i] += diffuse direct light of the point light (power rgb) at
- * hit i when its shadow ray is unoccluded, weight += 1: "integrate light
- * sample" (rt/cpu/bounce.cpp:132-151) for a Lambertian surface. */
+/* framebuffer[i] += the direct light of one point light of colour col at hit
+ * i when its shadow ray is valid and unoccluded, weight += 1: "integrate
+ * light sample" (rt/cpu/bounce.cpp:132-151) with pointlight::sample_Li
+ * (libgi/scene.cpp:500-508): col / dist^2 * albedo / pi * max(0, w_in . n),
+ * n the hit triangle's (unflipped) normal, zero where w_in is behind it. */
 int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *hits,
                           const hiprt_rays *shadow_rays, const uint8_t *d_occluded, uint32_t n, const float *light,
-                          const float *power, float *d_framebuffer, void *stream);
+                          const float *col, float *d_framebuffer, void *stream);
+
+/* ------------------------------------------------- direct-lighting steps
+ * The steps of the reference's wavefront direct lighting (gi/direct.cpp:
+ * 483-552, rt/cpu/bounce.cpp), selected by its "is" modes.  Materials are
+ * Lambertian; a hit's shading normal is its triangle's geometric normal.
+ *
+ * Light list (for HIPRT_DIRECT_LIGHT): every emissive triangle of the scene
+ * in triangle order (scene::find_light_geometry, libgi/scene.cpp:366-370),
+ * then the given point lights (6 floats each: pos xyz, col rgb), sampled in
+ * proportion to the luma of their power (scene::compute_light_distribution,
+ * libgi/scene.cpp:373-412; distribution_1d::sample_index).  *n_lights (may
+ * be NULL) receives the count.  Synchronous (reads the committed triangles
+ * back once). */
+int hiprt_scene_set_lights(hiprt_scene sc, const float *point_lights, uint32_t n_points, uint32_t *n_lights);
+
+enum { HIPRT_DIRECT_UNIFORM = 0, HIPRT_DIRECT_COSINE = 1, HIPRT_DIRECT_LIGHT = 2 };
+
+/* Sample step for camera ray i and its closest hit (PCG32 stream (seed, i, 0)):
+ *   UNIFORM  sample_uniform_dir (rt/cpu/bounce.cpp:11-39): a uniform
+ *            direction about the hit's normal flipped to the camera side,
+ *            pdf = 1/(2 pi);
+ *   COSINE   sample_cos_weighted_dir (bounce.cpp:41-72): cosine-weighted,
+ *            pdf = cos / pi;
+ *   LIGHT    sample_light_dir (bounce.cpp:74-106): a light by power, then a
+ *            point on it (trianglelight / pointlight ::sample_Li); the shadow
+ *            ray ends 1e-4 before it; pdf = area pdf x light pdf; the light's
+ *            colour into d_light_col (3 floats per ray).  Lights facing away
+ *            give the disabled ray.
+ * out: the sampled rays (the disabled ray o = 0, d = (1,1,1), t_max = -FLT_MAX
+ * for misses and emitters); d_pdf: the pdf per ray.  A camera hit on an
+ * emitter adds (emission, 1) to d_framebuffer (float4 rgb + weight per ray),
+ * (emission, 0) in LIGHT mode, as the reference's steps do. */
+int hiprt_sample_direct(hiprt_scene sc, int32_t mode, const hiprt_rays *cam_rays, const hiprt_hits *cam_hits,
+                        uint32_t n, uint64_t seed, const hiprt_ray_buffers *out, float *d_pdf, float *d_light_col,
+                        float *d_framebuffer, void *stream);
+/* integrate_dir_sample (bounce.cpp:108-130): framebuffer[i] += (emission of
+ * the surface shadow ray i hit x albedo/pi x max(0, w_in . n) / pdf, 1), zero
+ * where w_in is behind the camera hit's normal or nothing emits there. */
+int hiprt_integrate_dir_sample(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *cam_hits,
+                               const hiprt_rays *shadow_rays, const hiprt_hits *shadow_hits, const float *d_pdf,
+                               uint32_t n, float *d_framebuffer, void *stream);
+/* integrate_light_sample (bounce.cpp:132-151): framebuffer[i] += (light
+ * colour x albedo/pi x max(0, w_in . n) / pdf, 1) where shadow ray i is valid
+ * (t_max > 0) and unoccluded (d_occluded from an any-hit trace). */
+int hiprt_integrate_light_sample(hiprt_scene sc, const hiprt_rays *cam_rays, const hiprt_hits *cam_hits,
+                                 const hiprt_rays *shadow_rays, const uint8_t *d_occluded, const float *d_pdf,
+                                 const float *d_light_col, uint32_t n, float *d_framebuffer, void *stream);
 
 /* ------------------------------------------------------- wavefront path tracer
  * A diffuse path tracer over one committed scene and camera, with the control

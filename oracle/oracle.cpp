@@ -32,6 +32,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -887,6 +888,236 @@ void orc_path_trace(const float *pos, const uint32_t *idx, uint32_t n_tris, cons
     }
     if (level_rays)
         for (int L = 0; L < levels; ++L) level_rays[L] += counts[(size_t)L];
+}
+
+} // extern "C"
+
+namespace {
+
+// ---- wavefront direct lighting -------------------------------------------
+//
+// Lights: scene::find_light_geometry (libgi/scene.cpp:366-370) -- every
+// emissive triangle in triangle order -- then point lights as the pointlight
+// command appends them (driver/interaction.cpp:723-753).
+struct olight {
+    bool point;
+    v3 a, b, c;   // triangle corners, or a = position
+    v3 col;       // emission or point colour
+};
+
+inline float luma3(v3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }   // libgi/color.h
+
+struct olights {
+    std::vector<olight> L;
+    std::vector<float> f, cdf;
+    float integral = 0;
+    // scene::compute_light_distribution (libgi/scene.cpp:373-412) with
+    // trianglelight::power / pointlight::power (scene.cpp:494-527) and
+    // distribution_1d::build_cdf (libgi/discrete_distributions.cpp:25-38)
+    void build() {
+        const float pi = float(M_PI);
+        for (const olight &l : L) {
+            v3 p;
+            if (l.point) {
+                p = smul(4 * pi, l.col);
+            } else {
+                v3 e1 = sub(l.b, l.a), e2 = sub(l.c, l.a);
+                v3 x = cross(e1, e2);
+                p = mul(mul(mul(l.col, 0.5f), sqrtf(dot(x, x))), pi);
+            }
+            f.push_back(luma3(p));
+        }
+        size_t N = f.size();
+        cdf.assign(N + 1, 0.0f);
+        for (size_t i = 1; i < N + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1];
+        integral = cdf[N];
+        for (size_t i = 1; i < N + 1; ++i) cdf[i] = integral > 0 ? cdf[i] / integral : i / float(N);
+    }
+    // distribution_1d::sample_index (discrete_distributions.cpp:40-48)
+    uint32_t sample_index(float xi, float &pdf) const {
+        unsigned index = unsigned(std::lower_bound(cdf.begin(), cdf.end(), xi) - cdf.begin());
+        index = index > 0 ? index - 1 : index;
+        pdf = integral > 0.0f ? f[index] / integral : 0.0f;
+        return index;
+    }
+};
+
+struct direct_out {
+    ray shadow;
+    bool valid = false;   // shadow ray enabled (t_max > 0)
+    float pdf = 0;
+    v3 lightcol{0, 0, 0};
+};
+
+} // namespace
+
+extern "C" {
+
+// The wavefront direct-lighting algorithm of gi/direct.cpp:483-552 for the
+// camera rays of `cam` (pixel centres), one pass per seed in `seeds`:
+// closest hit of the camera ray, then by mode
+//   0 uniform: sample_uniform_dir (rt/cpu/bounce.cpp:11-39), closest hit,
+//              integrate_dir_sample (bounce.cpp:108-130);
+//   1 cosine:  sample_cos_weighted_dir (bounce.cpp:41-72), the same;
+//   2 light:   sample_light_dir (bounce.cpp:74-106) with trianglelight /
+//              pointlight ::sample_Li (libgi/scene.cpp:500-559), any hit,
+//              integrate_light_sample (bounce.cpp:132-151);
+// accumulating (rgb, weight) per pixel into fb (4 floats per pixel) in the
+// reference's order: the sample step's (emission, 1 -- or 0 in light mode)
+// for camera hits on emitters, then the integration step's (radiance, 1).
+// Random numbers per pixel i and seed s: PCG32 stream (s, i, 0); light mode
+// draws the light index, then the two for the point on the light.
+// Lambertian BRDF (lambertian_reflection::f, libgi/material.cpp:55-65); the
+// shading normal is the triangle's geometric normal (scenes without vertex
+// normals), flipped to the ray's side in the sample steps only (the
+// integration steps use diff_geom's normal unflipped, as the reference does);
+// a triangle light's normal is its geometric normal.  points: n_points x 6
+// floats (pos, col); mats: 8 floats per material; mat per triangle.
+void orc_direct(const float *pos, const uint32_t *idx, uint32_t n_tris, const uint32_t *mat, const float *mats,
+                const void *nodes, uint32_t n_nodes, uint32_t root, const uint32_t *index, const float *cam, int mode,
+                const float *points, uint32_t n_points, const uint64_t *seeds, uint32_t n_seeds, float *fb,
+                int threads) {
+    scene_view s{pos, idx, n_tris};
+    tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
+    pinhole ph(cam);
+    const float pi = float(M_PI);
+    const float one_over_pi = (1.0 / M_PI), one_over_2pi = (1.0 / (2 * M_PI));
+    auto material = [&](uint32_t t) { return mats + 8 * (size_t)(mat ? mat[t] : 0u); };
+    auto corner = [&](uint32_t t, int k) { return s.p(idx[3 * t + k]); };
+    olights lights;
+    for (uint32_t t = 0; t < n_tris; ++t) {
+        const float *m = material(t);
+        v3 e{m[4], m[5], m[6]};
+        if (e.x != 0 || e.y != 0 || e.z != 0) lights.L.push_back({false, corner(t, 0), corner(t, 1), corner(t, 2), e});
+    }
+    for (uint32_t k = 0; k < n_points; ++k) {
+        const float *p = points + 6 * k;
+        lights.L.push_back({true, v3{p[0], p[1], p[2]}, v3{0, 0, 0}, v3{0, 0, 0}, v3{p[3], p[4], p[5]}});
+    }
+    lights.build();
+    const int64_t n_px = (int64_t)ph.w * ph.h;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(set_threads(threads))
+    for (int64_t i = 0; i < n_px; ++i) {
+        int x = (int)(i % ph.w), y = (int)(i / ph.w);
+        float *px = fb + 4 * i;
+        for (uint32_t sd = 0; sd < n_seeds; ++sd) {
+            ray cr = ph.at(x, y, 0.0f, 0.0f);
+            hit is = closest_hit(s, tr, cr, nullptr);
+            direct_out o;
+            o.pdf = mode == 0 ? one_over_2pi : (mode == 1 ? one_over_pi : 0.0f);
+            v3 ng{0, 0, 0}, ns{0, 0, 0}, xh{0, 0, 0}, albedo{0, 0, 0};
+            if (is.t != FLT_MAX) {
+                v3 A = corner(is.ref, 0), B = corner(is.ref, 1), C = corner(is.ref, 2);
+                const float *m = material(is.ref);
+                albedo = v3{m[0], m[1], m[2]};
+                v3 e{m[4], m[5], m[6]};
+                xh = add(add(smul(1.0f - is.beta - is.gamma, A), smul(is.beta, B)), smul(is.gamma, C));
+                ns = normalize(cross(sub(B, A), sub(C, A)));   // diff_geom::ng (libgi/rt.cpp:7-15)
+                ng = ns;
+                if (dot(cr.d, ng) > 0) ng = v3{-ng.x, -ng.y, -ng.z};   // flip_normals_to_ray (util.h:226-233)
+                if (e.x != 0 || e.y != 0 || e.z != 0) {
+                    px[0] += e.x;
+                    px[1] += e.y;
+                    px[2] += e.z;
+                    px[3] += mode == 2 ? 0.0f : 1.0f;
+                } else {
+                    pcg32 g(stream_of(seeds[sd], (uint64_t)i, 0));
+                    if (mode == 0 || mode == 1) {
+                        float xi0 = g.next_float(), xi1 = g.next_float();
+                        v3 sdir;
+                        if (mode == 0) {   // uniform_sample_hemisphere (libgi/sampling.h:19-26)
+                            float z = xi0, r0 = 1 - z * z;
+                            float r = sqrtf(r0 < 0 ? -r0 : r0);
+                            float phi = 2 * pi * xi1;
+                            sdir = v3{r * cosf(phi), r * sinf(phi), z};
+                        } else {   // cosine_sample_hemisphere (sampling.h:32-36)
+                            float r = sqrtf(xi0), theta = 2 * pi * xi1;
+                            float dx = r * cosf(theta), dy = r * sinf(theta);
+                            float z = 1 - dx * dx - dy * dy;
+                            sdir = v3{dx, dy, z > 0 ? sqrtf(z) : 0.f};
+                        }
+                        // align (util.h:212-218)
+                        float sg = copysignf(1.f, ng.z);
+                        v3 w{sdir.x, sdir.y, sdir.z * sg}, hv{ng.x, ng.y, ng.z + sg};
+                        float k = dot(w, hv) / (1.f + (ng.z < 0 ? -ng.z : ng.z));
+                        v3 wi = sub(smul(k, hv), w);
+                        if (mode == 1) {
+                            float c = dot(wi, ng);
+                            o.pdf *= c < 0.0f ? 0.0f : c;   // cdot
+                        }
+                        o.shadow = make_ray(xh, wi, 1e-4f, FLT_MAX);
+                        o.valid = true;
+                    } else {
+                        float l_pdf;
+                        uint32_t l = lights.sample_index(g.next_float(), l_pdf);
+                        float xi0 = g.next_float(), xi1 = g.next_float();
+                        const olight &L = lights.L[l];
+                        v3 col = L.col, wi;
+                        float a_pdf = 1.0f, tmax;
+                        if (L.point) {   // pointlight::sample_Li (scene.cpp:500-508)
+                            wi = sub(L.a, xh);
+                            tmax = sqrtf(dot(wi, wi));
+                            wi = v3{wi.x / tmax, wi.y / tmax, wi.z / tmax};
+                            col = v3{col.x / (tmax * tmax), col.y / (tmax * tmax), col.z / (tmax * tmax)};
+                        } else {   // trianglelight::sample_Li (scene.cpp:534-559)
+                            float su0 = sqrtf(xi0);
+                            float bx = 1.0f - su0, by = xi1 * su0;   // uniform_sample_triangle (sampling.h:43-46)
+                            v3 target = add(add(smul(1.0f - bx - by, L.a), smul(bx, L.b)), smul(by, L.c));
+                            v3 nl = normalize(cross(sub(L.b, L.a), sub(L.c, L.a)));
+                            wi = sub(target, xh);
+                            v3 x2 = cross(sub(L.b, L.a), sub(L.c, L.a));
+                            float area = 0.5f * sqrtf(dot(x2, x2));
+                            tmax = sqrtf(dot(wi, wi));
+                            wi = v3{wi.x / tmax, wi.y / tmax, wi.z / tmax};
+                            float cos_l = dot(nl, v3{-wi.x, -wi.y, -wi.z});
+                            if (cos_l <= 0.0f) {
+                                col = v3{0, 0, 0};
+                                a_pdf = 0.0f;
+                            } else {
+                                a_pdf = tmax * tmax / (cos_l * area);
+                            }
+                        }
+                        if (col.x != 0 || col.y != 0 || col.z != 0) {
+                            o.shadow = make_ray(xh, wi, 1e-4f, tmax - 1e-4f);   // ray::length_exclusive
+                            o.valid = o.shadow.t_max > 0;
+                            o.lightcol = col;
+                        }
+                        o.pdf = a_pdf * l_pdf;
+                    }
+                }
+            }
+            // integration
+            v3 rad{0, 0, 0};
+            if (is.t != FLT_MAX && o.valid) {
+                v3 w_in = o.shadow.d;
+                float c = dot(w_in, ns);
+                float cosv = c < 0.0f ? 0.0f : c;
+                v3 fb_ = dot(w_in, ns) > 0 ? smul(one_over_pi, albedo) : v3{0, 0, 0};   // lambertian f
+                if (mode == 2) {
+                    if (!any_hit(s, tr, o.shadow)) {
+                        v3 lf = v3{o.lightcol.x * fb_.x, o.lightcol.y * fb_.y, o.lightcol.z * fb_.z};
+                        v3 lc = mul(lf, cosv);
+                        rad = v3{lc.x / o.pdf, lc.y / o.pdf, lc.z / o.pdf};
+                    }
+                } else {
+                    hit lh = closest_hit(s, tr, o.shadow, nullptr);
+                    if (lh.t != FLT_MAX) {
+                        const float *lm = material(lh.ref);
+                        v3 e{lm[4], lm[5], lm[6]};
+                        if (e.x != 0 || e.y != 0 || e.z != 0) {
+                            v3 ef = v3{e.x * fb_.x, e.y * fb_.y, e.z * fb_.z};
+                            v3 ec = mul(ef, cosv);
+                            rad = v3{ec.x / o.pdf, ec.y / o.pdf, ec.z / o.pdf};
+                        }
+                    }
+                }
+            }
+            px[0] += rad.x;
+            px[1] += rad.y;
+            px[2] += rad.z;
+            px[3] += 1.0f;
+        }
+    }
 }
 
 // Wall-clock seconds of one orc_trace_closest call (the CPU baseline leg).

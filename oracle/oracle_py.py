@@ -290,3 +290,26 @@ def path_trace(pos, idx, mat, materials, nodes, root, index, cam, seeds, max_dep
                          _p(seeds), c_uint32(len(seeds)),
                          _p(img), _p(levels), _p(sig), c_int(threads))
     return img, levels, sig.reshape(h, w)
+
+
+DIRECT_MODES = {"uniform": 0, "cosine": 1, "light": 2}
+
+
+def direct(pos, idx, mat, materials, nodes, root, index, cam, mode, seeds, points=None, threads=0):
+    """The wavefront direct-lighting algorithm (orc_direct) in mode "uniform",
+    "cosine" or "light", one pass per seed.  points: (k, 6) pos + col of point
+    lights.  Returns the framebuffer (h, w, 4): rgb sums and weights."""
+    pos, idx = _scene(pos, idx)
+    cam = np.ascontiguousarray(cam[:12], dtype=np.float32)
+    w, h = int(cam[10]), int(cam[11])
+    mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, 8)
+    mat = None if mat is None else np.ascontiguousarray(mat, dtype=np.uint32)
+    pts = np.zeros((0, 6), np.float32) if points is None else np.ascontiguousarray(points, np.float32).reshape(-1, 6)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    nodes = np.ascontiguousarray(nodes)
+    index = np.ascontiguousarray(index, dtype=np.uint32)
+    fb = np.zeros((h, w, 4), np.float32)
+    lib().orc_direct(_p(pos), _p(idx), c_uint32(len(idx)), _p(mat), _p(mats), _p(nodes), c_uint32(len(nodes)),
+                     c_uint32(root), _p(index), _p(cam), c_int(DIRECT_MODES[mode]), _p(pts), c_uint32(len(pts)),
+                     _p(seeds), c_uint32(len(seeds)), _p(fb), c_int(threads))
+    return fb

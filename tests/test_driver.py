@@ -95,41 +95,38 @@ def test_tracer_options_keep_the_image(tmp_path, torch_cuda):
 
 
 @pytest.mark.gpu
-def test_direct_wf_shadows_match_oracle(tmp_path, torch_cuda):
-    print("case: rtgi-hip direct-wf, hall 192x108, point light", flush=True)
+@pytest.mark.parametrize("mode", ["uniform", "cosine", "light"])
+def test_direct_wf_matches_oracle(tmp_path, torch_cuda, mode):
+    """direct-wf with "is <mode>" (uniform is the default) on the interior hall
+    with its emissive arches and a point light, 2 samples: the written image
+    is the oracle's framebuffer divided by its weights.  Sample k of the
+    sampling step draws from seed k (1, 2); light mode is bit-exact, the
+    others may differ where a cosf/sinf ulp turns a sample onto another
+    triangle (tests/test_gpu_direct.py)."""
+    print(f"case: rtgi-hip direct-wf, hall 192x108, is {mode}, 2 spp", flush=True)
     w, h = 192, 108
-    r = run_script(SCRIPT_HEAD.format(cfg=2, w=w, h=h, tracer="dynamic-while-while") +
-                   "pointlight replace pos 20 13.5 7.5 col 400 400 400\nalgo direct-wf\noutfile d.pfm\nrun\n",
-                   tmp_path)
+    r = run_script(SCRIPT_HEAD.format(cfg=2, w=w, h=h, tracer="persistent-while-while") +
+                   "pointlight replace pos 20 13.5 7.5 col 40 40 40\nsppx 2\nalgo direct-wf\n" +
+                   ("" if mode == "uniform" else f"is {mode}\n") + "outfile d.pfm\nrun\n", tmp_path)
     assert r.returncode == 0 and not r.stderr.strip(), r.stderr
     img = read_pfm(tmp_path / "d.pfm")
-    # the lit set must be exactly the pixels whose shadow ray the oracle finds unoccluded
     sc = hiprt.HostScene.generate(2)
     pos, idx, mat = sc.arrays()
     bvh = hiprt.Bvh(sc)
     cam = sc.cam.copy()
     cam[10], cam[11] = w, h
-    rays = hiprt.camera_rays(cam)
-    t, u, v, i = orc.trace_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, rays)
-    hit = t != np.float32(3.4028234663852886e38)
-    A, B, C = (pos[idx[i, k]].astype(np.float32) for k in range(3))
-    x = (np.float32(1) - u - v)[:, None] * A + u[:, None] * B + v[:, None] * C
-    light = np.array([20, 13.5, 7.5], np.float32)
-    to = light - x
-    dist = np.sqrt((to * to).sum(1))
-    shadow = np.zeros((8, len(t)), np.float32)
-    shadow[0:3] = x.T
-    shadow[3] = 1e-4
-    shadow[4:7] = (to / dist[:, None]).T
-    shadow[7] = np.where(hit, dist - np.float32(1e-4), -np.float32(3.4e38))
-    occ = orc.trace_any(pos, idx, bvh.nodes, bvh.root, bvh.index, shadow)
-    lit_img = img.reshape(-1, 3).sum(1) > 0
-    candidates = hit & (occ == 0)
-    # lit pixels are unoccluded hits (the numpy shadow rays here may differ
-    # from the device's in a last bit, so a grazing pixel or two may flip);
-    # unlit unoccluded hits face away from the light
-    assert (lit_img & ~candidates).sum() <= max(2, 1e-4 * len(t))
-    assert lit_img.sum() > 0.2 * candidates.sum()
+    fb = orc.direct(pos, idx, mat, sc.materials(), bvh.nodes, bvh.root, bvh.index, cam, mode,
+                    np.array([1, 2], np.uint64), np.array([[20, 13.5, 7.5, 40, 40, 40]], np.float32))
+    want = fb[..., :3] / fb[..., 3:4]
+    differ = (img.view(np.uint32) != want.view(np.uint32)).any(-1)
+    scale = np.maximum(np.abs(want).max(-1), 1e-30)
+    far = np.abs(img - want).max(-1) > 1e-5 * scale
+    print({"pixels_differ": int(differ.sum()), "pixels_beyond_rel": int(far.sum())})
+    assert want.sum() > 0
+    if mode == "light":
+        assert not differ.any()
+    else:   # last-bit cosine differences everywhere, another triangle rarely
+        assert far.sum() <= max(3, 4e-4 * far.size)
 
 
 @pytest.mark.gpu

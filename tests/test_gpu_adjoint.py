@@ -129,3 +129,35 @@ def test_autograd_function(c5):
     h2 = hiprt.new_hits(t.shape[0])
     hiprt.trace_closest(ds, rays.contiguous(), h2, tracer="persistent-while-while")
     assert torch.equal(h2["id"], tri) and torch.equal(h2["t"], t.detach())
+
+
+@pytest.mark.parametrize("n", [1, 257, 5000, 70_001])
+def test_vertex_gradients_accumulate_by_triangle(c5, n):
+    """The vertex gradients (rays grouped by hit triangle, summed per run,
+    added to the vertices) equal the per-ray corner gradients scattered to
+    the shared vertices in float64, and are added to what the array held.
+    Every third ray gets t_max just above t_min, so it misses."""
+    print(f"case: adjoint vertex accumulation, {n} rays, every third a miss", flush=True)
+    torch, ds = c5["torch"], c5["ds"]
+    rays = c5["rays"][:, :n].contiguous()
+    rays[7, ::3] = 2e-4
+    hits = hiprt.new_hits(n)
+    hiprt.trace_closest(ds, rays, hits, tracer="persistent-while-while")
+    rng = np.random.default_rng(n)
+    gt, gu, gv = (torch.from_numpy(rng.normal(size=n).astype(np.float32)).cuda() for _ in range(3))
+    nv, _ = hiprt.scene_counts(ds)
+    gvert = torch.ones((nv, 3), dtype=torch.float32, device="cuda")
+    corners = torch.empty((n, 9), dtype=torch.float32, device="cuda")
+    hiprt.adjoint_closest(ds, rays, hits, gt, gu, gv, grad_vertices=gvert, grad_corners=corners)
+    torch.cuda.synchronize()
+    t = hits["t"].cpu().numpy()
+    tri = hits["id"].cpu().numpy().view(np.uint32)
+    valid = np.nonzero(t != np.float32(3.4028234663852886e38))[0]
+    assert len(valid) < n or n == 1
+    cn = corners.cpu().numpy().astype(np.float64)
+    want = np.ones((nv, 3))
+    vidx = c5["idx"][tri[valid]]
+    for k in range(3):
+        np.add.at(want, vidx[:, k], cn[valid, 3 * k:3 * k + 3])
+    got = gvert.cpu().numpy()
+    assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, np.abs(want).max())
