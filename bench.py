@@ -519,51 +519,19 @@ def main():
     pt_kw = dict(tracer=tuned["tracer"], stack_config=tuned["stack_config"], work_chunks=tuned["work_chunks"],
                  refill_threshold=tuned["refill_threshold"], blocks=tuned["blocks"])
 
-    # The trace autotuner times one launch over rays already in pixel order
-    # with the tree warm in L2; inside a frame every level follows a shading
-    # pass and reads a sorted queue.  So the frame's configuration is chosen
-    # on warm-up frames (never the timed ones), in rays per second: the best
-    # of the autotuned one, its one-counter variant, the 4-wide and the binary
-    # tree with stack configuration 3, the defaults with 2, 4 or 8 samples per
-    # pixel in flight (consecutive frames' samples traced together, so every
-    # launch holds 2-8x the rays) and the binary tree with stack configuration
-    # 4 (a 20-entry LDS stack with registers capped for 8 waves per SIMD) and 8,
-    # 16, 32 or 64 samples in flight, when it beats
-    # the library defaults (binary tree, dynamic tracer, stack 0, eight work
-    # chunks, one sample) by more than 2%, the defaults otherwise, which keeps
-    # the choice stable between runs.  The autotuner's sample is the first
-    # bounce; the deeper bounces weigh the stack configurations and layouts
-    # differently, so the fixed candidates are tried on whole frames too.
+    # The frame configuration comes from the library's frame-level tuner
+    # (hiprt_pt_autotune): trial frames of each candidate -- tracer, layout,
+    # stack configuration at one sample per pixel, then samples in flight up
+    # to 64, then the parked-leaf limit -- in rays per second.  Its one-sample
+    # winner is the like-for-like batched configuration.
     ds4 = hiprt.DeviceScene(ctx, sc, bvh, hiprt.BVH4)
-    candidates = {"autotuned": (ds_tuned, pt_kw),
-                  "autotuned, one work counter": (ds_tuned, dict(pt_kw, work_chunks=1)),
-                  "4-wide tree, stack 3": (ds4, dict(tracer="dynamic-while-while", stack_config=3)),
-                  "binary tree, stack 3": (ds, dict(tracer="dynamic-while-while", stack_config=3)),
-                  "2 samples in flight": (ds, dict(samples=2)),
-                  "4 samples in flight": (ds, dict(samples=4)),
-                  "8 samples in flight": (ds, dict(samples=8)),
-                  "8 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
-                                                            samples=8)),
-                  "16 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
-                                                             samples=16)),
-                  "32 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
-                                                             samples=32)),
-                  "64 samples in flight, stack 4": (ds, dict(tracer="dynamic-while-while", stack_config=4,
-                                                             samples=64)),
-                  "library defaults": (ds, {})}
-    # Rates, not frame times: a render call with k samples in flight traces k
-    # samples per pixel.
-    trial = {}
-    for name, (dsc, kw) in candidates.items():
-        pt = hiprt.PathTracer(dsc, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, **kw)
-        ms, rays, _ = time_frames(torch, pt, 4, 1, stream)
-        trial[name] = rays / ms / 1e3
-        pt.close()
-    best_tuned = max((k for k in candidates if k != "library defaults"), key=trial.get)
-    choice = best_tuned if trial[best_tuned] > 1.02 * trial["library defaults"] else "library defaults"
-    one_choice = max((k for k in candidates if "samples" not in candidates[k][1]), key=trial.get)
-    log(f"frame configuration trial (Mrays/s): {trial}; timed frames use {choice}; one sample: {one_choice}")
-    ds_frame, frame_kw = candidates[choice]
+    t_tune = time.time()
+    frame_kw, frame_tune = hiprt.pt_autotune(ds, ds4, cam, max_samples=64)
+    frame_tune["seconds"] = round(time.time() - t_tune, 1)
+    one_kw, one_tune = hiprt.pt_autotune(ds, ds4, cam, max_samples=1)
+    log(f"frame tuner: {frame_kw} {frame_tune}; one sample: {one_kw} {one_tune}")
+    ds_frame = ds4 if frame_tune["layout"] == "bvh4" else ds
+    ds_one = ds4 if one_tune["layout"] == "bvh4" else ds
     samples = frame_kw.get("samples", 1)
 
     batched = hiprt.PathTracer(ds_frame, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True,
@@ -575,11 +543,10 @@ def main():
     log(f"plain:   {ms_p:.3f} ms/frame, {rays_p / 1e6:.3f} Mrays/frame")
     # like for like: the batched schedule with one sample per pixel per
     # launch, as the plain schedule traces
-    ds_one, one_kw = candidates[one_choice]
     one = hiprt.PathTracer(ds_one, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, **one_kw)
     ms_1, rays_1, _ = time_frames(torch, one, args.steps, args.warmup, stream)
     one.close()
-    log(f"batched, one sample per launch ({one_choice}): {ms_1:.3f} ms/frame, {rays_1 / 1e6:.3f} Mrays/frame")
+    log(f"batched, one sample per launch ({one_kw}): {ms_1:.3f} ms/frame, {rays_1 / 1e6:.3f} Mrays/frame")
     value = rays_b / ms_b / 1e3
     plain_value = rays_p / ms_p / 1e3
     one_value = rays_1 / ms_1 / 1e3
@@ -704,7 +671,7 @@ def main():
         # like for like: one sample per pixel per launch on both schedules
         "batched_over_plain": round(one_value / plain_value, 3),
         "one_sample_in_flight_mrays": round(one_value, 1),
-        "one_sample_config": one_choice,
+        "one_sample_config": dict(one_kw, layout=one_tune["layout"]),
         "batched_over_plain_at_value": round(value / plain_value, 3),
         "diffuse_rays_per_step_mrays": round(diffuse_per_step, 1),
         "diffuse_levels_trace_mrays": round(diffuse_trace, 1),
@@ -712,8 +679,8 @@ def main():
         "per_level": per_level,
         "trace_kernels_mrays": kernels,
         "autotuned": tuned,
-        "frame_config_trial_mrays": {k: round(v, 1) for k, v in trial.items()},
-        "frame_config": dict(frame_kw, layout=layout) if frame_kw else "library defaults",
+        "frame_config": dict(frame_kw, layout=layout),
+        "frame_tuner": frame_tune,
         "roofline": roof,
         "cpu_baseline": cpu,
         "adjoint_closest": adjoint,

@@ -30,6 +30,8 @@ rt::dev::persistent_cfg cfg_for(int tracer, const hiprt_tracer_opts *opts) {
     c.stats = nullptr;
     c.stack = (opts && opts->stack_config > 0 && opts->stack_config < rt::dev::kStackConfigs) ? opts->stack_config : 0;
     c.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
+    c.park = (opts && opts->park_leaves > 0) ? std::min(opts->park_leaves, rt::dev::kParkClosest)
+                                             : rt::dev::kParkClosest;
     return c;
 }
 
@@ -93,7 +95,7 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
         rt::scene_desc fmt{};
         fmt.bvh = bvh;
         fmt.tri = tri;
-        HIP_TRY(hiprt_detail::attach_spill(st, s, fmt, any != 0, true, need[g.first], cfg));
+        HIP_TRY(hiprt_detail::attach_spill(st, s, fmt, any != 0, true, need[g.first], cfg, opts ? opts->spill_limit : 0));
         auto *dst = reinterpret_cast<rt::dev::batch_desc *>(static_cast<char *>(table.p) + off);
         HIP_TRY(hipMemcpy(dst, g.second.data(), g.second.size() * sizeof(rt::dev::batch_desc), hipMemcpyHostToDevice));
         HIP_TRY(rt::dev::launch_trace_batch(dst, (uint32_t)g.second.size(), (uint32_t)totals[g.first], tri, bvh,
@@ -188,13 +190,13 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
         for (int tracer = HIPRT_TRACER_PERSISTENT_WHILE_WHILE; tracer < HIPRT_TRACER_COUNT; ++tracer) {
             std::vector<int> thresholds = tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE ? std::vector<int>{24, 40, 56}
                                                                                      : std::vector<int>{40};
-            hiprt_tracer_opts base{40, 0, nullptr, stack, 1};
+            hiprt_tracer_opts base{40, 0, nullptr, stack, 1, 0, 0};
             rt::dev::persistent_cfg pc = cfg_for(tracer, &base);
             int full = 0;
             HIP_TRY(rt::dev::persistent_grid(layouts[f]->desc, any != 0, pc, &full));
             for (int thr : thresholds)
                 for (int chunks : {1, 8}) {
-                    hiprt_tracer_opts o{thr, full, nullptr, stack, chunks};
+                    hiprt_tracer_opts o{thr, full, nullptr, stack, chunks, 0, 0};
                     if (int st = trace(layouts[f], tracer, &o, &h, nt)) return st;
                     float ms = 1e30f;
                     for (int rep = 0; rep < 2; ++rep) {
@@ -216,12 +218,39 @@ int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hip
                     if (!same) continue;
                     if (ms < best_ms) {
                         best_ms = ms;
-                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(nt / (ms * 1e3)), 0, stack, chunks};
+                        best = hiprt_tune_result{tracer, thr, o.blocks, f, (float)(nt / (ms * 1e3)), 0, stack, chunks,
+                                                 rt::dev::kParkClosest};
                     }
                 }
         }
     }
     if (best_ms >= 1e30f) return fail(HIPRT_ERR_UNSUPPORTED, "no configuration reproduced the plain kernel's hits");
+    // the postponement threshold of a speculative winner: parked leaves per lane
+    if (!any && best.tracer >= HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE) {
+        for (int park = 1; park < rt::dev::kParkClosest; ++park) {
+            hiprt_tracer_opts o{best.refill_threshold, best.blocks, nullptr, best.stack_config, best.work_chunks, 0,
+                                park};
+            hiprt_scene lay = layouts[best.bvh_format];
+            if (int st = trace(lay, best.tracer, &o, &h, nt)) return st;
+            HIP_TRY(hipEventRecord(e0, nullptr));
+            (void)trace(lay, best.tracer, &o, &h, nt);
+            HIP_TRY(hipEventRecord(e1, nullptr));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            ++count;
+            HIP_TRY(hipMemcpy(got_t.data(), h.d_t, 4ull * n, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(got_id.data(), h.d_id, 4ull * n, hipMemcpyDeviceToHost));
+            bool same = true;
+            for (uint32_t i = 0; i < n && same; ++i)
+                if (got_id[i] != ref_id[i] && std::memcmp(&got_t[i], &ref_t[i], 4) != 0) same = false;
+            if (same && ms < best_ms) {
+                best_ms = ms;
+                best.park_leaves = park;
+                best.mrays_per_s = (float)(nt / (ms * 1e3));
+            }
+        }
+    }
     best.candidates = count;
     *result = best;
     *tuned = layouts[best.bvh_format];

@@ -49,9 +49,10 @@ rt::scene_desc desc_for(const hiprt_scene_s *sc, bool alpha) {
 }
 
 hipError_t attach_spill(hiprt_stream_state *st, hipStream_t s, const rt::scene_desc &d, bool any, bool multi,
-                        int need, rt::dev::persistent_cfg &cfg) {
+                        int need, rt::dev::persistent_cfg &cfg, int spill_limit) {
     cfg.spill = nullptr;
     cfg.spill_cap = rt::dev::persistent_spill_entries(d.bvh, cfg.stack, need);
+    if (spill_limit > 0 && cfg.spill_cap > (uint32_t)spill_limit) cfg.spill_cap = (uint32_t)spill_limit;
     cfg.spill_bytes = 0;
     if (cfg.spill_cap == 0) return hipSuccess;
     int blocks = 0;
@@ -764,6 +765,8 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     cfg.stats = opts ? reinterpret_cast<unsigned long long *>(opts->d_fetch_stats) : nullptr;
     cfg.stack = stack;
     cfg.chunks = (opts && opts->work_chunks == 1) ? 1 : 8;
+    cfg.park = (opts && opts->park_leaves > 0 && opts->park_leaves < rt::dev::kParkClosest) ? opts->park_leaves
+                                                                                          : rt::dev::kParkClosest;
     uint32_t *counter = nullptr;
     if (noalloc) {
         char *base = nullptr;
@@ -774,6 +777,8 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
         if (!base) return fail(HIPRT_ERR_HIP, "the device's scratch is not set up");
         counter = reinterpret_cast<uint32_t *>(base);
         cfg.spill_cap = rt::dev::persistent_spill_entries(desc.bvh, cfg.stack, need);
+        if (opts && opts->spill_limit > 0 && cfg.spill_cap > (uint32_t)opts->spill_limit)
+            cfg.spill_cap = (uint32_t)opts->spill_limit;
         if (cfg.spill_cap) {
             int blocks = 0;
             HIP_TRY(rt::dev::persistent_grid(desc, any, cfg, &blocks));
@@ -791,7 +796,7 @@ int trace_common(hiprt_scene sc, const hiprt_rays *rays, uint32_t n, uint8_t *oc
     } else {
         hiprt_stream_state *st = sc->ctx->state(stream);
         if (!st) return fail(HIPRT_ERR_HIP, "cannot allocate the stream's work counters");
-        HIP_TRY(hiprt_detail::attach_spill(st, s, desc, any, false, need, cfg));
+        HIP_TRY(hiprt_detail::attach_spill(st, s, desc, any, false, need, cfg, opts ? opts->spill_limit : 0));
         counter = st->counter.as<uint32_t>();
     }
     HIP_TRY(rt::dev::launch_trace_persistent(desc, r, h, occluded, n, nullptr, counter, any, cfg, s));

@@ -122,8 +122,9 @@ rt::scene_desc desc_for(const hiprt_scene_s *sc, bool alpha);
 // trees whose traversal can hold `need` stack entries (none when the
 // configuration's VGPR and LDS entries hold them): the stream state's
 // buffer, grown once the stream's earlier launches, which may still use the
-// old one, have finished.  `multi` selects the descriptor-table kernel.
+// old one, have finished.  `multi` selects the descriptor-table kernel;
+// spill_limit > 0 caps the entries per lane (hiprt_tracer_opts::spill_limit).
 hipError_t attach_spill(hiprt_stream_state *st, hipStream_t s, const rt::scene_desc &d, bool any, bool multi,
-                        int need, rt::dev::persistent_cfg &cfg);
+                        int need, rt::dev::persistent_cfg &cfg, int spill_limit = 0);
 
 } // namespace hiprt_detail

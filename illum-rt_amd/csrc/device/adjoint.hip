@@ -71,8 +71,8 @@ __global__ __launch_bounds__(256) void k_adjoint_closest(adjoint_args a, uint32_
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     ray_adjoint r{v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f), v3(0.0f)};
-    const bool valid = a.hits.t[i] != FLT_MAX;
     const uint32_t tri = a.hits.id[i];
+    const bool valid = a.hits.t[i] < FLT_MAX && tri < a.n_tris;   // a hit on a triangle of this scene
     if (valid) r = adjoint_of(a, i, tri);
     if (a.keys) {
         a.keys[i] = valid ? tri : a.n_tris;

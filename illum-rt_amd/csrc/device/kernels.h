@@ -23,7 +23,8 @@ constexpr int kLdsStack = 28;
 constexpr int kPersistentWavesPerBlock = 4;
 constexpr int kStackConfigs = 5;
 // Leaves a closest-hit lane of the speculative persistent tracers parks
-// before it stops descending (at most 4).
+// before it stops descending: registers for at most 4; persistent_cfg::park
+// picks 1..4 at run time (default 4).
 constexpr int kParkClosest = 4;
 // Work counters of the persistent kernels: one per chunk, each in its own
 // 256-byte line; kCounterBytes of device memory hold all of them.
@@ -77,6 +78,7 @@ struct persistent_cfg {
     unsigned long long *stats = nullptr;   // non-null: counting variant (inner nodes, triangle tests)
     int stack = 0;        // stack configuration 0..kStackConfigs-1, see persistent_stack_cap
     int chunks = 8;       // 1: one work counter; 8: one contiguous chunk of rays per XCD
+    int park = kParkClosest;   // parked leaves per closest-hit lane of the speculative tracers (1..4)
     // Global spill buffer: spill_cap entries per lane of the grid
     // (persistent_spill_bytes); spill_cap 0 when the tree fits VGPRs + LDS.
     int32_t *spill = nullptr;

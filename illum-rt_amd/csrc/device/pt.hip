@@ -56,7 +56,7 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
                                                int level, const uint64_t *frame_seed, vec3f in_dir, float t, float beta,
                                                float gamma, uint32_t tri, vec3f &T, float *rad) {
     shade_out so{false, v3(0.0f), v3(0.0f)};
-    if (t == FLT_MAX) return so;
+    if (!ps.valid(t, tri)) return so;
     const float4 *g = ps.tri_geo + 3 * (size_t)tri;
     float4 g0 = g[0], g1 = g[1], g2 = g[2];
     vec3f A = f3(g0), B = f3(g1), C = f3(g2);
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void k_cosine_bounce(pt_scene ps, ray_soa in, 
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     float t = hits.t[i];
-    if (t == FLT_MAX || !(in.tmax[i] >= 0.0f)) {
+    if (!ps.valid(t, hits.id[i]) || !(in.tmax[i] >= 0.0f)) {
         write_disabled(out, i);
         return;
     }

@@ -23,6 +23,11 @@ struct ray_out {
 struct pt_scene {
     const float4 *tri_geo;
     const float4 *mats;
+    uint32_t n_tris;
+    // A hit the shading steps may use: finite t below FLT_MAX (misses are
+    // FLT_MAX, a traversal that lost stack entries is NaN) on a triangle of
+    // this scene.
+    __device__ __forceinline__ bool valid(float t, uint32_t id) const { return t < FLT_MAX && id < n_tris; }
 };
 
 struct pt_params {

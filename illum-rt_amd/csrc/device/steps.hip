@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_add_albedo(pt_scene ps, hit_soa hits, f
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     float4 acc = fb[i];
-    if (hits.t[i] != FLT_MAX) {
+    if (ps.valid(hits.t[i], hits.id[i])) {
         uint32_t m = __float_as_uint(ps.tri_geo[3 * (size_t)hits.id[i]].w);
         float4 a = ps.mats[2 * m];
         acc.x += a.x;
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_shadow_rays(pt_scene ps, ray_soa in, hi
     if (i >= n) return;
     vec3f o = v3(0.0f), d = v3(1.0f);
     float tmax = -FLT_MAX;
-    if (hits.t[i] != FLT_MAX && in.tmax[i] >= 0.0f) {
+    if (ps.valid(hits.t[i], hits.id[i]) && in.tmax[i] >= 0.0f) {
         const float4 *g = ps.tri_geo + 3 * (size_t)hits.id[i];
         vec3f A = f3(g[0]), B = f3(g[1]), C = f3(g[2]);
         o = hit_point(A, B, C, hits.u[i], hits.v[i]);
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_integrate_light(pt_scene ps, ray_soa ca
     if (i >= n) return;
     vec3f rad = v3(0.0f);
     bool valid_shadow = shadow.tmax[i] > 0.0f;
-    if (hits.t[i] != FLT_MAX && valid_shadow && !occluded[i]) {
+    if (ps.valid(hits.t[i], hits.id[i]) && valid_shadow && !occluded[i]) {
         const float4 *g = ps.tri_geo + 3 * (size_t)hits.id[i];
         float4 g0 = g[0];
         vec3f A = f3(g0), B = f3(g[1]), C = f3(g[2]);
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_sample_direct(pt_scene ps, light_table 
     float pdf = mode == kDirectUniform ? kOneOver2Pi : (mode == kDirectCosine ? kOneOverPi : 0.0f);
     vec3f o = v3(0.0f), d = v3(1.0f), lc = v3(0.0f);
     float tmax = -FLT_MAX;
-    if (hits.t[i] != FLT_MAX) {
+    if (ps.valid(hits.t[i], hits.id[i])) {
         const float4 *g = ps.tri_geo + 3 * (size_t)hits.id[i];
         float4 g0 = g[0];
         vec3f A = f3(g0), B = f3(g[1]), C = f3(g[2]);
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_integrate_dir(pt_scene ps, ray_soa cam,
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     vec3f rad = v3(0.0f);
-    if (hits.t[i] != FLT_MAX && shadow_hits.t[i] != FLT_MAX) {
+    if (ps.valid(hits.t[i], hits.id[i]) && ps.valid(shadow_hits.t[i], shadow_hits.id[i])) {
         uint32_t lm = __float_as_uint(ps.tri_geo[3 * (size_t)shadow_hits.id[i]].w);
         vec3f emi = f3(ps.mats[2 * lm + 1]);
         if (emi != v3(0.0f)) {
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void k_integrate_light_sample(pt_scene ps, ray
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     vec3f rad = v3(0.0f);
-    if (hits.t[i] != FLT_MAX && shadow.tmax[i] > 0.0f && !occluded[i]) {
+    if (ps.valid(hits.t[i], hits.id[i]) && shadow.tmax[i] > 0.0f && !occluded[i]) {
         const float4 *g = ps.tri_geo + 3 * (size_t)hits.id[i];
         float4 g0 = g[0];
         vec3f ns = geom_normal(f3(g0), f3(g[1]), f3(g[2]));

@@ -238,6 +238,7 @@ struct dhit {
     float t, u, v;
     uint32_t slot;
     bool found;
+    bool over;   // the traversal dropped a stack entry: the result is not known
 };
 
 __device__ __forceinline__ void store_hit(const hit_soa &h, uint32_t i, float t, float u, float v, uint32_t id) {
@@ -245,6 +246,15 @@ __device__ __forceinline__ void store_hit(const hit_soa &h, uint32_t i, float t,
     h.u[i] = u;
     h.v[i] = v;
     h.id[i] = id;
+}
+
+// The result of a ray whose traversal lost a stack entry (the host sizes
+// every stack for the tree, so this marks a sizing error, never a hit):
+// t = NaN and id 0xFFFFFFFF, occlusion byte 0xFF.  An any-hit ray that found
+// an occluder is unaffected (a lost entry can only hide occluders).
+__device__ __forceinline__ void store_lost(const hit_soa &h, uint8_t *occluded, uint32_t i) {
+    if (occluded) occluded[i] = 0xFF;
+    if (h.t) store_hit(h, i, __uint_as_float(0x7FC00000u), 0.0f, 0.0f, 0xffffffffu);
 }
 
 } // namespace dev

@@ -15,7 +15,8 @@
 //   * leaf postponement (SPEC): a lane that reaches a leaf parks it and keeps
 //     descending until every lane still searching has a parked leaf, so the
 //     wave runs triangle tests together instead of one lane at a time.  A
-//     closest-hit lane parks up to kParkClosest leaves (any-hit: one), so
+//     closest-hit lane parks up to `park` leaves (1..4, default
+//     kParkClosest = 4; any-hit: one), so
 //     lanes that find leaves early keep doing node work while the wave waits
 //     for its slowest lane's first leaf (node-loop lane occupancy on diffuse
 //     hall rays 33% with one parked leaf, 55% with four);
@@ -71,26 +72,31 @@ struct lane_stack {
     uint32_t gs;    // entries spilled to the global column (the oldest ones)
     int32_t *col;   // this lane's LDS column, entries 64 words apart
     spill_col g;
+    bool over;      // an entry was dropped: the host sized the stack too small
 
     __device__ __forceinline__ void reset() {
         rs = 0;
         ls = 0;
         ts = 0;
         gs = 0;
+        over = false;
     }
     // Below the registers: an LDS ring of LDS entries.  When it is full, its
     // oldest entry (the one in slot ts) moves to the spill column, so the
     // order of the whole stack is kept: registers newest, then the ring, then
     // the spill column.  The host sizes the column for the tree's stack
-    // need; entries beyond it would be dropped, never written out of bounds.
+    // need; entries beyond it are dropped, never written out of bounds, and
+    // the ray's result is then marked lost (store_lost).
     __device__ __forceinline__ void push_lds(int32_t x) {
         if (!RING) {
             // the host picks this form only when the tree's need fits
             if (ls < LDS) col[64 * ls++] = x;
+            else over = true;
             return;
         }
         if (ls == LDS) {
             if (gs < g.cap) g.p[(size_t)gs * g.stride] = col[64 * ts];
+            else over = true;
             ++gs;
         } else {
             ++ls;
@@ -201,7 +207,7 @@ constexpr int waves_hint(int reg, int lds, int bvh, bool multi) {
 template <int TRI, bool ANY, bool SPEC, bool REFILL, int REG, int LDS, bool COUNT, int BVH, bool MULTI, bool RING>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_hint(REG, LDS, BVH, MULTI))))
 void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occluded, uint32_t n_static,
-                        const uint32_t *count_ptr, uint32_t *counter, int refill_thr, unsigned long long *stats,
+                        const uint32_t *count_ptr, uint32_t *counter, int lane_limits, unsigned long long *stats,
                         const batch_desc *table, uint32_t n_desc, uint32_t n_chunks, int32_t *spill,
                         uint32_t spill_cap, ray_gather gather) {
     __shared__ int32_t lds_stack[kPersistentWavesPerBlock * LDS * 64];
@@ -222,7 +228,11 @@ void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occl
     dray r{};
     int32_t cur = kInvalidRef;
     // parked leaves (SPEC): p0 is the oldest, pc how many are held
-    constexpr int PARK = ANY ? 1 : kParkClosest;
+    // lane_limits: the refill threshold (low byte) and the parked-leaf limit
+    // (next byte), one argument so the kernel takes no extra SGPR
+    const int refill_thr = lane_limits & 0xFF;
+    constexpr int PARK = ANY ? 1 : kParkClosest;   // registers held for parked leaves
+    const int park = ANY ? 1 : ((lane_limits >> 8) & 0xFF);
     int32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
     int pc = 0;
     uint32_t ray_id = 0;
@@ -346,7 +356,7 @@ void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occl
                     }
                     if (SPEC) {
                         // park leaves (up to PARK), then keep descending
-                        while (cur < 0 && pc < PARK) {
+                        while (cur < 0 && pc < park) {
                             if (PARK > 3) p3 = pc == 3 ? cur : p3;
                             if (PARK > 2) p2 = pc == 2 ? cur : p2;
                             if (PARK > 1) p1 = pc == 1 ? cur : p1;
@@ -408,7 +418,9 @@ void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occl
                     ho = table[desc].hits;
                     occ = table[desc].occluded;
                 }
-                if (ANY) {
+                if (st.over && !(ANY && h.found)) {
+                    store_lost(ho, ANY ? occ : nullptr, ray_id);
+                } else if (ANY) {
                     if (occ) occ[ray_id] = h.found ? 1 : 0;
                     if (ho.t)
                         store_hit(ho, ray_id, h.found ? 4294967295.0f : FLT_MAX, 0.0f, 0.0f,
@@ -497,6 +509,14 @@ struct launch_args {
 
 __host__ __device__ constexpr uint32_t chunks_of(int c) { return c == 8 ? 8u : 1u; }
 
+// Refill threshold (1..64) and parked-leaf limit (1..kParkClosest) packed
+// into the kernel's lane_limits argument.
+inline int lane_limits_of(const persistent_cfg &cfg) {
+    int thr = cfg.refill_thr < 1 ? 1 : (cfg.refill_thr > 64 ? 64 : cfg.refill_thr);
+    int park = cfg.park < 1 ? kParkClosest : (cfg.park > kParkClosest ? kParkClosest : cfg.park);
+    return thr | (park << 8);
+}
+
 template <int TRI, bool ANY, bool SPEC, bool REFILL, int BVH, bool MULTI, int STK, bool RING>
 hipError_t launch_ring(const launch_args &a, const persistent_cfg &cfg, hipStream_t stream, bool grid_only,
                        int *grid_out) {
@@ -515,14 +535,14 @@ hipError_t launch_ring(const launch_args &a, const persistent_cfg &cfg, hipStrea
         if (cfg.stats) {
             hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, true, BVH, MULTI, true>),
                                dim3(blocks), dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr,
-                               a.counter, cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks),
+                               a.counter, lane_limits_of(cfg), cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks),
                                cfg.spill, cfg.spill_cap, a.gather);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((k_trace_persistent<TRI, ANY, SPEC, REFILL, R, L, false, BVH, MULTI, RING>), dim3(blocks),
                        dim3(256), 0, stream, a.sc, a.rays, a.hits, a.occluded, a.n, a.count_ptr, a.counter,
-                       cfg.refill_thr, cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill, cfg.spill_cap,
+                       lane_limits_of(cfg), cfg.stats, a.table, a.n_desc, chunks_of(cfg.chunks), cfg.spill, cfg.spill_cap,
                        a.gather);
     return hipGetLastError();
 }

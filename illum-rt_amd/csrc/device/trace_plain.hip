@@ -23,6 +23,7 @@ __device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray 
     int32_t cur = sc.root_ref;
     const node2 *nodes = static_cast<const node2 *>(sc.nodes);
     h.found = false;
+    h.over = false;
     h.t = ANY ? INFINITY : FLT_MAX;
     while (cur != kInvalidRef) {
         while (cur >= 0 && cur != kInvalidRef) {
@@ -38,6 +39,7 @@ __device__ __forceinline__ void traverse_plain(const scene_desc &sc, const dray 
                 int32_t far = left_first ? rr : rl;
                 cur = left_first ? rl : rr;
                 if (sp < kPlainStackCap - 1) stack[++sp] = far;
+                else h.over = true;
             } else if (hl) {
                 cur = rl;
             } else if (hr) {
@@ -81,7 +83,9 @@ __global__ __launch_bounds__(256) void k_trace_plain(scene_desc sc, ray_soa rays
         atomicAdd(stats, (unsigned long long)c_nodes);
         atomicAdd(stats + 1, (unsigned long long)c_tris);
     }
-    if (ANY) {
+    if (h.over && !(ANY && h.found)) {
+        store_lost(hits, ANY ? occluded : nullptr, i);
+    } else if (ANY) {
         if (occluded) occluded[i] = h.found ? 1 : 0;
         // The reference's CPU adapter marks an occluded ray with
         // t = ref = (unsigned)-1 and resets the others (rt/cpu/wavefront.cpp:60-70).

@@ -117,7 +117,7 @@ struct hiprt_pt_s {
     }
 
     rt::dev::pt_scene ps() const {
-        return rt::dev::pt_scene{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+        return rt::dev::pt_scene{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     }
 
     // The frame's launches on stream s (frame seed already set).  When
@@ -239,7 +239,7 @@ int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits
     rt::ray_soa in{rays->d_ox, rays->d_oy, rays->d_oz, rays->d_tmin, rays->d_dx, rays->d_dy, rays->d_dz, rays->d_tmax};
     rt::hit_soa h{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
     rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     HIP_TRY(rt::dev::launch_cosine_bounce(ps, in, h, o, seed, level, n, static_cast<hipStream_t>(stream)));
     return ok();
 }
@@ -259,7 +259,7 @@ int hiprt_add_hitpoint_albedo(hiprt_scene sc, const hiprt_hits *hits, uint32_t n
     if (!sc || !full_hits(hits) || (n && !d_framebuffer)) return fail(HIPRT_ERR_ARG, "null argument");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     HIP_TRY(rt::dev::launch_add_albedo(ps, hsoa(hits), reinterpret_cast<float4 *>(d_framebuffer), n,
                                        static_cast<hipStream_t>(stream)));
     return ok();
@@ -273,7 +273,7 @@ int hiprt_shadow_rays(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *
         return fail(HIPRT_ERR_ARG, "null output plane");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
     HIP_TRY(rt::dev::launch_shadow_rays(ps, soa(rays), hsoa(hits), rt::vec3f{light[0], light[1], light[2]}, o, n,
                                         static_cast<hipStream_t>(stream)));
@@ -288,7 +288,7 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
         return fail(HIPRT_ERR_ARG, "null argument");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     HIP_TRY(rt::dev::launch_integrate_light(ps, soa(cam_rays), hsoa(hits), soa(shadow_rays), d_occluded,
                                             rt::vec3f{light[0], light[1], light[2]},
                                             rt::vec3f{col[0], col[1], col[2]},
@@ -366,7 +366,7 @@ int hiprt_sample_direct(hiprt_scene sc, int32_t mode, const hiprt_rays *cam_rays
         return fail(HIPRT_ERR_ARG, "light sampling needs lights (hiprt_scene_set_lights found none)");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     rt::dev::light_table lt{sc->light_rec.as<const float4>(), sc->light_f.as<const float>(),
                             sc->light_cdf.as<const float>(), sc->n_lights, sc->light_integral};
     rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
@@ -384,7 +384,7 @@ int hiprt_integrate_dir_sample(hiprt_scene sc, const hiprt_rays *cam_rays, const
     if (n && (!d_pdf || !d_framebuffer)) return fail(HIPRT_ERR_ARG, "null pdf or framebuffer");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     HIP_TRY(rt::dev::launch_integrate_dir(ps, soa(cam_rays), hsoa(cam_hits), soa(shadow_rays), hsoa(shadow_hits),
                                           d_pdf, reinterpret_cast<float4 *>(d_framebuffer), n,
                                           static_cast<hipStream_t>(stream)));
@@ -400,7 +400,7 @@ int hiprt_integrate_light_sample(hiprt_scene sc, const hiprt_rays *cam_rays, con
         return fail(HIPRT_ERR_ARG, "null occlusion, pdf, light colour or framebuffer");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>()};
+    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
     HIP_TRY(rt::dev::launch_integrate_light_sample(ps, soa(cam_rays), hsoa(cam_hits), soa(shadow_rays), d_occluded,
                                                    d_pdf, d_light_col, reinterpret_cast<float4 *>(d_framebuffer), n,
                                                    static_cast<hipStream_t>(stream)));
@@ -522,6 +522,8 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pcfg.blocks = 0;
     p->pcfg.stack = stack;
     p->pcfg.chunks = params->work_chunks == 1 ? 1 : 8;
+    p->pcfg.park = (params->park_leaves > 0 && params->park_leaves < rt::dev::kParkClosest) ? params->park_leaves
+                                                                                          : rt::dev::kParkClosest;
     if (p->sorted) {
         for (int k = 0; k < 2; ++k) {
             HIP_TRY(p->rec[k].alloc(16ull * rt::dev::kRayRecord * n));
@@ -578,6 +580,92 @@ int hiprt_pt_render(hiprt_pt pt, uint32_t frame, void *stream) {
         HIP_TRY(pt->record(s, pt->clear_next));
     }
     pt->clear_next = false;
+    return ok();
+}
+
+int hiprt_pt_autotune(hiprt_scene bvh2_scene, hiprt_scene bvh4_scene, const float *cam,
+                      const hiprt_pt_params *base, int32_t max_samples, int32_t trial_frames,
+                      hiprt_pt_tune_result *out) {
+    if (!bvh2_scene || !cam || !base || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (bvh2_scene->desc.bvh != rt::BVH2) return fail(HIPRT_ERR_ARG, "bvh2_scene is not a binary-tree scene");
+    if (bvh4_scene && bvh4_scene->desc.bvh != rt::BVH4) return fail(HIPRT_ERR_ARG, "bvh4_scene is not a 4-wide scene");
+    const int frames = trial_frames > 0 ? trial_frames : 2;
+    const int max_s = max_samples > 1 ? std::min(max_samples, 64) : 1;
+    HIP_TRY(hipSetDevice(bvh2_scene->ctx->device));
+    struct events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    HIP_TRY(hipEventCreate(&ev.e[0]));
+    HIP_TRY(hipEventCreate(&ev.e[1]));
+    int count = 0;
+    // rays per millisecond of one candidate (negative on failure)
+    auto trial = [&](int layout, int tracer, int stack, int samples, int park, float *rate) -> int {
+        hiprt_pt_params p = *base;
+        p.batched = 1;
+        p.tracer = tracer | (base->tracer >= 0 ? (base->tracer & HIPRT_TRACER_ALPHA) : 0);
+        p.stack_config = stack;
+        p.samples = samples;
+        p.park_leaves = park;
+        hiprt_pt pt = nullptr;
+        int st = hiprt_pt_create(layout == HIPRT_BVH4 ? bvh4_scene : bvh2_scene, cam, &p, &pt);
+        if (st != HIPRT_OK) return st;
+        struct destroy_at_exit {
+            hiprt_pt p;
+            ~destroy_at_exit() { (void)hiprt_pt_destroy(p); }
+        } own{pt};
+        const int levels = p.max_depth + 1;
+        std::vector<uint64_t> t0(levels), t1(levels);
+        if ((st = hiprt_pt_render(pt, 0, nullptr)) != HIPRT_OK) return st;
+        if ((st = hiprt_pt_ray_totals(pt, t0.data(), levels)) != HIPRT_OK) return st;
+        HIP_TRY(hipEventRecord(ev.e[0], nullptr));
+        for (int f = 1; f <= frames; ++f)
+            if ((st = hiprt_pt_render(pt, (uint32_t)f, nullptr)) != HIPRT_OK) return st;
+        HIP_TRY(hipEventRecord(ev.e[1], nullptr));
+        HIP_TRY(hipEventSynchronize(ev.e[1]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+        if ((st = hiprt_pt_ray_totals(pt, t1.data(), levels)) != HIPRT_OK) return st;
+        double rays = 0;
+        for (int L = 0; L < levels; ++L) rays += (double)(t1[L] - t0[L]);
+        ++count;
+        *rate = ms > 0 ? (float)(rays / ms) : 0.0f;
+        return HIPRT_OK;
+    };
+    hiprt_pt_tune_result best{};
+    best.mrays_per_s = -1;
+    auto consider = [&](int layout, int tracer, int stack, int samples, int park) -> int {
+        float rate = 0;
+        int st = trial(layout, tracer, stack, samples, park, &rate);
+        if (st != HIPRT_OK) return st;
+        if (rate / 1e3f > best.mrays_per_s)
+            best = hiprt_pt_tune_result{tracer, stack, samples, layout, park, rate / 1e3f, 0};
+        return HIPRT_OK;
+    };
+    // 1. tracer x layout x stack configuration, one sample
+    for (int tracer : {HIPRT_TRACER_DYNAMIC_WHILE_WHILE, HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE}) {
+        for (int stack : {0, 3, 4})
+            if (int st = consider(HIPRT_BVH2, tracer, stack, 1, rt::dev::kParkClosest)) return st;
+        if (bvh4_scene)
+            for (int stack : {0, 3})
+                if (int st = consider(HIPRT_BVH4, tracer, stack, 1, rt::dev::kParkClosest)) return st;
+    }
+    // 2. samples in flight for the winner and for the binary tree with stack 4
+    const hiprt_pt_tune_result one = best;
+    for (int s = 2; s <= max_s; s *= 2) {
+        if (int st = consider(one.bvh_format, one.tracer, one.stack_config, s, rt::dev::kParkClosest)) return st;
+        if (!(one.bvh_format == HIPRT_BVH2 && one.stack_config == 4))
+            if (int st = consider(HIPRT_BVH2, one.tracer, 4, s, rt::dev::kParkClosest)) return st;
+    }
+    // 3. parked leaves
+    const hiprt_pt_tune_result win = best;
+    for (int park = 1; park < rt::dev::kParkClosest; ++park)
+        if (int st = consider(win.bvh_format, win.tracer, win.stack_config, win.samples, park)) return st;
+    best.candidates = count;
+    *out = best;
     return ok();
 }
 

@@ -52,7 +52,8 @@ class _Hits(ctypes.Structure):
 
 class _TracerOpts(ctypes.Structure):
     _fields_ = [("refill_threshold", c_int32), ("blocks", c_int32), ("d_fetch_stats", c_void_p),
-                ("stack_config", c_int32), ("work_chunks", c_int32)]
+                ("stack_config", c_int32), ("work_chunks", c_int32), ("spill_limit", c_int32),
+                ("park_leaves", c_int32)]
 
 
 class _BatchEntry(ctypes.Structure):
@@ -62,14 +63,19 @@ class _BatchEntry(ctypes.Structure):
 class _TuneResult(ctypes.Structure):
     _fields_ = [("tracer", c_int32), ("refill_threshold", c_int32), ("blocks", c_int32), ("bvh_format", c_int32),
                 ("mrays_per_s", c_float), ("candidates", c_int32), ("stack_config", c_int32),
-                ("work_chunks", c_int32)]
+                ("work_chunks", c_int32), ("park_leaves", c_int32)]
+
+
+class _PtTuneResult(ctypes.Structure):
+    _fields_ = [("tracer", c_int32), ("stack_config", c_int32), ("samples", c_int32), ("bvh_format", c_int32),
+                ("park_leaves", c_int32), ("mrays_per_s", c_float), ("candidates", c_int32)]
 
 
 class _PtParams(ctypes.Structure):
     _fields_ = [("max_depth", c_int32), ("rr_start", c_int32), ("rr_survive", c_float), ("seed", c_uint64),
                 ("tracer", c_int32), ("batched", c_int32), ("stack_config", c_int32), ("work_chunks", c_int32),
                 ("refill_threshold", c_int32), ("blocks", c_int32), ("ray_order", c_int32), ("samples", c_int32),
-                ("emitters_reflect", c_int32), ("pixel_centre", c_int32)]
+                ("emitters_reflect", c_int32), ("park_leaves", c_int32), ("pixel_centre", c_int32)]
 
 
 _lib = None
@@ -132,6 +138,8 @@ def lib():
                                             POINTER(_Rays), c_void_p]),
             "hiprt_pt_create": (c_int, [c_void_p, POINTER(c_float), POINTER(_PtParams), POINTER(c_void_p)]),
             "hiprt_pt_render": (c_int, [c_void_p, c_uint32, c_void_p]),
+            "hiprt_pt_autotune": (c_int, [c_void_p, c_void_p, POINTER(c_float), POINTER(_PtParams), c_int32, c_int32,
+                                          POINTER(_PtTuneResult)]),
             "hiprt_pt_ray_counts": (c_int, [c_void_p, POINTER(c_uint32), c_int32]),
             "hiprt_pt_image": (c_int, [c_void_p, c_void_p]),
             "hiprt_pt_ray_totals": (c_int, [c_void_p, POINTER(c_uint64), c_int32]),
@@ -411,18 +419,19 @@ def tracer_id(name):
     return i
 
 
-def _opts(refill_threshold=0, blocks=0, fetch_stats=None, stack_config=0, work_chunks=0):
+def _opts(refill_threshold=0, blocks=0, fetch_stats=None, stack_config=0, work_chunks=0, spill_limit=0,
+          park_leaves=0):
     return _TracerOpts(refill_threshold, blocks, fetch_stats.data_ptr() if fetch_stats is not None else None,
-                       stack_config, work_chunks)
+                       stack_config, work_chunks, spill_limit, park_leaves)
 
 
 def trace_closest(dscene, rays, hits, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
-                  fetch_stats=None, stack_config=0, work_chunks=0):
+                  fetch_stats=None, stack_config=0, work_chunks=0, spill_limit=0, park_leaves=0):
     """Closest hits of `rays` (8, n) into `hits`.  fetch_stats: optional int64
     CUDA tensor of 2 that receives inner-node fetches and triangle tests."""
     n = rays.shape[1]
     r, h = _rays_struct(rays), _hits_struct(hits)
-    o = _opts(refill_threshold, blocks, fetch_stats, stack_config, work_chunks)
+    o = _opts(refill_threshold, blocks, fetch_stats, stack_config, work_chunks, spill_limit, park_leaves)
     s = c_void_p(stream) if stream else None
     check(lib().hiprt_trace_closest(dscene.h, ctypes.byref(r), n, ctypes.byref(h), tracer_id(tracer),
                                     ctypes.byref(o), s))
@@ -440,9 +449,9 @@ def trace_closest_noalloc(dscene, rays, hits, tracer="persistent-while-while", s
 
 
 def trace_any(dscene, rays, occluded, hits=None, tracer="while-while", stream=None, refill_threshold=0, blocks=0,
-              stack_config=0, work_chunks=0):
+              stack_config=0, work_chunks=0, spill_limit=0):
     n = rays.shape[1]
-    r, o = _rays_struct(rays), _opts(refill_threshold, blocks, None, stack_config, work_chunks)
+    r, o = _rays_struct(rays), _opts(refill_threshold, blocks, None, stack_config, work_chunks, spill_limit)
     hp = ctypes.byref(_hits_struct(hits)) if hits is not None else None
     occ = c_void_p(occluded.data_ptr()) if occluded is not None else None
     s = c_void_p(stream) if stream else None
@@ -541,7 +550,7 @@ class PathTracer:
 
     def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None,
                  stack_config=0, work_chunks=0, refill_threshold=0, blocks=0, sort_rays=True, samples=1,
-                 emitters_reflect=False, jitter=True):
+                 emitters_reflect=False, jitter=True, park_leaves=0):
         self.dscene = dscene
         c = np.ascontiguousarray(cam[:12], dtype=np.float32)
         self.w, self.h = int(c[10]), int(c[11])
@@ -549,7 +558,7 @@ class PathTracer:
         tid = -1 if tracer is None else tracer_id(tracer)
         p = _PtParams(max_depth, rr_start, rr_survive, seed, tid, 1 if batched else 0, stack_config, work_chunks,
                       refill_threshold, blocks, 0 if sort_rays else 1, samples, 1 if emitters_reflect else 0,
-                      0 if jitter else 1)
+                      park_leaves, 0 if jitter else 1)
         self.h_ = c_void_p()
         check(lib().hiprt_pt_create(dscene.h, c.ctypes.data_as(POINTER(c_float)), ctypes.byref(p),
                                     ctypes.byref(self.h_)))
@@ -591,6 +600,23 @@ class PathTracer:
     def __del__(self):
         if _lib is not None:
             self.close()
+
+
+def pt_autotune(dscene, dscene4, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, max_samples=64,
+                trial_frames=2, sort_rays=True):
+    """hiprt_pt_autotune: the batched path tracer's configuration for this
+    scene and camera (dscene4: the 4-wide layout or None).  Returns the
+    PathTracer keyword arguments of the winner and the measurement."""
+    c = np.ascontiguousarray(cam[:12], dtype=np.float32)
+    base = _PtParams(max_depth, rr_start, rr_survive, seed, -1, 1, 0, 0, 0, 0, 0 if sort_rays else 1, 1, 0, 0, 0)
+    res = _PtTuneResult()
+    check(lib().hiprt_pt_autotune(dscene.h, dscene4.h if dscene4 is not None else None,
+                                  c.ctypes.data_as(POINTER(c_float)), ctypes.byref(base), max_samples, trial_frames,
+                                  ctypes.byref(res)))
+    kw = {"tracer": TRACERS[res.tracer], "stack_config": res.stack_config, "samples": res.samples,
+          "park_leaves": res.park_leaves}
+    return kw, {"layout": "bvh4" if res.bvh_format == BVH4 else "bvh2", "mrays_per_s": round(res.mrays_per_s, 1),
+                "candidates": res.candidates}
 
 
 def _dp(t):
@@ -641,5 +667,6 @@ def autotune(ctx, scene, bvh, sample_rays, any=False):
     ds = DeviceScene(ctx, None, None, handle=h)
     out = {"tracer": TRACERS[res.tracer], "refill_threshold": res.refill_threshold, "blocks": res.blocks,
            "bvh_format": "bvh4" if res.bvh_format == BVH4 else "bvh2", "mrays_per_s": res.mrays_per_s,
-           "candidates": res.candidates, "stack_config": res.stack_config, "work_chunks": res.work_chunks}
+           "candidates": res.candidates, "stack_config": res.stack_config, "work_chunks": res.work_chunks,
+           "park_leaves": res.park_leaves}
     return out, ds

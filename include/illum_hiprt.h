@@ -239,6 +239,14 @@ typedef struct {
                                  contiguous chunk of the rays per XCD, blocks start on
                                  their own XCD's chunk (fetch atomics spread over 8
                                  counters: +57% on short any-hit rays) */
+    int32_t spill_limit;      /* > 0: at most this many spill-column entries per lane
+                                 (bounds the spill memory); 0 = what the tree needs.  A
+                                 traversal that outgrows its stack never writes out of
+                                 bounds: its result is marked lost -- t = NaN, id
+                                 0xFFFFFFFF, occlusion byte 0xFF -- for that ray alone */
+    int32_t park_leaves;      /* speculative / dynamic closest hit: leaves a lane parks before
+                                 it stops descending, 1..4 (0 = 4), the postponement
+                                 threshold */
 } hiprt_tracer_opts;
 
 /* compute_closest_hit of batch_ray_tracer (libgi/wavefront-rt.h:145-149,
@@ -292,7 +300,8 @@ int hiprt_trace_batch(const hiprt_batch_entry *entries, uint32_t n_entries, int3
  * Times candidate configurations on a sample of a scene's rays -- tracer
  * variant (persistent / speculative / dynamic), refill threshold, stack
  * configuration, work chunks and BVH layout (binary vs collapsed 4-wide), on
- * the occupancy-derived grid -- and returns the fastest.  A sample of fewer
+ * the occupancy-derived grid, then the parked-leaf limit of the winner -- and
+ * returns the fastest.  A sample of fewer
  * than 2^20 rays is tiled up to that size before timing: a batch smaller than
  * the lanes a persistent grid holds (~300k) measures tail latency and ranks
  * configurations wrongly.  Every candidate's hits on the sample are compared
@@ -308,6 +317,7 @@ typedef struct {
     int32_t candidates;    /* configurations timed */
     int32_t stack_config;
     int32_t work_chunks;
+    int32_t park_leaves;   /* 1..4, timed for the speculative and dynamic tracers */
 } hiprt_tune_result;
 
 int hiprt_autotune(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, const hiprt_rays *sample, uint32_t n,
@@ -435,6 +445,7 @@ typedef struct {
     int32_t emitters_reflect; /* 0 (default): a path ends at the first emitter it meets, after
                                  adding throughput x emission (gi/pt.cpp:86-89); 1: emitters
                                  also reflect and the path goes on */
+    int32_t park_leaves;    /* batched: parked leaves per lane (1..4, 0 = 4) */
     int32_t pixel_centre;   /* 0 (default): each sample's camera ray passes through a random point
                                of its pixel -- offsets in [-0.5, 0.5)^2 from the path's own PCG32
                                stream, as simple_pt::sample_pixel jitters (gi/pt.cpp:36);
@@ -446,6 +457,34 @@ typedef struct hiprt_pt_s *hiprt_pt;
 int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *params, hiprt_pt *out);
 /* Renders one frame (one sample per pixel) into the accumulator, async. */
 int hiprt_pt_render(hiprt_pt pt, uint32_t frame, void *stream);
+
+/* Frame-level autotuner of the batched schedule (no reference counterpart;
+ * the reference's CUDA tracers are compiled with fixed settings,
+ * rt/cuda/trace-kernels.cu:5-10, rt/cuda/base.h:10-18).  Renders trial frames
+ * of the caller's camera and path parameters (`base`; its tracer, stack,
+ * samples and park fields are ignored) with candidate configurations and
+ * returns the one that traces the most rays per second:
+ *   1. tracer (speculative, dynamic) x layout (bvh2_scene; bvh4_scene when not
+ *      NULL) x stack configuration (0 and 3; 4 on the binary tree), one
+ *      sample per pixel;
+ *   2. samples in flight 2, 4, ... up to max_samples for the winner of 1 and
+ *      for the binary tree with stack configuration 4;
+ *   3. parked leaves 1..3 for the winner.
+ * Each candidate renders one warm-up and `trial_frames` (0: 2) timed frames
+ * on its own path tracer.  Images do not depend on the choice.  Synchronous. */
+typedef struct {
+    int32_t tracer;
+    int32_t stack_config;
+    int32_t samples;
+    int32_t bvh_format;     /* HIPRT_BVH2: bvh2_scene, HIPRT_BVH4: bvh4_scene */
+    int32_t park_leaves;
+    float mrays_per_s;      /* on the trial frames */
+    int32_t candidates;
+} hiprt_pt_tune_result;
+
+int hiprt_pt_autotune(hiprt_scene bvh2_scene, hiprt_scene bvh4_scene, const float *cam,
+                      const hiprt_pt_params *base, int32_t max_samples, int32_t trial_frames,
+                      hiprt_pt_tune_result *out);
 /* Live rays traced per level in the last frame (synchronizes the device). */
 int hiprt_pt_ray_counts(hiprt_pt pt, uint32_t *counts, int32_t n_levels);
 /* Live rays traced per level summed over every frame since creation

@@ -92,3 +92,26 @@ def test_path_tracer_matches_oracle(torch_cuda, hip_ctx, hall, name, opts, res, 
     assert differ.sum() <= MAX_DIFF_PATHS * n_paths
     assert sum_rel <= SUM_REL
     assert (level_diff <= COUNT_REL * want_levels + 1).all()
+
+
+def test_frame_tuner_choice_matches_oracle(torch_cuda, hip_ctx, hall):
+    """hiprt_pt_autotune picks a configuration by timing trial frames; the
+    image it renders meets the same bar against the oracle."""
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = 160, 90
+    ds4 = hiprt.DeviceScene(hip_ctx, hall["sc"], hall["bvh"], bvh_format=hiprt.BVH4)
+    print("case: frame tuner on a 160x90 hall frame, up to 4 samples in flight", flush=True)
+    kw, info = hiprt.pt_autotune(hall["ds"], ds4, cam, max_samples=4)
+    print(kw, info)
+    assert kw["samples"] in (1, 2, 4) and 1 <= kw["park_leaves"] <= 4 and info["candidates"] >= 12
+    ds = ds4 if info["layout"] == "bvh4" else hall["ds"]
+    pt = hiprt.PathTracer(ds, cam, seed=7, batched=True, **kw)
+    pt.render(0)
+    got, got_levels = pt.image(), pt.ray_totals()
+    pt.close()
+    ds4.close()
+    want, want_levels = _oracle(hall, cam, 7, kw["samples"])
+    n_paths = 160 * 90 * kw["samples"]
+    differ = (got.view(np.uint32) != want.view(np.uint32)).any(-1)
+    assert differ.sum() <= MAX_DIFF_PATHS * n_paths + 1
+    assert np.abs(got_levels.astype(np.int64) - want_levels.astype(np.int64)).max() <= COUNT_REL * n_paths + 1

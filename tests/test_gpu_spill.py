@@ -109,6 +109,50 @@ def test_spilled_traversal_matches_oracle(torch_cuda, deep, tracer, layout):
         assert (occ != want_any).mean() <= 1e-4, stack
 
 
+def _check_lost(torch, ds, rays, stack, tracer="persistent-while-while"):
+    """Capped and uncapped runs of the same rays: lost rays are marked, the
+    others equal (t, beta, gamma, id for the reference-order tracer, t for a
+    speculative one)."""
+    full = _closest(torch, ds, rays, tracer, stack_config=stack)
+    capped = _closest(torch, ds, rays, tracer, stack_config=stack, spill_limit=1)
+    lost = np.isnan(capped[0])
+    print({"lost": int(lost.sum()), "rays": rays.shape[1]})
+    assert lost.any() and (~lost).mean() > 0.3
+    assert np.all(capped[3][lost] == 0xFFFFFFFF) and not np.isnan(full[0]).any()
+    # the speculative tracers may return either triangle of an equal-t tie
+    exact = tracer == "persistent-while-while"
+    for a, b in zip(capped if exact else capped[:1], full if exact else full[:1]):
+        assert np.array_equal(a[~lost].view(np.uint32), b[~lost].view(np.uint32))
+    occ_full = _any(torch, ds, rays, tracer, stack_config=stack)
+    occ_cap = _any(torch, ds, rays, tracer, stack_config=stack, spill_limit=1)
+    lost_any = occ_cap == 0xFF
+    assert set(np.unique(occ_full).tolist()) <= {0, 1}
+    assert np.array_equal(occ_cap[~lost_any], occ_full[~lost_any])
+
+
+def test_capped_spill_column_marks_lost_rays_4wide(torch_cuda, deep):
+    """hiprt_tracer_opts.spill_limit = 1: a traversal that outgrows the stack
+    configuration's entries plus one spill entry drops entries (never writing
+    out of bounds); its result is marked lost -- t NaN, id 0xFFFFFFFF,
+    occlusion byte 0xFF -- and every other ray equals the uncapped run bit for
+    bit.  The nested scene's 4-wide tree with stack configuration 3, its
+    random rays plus rays on the plane x + y + z = 0 through the nesting's
+    corner (every box holds that corner, no triangle reaches the plane, so
+    these traversals hold the most entries)."""
+    torch = torch_cuda
+    pos, idx, sc, bvh, ds2, ds4 = deep
+    rng = np.random.default_rng(14)
+    graze = np.zeros((8, 2000), np.float32)
+    a = 2.0 ** -rng.uniform(5, 40, 2000)
+    graze[0], graze[1], graze[2] = -a, a, 0.0
+    d = np.array([1.0, -1.0, 0.0]) / np.sqrt(2.0)
+    graze[4], graze[5], graze[6] = d[0], d[1], d[2]
+    graze[7] = FLT_MAX
+    rays = np.ascontiguousarray(np.concatenate([nested_rays(20_000, 13), graze], 1))
+    print("case: nested scene, bvh4, persistent-while-while, stack 3, spill_limit 1", flush=True)
+    _check_lost(torch, ds4, rays, 3)
+
+
 def test_spilled_batch_and_work_chunks(torch_cuda, hip_ctx, deep):
     torch = torch_cuda
     pos, idx, sc, bvh, ds2, ds4 = deep
