@@ -20,8 +20,9 @@ by two rocprofv3 counter passes over a child process that renders the same
 configuration -- FETCH_SIZE, then WRITE_SIZE, each scaled by what the counter
 reads for a 1 GiB device copy in the same child -- and divided by the
 kernel's average launch time from HIP events on its stream.  A third pass
-reads the VALU counters (instructions, active lanes), the roof that binds.
-The counter CSVs stay under gpurun_out/bench_counters/.
+reads the VALU counters (instructions, active lanes), the roof that binds,
+and a kernel-trace pass times the same dispatches as rocprofv3 sees them.
+The CSVs stay under gpurun_out/bench_counters/.
 """
 import argparse
 import csv
@@ -118,16 +119,17 @@ def read_counters(outdir):
     return [rows[k] for k in sorted(rows, key=lambda k: k[1])]
 
 
-def counter_pass(counters, child_cfg, tag, timeout=300):
-    """Runs this script as a counter-collection child under rocprofv3 and
-    returns its per-dispatch counters (read_counters)."""
+def profiled_child(rp_args, child_cfg, tag, timeout=300):
+    """Runs this script as a counter child (counter_child) under
+    `rocprofv3 <rp_args>`, output CSVs in COUNTER_DIR/tag; returns that folder
+    and rocprofv3's exit code."""
     rp = rocprof_bin()
     if rp is None:
         raise RuntimeError("rocprofv3 not found")
     outdir = os.path.join(COUNTER_DIR, tag)
     shutil.rmtree(outdir, ignore_errors=True)
     os.makedirs(outdir, exist_ok=True)
-    cmd = [rp, "--pmc", *counters, "--output-format", "csv", "-d", outdir, "-o", "pass", "--",
+    cmd = [rp, *rp_args, "--output-format", "csv", "-d", outdir, "-o", "pass", "--",
            sys.executable, os.path.abspath(__file__), "--counter-child", json.dumps(child_cfg)]
     with open(os.path.join(outdir, "child.log"), "w") as logf:
         proc = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=logf,
@@ -137,11 +139,33 @@ def counter_pass(counters, child_cfg, tag, timeout=300):
         except subprocess.TimeoutExpired:
             os.killpg(proc.pid, signal.SIGKILL)
             proc.wait()
-            raise RuntimeError(f"counter pass {tag} timed out")
+            raise RuntimeError(f"rocprofv3 pass {tag} timed out")
+    return outdir, rc
+
+
+def counter_pass(counters, child_cfg, tag, timeout=300):
+    """Per-dispatch counters (read_counters) of a counter child run under
+    `rocprofv3 --pmc <counters>`."""
+    outdir, rc = profiled_child(["--pmc", *counters], child_cfg, tag, timeout)
     rows = read_counters(outdir)
     if not rows:
         raise RuntimeError(f"counter pass {tag}: no counter rows (rocprofv3 exit {rc})")
     return rows
+
+
+def trace_pass(child_cfg, kname, skip, tag="trace", timeout=300):
+    """Durations (ms) of the dispatches of kernel `kname` after the first
+    `skip`, from a counter child run under `rocprofv3 --kernel-trace --stats`."""
+    outdir, rc = profiled_child(["--kernel-trace", "--stats"], child_cfg, tag, timeout)
+    durs = []
+    for f in sorted(glob.glob(os.path.join(outdir, "**", "*kernel_trace.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            rows = sorted(csv.DictReader(fh), key=lambda r: int(r["Dispatch_Id"]))
+        durs += [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6 for r in rows
+                 if kname in r["Kernel_Name"]]
+    if len(durs) <= skip:
+        raise RuntimeError(f"kernel-trace pass: no dispatches of {kname} (rocprofv3 exit {rc})")
+    return durs[skip:]
 
 
 def counter_child(cfg):
@@ -187,7 +211,8 @@ def copy_factor(rows, counter):
 
 def measure_roofline(child_cfg, kname, levels, avg_launch_ms):
     """HBM bytes per launch of the frame kernel and its VALU figures, from three
-    counter passes; dict for the JSON line."""
+    counter passes, and its average launch time under the kernel trace (to
+    set beside the HIP-event time); dict for the JSON line."""
     out = {}
     fetch = counter_pass(["FETCH_SIZE"], child_cfg, "fetch")
     write = counter_pass(["WRITE_SIZE"], child_cfg, "write")
@@ -219,6 +244,10 @@ def measure_roofline(child_cfg, kname, levels, avg_launch_ms):
                        "active_lane_frac": round(sum(thr) / max(sum(act), 1.0) / 64.0, 4),
                        "basis": "SQ_INSTS_VALU x 4 cycles per wave64 instruction over 1024 SIMDs x launch time x "
                                 "2.4 GHz; active lanes = SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU"}
+    durs = trace_pass(child_cfg, kname, levels)
+    out["kernel_trace"] = {"avg_launch_ms": round(sum(durs) / len(durs), 4), "dispatches": len(durs),
+                           "basis": "rocprofv3 --kernel-trace over the counter child's frames, warm-up frame "
+                                    "excluded"}
     return out
 
 
@@ -291,6 +320,54 @@ def bench_adjoint(torch, hiprt, ds, d_diff, stream, reps=10):
     per_ray = 24 + 8 + 12 + 48 + 24 + 36 + 4 + 48 + 44
     return {"rays": n_adj, "ms": round(ms_adj, 4), "mrays": round(n_adj / ms_adj / 1e3, 1),
             "bytes_per_ray": per_ray, "achieved_GBs": round(n_adj * per_ray / (ms_adj * 1e-3) / 1e9, 1)}
+
+
+def bench_regen(torch, hiprt, ds, cam, kw, stream, total=64, frames=2):
+    """Path regeneration against samples in flight at equal memory: `total`
+    samples per pixel rendered by a pool of k path slots per pixel that
+    refill themselves (one render call), and by k samples in flight
+    (total / k render calls), for k = 1, 4, 16; frame configuration `kw`."""
+    base = {k: v for k, v in kw.items() if k != "samples"}
+    out = {"samples_per_pixel": total, "config": base}
+
+    def run(pt, calls):
+        f = 0
+        for _ in range(calls):   # warm-up
+            pt.render(f, stream.cuda_stream)
+            f += 1
+        torch.cuda.synchronize()
+        before = pt.ray_totals()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(frames * calls):
+            pt.render(f, stream.cuda_stream)
+            f += 1
+        e1.record(stream)
+        torch.cuda.synchronize()
+        rays = float((pt.ray_totals() - before).sum())
+        return e0.elapsed_time(e1) / frames, rays / frames
+
+    def one(extra, calls):
+        pt = hiprt.PathTracer(ds, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True,
+                              **base, **extra)
+        ms, rays = run(pt, calls)
+        mem = pt.memory()
+        pt.close()
+        return {"ms_per_frame": round(ms, 3), "mrays": round(rays / ms / 1e3, 1),
+                "device_mib": round(mem / 2 ** 20, 1)}, mem
+
+    for k in (1, 4, 16):
+        row = {}
+        row["in_flight"], m_if = one(dict(samples=k), total // k)
+        row["regen"], m_r = one(dict(samples=total, regen_pool=k), 1)
+        # a regeneration slot holds fewer bytes: the pool that fits the
+        # in-flight schedule's memory
+        k_eq = int(k * m_if / m_r)
+        if k_eq > k:
+            row[f"regen_pool_{k_eq}_equal_memory"], _ = one(dict(samples=total, regen_pool=k_eq), 1)
+        out[f"pool_{k}"] = row
+    return out
 
 
 def bench_device_build(torch, hiprt, ctx, stream, sc, d_diff, host_build_s):
@@ -446,6 +523,9 @@ def main():
     ap.add_argument("--stack-config", type=int, default=0, help="stack configuration for --kernel-only")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time the closest-hit adjoint on 1M of the frame's diffuse rays (for profiling)")
+    ap.add_argument("--regen-only", action="store_true",
+                    help="only compare path regeneration with samples in flight (stack configuration 4, "
+                         "dynamic-while-while, binary tree)")
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the rocprofv3 counter passes (roofline traffic and VALU figures)")
     ap.add_argument("--counter-child", help=argparse.SUPPRESS)
@@ -491,6 +571,13 @@ def main():
 
     if args.adjoint_only:
         print(json.dumps(bench_adjoint(torch, hiprt, ds, d_diff, stream, reps=20)), flush=True)
+        ds.close()
+        ctx.close()
+        return
+
+    if args.regen_only:
+        print(json.dumps(bench_regen(torch, hiprt, ds, cam, {"tracer": "dynamic-while-while", "stack_config": 4},
+                                     stream)), flush=True)
         ds.close()
         ctx.close()
         return
@@ -584,6 +671,9 @@ def main():
         ms, _ = time_trace(torch, hiprt, ds_tuned, rays, stream=stream, **pt_kw)
         kernels[f"{name}/autotuned"] = round(rays.shape[1] / ms / 1e3, 1)
     log(f"trace kernels alone (Mrays/s): {kernels}")
+
+    regen = bench_regen(torch, hiprt, ds_frame, cam, frame_kw, stream)
+    log(f"regeneration vs samples in flight: {regen}")
 
     adjoint = bench_adjoint(torch, hiprt, ds, d_diff, stream)
     log(f"adjoint: {adjoint}")
@@ -683,6 +773,7 @@ def main():
         "frame_tuner": frame_tune,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "regeneration": regen,
         "adjoint_closest": adjoint,
         "refit": refit,
     }

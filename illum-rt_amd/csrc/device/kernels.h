@@ -62,11 +62,13 @@ hipError_t launch_trace_plain(const scene_desc &sc, const ray_soa &rays, const h
 // Rays of a single-scene persistent launch read from records of `stride`
 // float4 (at least two) that begin (ox, oy, oz, tmin | dx, dy, dz, tmax)
 // instead of planes: ray idx is record perm[idx] (record idx when perm is
-// null); hits are still written at idx.
+// null); hits are still written at idx.  With fixed_t set every ray spans
+// [kRayEps, FLT_MAX] and the records' tmin/tmax words are not read.
 struct ray_gather {
     const float4 *aos = nullptr;
     const uint32_t *perm = nullptr;
     uint32_t stride = 2;
+    uint32_t fixed_t = 0;
 };
 
 // Tunables of the persistent kernel.

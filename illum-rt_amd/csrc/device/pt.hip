@@ -52,9 +52,11 @@ struct shade_out {
     vec3f org, dir;
 };
 
-__device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_params &pp, uint32_t path,
-                                               int level, const uint64_t *frame_seed, vec3f in_dir, float t, float beta,
-                                               float gamma, uint32_t tri, vec3f &T, float *rad) {
+// The path is pixel `pixel` of sample `sample`; its radiance accumulates in
+// rad[0..2].
+__device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_params &pp, uint32_t sample,
+                                               uint32_t pixel, int level, const uint64_t *frame_seed, vec3f in_dir,
+                                               float t, float beta, float gamma, uint32_t tri, vec3f &T, float *rad) {
     shade_out so{false, v3(0.0f), v3(0.0f)};
     if (!ps.valid(t, tri)) return so;
     const float4 *g = ps.tri_geo + 3 * (size_t)tri;
@@ -63,13 +65,12 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     uint32_t m = __float_as_uint(g0.w);
     float4 alb = ps.mats[2 * m], emi = ps.mats[2 * m + 1];
     if (emi.x != 0.0f || emi.y != 0.0f || emi.z != 0.0f) {
-        rad[3 * path + 0] += T.x * emi.x;
-        rad[3 * path + 1] += T.y * emi.y;
-        rad[3 * path + 2] += T.z * emi.z;
+        rad[0] += T.x * emi.x;
+        rad[1] += T.y * emi.y;
+        rad[2] += T.z * emi.z;
         if (!pp.emitters_reflect) return so;
     }
     if (level >= pp.max_depth) return so;
-    const uint32_t sample = path / pp.n_px, pixel = path - sample * pp.n_px;
     pcg32 rng(pcg_stream_id(frame_seed[sample], pixel, (uint64_t)level));
     float xi0 = rng.next_float(), xi1 = rng.next_float();
     float scale = 1.0f;
@@ -85,6 +86,14 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     // a path that can no longer carry light ends (gi/pt.cpp:110-111)
     so.live = T.x != 0.0f || T.y != 0.0f || T.z != 0.0f;
     return so;
+}
+
+// shade_one for path `path` = sample * n_px + pixel, radiance in rad[3 path ..].
+__device__ __forceinline__ shade_out shade_path(const pt_scene &ps, const pt_params &pp, uint32_t path, int level,
+                                                const uint64_t *frame_seed, vec3f in_dir, float t, float beta,
+                                                float gamma, uint32_t tri, vec3f &T, float *rad) {
+    const uint32_t sample = path / pp.n_px, pixel = path - sample * pp.n_px;
+    return shade_one(ps, pp, sample, pixel, level, frame_seed, in_dir, t, beta, gamma, tri, T, rad + 3ull * path);
 }
 
 __device__ __forceinline__ vec3f load3(const float *p, uint32_t i) { return vec3f{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
@@ -176,7 +185,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_plain(pt_scene ps, pt_par
     if (i < n && rays.tmax[i] >= 0.0f) {
         vec3f in_dir{rays.dx[i], rays.dy[i], rays.dz[i]};
         vec3f T = load3(thr, i);
-        shade_out so = shade_one(ps, pp, i, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i],
+        shade_out so = shade_path(ps, pp, i, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i],
                                  hits.id[i], T, rad);
         live = so.live;
         if (live) store3(thr, i, T);
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
         path = path_in[i];
         vec3f in_dir{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
         vec3f T = load3(thr, path);
-        so = shade_one(ps, pp, path, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
+        so = shade_path(ps, pp, path, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
                        rad);
         live = so.live;
         if (live) store3(thr, path, T);
@@ -295,7 +304,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
             in_dir = vec3f{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
             T = load3(thr, path);
         }
-        so = shade_one(ps, pp, path, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
+        so = shade_path(ps, pp, path, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
                        rad);
         live = so.live;
     }
@@ -306,6 +315,113 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
         out_rec[3ull * slot + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, FLT_MAX);
         out_rec[3ull * slot + 2] = make_float4(__uint_as_float(path), T.x, T.y, T.z);
         keys[slot] = ray_key(grid, so.org, so.dir);
+    }
+}
+
+// ---- path regeneration ----
+//
+// A fixed pool of lanes x n_px path slots.  Slot j renders pixel j % n_px,
+// samples j / n_px, j / n_px + lanes, ... of the render call, one after
+// another: when a path ends, the shade step that ended it appends the camera
+// ray of the slot's next sample in its place, so every iteration traces a
+// full pool until the slots run out of samples.  The queue holds records
+// (origin, - | direction, sample << 8 | level | slot, throughput), traced
+// with ray_gather::fixed_t (every ray of the path tracer spans
+// [kRayEps, FLT_MAX]).  With one lane per pixel a pixel's paths end in
+// sample order and add their radiance straight to the image, in the same
+// order as the in-flight schedule adds its samples; with more lanes each
+// slot accumulates its own samples in `acc` and the lanes are added at the
+// end of the render call.
+
+// Key of a camera ray: above every bounce ray's key (bit kRayKeyBits), so
+// that camera rays and bounce rays do not share wavefronts; then its
+// direction octant and the Morton code of its pixel's cell in a 64 x 64
+// grid over the image (camera rays share one origin, so the origin cell
+// would not tell them apart).
+__device__ __forceinline__ uint32_t camera_key(const cam_frame &f, int x, int y, vec3f d) {
+    uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
+    uint32_t cx = (uint32_t)((uint64_t)x * 64u / (uint32_t)f.w), cy = (uint32_t)((uint64_t)y * 64u / (uint32_t)f.h);
+    uint32_t m = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) m |= (((cx >> b) & 1u) << (2 * b + 1)) | (((cy >> b) & 1u) << (2 * b));
+    return (1u << kRayKeyBits) | (oct << 12) | m;
+}
+
+// Camera-ray record of sample `sample` of pixel `px` for slot `slot`, at
+// queue position `at`.  The ray is k_camera_rays' ray for that path.
+__device__ __forceinline__ void write_camera_record(const regen_params &rp, uint32_t slot, uint32_t px,
+                                                    uint32_t sample, float4 *out_rec, uint32_t *keys, uint32_t at) {
+    int x = (int)(px % (uint32_t)rp.f.w), y = (int)(px / (uint32_t)rp.f.w);
+    float ox = 0.0f, oy = 0.0f;
+    if (rp.jitter) pixel_jitter(rp.seeds[sample], px, ox, oy);
+    vec3f d = cam_dir(rp.f, x, y, ox, oy);
+    out_rec[3ull * at] = make_float4(rp.f.pos.x, rp.f.pos.y, rp.f.pos.z, 0.0f);
+    out_rec[3ull * at + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(sample << 8));
+    out_rec[3ull * at + 2] = make_float4(__uint_as_float(slot), 1.0f, 1.0f, 1.0f);
+    if (keys) keys[at] = camera_key(rp.f, x, y, d);
+}
+
+// The pool's first paths: slot j < lanes * n_px gets sample j / n_px of its
+// pixel at queue position j; clears the slot's accumulator.
+__global__ __launch_bounds__(256) void k_regen_init(regen_params rp, float4 *out_rec, uint32_t *keys) {
+    uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= rp.lanes * rp.n_px) return;
+    uint32_t lane = j / rp.n_px, px = j - lane * rp.n_px;
+    write_camera_record(rp, j, px, lane, out_rec, keys, j);
+    if (rp.acc) rp.acc[3ull * j] = rp.acc[3ull * j + 1] = rp.acc[3ull * j + 2] = 0.0f;
+}
+
+// One regeneration iteration's shade step: queue slot i < *count_in is the
+// ray of record perm[i] (i without a permutation), its hit is slot i.  A
+// path that goes on is appended with level + 1; one that ends appends its
+// slot's next sample's camera ray, if the slot has one left.  Rays shaded
+// per level are added to rp.level_counts.
+__global__ __launch_bounds__(kShadeBlock) void k_shade_regen(pt_scene ps, pt_params pp, regen_params rp,
+                                                            const float4 *in_rec, const uint32_t *perm, hit_soa hits,
+                                                            float4 *out_rec, uint32_t *keys, const uint32_t *count_in,
+                                                            uint32_t *count_out, sort_grid grid) {
+    __shared__ uint32_t sh[kShadeWaves + 1];
+    __shared__ uint32_t per_level[kMaxPathLevels];
+    const uint32_t i = blockIdx.x * (uint32_t)kShadeBlock + threadIdx.x;
+    const uint32_t n = *count_in;
+    if (blockIdx.x * (uint32_t)kShadeBlock >= n) return;   // whole block past the queue: uniform exit
+    const int levels = pp.max_depth + 1;
+    if ((int)threadIdx.x < levels) per_level[threadIdx.x] = 0u;
+    __syncthreads();
+    bool emit = false, live = false;
+    shade_out so{};
+    vec3f T{};
+    uint32_t slot = 0, px = 0, sample = 0, next_word = 0;
+    if (i < n) {
+        uint32_t src = perm ? perm[i] : i;
+        float4 d = in_rec[3ull * src + 1];
+        float4 pr = in_rec[3ull * src + 2];
+        uint32_t word = __float_as_uint(d.w);
+        sample = word >> 8;
+        int level = (int)(word & 255u);
+        slot = __float_as_uint(pr.x);
+        px = slot % rp.n_px;
+        T = vec3f{pr.y, pr.z, pr.w};
+        float *r3 = rp.acc ? rp.acc + 3ull * slot : rp.image + 3ull * px;
+        so = shade_one(ps, pp, sample, px, level, rp.seeds, vec3f{d.x, d.y, d.z}, hits.t[i], hits.u[i], hits.v[i],
+                       hits.id[i], T, r3);
+        atomicAdd(&per_level[level < levels ? level : levels - 1], 1u);
+        live = so.live;
+        next_word = (sample << 8) | (uint32_t)(level + 1);
+        emit = live || sample + rp.lanes < rp.samples;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < levels && per_level[threadIdx.x])
+        atomicAdd(rp.level_counts + threadIdx.x, per_level[threadIdx.x]);
+    uint32_t at = block_append(emit, count_out, sh);
+    if (!emit) return;
+    if (live) {
+        out_rec[3ull * at] = make_float4(so.org.x, so.org.y, so.org.z, 0.0f);
+        out_rec[3ull * at + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, __uint_as_float(next_word));
+        out_rec[3ull * at + 2] = make_float4(__uint_as_float(slot), T.x, T.y, T.z);
+        if (keys) keys[at] = ray_key(grid, so.org, so.dir);
+    } else {
+        write_camera_record(rp, slot, px, sample + rp.lanes, out_rec, keys, at);
     }
 }
 
@@ -396,6 +512,23 @@ hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level
     hipLaunchKernelGGL(k_shade_stage, dim3((n_max + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, ps, pp,
                        level, frame_seed, rays_in, path_in, thr, in_rec, perm, hits, out_rec, keys, rad, counts,
                        grid);
+    return hipGetLastError();
+}
+
+hipError_t launch_regen_init(const regen_params &rp, float4 *out_rec, uint32_t *keys, hipStream_t s) {
+    const uint32_t n = rp.lanes * rp.n_px;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_regen_init, dim3((n + 255) / 256), dim3(256), 0, s, rp, out_rec, keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade_regen(const pt_scene &ps, const pt_params &pp, const regen_params &rp,
+                              const float4 *in_rec, const uint32_t *perm, const hit_soa &hits, float4 *out_rec,
+                              uint32_t *keys, const uint32_t *count_in, uint32_t *count_out, const sort_grid &grid,
+                              uint32_t n_max, hipStream_t s) {
+    if (pp.max_depth + 1 > kMaxPathLevels) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_shade_regen, dim3((n_max + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, ps, pp,
+                       rp, in_rec, perm, hits, out_rec, keys, count_in, count_out, grid);
     return hipGetLastError();
 }
 

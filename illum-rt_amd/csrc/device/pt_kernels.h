@@ -96,6 +96,31 @@ hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level
                               const float4 *in_rec, const uint32_t *perm, const hit_soa &hits, float4 *out_rec,
                               uint32_t *keys, float *rad, uint32_t *counts, const sort_grid &grid, uint32_t n_max,
                               hipStream_t s);
+// Path regeneration (a fixed pool of lanes * n_px path slots, each slot
+// rendering its pixel's samples one after another; device/pt.hip).
+constexpr int kMaxPathLevels = 65;   // max_depth <= 64
+constexpr int kRegenKeyBits = kRayKeyBits + 1;   // bounce-ray keys, then camera-ray keys
+struct regen_params {
+    cam_frame f;
+    const uint64_t *seeds;     // frame seed of each sample of the render call
+    int jitter;                // camera rays through pixel_jitter offsets
+    uint32_t n_px;
+    uint32_t samples;          // samples per pixel in the render call (< 2^24)
+    uint32_t lanes;            // slots per pixel (<= samples)
+    float *acc;                // per-slot radiance when lanes > 1, else null
+    float *image;              // per-pixel radiance the paths add to when acc is null
+    uint32_t *level_counts;    // rays shaded per level, added to
+};
+// Writes the first sample of every slot to out_rec (and keys, when non-null)
+// in slot order; the queue length is lanes * n_px.
+hipError_t launch_regen_init(const regen_params &rp, float4 *out_rec, uint32_t *keys, hipStream_t s);
+// One iteration's shade step over the *count_in rays of in_rec (through
+// perm when non-null); appends continued paths and regenerated camera rays
+// to out_rec / keys and counts them in *count_out.
+hipError_t launch_shade_regen(const pt_scene &ps, const pt_params &pp, const regen_params &rp,
+                              const float4 *in_rec, const uint32_t *perm, const hit_soa &hits, float4 *out_rec,
+                              uint32_t *keys, const uint32_t *count_in, uint32_t *count_out, const sort_grid &grid,
+                              uint32_t n_max, hipStream_t s);
 hipError_t launch_set_u64(uint64_t *p, uint64_t v, hipStream_t s);
 // totals[i] += counts[i] for i < levels.
 hipError_t launch_add_counts(unsigned long long *totals, const uint32_t *counts, int levels, hipStream_t s);

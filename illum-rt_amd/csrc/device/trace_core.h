@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../core/layout.h"
+#include "../core/raygen.h"
 
 #include <cfloat>
 #include <cstdint>
@@ -42,18 +43,19 @@ __device__ __forceinline__ dray load_ray(const ray_soa &r, uint32_t i) {
 }
 
 // The same ray from a record of `stride` float4 that begins
-// (ox, oy, oz, tmin | dx, dy, dz, tmax).
-__device__ __forceinline__ dray load_ray_aos(const float4 *p, uint32_t i, uint32_t stride) {
+// (ox, oy, oz, tmin | dx, dy, dz, tmax); with `fixed_t` the record's .w words
+// carry other data and the ray spans [kRayEps, FLT_MAX].
+__device__ __forceinline__ dray load_ray_aos(const float4 *p, uint32_t i, uint32_t stride, bool fixed_t) {
     float4 a = p[(size_t)stride * i], b = p[(size_t)stride * i + 1];
     dray q;
     q.ox = a.x;
     q.oy = a.y;
     q.oz = a.z;
-    q.tmin = a.w;
+    q.tmin = fixed_t ? kRayEps : a.w;
     q.dx = b.x;
     q.dy = b.y;
     q.dz = b.z;
-    q.tmax = b.w;
+    q.tmax = fixed_t ? FLT_MAX : b.w;
     q.ix = 1.0f / q.dx;
     q.iy = 1.0f / q.dy;
     q.iz = 1.0f / q.dz;

@@ -276,7 +276,8 @@ void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occl
                         r = load_ray(bd.rays, ray_id);
                     } else {
                         ray_id = idx;
-                        if (gather.aos) r = load_ray_aos(gather.aos, gather.perm ? gather.perm[idx] : idx, gather.stride);
+                        if (gather.aos) r = load_ray_aos(gather.aos, gather.perm ? gather.perm[idx] : idx, gather.stride,
+                                                         gather.fixed_t != 0);
                         else r = load_ray(rays, idx);
                     }
                     cur = ls.root;

@@ -110,10 +110,97 @@ struct hiprt_pt_s {
     hipStream_t cap = nullptr;
     hipGraphExec_t exec = nullptr;
     bool clear_next = true;
+    // path regeneration: `lanes` slots per pixel (0: off); queue lengths of
+    // the two record buffers, per-slot radiance (lanes > 1), the graph of
+    // kRegenChunk iterations, and the host's view of the queue length
+    uint32_t lanes = 0;
+    dbuf qcount, acc;
+    hipGraphExec_t regen_exec = nullptr;
+    uint32_t *host_q = nullptr;
+    hipEvent_t regen_ev[2] = {nullptr, nullptr};
+    static constexpr int kRegenChunk = 8;   // even: every chunk starts on record buffer 0
 
     ~hiprt_pt_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
+        if (regen_exec) (void)hipGraphExecDestroy(regen_exec);
+        for (hipEvent_t e : regen_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (host_q) (void)hipHostFree(host_q);
         if (cap) (void)hipStreamDestroy(cap);
+    }
+
+    rt::dev::regen_params regen() const {
+        return rt::dev::regen_params{f,     frame_seed.as<const uint64_t>(), jitter ? 1 : 0, n_px, samples, lanes,
+                                     lanes > 1 ? acc.as<float>() : nullptr, rad.as<float>(), counts.as<uint32_t>()};
+    }
+
+    rt::dev::radix_sort_buffers sort_buffers() const {
+        return rt::dev::radix_sort_buffers{keys.as<uint32_t>(),        keys_tmp.as<uint32_t>(),
+                                           vals_tmp.as<uint32_t>(),    perm.as<uint32_t>(),
+                                           sort_counts.as<uint32_t>(), sort_totals.as<uint32_t>()};
+    }
+
+    // kRegenChunk regeneration iterations: trace the queue in record buffer
+    // a, shade it into buffer b (continued paths and regenerated camera
+    // rays), sort b's keys.
+    hipError_t record_regen_chunk(hipStream_t s) {
+        using namespace rt::dev;
+        uint32_t *q = qcount.as<uint32_t>();
+        const regen_params rp = regen();
+        for (int it = 0; it < kRegenChunk; ++it) {
+            const int a = it & 1, b = a ^ 1;
+            ray_gather g;
+            g.aos = rec[a].as<const float4>();
+            g.perm = sorted ? perm.as<const uint32_t>() : nullptr;
+            g.stride = kRayRecord;
+            g.fixed_t = 1;
+            hipError_t e = launch_trace_persistent(hiprt_detail::desc_for(sc, alpha), rt::ray_soa{}, h, nullptr, n,
+                                                   q + a, work.as<uint32_t>(), false, pcfg, s, g);
+            if (e == hipSuccess) e = launch_fill_u32(q + b, 0u, 1u, s);
+            if (e == hipSuccess)
+                e = launch_shade_regen(ps(), pp, rp, rec[a].as<const float4>(), g.perm, h, rec[b].as<float4>(),
+                                       sorted ? keys.as<uint32_t>() : nullptr, q + a, q + b, grid, n, s);
+            if (e == hipSuccess && sorted) e = launch_radix_sort(sort_buffers(), n, q + b, kRegenKeyBits, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+
+    // One render call with path regeneration (frame seeds already set).
+    int render_regen(hipStream_t s) {
+        using namespace rt::dev;
+        uint32_t *q = qcount.as<uint32_t>();
+        const int levels = pp.max_depth + 1;
+        if (clear_next) HIP_TRY(launch_fill_u32(rad.as<uint32_t>(), 0u, 3u * n_px, s));
+        HIP_TRY(launch_fill_u32(counts.as<uint32_t>(), 0u, (uint32_t)levels + 1, s));
+        HIP_TRY(launch_fill_u32(q, n, 1u, s));
+        HIP_TRY(launch_fill_u32(q + 1, 0u, 1u, s));
+        HIP_TRY(launch_regen_init(regen(), rec[0].as<float4>(), sorted ? keys.as<uint32_t>() : nullptr, s));
+        if (sorted) HIP_TRY(launch_radix_sort(sort_buffers(), n, q, kRegenKeyBits, s));
+        // every slot ends within ceil(samples / lanes) paths of at most
+        // max_depth + 1 iterations each
+        const uint64_t bound = (uint64_t)((samples + lanes - 1) / lanes) * (uint64_t)levels;
+        const uint64_t chunks = (bound + kRegenChunk - 1) / kRegenChunk;
+        bool drained = false;
+        uint64_t c = 0;
+        for (; c < chunks && !drained; ++c) {
+            HIP_TRY(hipGraphLaunch(regen_exec, s));
+            HIP_TRY(hipMemcpyAsync(host_q + (c & 1), q, 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipEventRecord(regen_ev[c & 1], s));
+            if (c >= 1) {   // chunk c is queued behind the one waited for
+                HIP_TRY(hipEventSynchronize(regen_ev[(c - 1) & 1]));
+                drained = host_q[(c - 1) & 1] == 0u;
+            }
+        }
+        if (!drained) {
+            HIP_TRY(hipEventSynchronize(regen_ev[(c - 1) & 1]));
+            if (host_q[(c - 1) & 1] != 0u)
+                return fail(HIPRT_ERR_HIP, "path regeneration did not drain its queue within the iteration bound");
+        }
+        if (lanes > 1) HIP_TRY(launch_add_samples(rad.as<float>(), acc.as<const float>(), n_px, lanes, s));
+        HIP_TRY(launch_add_counts(totals.as<unsigned long long>(), counts.as<const uint32_t>(), levels, s));
+        clear_next = false;
+        return ok();
     }
 
     rt::dev::pt_scene ps() const {
@@ -473,7 +560,14 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     if (npix * samples > 0x40000000) return fail(HIPRT_ERR_ARG, "image too large for that many samples in flight");
     p->n_px = (uint32_t)npix;
     p->samples = samples;
-    p->n = (uint32_t)npix * samples;
+    if (params->regen_pool < 0) return fail(HIPRT_ERR_ARG, "regen_pool must be >= 0");
+    if (params->regen_pool > 0) {
+        if (!params->batched) return fail(HIPRT_ERR_ARG, "path regeneration needs the batched schedule");
+        if (samples >= (1u << 24)) return fail(HIPRT_ERR_ARG, "path regeneration takes fewer than 2^24 samples");
+        p->lanes = std::min((uint32_t)params->regen_pool, samples);
+    }
+    // path slots: the regeneration pool, or every sample's paths at once
+    p->n = (uint32_t)npix * (p->lanes ? p->lanes : samples);
     p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive, (uint32_t)npix,
                                params->emitters_reflect ? 1 : 0};
     p->seed = params->seed;
@@ -494,22 +588,32 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     if (!p->batched && sc->stack_need > rt::dev::kPlainStackCap - 1)
         return fail(HIPRT_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     uint32_t n = p->n;
+    const bool regen = p->lanes > 0;
     p->sorted = p->batched && params->ray_order == 0 && p->pp.max_depth > 0;
-    HIP_TRY(p->rays[0].alloc(n));
-    if (p->batched) {
-        HIP_TRY(p->paths[0].alloc(4ull * n));
-        if (!p->sorted) {
-            HIP_TRY(p->rays[1].alloc(n));
-            HIP_TRY(p->paths[1].alloc(4ull * n));
+    if (!regen) {
+        HIP_TRY(p->rays[0].alloc(n));
+        if (p->batched) {
+            HIP_TRY(p->paths[0].alloc(4ull * n));
+            if (!p->sorted) {
+                HIP_TRY(p->rays[1].alloc(n));
+                HIP_TRY(p->paths[1].alloc(4ull * n));
+            }
         }
+        HIP_TRY(p->thr.alloc(12ull * n));
+        if (samples > 1) HIP_TRY(p->rad_paths.alloc(12ull * n));
+    } else {
+        // records in both parities, queue lengths, per-slot radiance
+        for (int k = 0; k < 2; ++k) HIP_TRY(p->rec[k].alloc(16ull * rt::dev::kRayRecord * n));
+        HIP_TRY(p->qcount.alloc(8));
+        if (p->lanes > 1) HIP_TRY(p->acc.alloc(12ull * n));
+        HIP_TRY(hipHostMalloc((void **)&p->host_q, 8));
+        for (hipEvent_t &e : p->regen_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     HIP_TRY(p->hits.alloc(16ull * n));
     float *hb = p->hits.as<float>();
     p->h = rt::hit_soa{hb, hb + n, hb + 2ull * n, reinterpret_cast<uint32_t *>(hb + 3ull * n)};
-    HIP_TRY(p->thr.alloc(12ull * n));
     HIP_TRY(p->rad.alloc(12ull * p->n_px));
     HIP_TRY(hipMemset(p->rad.p, 0, 12ull * p->n_px));
-    if (samples > 1) HIP_TRY(p->rad_paths.alloc(12ull * n));
     HIP_TRY(p->counts.alloc(4ull * (p->pp.max_depth + 2)));
     HIP_TRY(hipMemset(p->counts.p, 0, 4ull * (p->pp.max_depth + 2)));
     HIP_TRY(p->totals.alloc(8ull * (p->pp.max_depth + 1)));
@@ -525,9 +629,8 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pcfg.park = (params->park_leaves > 0 && params->park_leaves < rt::dev::kParkClosest) ? params->park_leaves
                                                                                           : rt::dev::kParkClosest;
     if (p->sorted) {
-        for (int k = 0; k < 2; ++k) {
-            HIP_TRY(p->rec[k].alloc(16ull * rt::dev::kRayRecord * n));
-        }
+        if (!regen)
+            for (int k = 0; k < 2; ++k) HIP_TRY(p->rec[k].alloc(16ull * rt::dev::kRayRecord * n));
         HIP_TRY(p->keys.alloc(4ull * n));
         HIP_TRY(p->keys_tmp.alloc(4ull * n));
         HIP_TRY(p->vals_tmp.alloc(4ull * n));
@@ -551,7 +654,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
         }
         HIP_TRY(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-        hipError_t rec = p->record(p->cap, false);
+        hipError_t rec = regen ? p->record_regen_chunk(p->cap) : p->record(p->cap, false);
         hipGraph_t g = nullptr;
         hipError_t end = hipStreamEndCapture(p->cap, &g);
         if (rec != hipSuccess) {
@@ -559,7 +662,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
             return fail(HIPRT_ERR_HIP, std::string("recording the frame graph: ") + hipGetErrorString(rec));
         }
         HIP_TRY(end);
-        hipError_t inst = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+        hipError_t inst = hipGraphInstantiate(regen ? &p->regen_exec : &p->exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HIP_TRY(inst);
     }
@@ -574,6 +677,7 @@ int hiprt_pt_render(hiprt_pt pt, uint32_t frame, void *stream) {
     // render call `frame` traces the samples of frames frame*samples .. +samples-1
     HIP_TRY(rt::dev::launch_set_seeds(pt->frame_seed.as<uint64_t>(),
                                       pt->seed * 0x100000001B3ULL + (uint64_t)frame * pt->samples, pt->samples, s));
+    if (pt->lanes) return pt->render_regen(s);
     if (pt->batched && !pt->clear_next) {
         HIP_TRY(hipGraphLaunch(pt->exec, s));
     } else {
@@ -610,6 +714,7 @@ int hiprt_pt_autotune(hiprt_scene bvh2_scene, hiprt_scene bvh4_scene, const floa
         p.stack_config = stack;
         p.samples = samples;
         p.park_leaves = park;
+        p.regen_pool = 0;
         hiprt_pt pt = nullptr;
         int st = hiprt_pt_create(layout == HIPRT_BVH4 ? bvh4_scene : bvh2_scene, cam, &p, &pt);
         if (st != HIPRT_OK) return st;
@@ -692,6 +797,7 @@ int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_le
     if (!pt) return fail(HIPRT_ERR_ARG, "null path tracer");
     int levels = pt->pp.max_depth + 1;
     if (n_levels < levels) return fail(HIPRT_ERR_ARG, "arrays shorter than max_depth + 1");
+    if (pt->lanes) return fail(HIPRT_ERR_UNSUPPORTED, "path regeneration mixes levels in every launch");
     hipStream_t s = static_cast<hipStream_t>(stream);
     HIP_TRY(hipSetDevice(pt->sc->ctx->device));
     std::vector<hipEvent_t> ev(2 * levels, nullptr);
@@ -720,6 +826,19 @@ int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_le
         HIP_TRY(hipStreamSynchronize(s));
         HIP_TRY(hipMemcpy(fetch_stats, st.p, 16ull * levels, hipMemcpyDeviceToHost));
     }
+    return ok();
+}
+
+int hiprt_pt_memory(hiprt_pt pt, uint64_t *bytes) {
+    if (!pt || !bytes) return fail(HIPRT_ERR_ARG, "null argument");
+    const dbuf *bufs[] = {&pt->rays[0].mem, &pt->rays[1].mem, &pt->hits,     &pt->paths[0],  &pt->paths[1],
+                          &pt->thr,         &pt->rad,         &pt->counts,   &pt->totals,    &pt->frame_seed,
+                          &pt->work,        &pt->spill,       &pt->rad_paths, &pt->rec[0],   &pt->rec[1],
+                          &pt->keys,        &pt->keys_tmp,    &pt->vals_tmp, &pt->perm,      &pt->sort_counts,
+                          &pt->sort_totals, &pt->qcount,      &pt->acc};
+    uint64_t sum = 0;
+    for (const dbuf *b : bufs) sum += b->bytes;
+    *bytes = sum;
     return ok();
 }
 

@@ -75,7 +75,8 @@ class _PtParams(ctypes.Structure):
     _fields_ = [("max_depth", c_int32), ("rr_start", c_int32), ("rr_survive", c_float), ("seed", c_uint64),
                 ("tracer", c_int32), ("batched", c_int32), ("stack_config", c_int32), ("work_chunks", c_int32),
                 ("refill_threshold", c_int32), ("blocks", c_int32), ("ray_order", c_int32), ("samples", c_int32),
-                ("emitters_reflect", c_int32), ("park_leaves", c_int32), ("pixel_centre", c_int32)]
+                ("emitters_reflect", c_int32), ("park_leaves", c_int32), ("pixel_centre", c_int32),
+                ("regen_pool", c_int32)]
 
 
 _lib = None
@@ -142,6 +143,7 @@ def lib():
                                           POINTER(_PtTuneResult)]),
             "hiprt_pt_ray_counts": (c_int, [c_void_p, POINTER(c_uint32), c_int32]),
             "hiprt_pt_image": (c_int, [c_void_p, c_void_p]),
+            "hiprt_pt_memory": (c_int, [c_void_p, POINTER(c_uint64)]),
             "hiprt_pt_ray_totals": (c_int, [c_void_p, POINTER(c_uint64), c_int32]),
             "hiprt_pt_profile": (c_int, [c_void_p, c_uint32, c_void_p, POINTER(c_float), POINTER(c_uint64),
                                          POINTER(c_uint32), c_int32]),
@@ -550,7 +552,9 @@ class PathTracer:
 
     def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None,
                  stack_config=0, work_chunks=0, refill_threshold=0, blocks=0, sort_rays=True, samples=1,
-                 emitters_reflect=False, jitter=True, park_leaves=0):
+                 emitters_reflect=False, jitter=True, park_leaves=0, regen_pool=0):
+        """regen_pool k > 0: path regeneration over k path slots per pixel
+        (hiprt_pt_params.regen_pool) instead of all `samples` in flight."""
         self.dscene = dscene
         c = np.ascontiguousarray(cam[:12], dtype=np.float32)
         self.w, self.h = int(c[10]), int(c[11])
@@ -558,7 +562,7 @@ class PathTracer:
         tid = -1 if tracer is None else tracer_id(tracer)
         p = _PtParams(max_depth, rr_start, rr_survive, seed, tid, 1 if batched else 0, stack_config, work_chunks,
                       refill_threshold, blocks, 0 if sort_rays else 1, samples, 1 if emitters_reflect else 0,
-                      park_leaves, 0 if jitter else 1)
+                      park_leaves, 0 if jitter else 1, regen_pool)
         self.h_ = c_void_p()
         check(lib().hiprt_pt_create(dscene.h, c.ctypes.data_as(POINTER(c_float)), ctypes.byref(p),
                                     ctypes.byref(self.h_)))
@@ -583,6 +587,12 @@ class PathTracer:
         rays = (c_uint32 * self.levels)()
         check(lib().hiprt_pt_profile(self.h_, frame, _stream(stream), ms, st, rays, self.levels))
         return (np.array(ms, np.float64), np.array(st, np.uint64).reshape(-1, 2), np.array(rays, np.uint64))
+
+    def memory(self):
+        """Device bytes the path tracer holds (hiprt_pt_memory)."""
+        b = c_uint64()
+        check(lib().hiprt_pt_memory(self.h_, ctypes.byref(b)))
+        return int(b.value)
 
     def image(self):
         img = np.empty((self.h, self.w, 3), np.float32)

@@ -450,6 +450,18 @@ typedef struct {
                                of its pixel -- offsets in [-0.5, 0.5)^2 from the path's own PCG32
                                stream, as simple_pt::sample_pixel jitters (gi/pt.cpp:36);
                                1: through the pixel centre */
+    int32_t regen_pool;     /* batched: 0 (default) traces the samples of a render call in flight
+                               together, level by level; k > 0 runs path regeneration over a
+                               fixed pool of k slots per pixel (at most `samples`): each slot
+                               renders its pixel's samples one after another, and a path that
+                               ends is replaced in the next iteration by the camera ray of its
+                               slot's next sample, so every trace launch holds a full pool.
+                               With k = 1 the image equals the in-flight schedule's bit for bit
+                               (a pixel's samples end, and add, in sample order); with k > 1
+                               each slot sums its own samples and the slots are added at the
+                               end, which reassociates the per-pixel sum.  render() then
+                               waits on the device: it reads the pool's queue length every
+                               few iterations to know when the frame is done. */
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;
@@ -490,13 +502,16 @@ int hiprt_pt_ray_counts(hiprt_pt pt, uint32_t *counts, int32_t n_levels);
 /* Live rays traced per level summed over every frame since creation
  * (synchronizes the device). */
 int hiprt_pt_ray_totals(hiprt_pt pt, uint64_t *totals, int32_t n_levels);
-/* Runs frame `frame` twice outside the graph: once timing each level's trace
+/* Runs frame `frame` twice outside the graph (not with regen_pool): once timing each level's trace
  * launch with HIP events on `stream` (ms_per_level), once with the counting
  * trace variant (fetch_stats: inner-node fetches and triangle tests, 2 per
  * level).  rays_per_level receives the live rays of each level.  Both frames
  * add to the radiance accumulator.  Synchronizes the device. */
 int hiprt_pt_profile(hiprt_pt pt, uint32_t frame, void *stream, float *ms_per_level, uint64_t *fetch_stats,
                      uint32_t *rays_per_level, int32_t n_levels);
+/* Device memory the path tracer holds (its path slots, queues, sort
+ * scratch, accumulators), in bytes; the scene is not counted. */
+int hiprt_pt_memory(hiprt_pt pt, uint64_t *bytes);
 /* Accumulated radiance, w*h*3 floats (synchronizes the device). */
 int hiprt_pt_image(hiprt_pt pt, float *rgb);
 int hiprt_pt_reset(hiprt_pt pt, void *stream);

@@ -115,3 +115,72 @@ def test_frame_tuner_choice_matches_oracle(torch_cuda, hip_ctx, hall):
     differ = (got.view(np.uint32) != want.view(np.uint32)).any(-1)
     assert differ.sum() <= MAX_DIFF_PATHS * n_paths + 1
     assert np.abs(got_levels.astype(np.int64) - want_levels.astype(np.int64)).max() <= COUNT_REL * n_paths + 1
+
+
+REGEN_CASES = [
+    # name, PathTracer options, frame size, samples per render call
+    ("one slot per pixel, sorted, persistent-while-while", dict(tracer="persistent-while-while", regen_pool=1),
+     (160, 90), 8),
+    ("three slots per pixel, sorted, dynamic-while-while, stack 4",
+     dict(stack_config=4, regen_pool=3), (160, 90), 8),
+    ("two slots per pixel, shading order, persistent-while-while",
+     dict(tracer="persistent-while-while", regen_pool=2, sort_rays=False), (97, 61), 6),
+    ("four slots per pixel, speculative, 4-wide tree", dict(tracer="persistent-speculative-while-while",
+                                                            regen_pool=4, bvh4=True), (128, 72), 4),
+]
+
+# With more than one slot per pixel each slot sums its own samples and the
+# slots are added at the end, so a pixel's sum is reassociated: per pixel the
+# image agrees with the oracle within PIXEL_REL (S float additions of
+# non-negative terms), except where a path took another way (MAX_DIFF_PATHS).
+PIXEL_REL = 1e-5
+
+
+@pytest.mark.parametrize("name,opts,res,samples", REGEN_CASES, ids=[c[0] for c in REGEN_CASES])
+def test_regeneration_matches_oracle(torch_cuda, hip_ctx, hall, name, opts, res, samples):
+    print(f"case: hall path regeneration vs oracle, {name}, {res[0]}x{res[1]}, {samples} samples", flush=True)
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = res
+    opts = dict(opts)
+    ds = hall["ds"]
+    own = None
+    if opts.pop("bvh4", False):
+        own = ds = hiprt.DeviceScene(hip_ctx, hall["sc"], hall["bvh"], bvh_format=hiprt.BVH4)
+    pt = hiprt.PathTracer(ds, cam, seed=7, batched=True, samples=samples, **opts)
+    pt.render(0)
+    got, got_levels = pt.image(), pt.ray_totals()
+    pt.close()
+    if own is not None:
+        own.close()
+    want, want_levels = _oracle(hall, cam, 7, samples)
+    n_paths = res[0] * res[1] * samples
+    scale = np.maximum(np.abs(want).max(-1), 1e-30)
+    far = np.abs(got - want).max(-1) > PIXEL_REL * scale
+    level_diff = np.abs(got_levels.astype(np.int64) - want_levels.astype(np.int64))
+    print({"pixels_beyond_rel": int(far.sum()), "paths": n_paths, "levels_gpu": got_levels.tolist(),
+           "levels_oracle": want_levels.tolist()})
+    assert np.isfinite(got).all() and want.sum() > 0
+    assert want_levels[0] == n_paths and got_levels[0] == n_paths
+    assert far.sum() <= MAX_DIFF_PATHS * n_paths
+    assert (level_diff <= COUNT_REL * want_levels + 1).all()
+
+
+def test_regeneration_with_one_slot_equals_samples_in_flight(torch_cuda, hip_ctx, hall):
+    """One slot per pixel adds a pixel's samples in sample order, as the
+    in-flight schedule does: with the deterministic tracer the two images
+    and their per-level ray counts are identical, over two render calls."""
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = 200, 120
+    print("case: regeneration (1 slot per pixel) vs 16 samples in flight, 200x120, 2 render calls", flush=True)
+    out = []
+    for kw in (dict(samples=16), dict(samples=16, regen_pool=1)):
+        pt = hiprt.PathTracer(hall["ds"], cam, seed=11, batched=True, tracer="persistent-while-while",
+                              stack_config=4, **kw)
+        pt.render(0)
+        pt.render(1)
+        out.append((pt.image(), pt.ray_totals(), pt.ray_counts()))
+        pt.close()
+    (a, ta, ca), (b, tb, cb) = out
+    assert a.sum() > 0
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(ta, tb) and np.array_equal(ca, cb)
