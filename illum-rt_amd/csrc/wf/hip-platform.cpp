@@ -456,8 +456,14 @@ bool platform::interprete(const std::string &command, std::istringstream &in) {
         } else if (name == "samples") {
             if (v < 1 || v > 64) error("Parameter error, samples must be 1..64");
             samples = v;
+        } else if (name == "park") {
+            if (v < 1 || v > 4) error("Parameter error, park must be 1..4");
+            opts.park_leaves = v;
+        } else if (name == "regen") {
+            if (v < 0 || v > 64) error("Parameter error, regen must be 0..64 (path slots per pixel, 0 = off)");
+            regen_pool = v;
         } else {
-            error("Unknown tracer option " << name << " (refill, blocks, stack, chunks, ray-order, samples)");
+            error("Unknown tracer option " << name << " (refill, blocks, stack, chunks, ray-order, samples, park, regen)");
         }
         return true;
     }
@@ -576,7 +582,8 @@ public:
 // sample, the accumulator averaged into the framebuffer at the end.  With
 // "tracer-option samples k" each render call traces k samples per pixel
 // together (k must divide sppx), so sppx / k calls render the same sppx
-// frames.
+// frames.  "tracer-option regen k" renders them with path regeneration over
+// k path slots per pixel instead of all k samples in flight.
 class pt_wf : public simple_algorithm {
     hiprt_pt pt = nullptr;
     int frames = 0;    // render calls so far
@@ -603,6 +610,8 @@ public:
                                      " is not a multiple of tracer-option samples " + std::to_string(samples));
         hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, o.stack_config, o.work_chunks, o.refill_threshold, o.blocks,
                           pf->ray_order, samples};
+        p.park_leaves = o.park_leaves;
+        p.regen_pool = pf->regen_pool;
         check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
         frames = 0;
     }

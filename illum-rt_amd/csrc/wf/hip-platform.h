@@ -96,6 +96,7 @@ public:
     hiprt_tracer_opts opts{40, 0, nullptr, 0, 8, 0};
     int ray_order = 0;   // path tracer bounce queues: 0 sorted, 1 shading order
     int samples = 1;     // path tracer: samples per pixel traced together (divides sppx)
+    int regen_pool = 0;  // path tracer: path slots per pixel of path regeneration (0: off)
     hiprt_bvh bvh = nullptr;
     hiprt_scene ds = nullptr;          // device scene of the last commit
     wf::scene *sd = nullptr;

@@ -131,7 +131,9 @@ def test_direct_wf_matches_oracle(tmp_path, torch_cuda, mode):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("options,samples", [("", 1), ("tracer-option ray-order 1\ntracer-option stack 3\n", 1),
-                                             ("tracer-option stack 4\ntracer-option samples 2\n", 2)])
+                                             ("tracer-option stack 4\ntracer-option samples 2\n", 2),
+                                             ("tracer-option samples 2\ntracer-option regen 1\ntracer-option park 3\n",
+                                              2)])
 def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options, samples):
     print(f"case: rtgi-hip pt-wf, hall 128x72, 2 spp vs hiprt.PathTracer, options {options!r}", flush=True)
     w, h = 128, 72
