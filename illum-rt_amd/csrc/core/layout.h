@@ -24,9 +24,10 @@
 //           -- holds exactly the differences the Shirley/Cramer test of
 //              libgi/intersect.h:19-70 computes, so the kernel reproduces the
 //              CPU test bit for bit.
-//   woop:   row0..row2 = the three rows of the inverse of [B-A, C-A, N] with
-//           w = -row.A, computed in FP64 and rounded; tri ids live in a
-//           separate per-slot array.
+//   woop:   row0..row2 = the three rows of the inverse of [B-A, C-A, N],
+//           computed in FP64 and rounded, with w = A.x, A.y, A.z (the
+//           triangle's local origin); tri ids live in a separate per-slot
+//           array.
 #pragma once
 
 #include <cstdint>

@@ -42,30 +42,30 @@ RT_HD tri_rec make_exact_rec(vec3f A, vec3f B, vec3f C, uint32_t tri_id) {
     return r;
 }
 
-// Woop record: rows of inverse([B-A, C-A, N]) with N = (B-A)x(C-A), in FP64.
-// For a point p, (row0.(p-A), row1.(p-A), row2.(p-A)) = (beta, gamma, s):
-// beta and gamma are the barycentric weights of B and C (the reference's
-// beta/gamma) and s the signed offset along N.  Stored as (row.xyz, -row.A)
-// and rounded to fp32.  Degenerate triangles get row2 = (0,0,0,1), which
-// makes every t = -inf, i.e. no hit.
+// Woop record: rows of inverse([B-A, C-A, N]) with N = (B-A)x(C-A), in FP64,
+// rounded to fp32, with the triangle's corner A as the local origin: row k
+// is (q_k.xyz, A[k]).  For a point p, (q0.(p-A), q1.(p-A), q2.(p-A)) =
+// (beta, gamma, s): beta and gamma are the barycentric weights of B and C
+// (the reference's beta/gamma) and s the signed offset along N.  The test
+// forms p - A from the ray origin first, as the Shirley test forms A - o, so
+// no row carries an offset term q.A whose rounding error grows with |A|
+// relative to the triangle.  Degenerate triangles get q = 0: every t is then
+// -0/0 = NaN, i.e. no hit.
 RT_HD tri_rec make_woop_rec(vec3f A, vec3f B, vec3f C) {
     dvec a = dv(A), e1 = dsub(dv(B), a), e2 = dsub(dv(C), a);
     dvec n = dcross(e1, e2);
     double det = ddot(n, n);   // det([e1 e2 n]) = |e1 x e2|^2
     tri_rec r{};
-    if (!(det > 0.0 && det < HUGE_VAL)) {
-        r.r2[3] = 1.0f;
-        return r;
-    }
+    float *out[3] = {r.r0, r.r1, r.r2};
+    const float corner[3] = {A.x, A.y, A.z};
+    for (int k = 0; k < 3; ++k) out[k][3] = corner[k];
+    if (!(det > 0.0 && det < HUGE_VAL)) return r;
     // inverse of a matrix with columns c0,c1,c2: rows (c1xc2, c2xc0, c0xc1)/det
     dvec rows[3] = {dcross(e2, n), dcross(n, e1), dcross(e1, e2)};
-    float *out[3] = {r.r0, r.r1, r.r2};
     for (int k = 0; k < 3; ++k) {
-        dvec q{rows[k].x / det, rows[k].y / det, rows[k].z / det};
-        out[k][0] = (float)q.x;
-        out[k][1] = (float)q.y;
-        out[k][2] = (float)q.z;
-        out[k][3] = (float)(-ddot(q, a));
+        out[k][0] = (float)(rows[k].x / det);
+        out[k][1] = (float)(rows[k].y / det);
+        out[k][2] = (float)(rows[k].z / det);
     }
     return r;
 }

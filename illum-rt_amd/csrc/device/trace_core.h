@@ -161,20 +161,23 @@ __device__ __forceinline__ bool tri_exact(const float4 &r0, const float4 &r1, co
     return true;
 }
 
-// Woop test: t from the plane row, then the two barycentric rows; the same
-// acceptance rule as the exact test.  FMAs are explicit: this format is not
+// Woop test on a local-origin record (core/records.h): the ray origin
+// relative to the corner A (rows' .w), t from the plane row, then the two
+// barycentric rows; the same acceptance rule as the exact test.  FMAs and
+// the hardware reciprocal (1 ulp) are explicit: this format is not
 // bit-compatible with the CPU test in any case.
 __device__ __forceinline__ bool tri_woop(const float4 &m0, const float4 &m1, const float4 &m2, const dray &r,
                                          float best, float &t_out, float &u_out, float &v_out) {
-    float oz = fmaf(m2.x, r.ox, fmaf(m2.y, r.oy, fmaf(m2.z, r.oz, m2.w)));
+    const float px = r.ox - m0.w, py = r.oy - m1.w, pz = r.oz - m2.w;
+    float oz = fmaf(m2.x, px, fmaf(m2.y, py, m2.z * pz));
     float dz = fmaf(m2.x, r.dx, fmaf(m2.y, r.dy, m2.z * r.dz));
-    float tt = -oz / dz;
+    float tt = -oz * __builtin_amdgcn_rcpf(dz);
     if (!(tt > r.tmin && tt < r.tmax && tt < best)) return false;
-    float ou = fmaf(m0.x, r.ox, fmaf(m0.y, r.oy, fmaf(m0.z, r.oz, m0.w)));
+    float ou = fmaf(m0.x, px, fmaf(m0.y, py, m0.z * pz));
     float du = fmaf(m0.x, r.dx, fmaf(m0.y, r.dy, m0.z * r.dz));
     float u = fmaf(tt, du, ou);
     if (!(u > 0)) return false;
-    float ov = fmaf(m1.x, r.ox, fmaf(m1.y, r.oy, fmaf(m1.z, r.oz, m1.w)));
+    float ov = fmaf(m1.x, px, fmaf(m1.y, py, m1.z * pz));
     float dv = fmaf(m1.x, r.dx, fmaf(m1.y, r.dy, m1.z * r.dz));
     float v = fmaf(tt, dv, ov);
     if (!(v > 0 && u + v <= 1)) return false;

@@ -129,7 +129,8 @@ int hiprt_destroy(hiprt_context ctx);
 /* ------------------------------------------------------------ device scenes
  * Layouts: bvh_format 0 = binary (64-B Aila-Laine nodes), 1 = collapsed 4-wide
  * (128-B nodes); tri_format 0 = exact (A, A-B, A-C records: bit-identical to
- * the CPU Shirley test), 1 = Woop (affine rows).  48-B triangle records. */
+ * the CPU Shirley test), 1 = Woop (rows of the inverse edge matrix about the
+ * corner A as a local origin).  48-B triangle records. */
 enum { HIPRT_BVH2 = 0, HIPRT_BVH4 = 1 };
 enum { HIPRT_TRI_EXACT = 0, HIPRT_TRI_WOOP = 1 };
 

@@ -88,7 +88,8 @@ def test_tracer_options_keep_the_image(tmp_path, torch_cuda):
     assert r0.returncode == 0 and not r0.stderr.strip(), r0.stderr
     err = r1.stderr.splitlines()
     assert err == ["command (tracer-option): Parameter error, stack configuration must be 0..4",
-                   "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks, ray-order, samples)"], err
+                   "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks, ray-order, samples, "
+                   "park, regen)"], err
     # primary-wf shades hits with the albedo, so the images agree unless two hits tie in t
     a, b = read_pfm(tmp_path / "a.pfm"), read_pfm(tmp_path / "b.pfm")
     assert np.array_equal(a, b)

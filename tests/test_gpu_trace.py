@@ -4,7 +4,7 @@ Every case traces the same rays through the same BVH on the GPU (through the
 C ABI) and in the oracle.  Exact triangle records with the plain kernel must
 be bit-identical to the oracle; the persistent kernels may differ only where
 two triangles have equal t; Woop records are held to the tie/grazing bar of
-tests/parity.py.
+tests/parity.py and to the CPU tracer's own accuracy against FP64.
 """
 import numpy as np
 import pytest
@@ -103,24 +103,37 @@ def test_work_chunks_match_up_to_ties(torch_cuda, c1, tracer, chunks, blocks):
 @pytest.mark.parametrize("tracer", ["while-while", "dynamic-while-while"])
 def test_woop_within_tie_and_grazing_bar(torch_cuda, c1, tracer):
     """Woop records: hit agreement >= 99.999% and every mismatch a tie or a
-    grazing hit.  Their t, beta, gamma come from affine rows whose offset term
-    carries an absolute error of about eps * |A| / triangle size, so they are
-    less accurate than the reference's fp32 Shirley test (which the exact
-    records reproduce bit for bit): on this workload ~95% of t within 1e-6
-    relative of FP64 against ~99.5% for the CPU tracer.  The bounds below
-    guard that measured accuracy from regressing."""
+    grazing hit, and their accuracy against FP64 no worse than the reference's
+    own fp32 Shirley test on the same hits (the oracle's hits, which the exact
+    records reproduce bit for bit): at least the CPU tracer's fraction of t
+    within 1e-6 relative, t's largest error within twice the CPU's, and beta's
+    and gamma's 99th-percentile errors within twice the CPU's.  The records
+    hold the triangle's corner as a local origin (core/records.h), so no row
+    carries an offset term whose rounding grows with |A|; per ray the two
+    methods round differently, so the comparison is over the distribution."""
     print(f"case: C1 camera+random rays, {tracer}, woop records", flush=True)
     got = _run_closest(torch_cuda, c1["woop"], c1["rays"], tracer)
     rep = classify(orc, c1["pos"], c1["idx"], c1["rays"], got, c1["ref"])
     print(rep)
     assert rep["unexplained"] == 0, rep
     assert rep["agree_frac"] >= 0.99999, rep
-    _, et, eu, ev = fp64_error(orc, c1["pos"], c1["idx"], c1["rays"], got, sample=50000)
-    print({"t_frac_1e-6": float((et <= 1e-6).mean()), "t_max": float(et.max()), "u_max": float(eu.max()),
-           "v_max": float(ev.max()), "u_p99": float(np.quantile(eu, 0.99))})
-    assert (et <= 1e-6).mean() >= 0.90
-    assert et.max() <= 1e-2
-    assert eu.max() <= 1e-3 and ev.max() <= 1e-3
+    # the same rays for both: those where the two hit the same triangle
+    tw, uw, vw, iw = got
+    tr, ur, vr, ir = c1["ref"]
+    both = np.nonzero((tw != FLT_MAX) & (tr != FLT_MAX) & (iw == ir))[0]
+    both = np.random.default_rng(3).choice(both, min(50000, len(both)), replace=False)
+    sub = lambda h: tuple(x[both] for x in h)
+    _, et, eu, ev = fp64_error(orc, c1["pos"], c1["idx"], c1["rays"][:, both], sub(got))
+    _, ct, cu, cv = fp64_error(orc, c1["pos"], c1["idx"], c1["rays"][:, both], sub(c1["ref"]))
+    q = lambda x: float(np.quantile(x, 0.99))
+    stats = {"t_frac_1e-6": float((et <= 1e-6).mean()), "cpu_t_frac_1e-6": float((ct <= 1e-6).mean()),
+             "t_max": float(et.max()), "cpu_t_max": float(ct.max()), "u_p99": q(eu), "cpu_u_p99": q(cu),
+             "v_p99": q(ev), "cpu_v_p99": q(cv), "u_max": float(eu.max()), "v_max": float(ev.max())}
+    print(stats)
+    assert stats["t_frac_1e-6"] >= stats["cpu_t_frac_1e-6"]
+    assert stats["t_max"] <= 2 * stats["cpu_t_max"]
+    assert stats["u_p99"] <= 2 * stats["cpu_u_p99"] and stats["v_p99"] <= 2 * stats["cpu_v_p99"]
+    assert stats["u_max"] <= 1e-3 and stats["v_max"] <= 1e-3
 
 
 @pytest.mark.parametrize("tracer", TRACERS)
