@@ -229,6 +229,21 @@ int hiprt_scene_get_texcoords(hiprt_host_scene s, float *tc, uint32_t *n_vert) {
     return ok();
 }
 
+int hiprt_scene_set_normals(hiprt_host_scene s, const float *normals, uint32_t n_vert) {
+    if (!s || (n_vert && !normals)) return fail(HIPRT_ERR_ARG, "null argument");
+    if (n_vert && n_vert != s->sc.pos.size()) return fail(HIPRT_ERR_ARG, "normal count differs from the vertex count");
+    s->sc.norm.assign(normals, normals + 3ull * n_vert);
+    return ok();
+}
+
+int hiprt_scene_get_normals(hiprt_host_scene s, float *normals, uint32_t *n_vert) {
+    if (!s) return fail(HIPRT_ERR_ARG, "null scene");
+    const bool has = !s->sc.norm.empty() && s->sc.norm.size() == 3 * s->sc.pos.size();
+    if (n_vert) *n_vert = has ? (uint32_t)s->sc.pos.size() : 0u;
+    if (normals && has) std::memcpy(normals, s->sc.norm.data(), s->sc.norm.size() * 4);
+    return ok();
+}
+
 int hiprt_scene_set_texture(hiprt_host_scene s, uint32_t material, const float *rgba, uint32_t w, uint32_t h) {
     if (!s) return fail(HIPRT_ERR_ARG, "null scene");
     if (material >= s->sc.mats.size()) return fail(HIPRT_ERR_ARG, "material id out of range");
@@ -506,6 +521,15 @@ int hiprt_scene_commit(hiprt_context ctx, hiprt_host_scene s, hiprt_bvh b, int32
     HIP_TRY(ds->tri_geo.upload(geo.data(), geo.size() * 4));
     HIP_TRY(ds->mats.upload(mats.data(), mats.size() * 4));
     HIP_TRY(ds->tri_vidx.upload(sc.idx.data(), sc.idx.size() * 4));
+    // Shading normals, when the scene has them: the three corners' normals
+    // per triangle (3 float4, w unused), so a hit reads one 48-byte row.
+    if (!sc.norm.empty() && sc.norm.size() == 3 * sc.pos.size() && nt) {
+        std::vector<float> tn(12ull * nt, 0.0f);
+        for (uint32_t t = 0; t < nt; ++t)
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) tn[12ull * t + 4 * k + c] = sc.norm[3ull * sc.idx[3ull * t + k] + c];
+        HIP_TRY(ds->tri_ns.upload(tn.data(), tn.size() * 4));
+    }
     // Opacity data for the alpha-aware tracers, when a material has a texture.
     bool textured = false;
     for (const rt::material &m : sc.mats) textured = textured || (m.tex >= 0 && (size_t)m.tex < sc.texs.size());

@@ -93,6 +93,7 @@ struct hiprt_scene_s {
     hiprt_detail::dbuf nodes, tris, slot_ids;    // traversal data
     hiprt_detail::dbuf tri_geo, mats;            // shading data (scenes committed from a host scene)
     hiprt_detail::dbuf tri_vidx;                 // vertex indices per triangle (adjoint scatter)
+    hiprt_detail::dbuf tri_ns;                   // corner shading normals per triangle (scenes with normals)
     hiprt_detail::dbuf alpha_tri, alpha_tex, alpha_mask;   // opacity data (scenes with textures)
     uint32_t n_tris = 0, n_vert = 0;
     int stack_need = 0;                          // most stack entries any traversal can hold

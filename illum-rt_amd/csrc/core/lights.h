@@ -11,7 +11,8 @@
 //                   divided by their total; sample_index = lower_bound - 1
 //                   (libgi/discrete_distributions.cpp:25-52)
 //   sample_Li       triangle: uniform point (uniform_sample_triangle,
-//                   libgi/sampling.h:43-46), pdf = dist^2 / (cos_light area),
+//                   libgi/sampling.h:43-46), pdf = dist^2 / (cos_light area)
+//                   with the light's normal interpolated at that point,
 //                   nothing behind the light (libgi/scene.cpp:534-559);
 //                   point: col / dist^2, pdf 1 (libgi/scene.cpp:500-508)
 //   hemispheres     uniform_sample_hemisphere, cosine_sample_hemisphere
@@ -32,8 +33,8 @@ constexpr float kPiF = 3.14159265358979323846f;                              // 
 enum : uint32_t { kLightTriangle = 0, kLightPoint = 1 };
 
 // One light, as 4 float4 on the device: (P0, type bits), (P1, area),
-// (P2, 0), (colour, 0).  Triangle: P0..P2 = A, B, C and colour = emission;
-// point: P0 = position and colour = col.
+// (P2, triangle id bits), (colour, 0).  Triangle: P0..P2 = A, B, C and
+// colour = emission; point: P0 = position and colour = col.
 struct light_rec {
     float r[16];
 };

@@ -75,11 +75,13 @@ RT_HD vec3f hit_point(vec3f A, vec3f B, vec3f C, float beta, float gamma) {
 }
 RT_HD vec3f geom_normal(vec3f A, vec3f B, vec3f C) { return normalize(cross(B - A, C - A)); }
 
-// Cosine-weighted bounce direction about the normal flipped to face away from
-// the incoming direction (flip_normals_to_ray: flip when dot(d, n) > 0).
-RT_HD vec3f cosine_bounce(vec3f n, vec3f in_dir, float xi0, float xi1) {
-    if (dot(in_dir, n) > 0) n = -n;
-    return align(cosine_hemisphere(xi0, xi1), n);
+// Cosine-weighted bounce direction about the geometric normal ng flipped to
+// face away from the incoming direction; flip_normals_to_ray decides the flip
+// by the shading normal ns (flip when dot(d, ns) > 0), and
+// sample_cos_weighted_dir aligns the sample to ng (rt/cpu/bounce.cpp:41-72).
+RT_HD vec3f cosine_bounce(vec3f ng, vec3f ns, vec3f in_dir, float xi0, float xi1) {
+    if (dot(in_dir, ns) > 0) ng = -ng;
+    return align(cosine_hemisphere(xi0, xi1), ng);
 }
 
 // Uniform direction on the sphere: z = 1 - 2 xi0, phi = 2 pi xi1.

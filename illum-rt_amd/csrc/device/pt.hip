@@ -5,8 +5,9 @@
 //
 // Semantics follow the reference's pieces: camera rays of cam_ray
 // (libgi/camera.cpp:44-61); hit point and geometric normal of diff_geom
-// (libgi/rt.cpp:7-15); cosine-weighted bounce about the normal flipped to the
-// ray side (rt/cpu/bounce.cpp:41-72); a disabled ray is o = 0, d = (1,1,1),
+// (libgi/rt.cpp:7-15); cosine-weighted bounce about the geometric normal
+// flipped to the ray side by the shading normal (rt/cpu/bounce.cpp:41-72,
+// libgi/util.h:226-232); a disabled ray is o = 0, d = (1,1,1),
 // t_max = -FLT_MAX (bounce.cpp:50-51).  The path loop is simple_pt's
 // (gi/pt.cpp:72-115): a miss ends the path (no sky light), an emitter adds
 // throughput x emission and ends it (unless emitters_reflect is set), a
@@ -80,7 +81,7 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
         scale = 1.0f / pp.rr_survive;
     }
     vec3f n = geom_normal(A, B, C);
-    so.dir = cosine_bounce(n, in_dir, xi0, xi1);
+    so.dir = cosine_bounce(n, ps.shading_normal(tri, beta, gamma, n), in_dir, xi0, xi1);
     so.org = hit_point(A, B, C, beta, gamma);
     T = vec3f{T.x * alb.x * scale, T.y * alb.y * scale, T.z * alb.z * scale};
     // a path that can no longer carry light ends (gi/pt.cpp:110-111)
@@ -442,7 +443,9 @@ __global__ __launch_bounds__(256) void k_cosine_bounce(pt_scene ps, ray_soa in, 
     vec3f A = f3(g0), B = f3(g1), C = f3(g2);
     pcg32 rng(pcg_stream_id(seed, i, (uint64_t)level));
     float xi0 = rng.next_float(), xi1 = rng.next_float();
-    vec3f dir = cosine_bounce(geom_normal(A, B, C), vec3f{in.dx[i], in.dy[i], in.dz[i]}, xi0, xi1);
+    vec3f ng = geom_normal(A, B, C);
+    vec3f dir = cosine_bounce(ng, ps.shading_normal(hits.id[i], hits.u[i], hits.v[i], ng),
+                              vec3f{in.dx[i], in.dy[i], in.dz[i]}, xi0, xi1);
     write_ray(out, i, hit_point(A, B, C, hits.u[i], hits.v[i]), kRayEps, dir, FLT_MAX);
 }
 

@@ -24,10 +24,22 @@ struct pt_scene {
     const float4 *tri_geo;
     const float4 *mats;
     uint32_t n_tris;
+    // corner shading normals, 3 float4 per triangle; null when the scene has
+    // none and the geometric normal shades
+    const float4 *tri_ns = nullptr;
     // A hit the shading steps may use: finite t below FLT_MAX (misses are
     // FLT_MAX, a traversal that lost stack entries is NaN) on a triangle of
     // this scene.
     __device__ __forceinline__ bool valid(float t, uint32_t id) const { return t < FLT_MAX && id < n_tris; }
+    // diff_geom::ns (libgi/rt.cpp:7-15): the corner normals interpolated at
+    // (beta, gamma) and normalized; ng for a scene without normals.
+    __device__ __forceinline__ vec3f shading_normal(uint32_t tri, float beta, float gamma, vec3f ng) const {
+        if (!tri_ns) return ng;
+        const float4 *q = tri_ns + 3 * (size_t)tri;
+        float4 a = q[0], b = q[1], c = q[2];
+        float w = 1.0f - beta - gamma;
+        return normalize(w * vec3f{a.x, a.y, a.z} + beta * vec3f{b.x, b.y, b.z} + gamma * vec3f{c.x, c.y, c.z});
+    }
 };
 
 struct pt_params {

@@ -12,11 +12,12 @@
 // (rgb, weight), as the reference's.  Materials are Lambertian
 // (lambertian_reflection::f, libgi/material.cpp:55-65: albedo / pi where the
 // incoming direction lies on the side of the shading normal).  The shading
-// normal of these scenes is the triangle's geometric normal (scenes come
-// without vertex normals, which the reference's loader fills with face
-// normals); the sample steps flip it to the camera ray's side
-// (flip_normals_to_ray), the integration steps use it unflipped, as the
-// reference does.
+// normal is the corners' normals interpolated at the hit (pt_scene::
+// shading_normal), or the geometric normal in a scene without normals (the
+// face normals the reference's loader generates then); the sample steps flip
+// the normals to the camera ray's side by it (flip_normals_to_ray) and align
+// their samples to the geometric normal, the integration steps weigh by it
+// unflipped, as the reference does.
 #include "pt_kernels.h"
 #include "steps_kernels.h"
 
@@ -105,7 +106,8 @@ __global__ __launch_bounds__(256) void k_integrate_light(pt_scene ps, ray_soa ca
         vec3f c = col / (dist * dist);
         vec3f wi{shadow.dx[i], shadow.dy[i], shadow.dz[i]};
         float cosv;
-        vec3f f = lambert_cos(f3(ps.mats[2 * __float_as_uint(g0.w)]), geom_normal(A, B, C), wi, cosv);
+        vec3f ns = ps.shading_normal(hits.id[i], hits.u[i], hits.v[i], geom_normal(A, B, C));
+        vec3f f = lambert_cos(f3(ps.mats[2 * __float_as_uint(g0.w)]), ns, wi, cosv);
         rad = c * f * cosv / 1.0f;
     }
     add_fb(fb, i, rad, 1.0f);
@@ -126,8 +128,9 @@ __global__ __launch_bounds__(256) void k_sample_direct(pt_scene ps, light_table 
         uint32_t m = __float_as_uint(g0.w);
         vec3f emi = f3(ps.mats[2 * m + 1]);
         vec3f ng = geom_normal(A, B, C);
+        vec3f ns = ps.shading_normal(hits.id[i], hits.u[i], hits.v[i], ng);
         vec3f in_dir{cam.dx[i], cam.dy[i], cam.dz[i]};
-        if (dot(in_dir, ng) > 0) ng = -ng;   // flip_normals_to_ray
+        if (dot(in_dir, ns) > 0) ng = -ng;   // flip_normals_to_ray (ns too, unused here)
         vec3f x = hit_point(A, B, C, hits.u[i], hits.v[i]);
         if (emi != v3(0.0f)) {
             add_fb(fb, i, emi, mode == kDirectLight ? 0.0f : 1.0f);
@@ -162,7 +165,8 @@ __global__ __launch_bounds__(256) void k_sample_direct(pt_scene ps, light_table 
                     float su0 = sqrtf(xi0);
                     float bx = 1.0f - su0, by = xi1 * su0;
                     vec3f target = (1.0f - bx - by) * la + bx * lb + by * lcn;
-                    vec3f ln = geom_normal(la, lb, lcn);
+                    // the light's normal at the sampled point (its triangle id in r2.w)
+                    vec3f ln = ps.shading_normal(__float_as_uint(r2.w), bx, by, geom_normal(la, lb, lcn));
                     w_i = target - x;
                     dist = sqrtf(dot(w_i, w_i));
                     w_i = w_i / dist;
@@ -211,7 +215,7 @@ __global__ __launch_bounds__(256) void k_integrate_dir(pt_scene ps, ray_soa cam,
         if (emi != v3(0.0f)) {
             const float4 *g = ps.tri_geo + 3 * (size_t)hits.id[i];
             float4 g0 = g[0];
-            vec3f ns = geom_normal(f3(g0), f3(g[1]), f3(g[2]));
+            vec3f ns = ps.shading_normal(hits.id[i], hits.u[i], hits.v[i], geom_normal(f3(g0), f3(g[1]), f3(g[2])));
             vec3f w_in{shadow.dx[i], shadow.dy[i], shadow.dz[i]};
             float cosv;
             vec3f f = lambert_cos(f3(ps.mats[2 * __float_as_uint(g0.w)]), ns, w_in, cosv);
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(256) void k_integrate_light_sample(pt_scene ps, ray
     if (ps.valid(hits.t[i], hits.id[i]) && shadow.tmax[i] > 0.0f && !occluded[i]) {
         const float4 *g = ps.tri_geo + 3 * (size_t)hits.id[i];
         float4 g0 = g[0];
-        vec3f ns = geom_normal(f3(g0), f3(g[1]), f3(g[2]));
+        vec3f ns = ps.shading_normal(hits.id[i], hits.u[i], hits.v[i], geom_normal(f3(g0), f3(g[1]), f3(g[2])));
         vec3f w_in{shadow.dx[i], shadow.dy[i], shadow.dz[i]};
         vec3f lc{light_col[3 * (size_t)i], light_col[3 * (size_t)i + 1], light_col[3 * (size_t)i + 2]};
         float cosv;

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <fstream>
 #include <map>
+#include <tuple>
 #include <sstream>
 
 namespace rt {
@@ -302,6 +303,7 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
     std::vector<vec3f> normals;
     std::vector<float> uvs;                 // "vt" entries, (u, v) pairs
     std::vector<long> corner_vt;            // per triangle corner: vt index, -1 none
+    std::vector<long> corner_vn;            // per triangle corner: vn index, -1 none
     std::vector<std::string> mtl_names;
     std::vector<mtl_entry> mtl;
     bool legacy = false;        // "usemtl m<id>" without a material library (write_obj's files)
@@ -371,13 +373,15 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
             for (size_t k = 2; k < ids.size(); ++k) {
                 uint32_t a = (uint32_t)ids[0], b = (uint32_t)ids[k - 1], c = (uint32_t)ids[k];
                 long ta = tids[0], tb = tids[k - 1], tcn = tids[k];
-                if (nids[0] >= 0 && nids[k - 1] >= 0 && nids[k] >= 0) {
+                long na = nids[0], nb = nids[k - 1], nc = nids[k];
+                if (na >= 0 && nb >= 0 && nc >= 0) {
                     vec3f pa = out.pos[a], pb = out.pos[b], pc = out.pos[c];
                     vec3f g = cross(pb - pa, pc - pa);
-                    vec3f m = (normals[nids[0]] + normals[nids[k - 1]] + normals[nids[k]]) * 0.333f;
+                    vec3f m = (normals[na] + normals[nb] + normals[nc]) * 0.333f;
                     if (!(dot(g, m) > 0)) {
                         std::swap(b, c);
                         std::swap(tb, tcn);
+                        std::swap(nb, nc);
                     }
                 }
                 out.idx.push_back(a);
@@ -386,6 +390,9 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
                 corner_vt.push_back(ta);
                 corner_vt.push_back(tb);
                 corner_vt.push_back(tcn);
+                corner_vn.push_back(na);
+                corner_vn.push_back(nb);
+                corner_vn.push_back(nc);
                 out.mat.push_back(cur_mat);
                 tri_named.push_back(cur_named);
             }
@@ -408,24 +415,46 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
             }
         }
     }
-    if (!uvs.empty()) {
-        // Texture coordinates belong to face corners (assimp gives every
-        // corner its own vertex, libgi/scene.cpp:251-262): each distinct
-        // (position, vt) pair becomes a vertex of its own after the "v"
-        // vertices, which keep (0, 0) for corners without a coordinate.
+    bool with_vn = false;
+    for (long n : corner_vn) with_vn = with_vn || n >= 0;
+    if (!uvs.empty() || with_vn) {
+        // Texture coordinates and normals belong to face corners (assimp
+        // gives every corner its own vertex, libgi/scene.cpp:251-262): each
+        // distinct (position, vt, vn) corner becomes a vertex of its own
+        // after the "v" vertices, which keep (0, 0) and no normal.  With
+        // normals in the file, a corner without one gets its face's normal,
+        // on a vertex of its own.
         const size_t nv = out.pos.size();
-        out.tc.assign(2 * nv, 0.0f);
-        std::map<std::pair<uint32_t, long>, uint32_t> split;
+        if (!uvs.empty()) out.tc.assign(2 * nv, 0.0f);
+        if (with_vn) out.norm.assign(3 * nv, 0.0f);
+        std::map<std::tuple<uint32_t, long, long>, uint32_t> split;
         for (size_t k = 0; k < out.idx.size(); ++k) {
-            long t = corner_vt[k];
-            if (t < 0) continue;
-            auto key = std::make_pair(out.idx[k], t);
+            long t = uvs.empty() ? -1 : corner_vt[k];
+            long n = with_vn ? corner_vn[k] : -1;
+            if (t < 0 && !with_vn) continue;
+            if (with_vn && n < 0) n = -2 - (long)(k / 3);   // this face's own normal
+            auto key = std::make_tuple(out.idx[k], t, n);
             auto it = split.find(key);
             if (it == split.end()) {
                 uint32_t id = (uint32_t)out.pos.size();
                 out.pos.push_back(out.pos[out.idx[k]]);
-                out.tc.push_back(uvs[2 * t]);
-                out.tc.push_back(uvs[2 * t + 1]);
+                if (!uvs.empty()) {
+                    out.tc.push_back(t >= 0 ? uvs[2 * t] : 0.0f);
+                    out.tc.push_back(t >= 0 ? uvs[2 * t + 1] : 0.0f);
+                }
+                if (with_vn) {
+                    vec3f nn;
+                    if (n >= 0) {
+                        nn = normals[n];
+                    } else {
+                        const size_t f = 3 * (k / 3);
+                        vec3f pa = out.pos[out.idx[f]], pb = out.pos[out.idx[f + 1]], pc = out.pos[out.idx[f + 2]];
+                        nn = normalize(cross(pb - pa, pc - pa));
+                    }
+                    out.norm.push_back(nn.x);
+                    out.norm.push_back(nn.y);
+                    out.norm.push_back(nn.z);
+                }
                 it = split.emplace(key, id).first;
             }
             out.idx[k] = it->second;

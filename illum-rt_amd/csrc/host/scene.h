@@ -3,8 +3,9 @@
 // minimal OBJ reader/writer.
 //
 // The reference keeps vertices as {pos, norm, tc} and triangles as
-// {a, b, c, material_id} (libgi/rt.h:35-47); only positions and indices take
-// part in intersection, so that is all the tracer stores.
+// {a, b, c, material_id} (libgi/rt.h:35-47); positions and indices take part
+// in intersection, shading normals and texture coordinates in shading and
+// the opacity test.
 #pragma once
 
 #include "../core/vec.h"
@@ -35,6 +36,10 @@ struct scene {
     std::vector<uint32_t> mat;     // material id per triangle
     std::vector<material> mats;
     std::vector<float> tc;         // per vertex (u, v), empty when the scene has no texture coordinates
+    // per vertex shading normal (x, y, z), vertex::norm; empty when the scene
+    // has none, and then the geometric normal shades (the face normals the
+    // reference's loader generates for meshes without normals)
+    std::vector<float> norm;
     std::vector<texture> texs;
 
     uint32_t n_tris() const { return (uint32_t)(idx.size() / 3); }
@@ -85,6 +90,11 @@ uint32_t hall_debris_for_config3();
 // and takes materials from "mtllib" libraries (newmtl/Kd/Ks/Ke, resolved as
 // libgi/scene.cpp:317-332 does: material 0 is the default, 1 + k the k-th
 // library material) or, without a library, from "usemtl m<id>" names.
+// Texture coordinates and shading normals belong to face corners, as assimp
+// gives every corner its own vertex: each distinct (v, vt, vn) corner becomes
+// a vertex of its own after the file's "v" vertices.  In a file with normals,
+// a corner without one gets its face's normal (the flat normal assimp
+// generates for meshes without normals).
 bool write_obj(const scene &s, const std::string &path);
 bool read_obj(const std::string &path, scene &out, std::string &err);
 

@@ -62,6 +62,12 @@ rt::dev::sort_grid grid_of(const rt::flat_scene &h) {
     return g;
 }
 
+// Shading data of a committed scene for the device steps.
+rt::dev::pt_scene pt_scene_of(const hiprt_scene_s *sc) {
+    return rt::dev::pt_scene{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris,
+                             sc->tri_ns.as<const float4>()};
+}
+
 rt::cam_frame frame_from(const float *cam) {
     return rt::make_cam_frame(rt::vec3f{cam[0], cam[1], cam[2]}, rt::vec3f{cam[3], cam[4], cam[5]},
                               rt::vec3f{cam[6], cam[7], cam[8]}, cam[9], (int)cam[10], (int)cam[11]);
@@ -203,9 +209,7 @@ struct hiprt_pt_s {
         return ok();
     }
 
-    rt::dev::pt_scene ps() const {
-        return rt::dev::pt_scene{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
-    }
+    rt::dev::pt_scene ps() const { return pt_scene_of(sc); }
 
     // The frame's launches on stream s (frame seed already set).  When
     // `ev` is non-null, events ev[2L], ev[2L+1] bracket level L's trace;
@@ -326,7 +330,7 @@ int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits
     rt::ray_soa in{rays->d_ox, rays->d_oy, rays->d_oz, rays->d_tmin, rays->d_dx, rays->d_dy, rays->d_dz, rays->d_tmax};
     rt::hit_soa h{hits->d_t, hits->d_u, hits->d_v, hits->d_id};
     rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
+    rt::dev::pt_scene ps = pt_scene_of(sc);
     HIP_TRY(rt::dev::launch_cosine_bounce(ps, in, h, o, seed, level, n, static_cast<hipStream_t>(stream)));
     return ok();
 }
@@ -346,7 +350,7 @@ int hiprt_add_hitpoint_albedo(hiprt_scene sc, const hiprt_hits *hits, uint32_t n
     if (!sc || !full_hits(hits) || (n && !d_framebuffer)) return fail(HIPRT_ERR_ARG, "null argument");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
+    rt::dev::pt_scene ps = pt_scene_of(sc);
     HIP_TRY(rt::dev::launch_add_albedo(ps, hsoa(hits), reinterpret_cast<float4 *>(d_framebuffer), n,
                                        static_cast<hipStream_t>(stream)));
     return ok();
@@ -360,7 +364,7 @@ int hiprt_shadow_rays(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *
         return fail(HIPRT_ERR_ARG, "null output plane");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
+    rt::dev::pt_scene ps = pt_scene_of(sc);
     rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
     HIP_TRY(rt::dev::launch_shadow_rays(ps, soa(rays), hsoa(hits), rt::vec3f{light[0], light[1], light[2]}, o, n,
                                         static_cast<hipStream_t>(stream)));
@@ -375,7 +379,7 @@ int hiprt_integrate_light(hiprt_scene sc, const hiprt_rays *cam_rays, const hipr
         return fail(HIPRT_ERR_ARG, "null argument");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
+    rt::dev::pt_scene ps = pt_scene_of(sc);
     HIP_TRY(rt::dev::launch_integrate_light(ps, soa(cam_rays), hsoa(hits), soa(shadow_rays), d_occluded,
                                             rt::vec3f{light[0], light[1], light[2]},
                                             rt::vec3f{col[0], col[1], col[2]},
@@ -408,7 +412,7 @@ int hiprt_scene_set_lights(hiprt_scene sc, const float *point_lights, uint32_t n
         if (e == rt::v3(0.0f)) continue;
         rt::vec3f A{g[0], g[1], g[2]}, B{g[4], g[5], g[6]}, C{g[8], g[9], g[10]};
         recs.push_back(rt::light_rec{{A.x, A.y, A.z, bits(rt::kLightTriangle), B.x, B.y, B.z, rt::tri_area(A, B, C),
-                                      C.x, C.y, C.z, 0.0f, e.x, e.y, e.z, 0.0f}});
+                                      C.x, C.y, C.z, bits(t), e.x, e.y, e.z, 0.0f}});
         f.push_back(rt::triangle_light_power(A, B, C, e));
     }
     for (uint32_t k = 0; k < n_points; ++k) {
@@ -453,7 +457,7 @@ int hiprt_sample_direct(hiprt_scene sc, int32_t mode, const hiprt_rays *cam_rays
         return fail(HIPRT_ERR_ARG, "light sampling needs lights (hiprt_scene_set_lights found none)");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
+    rt::dev::pt_scene ps = pt_scene_of(sc);
     rt::dev::light_table lt{sc->light_rec.as<const float4>(), sc->light_f.as<const float>(),
                             sc->light_cdf.as<const float>(), sc->n_lights, sc->light_integral};
     rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
@@ -471,7 +475,7 @@ int hiprt_integrate_dir_sample(hiprt_scene sc, const hiprt_rays *cam_rays, const
     if (n && (!d_pdf || !d_framebuffer)) return fail(HIPRT_ERR_ARG, "null pdf or framebuffer");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
+    rt::dev::pt_scene ps = pt_scene_of(sc);
     HIP_TRY(rt::dev::launch_integrate_dir(ps, soa(cam_rays), hsoa(cam_hits), soa(shadow_rays), hsoa(shadow_hits),
                                           d_pdf, reinterpret_cast<float4 *>(d_framebuffer), n,
                                           static_cast<hipStream_t>(stream)));
@@ -487,7 +491,7 @@ int hiprt_integrate_light_sample(hiprt_scene sc, const hiprt_rays *cam_rays, con
         return fail(HIPRT_ERR_ARG, "null occlusion, pdf, light colour or framebuffer");
     if (!sc->tri_geo.p) return fail(HIPRT_ERR_UNSUPPORTED, "scene has no shading data (commit it from a host scene)");
     HIP_TRY(hipSetDevice(sc->ctx->device));
-    rt::dev::pt_scene ps{sc->tri_geo.as<const float4>(), sc->mats.as<const float4>(), sc->n_tris};
+    rt::dev::pt_scene ps = pt_scene_of(sc);
     HIP_TRY(rt::dev::launch_integrate_light_sample(ps, soa(cam_rays), hsoa(cam_hits), soa(shadow_rays), d_occluded,
                                                    d_pdf, d_light_col, reinterpret_cast<float4 *>(d_framebuffer), n,
                                                    static_cast<hipStream_t>(stream)));

@@ -104,6 +104,8 @@ def lib():
             "hiprt_scene_materials": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_set_texcoords": (c_int, [c_void_p, c_void_p, c_uint32]),
             "hiprt_scene_get_texcoords": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
+            "hiprt_scene_set_normals": (c_int, [c_void_p, c_void_p, c_uint32]),
+            "hiprt_scene_get_normals": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_set_texture": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_uint32]),
             "hiprt_scene_free": (None, [c_void_p]),
             "hiprt_bvh_build": (c_int, [c_void_p, POINTER(_BvhParams), POINTER(c_void_p)]),
@@ -253,6 +255,25 @@ class HostScene:
         tc = np.zeros((nv.value, 2), np.float32)
         check(lib().hiprt_scene_get_texcoords(self.h, _ptr(tc), ctypes.byref(nv)))
         return tc
+
+    def set_normals(self, normals):
+        """Per-vertex shading normals (n_vert, 3) float32; None removes them."""
+        if normals is None:
+            check(lib().hiprt_scene_set_normals(self.h, None, 0))
+            return
+        n = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        check(lib().hiprt_scene_set_normals(self.h, _ptr(n), len(n)))
+
+    def normals(self):
+        """Per-vertex shading normals (n_vert, 3) float32, or None when the
+        scene has none."""
+        nv = c_uint32()
+        check(lib().hiprt_scene_get_normals(self.h, None, ctypes.byref(nv)))
+        if nv.value == 0:
+            return None
+        n = np.zeros((nv.value, 3), np.float32)
+        check(lib().hiprt_scene_get_normals(self.h, _ptr(n), ctypes.byref(nv)))
+        return n
 
     def set_texture(self, material, rgba):
         """Attach an RGBA texture (h, w, 4) float32 as the albedo texture of a

@@ -83,6 +83,20 @@ int hiprt_scene_set_texcoords(hiprt_host_scene s, const float *tc, uint32_t n_ve
  * "vt" coordinates arrive here: every distinct (position, vt) corner pair is
  * a vertex of its own, after the file's "v" vertices. */
 int hiprt_scene_get_texcoords(hiprt_host_scene s, float *tc, uint32_t *n_vert);
+/* Per-vertex shading normals, n_vert (x, y, z) triples: vertex::norm of
+ * libgi/rt.h, interpolated at a hit into diff_geom::ns (libgi/rt.cpp:7-15).
+ * The wavefront steps flip normals to the ray's side by it
+ * (flip_normals_to_ray, libgi/util.h:226-232), the Lambertian integration
+ * weighs by it, and a triangle light's sample point takes its interpolated
+ * normal (trianglelight::sample_Li, libgi/scene.cpp:534-559).  n_vert must be
+ * the scene's vertex count, or 0 to remove them (the geometric normal then
+ * shades).  Read at commit; a refit keeps the committed normals. */
+int hiprt_scene_set_normals(hiprt_host_scene s, const float *normals, uint32_t n_vert);
+/* The shading normals: *n_vert = the vertex count, or 0 for a scene without
+ * them; normals (3 floats per vertex) may be NULL.  An OBJ file's "vn"
+ * normals arrive here: every distinct (v, vt, vn) corner is a vertex of its
+ * own, and a corner without "vn" in a file with normals gets its face's. */
+int hiprt_scene_get_normals(hiprt_host_scene s, float *normals, uint32_t *n_vert);
 /* Attaches an RGBA texture (w*h texels of 4 floats, texel (x, y) at
  * 4*(y*w + x)) to material `material` as its albedo texture, the
  * material::albedo_tex that libgi/scene.cpp:340-349 loads; rgba NULL detaches

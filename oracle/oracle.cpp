@@ -51,6 +51,14 @@ inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline v3 cross(v3 a, v3 b) { return v3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
 inline v3 normalize(v3 v) { return mul(v, 1.0f / sqrtf(dot(v, v))); }
 
+// diff_geom::ns (libgi/rt.cpp:7-15): the corners' normals (vn, 3 floats per
+// vertex) interpolated at (beta, gamma) and normalized.
+inline v3 shading_normal(const float *vn, const uint32_t *idx, uint32_t tri, float beta, float gamma) {
+    auto nv = [&](uint32_t v) { return v3{vn[3 * v], vn[3 * v + 1], vn[3 * v + 2]}; };
+    return normalize(add(add(smul(1.0f - beta - gamma, nv(idx[3 * tri])), smul(beta, nv(idx[3 * tri + 1]))),
+                         smul(gamma, nv(idx[3 * tri + 2]))));
+}
+
 struct ray {
     v3 o, d, id;
     float t_min, t_max;
@@ -627,12 +635,15 @@ void orc_camera_rays(const float *pos, const float *dir, const float *up, float 
 }
 
 // Cosine-weighted bounce ray from hit (t, beta, gamma, id) of incoming ray i,
-// driven by the given uniform numbers xi0/xi1 (rt/cpu/bounce.cpp:41-72).
-// Rays without a hit get the reference's disabled ray: o = 0, d = (1,1,1),
-// t_max = -FLT_MAX.
-void orc_cosine_rays(const float *pos, const uint32_t *idx, const float *const *in_planes, const float *hit_t,
-                     const float *hit_u, const float *hit_v, const uint32_t *hit_id, const float *xi0,
-                     const float *xi1, uint32_t n, float *const *out) {
+// driven by the given uniform numbers xi0/xi1 (rt/cpu/bounce.cpp:41-72): the
+// normals flipped to the ray's side by the shading normal
+// (flip_normals_to_ray, libgi/util.h:226-232; vn: 3 floats per vertex
+// interpolated as diff_geom::ns, null: the geometric normal), the sample
+// aligned to the geometric normal.  Rays without a hit get the reference's
+// disabled ray: o = 0, d = (1,1,1), t_max = -FLT_MAX.
+void orc_cosine_rays(const float *pos, const uint32_t *idx, const float *vn, const float *const *in_planes,
+                     const float *hit_t, const float *hit_u, const float *hit_v, const uint32_t *hit_id,
+                     const float *xi0, const float *xi1, uint32_t n, float *const *out) {
     const float pi = float(M_PI);
     for (uint32_t i = 0; i < n; ++i) {
         float o[3] = {0, 0, 0}, d[3] = {1, 1, 1}, tmax = -FLT_MAX;
@@ -644,8 +655,9 @@ void orc_cosine_rays(const float *pos, const uint32_t *idx, const float *const *
             float beta = hit_u[i], gamma = hit_v[i];
             v3 x = add(add(smul(1.0f - beta - gamma, A), smul(beta, B)), smul(gamma, C));
             v3 ng = normalize(cross(sub(B, A), sub(C, A)));
+            v3 ns = vn ? shading_normal(vn, idx, tr, beta, gamma) : ng;
             v3 rd{in_planes[4][i], in_planes[5][i], in_planes[6][i]};
-            if (dot(rd, ng) > 0) ng = v3{-ng.x, -ng.y, -ng.z};
+            if (dot(rd, ns) > 0) ng = v3{-ng.x, -ng.y, -ng.z};
             // uniform_sample_disk + cosine_sample_hemisphere
             float r = sqrtf(xi0[i]);
             float theta = 2 * pi * xi1[i];
@@ -901,8 +913,9 @@ namespace {
 // command appends them (driver/interaction.cpp:723-753).
 struct olight {
     bool point;
-    v3 a, b, c;   // triangle corners, or a = position
-    v3 col;       // emission or point colour
+    v3 a, b, c;      // triangle corners, or a = position
+    v3 col;          // emission or point colour
+    uint32_t tri;    // the emitting triangle
 };
 
 inline float luma3(v3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }   // libgi/color.h
@@ -967,16 +980,20 @@ extern "C" {
 // for camera hits on emitters, then the integration step's (radiance, 1).
 // Random numbers per pixel i and seed s: PCG32 stream (s, i, 0); light mode
 // draws the light index, then the two for the point on the light.
-// Lambertian BRDF (lambertian_reflection::f, libgi/material.cpp:55-65); the
-// shading normal is the triangle's geometric normal (scenes without vertex
-// normals), flipped to the ray's side in the sample steps only (the
-// integration steps use diff_geom's normal unflipped, as the reference does);
-// a triangle light's normal is its geometric normal.  points: n_points x 6
+// Lambertian BRDF (lambertian_reflection::f, libgi/material.cpp:55-65).  The
+// shading normal ns is the corners' normals vn interpolated at the hit
+// (diff_geom::ns), or, with vn null, the geometric normal (the face normals
+// the reference's loader generates for meshes without normals).  The sample
+// steps flip the normals to the camera ray's side by ns (flip_normals_to_ray)
+// and align their samples to the flipped geometric normal; the integration
+// steps weigh by ns unflipped, as the reference does; a triangle light's
+// normal at its sampled point is its corners' normals interpolated there
+// (trianglelight::sample_Li), or its geometric normal.  points: n_points x 6
 // floats (pos, col); mats: 8 floats per material; mat per triangle.
 void orc_direct(const float *pos, const uint32_t *idx, uint32_t n_tris, const uint32_t *mat, const float *mats,
-                const void *nodes, uint32_t n_nodes, uint32_t root, const uint32_t *index, const float *cam, int mode,
-                const float *points, uint32_t n_points, const uint64_t *seeds, uint32_t n_seeds, float *fb,
-                int threads) {
+                const float *vn, const void *nodes, uint32_t n_nodes, uint32_t root, const uint32_t *index,
+                const float *cam, int mode, const float *points, uint32_t n_points, const uint64_t *seeds,
+                uint32_t n_seeds, float *fb, int threads) {
     scene_view s{pos, idx, n_tris};
     tree tr{static_cast<const node *>(nodes), n_nodes, root, index};
     pinhole ph(cam);
@@ -988,11 +1005,11 @@ void orc_direct(const float *pos, const uint32_t *idx, uint32_t n_tris, const ui
     for (uint32_t t = 0; t < n_tris; ++t) {
         const float *m = material(t);
         v3 e{m[4], m[5], m[6]};
-        if (e.x != 0 || e.y != 0 || e.z != 0) lights.L.push_back({false, corner(t, 0), corner(t, 1), corner(t, 2), e});
+        if (e.x != 0 || e.y != 0 || e.z != 0) lights.L.push_back({false, corner(t, 0), corner(t, 1), corner(t, 2), e, t});
     }
     for (uint32_t k = 0; k < n_points; ++k) {
         const float *p = points + 6 * k;
-        lights.L.push_back({true, v3{p[0], p[1], p[2]}, v3{0, 0, 0}, v3{0, 0, 0}, v3{p[3], p[4], p[5]}});
+        lights.L.push_back({true, v3{p[0], p[1], p[2]}, v3{0, 0, 0}, v3{0, 0, 0}, v3{p[3], p[4], p[5]}, 0});
     }
     lights.build();
     const int64_t n_px = (int64_t)ph.w * ph.h;
@@ -1012,9 +1029,9 @@ void orc_direct(const float *pos, const uint32_t *idx, uint32_t n_tris, const ui
                 albedo = v3{m[0], m[1], m[2]};
                 v3 e{m[4], m[5], m[6]};
                 xh = add(add(smul(1.0f - is.beta - is.gamma, A), smul(is.beta, B)), smul(is.gamma, C));
-                ns = normalize(cross(sub(B, A), sub(C, A)));   // diff_geom::ng (libgi/rt.cpp:7-15)
-                ng = ns;
-                if (dot(cr.d, ng) > 0) ng = v3{-ng.x, -ng.y, -ng.z};   // flip_normals_to_ray (util.h:226-233)
+                ng = normalize(cross(sub(B, A), sub(C, A)));   // diff_geom::ng (libgi/rt.cpp:7-15)
+                ns = vn ? shading_normal(vn, idx, is.ref, is.beta, is.gamma) : ng;
+                if (dot(cr.d, ns) > 0) ng = v3{-ng.x, -ng.y, -ng.z};   // flip_normals_to_ray (util.h:226-233)
                 if (e.x != 0 || e.y != 0 || e.z != 0) {
                     px[0] += e.x;
                     px[1] += e.y;
@@ -1063,7 +1080,8 @@ void orc_direct(const float *pos, const uint32_t *idx, uint32_t n_tris, const ui
                             float su0 = sqrtf(xi0);
                             float bx = 1.0f - su0, by = xi1 * su0;   // uniform_sample_triangle (sampling.h:43-46)
                             v3 target = add(add(smul(1.0f - bx - by, L.a), smul(bx, L.b)), smul(by, L.c));
-                            v3 nl = normalize(cross(sub(L.b, L.a), sub(L.c, L.a)));
+                            v3 nl = vn ? shading_normal(vn, idx, L.tri, bx, by)
+                                       : normalize(cross(sub(L.b, L.a), sub(L.c, L.a)));
                             wi = sub(target, xh);
                             v3 x2 = cross(sub(L.b, L.a), sub(L.c, L.a));
                             float area = 0.5f * sqrtf(dot(x2, x2));

@@ -195,8 +195,10 @@ def camera_rays(pos, dir, up, fovy, w, h, ox=None, oy=None):
     return out
 
 
-def cosine_rays(pos, idx, in_rays, t, u, v, ids, xi0, xi1):
+def cosine_rays(pos, idx, in_rays, t, u, v, ids, xi0, xi1, normals=None):
+    """orc_cosine_rays; normals: optional (n_vert, 3) shading normals."""
     pos, idx = _scene(pos, idx)
+    vn = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
     in_rays, inp = _planes(in_rays)
     n = in_rays.shape[1]
     out = np.empty((8, n), np.float32)
@@ -207,7 +209,8 @@ def cosine_rays(pos, idx, in_rays, t, u, v, ids, xi0, xi1):
     ids = np.ascontiguousarray(ids, np.uint32)
     xi0 = np.ascontiguousarray(xi0, np.float32)
     xi1 = np.ascontiguousarray(xi1, np.float32)
-    lib().orc_cosine_rays(_p(pos), _p(idx), inp, _p(t), _p(u), _p(v), _p(ids), _p(xi0), _p(xi1), c_uint32(n), arr)
+    lib().orc_cosine_rays(_p(pos), _p(idx), _p(vn), inp, _p(t), _p(u), _p(v), _p(ids), _p(xi0), _p(xi1), c_uint32(n),
+                          arr)
     return out
 
 
@@ -295,11 +298,13 @@ def path_trace(pos, idx, mat, materials, nodes, root, index, cam, seeds, max_dep
 DIRECT_MODES = {"uniform": 0, "cosine": 1, "light": 2}
 
 
-def direct(pos, idx, mat, materials, nodes, root, index, cam, mode, seeds, points=None, threads=0):
+def direct(pos, idx, mat, materials, nodes, root, index, cam, mode, seeds, points=None, threads=0, normals=None):
     """The wavefront direct-lighting algorithm (orc_direct) in mode "uniform",
     "cosine" or "light", one pass per seed.  points: (k, 6) pos + col of point
-    lights.  Returns the framebuffer (h, w, 4): rgb sums and weights."""
+    lights; normals: optional (n_vert, 3) shading normals.  Returns the
+    framebuffer (h, w, 4): rgb sums and weights."""
     pos, idx = _scene(pos, idx)
+    vn = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
     cam = np.ascontiguousarray(cam[:12], dtype=np.float32)
     w, h = int(cam[10]), int(cam[11])
     mats = np.ascontiguousarray(materials, dtype=np.float32).reshape(-1, 8)
@@ -309,7 +314,7 @@ def direct(pos, idx, mat, materials, nodes, root, index, cam, mode, seeds, point
     nodes = np.ascontiguousarray(nodes)
     index = np.ascontiguousarray(index, dtype=np.uint32)
     fb = np.zeros((h, w, 4), np.float32)
-    lib().orc_direct(_p(pos), _p(idx), c_uint32(len(idx)), _p(mat), _p(mats), _p(nodes), c_uint32(len(nodes)),
+    lib().orc_direct(_p(pos), _p(idx), c_uint32(len(idx)), _p(mat), _p(mats), _p(vn), _p(nodes), c_uint32(len(nodes)),
                      c_uint32(root), _p(index), _p(cam), c_int(DIRECT_MODES[mode]), _p(pts), c_uint32(len(pts)),
                      _p(seeds), c_uint32(len(seeds)), _p(fb), c_int(threads))
     return fb

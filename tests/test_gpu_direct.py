@@ -123,3 +123,40 @@ def test_point_light_pair_equals_light_mode(torch_cuda, hip_ctx, hall):
     ds.close()
     assert a[:, :3].sum() > 0
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", ["uniform", "cosine", "light"])
+def test_direct_modes_with_shading_normals_match_oracle(torch_cuda, hip_ctx, hall, mode):
+    """Random unit vertex normals: the sample steps flip by the interpolated
+    shading normal, the integration weighs by it and a triangle light's cosine
+    takes its normal interpolated at the sampled point, as in the oracle."""
+    sc = hiprt.HostScene.generate(hall["cfg"])
+    rng = np.random.default_rng(12)
+    nrm = rng.normal(size=(len(hall["pos"]), 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    sc.set_normals(nrm)
+    ds = hiprt.DeviceScene(hip_ctx, sc, hall["bvh"])
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = 256, 144
+    seeds = orc.frame_seeds(4, 0, 2)
+    print(f"case: direct lighting with random vertex normals, hall config {hall['cfg']}, {mode}, 256x144", flush=True)
+    got = _gpu(torch_cuda, ds, cam, mode, seeds, None)
+    ds.close()
+    b = hall["bvh"]
+    want = orc.direct(hall["pos"], hall["idx"], hall["mat"], hall["mats"], b.nodes, b.root, b.index, cam, mode, seeds,
+                      None, normals=nrm)
+    plain = orc.direct(hall["pos"], hall["idx"], hall["mat"], hall["mats"], b.nodes, b.root, b.index, cam, mode, seeds,
+                       None)
+    differ = (got.view(np.uint32) != want.view(np.uint32)).any(-1)
+    scale = np.maximum(np.abs(want[..., :3]).max(-1), 1e-30)
+    far = (np.abs(got[..., :3] - want[..., :3]).max(-1) > PIXEL_REL * scale)
+    print({"pixels_differ": int(differ.sum()), "pixels_beyond_rel": int(far.sum())})
+    assert np.array_equal(got[..., 3], want[..., 3])
+    assert want[..., :3].sum() > 0
+    if mode == "light":
+        assert not differ.any()
+    else:
+        assert far.sum() <= MAX_DIFF * far.size * len(seeds)
+    # the normals change most lit pixels
+    lit = (want[..., :3].sum(-1) > 0) | (plain[..., :3].sum(-1) > 0)
+    assert (plain.view(np.uint32) != want.view(np.uint32)).any(-1)[lit].mean() > 0.3

@@ -81,6 +81,43 @@ def test_bounce_rays_match_oracle_and_trace_exactly(torch_cuda, hall):
     assert bit_exact(_hits_np(hits2), ref)
 
 
+def test_bounce_rays_with_shading_normals_match_oracle(torch_cuda, hip_ctx, hall):
+    """With per-vertex shading normals (random unit vectors, so that about
+    half the hits flip differently from the geometric rule) the bounce step
+    flips by the interpolated shading normal, as the oracle does."""
+    print("case: hall 480x270 primary hits -> cosine bounce with random vertex normals", flush=True)
+    torch = torch_cuda
+    sc = hiprt.HostScene.generate(3)
+    pos, idx, _ = sc.arrays()
+    rng = np.random.default_rng(5)
+    nrm = rng.normal(size=(len(pos), 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    sc.set_normals(nrm)
+    ds = hiprt.DeviceScene(hip_ctx, sc, hall["bvh"])
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = 480, 270
+    rays = hiprt.camera_rays(cam)
+    n = rays.shape[1]
+    d_rays = torch.from_numpy(rays).cuda()
+    hits = hiprt.new_hits(n)
+    hiprt.trace_closest(ds, d_rays, hits, tracer="while-while")
+    out = torch.empty_like(d_rays)
+    hiprt.bounce_cosine(ds, d_rays, hits, out, 91, 2)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    ds.close()
+    h = _hits_np(hits)
+    g = Pcg32(stream_id(91, np.arange(n, dtype=np.uint64), 2))
+    xi0, xi1 = g.next_float(), g.next_float()
+    want = orc.cosine_rays(pos, idx, rays, *h, xi0, xi1, normals=nrm)
+    plain = orc.cosine_rays(pos, idx, rays, *h, xi0, xi1)
+    assert np.array_equal(got[[0, 1, 2, 3, 7]].view(np.uint32), want[[0, 1, 2, 3, 7]].view(np.uint32))
+    assert np.max(np.abs(got[4:7] - want[4:7])) < 1e-5
+    # the normals change the side of a large share of the bounces
+    flipped = (np.sign(want[4:7]) != np.sign(plain[4:7])).any(0) & (want[7] > 0)
+    assert flipped.mean() > 0.2
+
+
 @pytest.mark.parametrize("res,stack,chunks,sort", [((320, 180), 0, 1, True), ((320, 180), 1, 8, True),
                                                    ((320, 180), 0, 8, False), ((333, 97), 3, 8, True),
                                                    ((333, 97), 3, 1, False), ((333, 97), 4, 8, True)])

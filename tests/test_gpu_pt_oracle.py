@@ -184,3 +184,38 @@ def test_regeneration_with_one_slot_equals_samples_in_flight(torch_cuda, hip_ctx
     assert a.sum() > 0
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert np.array_equal(ta, tb) and np.array_equal(ca, cb)
+
+
+def test_path_tracer_with_shading_normals_matches_oracle(torch_cuda, hip_ctx, hall):
+    """The hall with random unit vertex normals: every bounce flips by the
+    interpolated shading normal (flip_normals_to_ray), on the GPU (plain and
+    batched schedules) as in the oracle's path loop; the image differs from
+    the one without normals."""
+    sc = hiprt.HostScene.generate(3)
+    pos, idx, mat = sc.arrays()
+    rng = np.random.default_rng(8)
+    nrm = rng.normal(size=(len(pos), 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    sc.set_normals(nrm)
+    ds = hiprt.DeviceScene(hip_ctx, sc, hall["bvh"])
+    cam = hall["cam"].copy()
+    cam[10], cam[11] = 160, 90
+    want, want_levels = _oracle(hall, cam, 5, 2, normals=nrm)
+    plain_img, _ = _oracle(hall, cam, 5, 2)
+    n_paths = 160 * 90 * 2
+    for opts in (dict(batched=False), dict(batched=True, tracer="persistent-while-while", samples=2)):
+        print(f"case: hall 160x90 with random vertex normals, {opts}", flush=True)
+        pt = hiprt.PathTracer(ds, cam, seed=5, **opts)
+        for f in range(2 // opts.get("samples", 1)):
+            pt.render(f)
+        got, got_levels = pt.image(), pt.ray_totals()
+        pt.close()
+        differ = (got.view(np.uint32) != want.view(np.uint32)).any(-1)
+        print({"pixels_differ": int(differ.sum()), "levels_gpu": got_levels.tolist(),
+               "levels_oracle": want_levels.tolist()})
+        assert differ.sum() <= MAX_DIFF_PATHS * n_paths + 1
+        assert (np.abs(got_levels.astype(np.int64) - want_levels.astype(np.int64)) <= COUNT_REL * want_levels + 1).all()
+    ds.close()
+    # the normals change most lit pixels
+    lit = (want.sum(-1) > 0) | (plain_img.sum(-1) > 0)
+    assert (plain_img.view(np.uint32) != want.view(np.uint32)).any(-1)[lit].mean() > 0.3

@@ -117,15 +117,42 @@ def test_obj_materials_from_a_library(tmp_path):
 def test_obj_winding_follows_shading_normals(tmp_path):
     """A triangle whose geometric normal points away from the mean of its
     corner normals has b and c swapped (libgi/scene.cpp:271-277); one whose
-    normals agree, or that has no normals, keeps its order."""
+    normals agree, or that has no normals, keeps its order.  Every (v, vn)
+    corner is a vertex of its own carrying its normal; the corners of the face
+    without normals get the face's normal."""
     (tmp_path / "n.obj").write_text(
         "v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nvn 0 0 -1\n"
         "f 1//1 2//1 3//1\n"            # geometric normal +z, shading +z: kept
         "f 1//2 2//2 3//2\n"            # shading -z: swapped
         "f -3//-1 -2//-1 -1//-1\n"      # relative indices, shading -z: swapped
-        "f 1 2 3\n")                    # no normals: kept
-    _, idx, _ = hiprt.HostScene.load_obj(tmp_path / "n.obj").arrays()
-    assert idx.tolist() == [[0, 1, 2], [0, 2, 1], [0, 2, 1], [0, 1, 2]]
+        "f 1 2 3\n")                    # no normals: kept, face normal +z
+    sc = hiprt.HostScene.load_obj(tmp_path / "n.obj")
+    pos, idx, _ = sc.arrays()
+    nrm = sc.normals()
+    P = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+    kept, swapped = [P[0], P[1], P[2]], [P[0], P[2], P[1]]
+    assert pos[idx].tolist() == [kept, swapped, swapped, kept]
+    up, down = [0, 0, 1], [0, 0, -1]
+    assert nrm[idx].tolist() == [[up] * 3, [down] * 3, [down] * 3, [up] * 3]
+    assert len(pos) == 3 + 3 + 3 + 3   # the file's vertices, then one per distinct (v, vn) or face corner
+    assert hiprt.HostScene.load_obj(tmp_path / "n.obj").normals() is not None
+
+
+def test_scene_normals_round_trip_and_counts(tmp_path):
+    """hiprt_scene_set_normals takes one normal per vertex (or none); a scene
+    without normals reports none, and an OBJ without "vn" has none."""
+    sc = hiprt.HostScene.generate(1)
+    pos, _, _ = sc.arrays()
+    assert sc.normals() is None
+    n = np.tile(np.array([[0, 1, 0]], np.float32), (len(pos), 1))
+    sc.set_normals(n)
+    assert np.array_equal(sc.normals(), n)
+    with pytest.raises(hiprt.HiprtError):
+        sc.set_normals(n[:-1])
+    sc.set_normals(None)
+    assert sc.normals() is None
+    (tmp_path / "p.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    assert hiprt.HostScene.load_obj(tmp_path / "p.obj").normals() is None
 
 
 def test_obj_texture_coordinates_per_corner(tmp_path):
@@ -136,16 +163,26 @@ def test_obj_texture_coordinates_per_corner(tmp_path):
     coordinates with their corners."""
     (tmp_path / "t.obj").write_text(
         "v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\n"
-        "vt 0.25 0.5\nvt 0.75 0.5\nvt 0.5 1.5\nvn 0 0 -1\n"
+        "vt 0.25 0.5\nvt 0.75 0.5\nvt 0.5 1.5\n"
         "f 1/1 2/2 3/3\n"            # three new vertices 4, 5, 6
-        "f 2/2 4 3/-1\n"             # (2, vt2) reused, 4 plain, (3, vt3) reused
-        "f 1/1/1 2/2/1 3/3/1\n")     # shading -z: b and c swap, with their coordinates
+        "f 2/2 4 3/-1\n")            # (2, vt2) reused, 4 plain, (3, vt3) reused
     sc = hiprt.HostScene.load_obj(tmp_path / "t.obj")
     pos, idx, _ = sc.arrays()
     tc = sc.texcoords()
-    assert len(pos) == 7 and idx.tolist() == [[4, 5, 6], [5, 3, 6], [4, 6, 5]]
+    assert len(pos) == 7 and idx.tolist() == [[4, 5, 6], [5, 3, 6]]
     assert np.array_equal(pos[4:], pos[[0, 1, 2]])
     assert tc.tolist() == [[0, 0], [0, 0], [0, 0], [0, 0], [0.25, 0.5], [0.75, 0.5], [0.5, 1.5]]
+    assert sc.normals() is None
+    # with normals in the file: the swap moves coordinates and normals with their corners
+    (tmp_path / "w.obj").write_text(
+        "v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0.25 0.5\nvt 0.75 0.5\nvt 0.5 1.5\nvn 0 0 -1\n"
+        "f 1/1/1 2/2/1 3/3/1\n")     # shading -z: b and c swap
+    sc = hiprt.HostScene.load_obj(tmp_path / "w.obj")
+    pos, idx, _ = sc.arrays()
+    assert idx.tolist() == [[3, 4, 5]]
+    assert pos[idx[0]].tolist() == [[0, 0, 0], [0, 1, 0], [1, 0, 0]]
+    assert sc.texcoords()[idx[0]].tolist() == [[0.25, 0.5], [0.5, 1.5], [0.75, 0.5]]
+    assert sc.normals()[idx[0]].tolist() == [[0, 0, -1]] * 3
     (tmp_path / "b.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nf 1/1 2/2 3/1\n")
     with pytest.raises(hiprt.HiprtError, match="texture coordinate index"):
         hiprt.HostScene.load_obj(tmp_path / "b.obj")

@@ -57,10 +57,10 @@ def _walls():
     return P, tris
 
 
-def _run(pos, idx, mat, mats, mode, points=None, cam=CAM, seeds=SEEDS):
+def _run(pos, idx, mat, mats, mode, points=None, cam=CAM, seeds=SEEDS, normals=None):
     sc = hiprt.HostScene.from_arrays(pos, idx, mat)
     bvh = hiprt.Bvh(sc)
-    return orc.direct(pos, idx, mat, mats, bvh.nodes, bvh.root, bvh.index, cam, mode, seeds, points)
+    return orc.direct(pos, idx, mat, mats, bvh.nodes, bvh.root, bvh.index, cam, mode, seeds, points, normals=normals)
 
 
 A = 0.5
@@ -117,8 +117,9 @@ def _light_tri(y, big, down=True):
     return P, [(0, 1, 2)] if down else [(0, 2, 1)]
 
 
-def _tri_estimate(x, L, e, xi0, xi1, p_light):
-    """e a/pi cos_x cos_l area / dist^2 / p_light for a point of triangle L."""
+def _tri_estimate(x, L, e, xi0, xi1, p_light, light_normal=None):
+    """e a/pi cos_x cos_l area / dist^2 / p_light for a point of triangle L
+    (cos_l against light_normal when given, else the geometric normal)."""
     a, b, c = (np.asarray(p, np.float64) for p in L)
     su0 = np.sqrt(xi0)
     bx, by = 1 - su0, xi1 * su0
@@ -126,6 +127,8 @@ def _tri_estimate(x, L, e, xi0, xi1, p_light):
     nl = np.cross(b - a, c - a)
     area = 0.5 * np.linalg.norm(nl)
     nl /= np.linalg.norm(nl)
+    if light_normal is not None:
+        nl = np.asarray(light_normal, np.float64)
     w = target - x
     dist = np.linalg.norm(w, axis=1)
     w /= dist[:, None]
@@ -146,6 +149,56 @@ def test_light_mode_one_triangle():
         want = want + _tri_estimate(x, L, E, xi0, xi1, 1.0)
     assert np.all(fb[:, 3] == len(SEEDS))
     np.testing.assert_allclose(fb[:, :3], want, rtol=2e-5, atol=1e-7)
+
+
+def test_cosine_mode_weighs_by_the_shading_normal():
+    """With the floor's normals tilted by 0.5 rad from its geometric normal,
+    each cosine sample w (drawn about the geometric normal, pdf cos_g / pi)
+    adds e * a * max(0, w . ns) / cos_g: integrate_dir_sample weighs by the
+    shading normal (rt/cpu/bounce.cpp:108-130)."""
+    F, ft = _floor(up=True)
+    W, wt = _walls()
+    pos, idx, mat = _scene([(F, ft, 0), (W, wt, 1)])
+    tilt = np.array([np.sin(0.5), np.cos(0.5), 0.0])
+    normals = np.zeros((len(pos), 3), np.float32)
+    normals[:4] = tilt
+    normals[4:] = [0, 1, 0]   # the emitting walls: only their emission counts
+    fb = _run(pos, idx, mat, MATS, "cosine", normals=normals).reshape(-1, 4)
+    n = len(fb)
+    want = 0.0
+    for s in SEEDS:
+        xi0, xi1 = _uniforms(s, n, 2)
+        r, th = np.sqrt(xi0), 2 * np.pi * xi1
+        dx, dy = r * np.cos(th), r * np.sin(th)
+        z = np.sqrt(np.maximum(0.0, 1 - dx * dx - dy * dy))
+        w = np.stack([-dx, z, dy], 1)   # align(sample, +y)
+        c = w @ tilt
+        want = want + A * E[None, :] * (np.maximum(c, 0) / z)[:, None]
+    assert np.all(fb[:, 3] == len(SEEDS))
+    np.testing.assert_allclose(fb[:, :3], want, rtol=1e-4, atol=1e-6)
+
+
+def test_light_mode_uses_the_lights_interpolated_normal():
+    """trianglelight::sample_Li takes the light's vertex normals interpolated
+    at the sampled point for cos_light (libgi/scene.cpp:534-559): tilting
+    them changes the estimate accordingly."""
+    F, ft = _floor(up=True)
+    L, lt = _light_tri(1.0, big=True)
+    pos, idx, mat = _scene([(F, ft, 0), (L, lt, 1)])
+    nl = np.array([np.sin(0.4), -np.cos(0.4), 0.0])
+    normals = np.zeros((len(pos), 3), np.float32)
+    normals[:4] = [0, 1, 0]   # the floor's shading normal = its geometric normal
+    normals[4:] = nl
+    fb = _run(pos, idx, mat, MATS, "light", normals=normals).reshape(-1, 4)
+    x = _floor_points()
+    want = 0.0
+    for s in SEEDS:
+        _, xi0, xi1 = _uniforms(s, len(x), 3)
+        want = want + _tri_estimate(x, L, E, xi0, xi1, 1.0, light_normal=nl)
+    assert np.all(fb[:, 3] == len(SEEDS))
+    np.testing.assert_allclose(fb[:, :3], want, rtol=5e-5, atol=1e-7)
+    plain = _run(pos, idx, mat, MATS, "light").reshape(-1, 4)
+    assert not np.allclose(plain[:, :3], fb[:, :3], rtol=1e-3)
 
 
 def test_light_mode_light_facing_away_and_floor_facing_down():

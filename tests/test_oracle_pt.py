@@ -96,6 +96,26 @@ def test_emitters_reflect_is_an_option():
     assert img.sum() > 1.2 * n * 4.5
 
 
+def test_floor_normals_decide_the_flip():
+    """flip_normals_to_ray flips by the shading normal (libgi/util.h:226-232).
+    The box's floor is wound with its geometric normal pointing down (out of
+    the box).  With vertex normals pointing up, a camera ray from above is not
+    on their side, so nothing flips, the cosine bounce leaves about the
+    downward geometric normal, out of the box, and no path meets an emitter.
+    With the normals down (equal to the geometric normal) both flip, the
+    bounce goes up and every pixel is a * e, as without normals.  Flipping by
+    the geometric normal would light the floor in both."""
+    pos, idx, mat = _box(0, 1)
+    mats = np.array([[0.5, 0.5, 0.5, 0, 0, 0, 0, 0], [0.3, 0.3, 0.3, 0, 2, 3, 4, 0]], np.float32)
+    seeds = orc.frame_seeds(3, 0, 1)
+    n = 32 * 32
+    up = np.tile(np.float32([[0, 1, 0]]), (len(pos), 1))
+    img, levels, _ = _run(pos, idx, mat, mats, seeds, normals=up)
+    assert levels.tolist() == [n, n, 0, 0, 0] and not img.any()
+    img, levels, _ = _run(pos, idx, mat, mats, seeds, normals=-up)
+    assert np.array_equal(img.reshape(-1, 3), np.tile(np.float32([1.0, 1.5, 2.0]), (n, 1)))
+
+
 def test_black_floor_ends_every_path():
     pos, idx, mat = _box(0, 1)
     mats = np.array([[0, 0, 0, 0, 0, 0, 0, 0], [0.3, 0.3, 0.3, 0, 2, 3, 4, 0]], np.float32)
