@@ -212,6 +212,24 @@ int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_material
     return ok();
 }
 
+int hiprt_scene_material_name(hiprt_host_scene s, uint32_t material, char *name, uint32_t size) {
+    if (!s || !name || size == 0) return fail(HIPRT_ERR_ARG, "null argument");
+    if (material >= s->sc.mats.size()) return fail(HIPRT_ERR_ARG, "material id out of range");
+    const std::string n = material < s->sc.mat_names.size() ? s->sc.mat_names[material] : std::string();
+    if (n.size() + 1 > size) return fail(HIPRT_ERR_ARG, "name buffer too small");
+    std::memcpy(name, n.c_str(), n.size() + 1);
+    return ok();
+}
+
+int hiprt_scene_set_material(hiprt_host_scene s, uint32_t material, const float *albedo, const float *emissive) {
+    if (!s) return fail(HIPRT_ERR_ARG, "null scene");
+    if (material >= s->sc.mats.size()) return fail(HIPRT_ERR_ARG, "material id out of range");
+    rt::material &m = s->sc.mats[material];
+    if (albedo) m.albedo = rt::vec3f{albedo[0], albedo[1], albedo[2]};
+    if (emissive) m.emissive = rt::vec3f{emissive[0], emissive[1], emissive[2]};
+    return ok();
+}
+
 int hiprt_scene_set_texcoords(hiprt_host_scene s, const float *tc, uint32_t n_vert) {
     if (!s || (n_vert && !tc)) return fail(HIPRT_ERR_ARG, "null argument");
     if (n_vert != s->sc.pos.size()) return fail(HIPRT_ERR_ARG, "texture coordinate count differs from the vertex count");

@@ -465,13 +465,16 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
         // 1 + k: the k-th material of the libraries, in definition order
         out.mats.assign(1, material{v3(0.8f), v3(0.0f)});
         for (const mtl_entry &e : mtl) out.mats.push_back(e.resolve());
+        out.mat_names.assign(1, "default");
+        out.mat_names.insert(out.mat_names.end(), mtl_names.begin(), mtl_names.end());
         for (size_t t = 0; t < out.mat.size(); ++t) out.mat[t] = tri_named[t] >= 0 ? (uint32_t)tri_named[t] + 1 : 0u;
         return true;
     }
-    (void)legacy;
     uint32_t max_mat = 0;
     for (uint32_t m : out.mat) max_mat = m > max_mat ? m : max_mat;
     out.mats.assign(max_mat + 1, material{v3(0.8f), v3(0.0f)});
+    out.mat_names.clear();
+    for (uint32_t m = 0; m <= max_mat; ++m) out.mat_names.push_back(legacy ? "m" + std::to_string(m) : "default");
     return true;
 }
 

@@ -35,6 +35,9 @@ struct scene {
     std::vector<uint32_t> idx;     // 3 per triangle
     std::vector<uint32_t> mat;     // material id per triangle
     std::vector<material> mats;
+    // material names (material::name of the reference), one per material, or
+    // empty for scenes whose materials have none
+    std::vector<std::string> mat_names;
     std::vector<float> tc;         // per vertex (u, v), empty when the scene has no texture coordinates
     // per vertex shading normal (x, y, z), vertex::norm; empty when the scene
     // has none, and then the geometric normal shades (the face normals the

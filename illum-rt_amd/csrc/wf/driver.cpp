@@ -15,16 +15,22 @@
 // reference; "run" checks the same preconditions.
 //
 // Commands: at, look, up, fovy, resolution, sppx, load, generate, pointlight,
-// platform, raytracer, tracer-option, bvh, triangles, add-scene-step, commit, algo, run,
-// outfile, stats, echo, quit, and the algorithm's own (direct-wf: is
-// uniform|cosine|light).  Images are written as PFM (float) or, for any
-// other extension, 8-bit PPM with a 2.2 gamma.
+// material, default-brdf, platform, raytracer, tracer-option, bvh, triangles,
+// add-scene-step, commit, algo, run, outfile, stats, echo, disable, with / },
+// quit, and the algorithm's own (direct-wf: is uniform|cosine|light).
+// Materials are Lambertian: "material roughness|ior|brdf" and "default-brdf"
+// are accepted and kept for "material show" but do not change the shading.
+// "with gl|cuda|optix {" blocks are skipped (those platforms are not built
+// here); "with hip {" blocks run unless "disable hip" came first.  Images are
+// written as PFM (float) or, for any other extension, 8-bit PPM with a 2.2
+// gamma.
 #include "hip-platform.h"
 
 #include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -43,6 +49,28 @@ update_counters uc;
 unique_ptr<wf::simple_algorithm> algo;
 string algo_name;
 string outfile = "out.pfm";
+bool skip_block = false;                 // inside a "with <missing> {" block
+map<string, bool> disabled_features;
+int selected_mat = -1;                   // "material select", -1: none
+// parameters of the reference's other BRDFs, kept per material for "material show"
+struct brdf_params {
+    float roughness = 0.1f, ior = 1.3f;
+    string brdf;
+};
+map<int, brdf_params> mat_params;
+string default_brdf = "lambert";
+
+string material_name(hiprt_host_scene s, uint32_t m) {
+    char buf[256];
+    return hiprt_scene_material_name(s, m, buf, sizeof buf) == HIPRT_OK ? string(buf) : string();
+}
+
+string rest_of_line(istringstream &in) {
+    string r;
+    while (isspace(in.peek())) in.get();
+    getline(in, r);
+    return r;
+}
 
 #define error(x)                                                              \
     {                                                                         \
@@ -97,9 +125,96 @@ void eval(const string &line) {
     in >> command;
     ++uc.cmdid;
     wf::render_context *rc = wf::rc;
+    if (skip_block) {   // interaction.cpp:238-244
+        if (line == "}") skip_block = false;
+        return;
+    }
     try {
         if (command.empty() || command[0] == '#' || command.substr(0, 2) == "//") {
             return;
+        } else if (command == "disable") {
+            string what;
+            in >> what;
+            check_in_complete("Syntax error, expecting only one feature to disable");
+            cout << "DISABLING " << what << endl;
+            disabled_features[what] = true;
+        } else if (command == "with") {
+            // interaction.cpp:263-297; the platforms this build has: hip
+            string what, rest;
+            in >> what;
+            bool missing;
+            if (what == "gl" || what == "cuda" || what == "optix") missing = true;
+            else if (what == "hip") missing = disabled_features[what];
+            else error("Syntax error: expected one of gl, cuda, optix, hip");
+            in >> rest;
+            check_in_complete("Syntax error, expecting only {");
+            if (rest != "{") error("Syntax error, expecting {");
+            skip_block = missing;
+            if (skip_block) cout << "Skipping a block due to missing " << what << endl;
+        } else if (command == "}") {
+            check_in_complete("Syntax error: } must be on an otherwise empty line");
+        } else if (command == "material") {
+            // interaction.cpp:624-706
+            string sub;
+            in >> sub;
+            hiprt_host_scene s = rc->scene.geometry;
+            uint32_t n_mats = 0;
+            if (s) hiprt_scene_materials(s, nullptr, &n_mats);
+            if (in.eof() && sub == "list") {
+                for (uint32_t m = 0; m < n_mats; ++m) cout << material_name(s, m) << endl;
+                return;
+            }
+            if (in.fail()) error("Syntax error, requires material name, command and subsequent arguments");
+            if (sub == "select") {
+                string name = rest_of_line(in);
+                int found = -1;
+                for (uint32_t m = 0; m < n_mats && found < 0; ++m)
+                    if (material_name(s, m) == name) found = (int)m;
+                if (found < 0) error("No material called '" << name << "'");
+                selected_mat = found;
+                return;
+            }
+            if (sub == "blacklist") return;
+            if (selected_mat < 0 || !s || (uint32_t)selected_mat >= n_mats) error("No material selected");
+            if (sub == "albedo" || sub == "emissive") {
+                float v[3];
+                if (!read3(in, v)) error("Expects a color triplet");
+                check_in_complete("Expects a color triplet");
+                if (hiprt_scene_set_material(s, (uint32_t)selected_mat, sub == "albedo" ? v : nullptr,
+                                             sub == "emissive" ? v : nullptr) != HIPRT_OK)
+                    error(hiprt_last_error());
+                uc.scene_touched_at = uc.cmdid;
+            } else if (sub == "roughness" || sub == "ior") {
+                float x;
+                in >> x;
+                check_in_complete("Expects a floating point value");
+                (sub == "roughness" ? mat_params[selected_mat].roughness : mat_params[selected_mat].ior) = x;
+            } else if (sub == "brdf") {
+                string name;
+                in >> name;
+                check_in_complete("Expected a single (no whitespace) string value");
+                mat_params[selected_mat].brdf = name;
+            } else if (sub == "texture") {
+                error("Texture files are not decoded here; attach RGBA texels with hiprt_scene_set_texture");
+            } else if (sub == "show") {
+                check_in_complete("Does not take further arguments");
+                vector<float> t(8ull * n_mats);
+                hiprt_scene_materials(s, t.data(), &n_mats);
+                const float *m = &t[8ull * selected_mat];
+                const brdf_params &bp = mat_params[selected_mat];
+                cout << "albedo     " << m[0] << " " << m[1] << " " << m[2] << endl;
+                cout << "albedo-tex none" << endl;
+                cout << "emissive   " << m[4] << " " << m[5] << " " << m[6] << endl;
+                cout << "roughness  " << bp.roughness << endl;
+                cout << "ior        " << bp.ior << endl;
+            } else {
+                error("Unknown subcommand");
+            }
+        } else if (command == "default-brdf") {
+            string name;
+            in >> name;
+            check_in_complete("Expected a single (no whitespace) string value");
+            default_brdf = name;
         } else if (command == "at" || command == "look" || command == "up") {
             float v[3];
             if (!read3(in, v)) error("Syntax error, requires 3 numerical components");
@@ -132,6 +247,8 @@ void eval(const string &line) {
             if (hiprt_scene_load_obj(file.c_str(), &s) != HIPRT_OK) error(hiprt_last_error());
             if (rc->scene.geometry) hiprt_scene_free(rc->scene.geometry);
             rc->scene.geometry = s;
+            selected_mat = -1;
+            mat_params.clear();
             uc.scene_touched_at = uc.cmdid;
         } else if (command == "generate") {
             int cfg;
@@ -142,6 +259,8 @@ void eval(const string &line) {
             if (hiprt_scene_generate(cfg, &s, cam) != HIPRT_OK) error(hiprt_last_error());
             if (rc->scene.geometry) hiprt_scene_free(rc->scene.geometry);
             rc->scene.geometry = s;
+            selected_mat = -1;
+            mat_params.clear();
             for (int k = 0; k < 3; ++k) {
                 rc->scene.cam_pos[k] = cam[k];
                 rc->scene.cam_dir[k] = cam[3 + k];

@@ -105,6 +105,8 @@ def lib():
             "hiprt_scene_set_texcoords": (c_int, [c_void_p, c_void_p, c_uint32]),
             "hiprt_scene_get_texcoords": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_set_normals": (c_int, [c_void_p, c_void_p, c_uint32]),
+            "hiprt_scene_material_name": (c_int, [c_void_p, c_uint32, c_char_p, c_uint32]),
+            "hiprt_scene_set_material": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
             "hiprt_scene_get_normals": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_set_texture": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_uint32]),
             "hiprt_scene_free": (None, [c_void_p]),
@@ -255,6 +257,23 @@ class HostScene:
         tc = np.zeros((nv.value, 2), np.float32)
         check(lib().hiprt_scene_get_texcoords(self.h, _ptr(tc), ctypes.byref(nv)))
         return tc
+
+    def material_names(self):
+        """Names of the materials ("" where a scene's materials have none)."""
+        n = len(self.materials())
+        out = []
+        for m in range(n):
+            buf = ctypes.create_string_buffer(256)
+            check(lib().hiprt_scene_material_name(self.h, m, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
+    def set_material(self, material, albedo=None, emissive=None):
+        """Albedo and/or emission (rgb) of one material; None keeps a value."""
+        a = None if albedo is None else np.ascontiguousarray(albedo, dtype=np.float32).reshape(3)
+        e = None if emissive is None else np.ascontiguousarray(emissive, dtype=np.float32).reshape(3)
+        check(lib().hiprt_scene_set_material(self.h, material, _ptr(a) if a is not None else None,
+                                             _ptr(e) if e is not None else None))
 
     def set_normals(self, normals):
         """Per-vertex shading normals (n_vert, 3) float32; None removes them."""

@@ -73,6 +73,16 @@ int hiprt_scene_copy(hiprt_host_scene s, float *pos, uint32_t *idx, uint32_t *ma
 /* Material table: 8 floats per material (albedo rgb, 0, emission rgb, 0), the
  * layout hiprt_scene_build takes; *n_materials = count (table may be NULL). */
 int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_materials);
+/* Name of material `material` (material::name, which "material select"
+ * looks up, driver/interaction.cpp:624-635), NUL-terminated into name[size];
+ * "" for a scene whose materials have no names.  An OBJ file's materials are
+ * "default" (faces without a known material) and then the library's
+ * materials in definition order, or "m<id>" for "usemtl m<id>" files. */
+int hiprt_scene_material_name(hiprt_host_scene s, uint32_t material, char *name, uint32_t size);
+/* Sets material `material`'s albedo and/or emission (3 floats each; NULL
+ * keeps the value), as "material albedo" / "material emissive" do
+ * (driver/interaction.cpp:642-651).  Takes effect at the next commit. */
+int hiprt_scene_set_material(hiprt_host_scene s, uint32_t material, const float *albedo, const float *emissive);
 /* Per-vertex texture coordinates, n_vert (u, v) pairs: vertex::tc of
  * libgi/rt.h, which the reference fills from assimp (libgi/scene.cpp:253-262).
  * n_vert must be the scene's vertex count.  Only alpha-aware tracing reads

@@ -47,6 +47,33 @@ def test_driver_errors_follow_the_reference(tmp_path):
     assert len(err) == 9
 
 
+def test_driver_blocks_and_material_commands(tmp_path):
+    """The reference's "with <feature> {" blocks (interaction.cpp:263-297):
+    gl, cuda and optix are not built here, so their blocks are skipped up to
+    the closing "}"; "with hip {" runs unless hip was disabled.  "material"
+    (interaction.cpp:624-706): list, select by name, albedo/emissive, the
+    reference's error messages."""
+    (tmp_path / "lib.mtl").write_text("newmtl floor\nKd 0.5 0.5 0.5\nnewmtl lamp\nKd 0.5 0.5 0.5\n")
+    (tmp_path / "s.obj").write_text("mtllib lib.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl floor\nf 1 2 3\n")
+    r = run_script("with cuda {\n\tplatform cuda\n\techo never\n}\necho after\nwith hip {\necho inside\n}\n"
+                   "disable hip\nwith hip {\necho never\n}\nwith vulkan {\n"
+                   "load s.obj\nmaterial list\nmaterial albedo 1 1 1\nmaterial select nope\nmaterial select lamp\n"
+                   "material emissive 5 4 3\nmaterial roughness 0.3\nmaterial show\nmaterial frobnicate\n"
+                   "default-brdf layered-gtr2\n", tmp_path)
+    assert r.returncode == 0
+    out = r.stdout.splitlines()
+    assert out[:5] == ["Skipping a block due to missing cuda", "after", "inside", "DISABLING hip",
+                       "Skipping a block due to missing hip"]
+    assert "never" not in out
+    assert out[5:8] == ["default", "floor", "lamp"]
+    assert "emissive   5 4 3" in out and "roughness  0.3" in out
+    err = r.stderr.splitlines()
+    assert err == ["command (with): Syntax error: expected one of gl, cuda, optix, hip",
+                   "command (material): No material selected",
+                   "command (material): No material called 'nope'",
+                   "command (material): Unknown subcommand"], err
+
+
 SCRIPT_HEAD = """platform hip
 generate {cfg}
 resolution {w} {h}
@@ -176,3 +203,34 @@ def test_unknown_tracer_and_stale_bvh(tmp_path, torch_cuda):
     err = r.stderr.splitlines()
     assert "command (raytracer): There is no such hip ray tracer variant" in err
     assert any(e.startswith("command (run): We have to have a scene loaded") for e in err)
+
+
+@pytest.mark.gpu
+def test_material_commands_light_an_obj_scene(tmp_path, torch_cuda):
+    """An OBJ scene whose lamp material has no emission in its library gets
+    one by "material select lamp" / "material emissive" before commit, as
+    RTGI's scripts light sibenik's windows; direct-wf in light mode then
+    equals the oracle over the edited material table, bit for bit."""
+    print("case: rtgi-hip direct-wf light mode on an OBJ floor under a lamp, material emissive set by script",
+          flush=True)
+    (tmp_path / "lib.mtl").write_text("newmtl floor\nKd 0.6 0.6 0.6\nnewmtl lamp\nKd 0.5 0.5 0.5\n")
+    (tmp_path / "s.obj").write_text(
+        "mtllib lib.mtl\nv -2 0 -2\nv 2 0 -2\nv 2 0 2\nv -2 0 2\nv -0.5 1 -0.5\nv 0.5 1 -0.5\nv -0.5 1 0.5\n"
+        "usemtl floor\nf 1 3 2\nf 1 4 3\nusemtl lamp\nf 5 6 7\n")
+    w, h = 64, 48
+    r = run_script("platform hip\nload s.obj\nat 0.1 0.7 0.05\nlook 0 -1 0\nup 0 0 1\nfovy 60\n"
+                   f"resolution {w} {h}\nraytracer default\nmaterial select lamp\nmaterial emissive 5 4 3\n"
+                   "add-scene-step build accel struct\ncommit\nsppx 2\nalgo direct-wf\nis light\n"
+                   "outfile d.pfm\nrun\n", tmp_path)
+    assert r.returncode == 0 and not r.stderr.strip(), r.stderr
+    img = read_pfm(tmp_path / "d.pfm")
+    sc = hiprt.HostScene.load_obj(tmp_path / "s.obj")
+    sc.set_material(2, emissive=[5, 4, 3])
+    pos, idx, mat = sc.arrays()
+    bvh = hiprt.Bvh(sc)
+    cam = np.array([0.1, 0.7, 0.05, 0, -1, 0, 0, 0, 1, 60, w, h], np.float32)
+    fb = orc.direct(pos, idx, mat, sc.materials(), bvh.nodes, bvh.root, bvh.index, cam, "light",
+                    np.array([1, 2], np.uint64), None)
+    want = fb[..., :3] / fb[..., 3:4]
+    assert want.sum() > 0
+    assert np.array_equal(img.view(np.uint32), want.view(np.uint32))

@@ -114,6 +114,26 @@ def test_obj_materials_from_a_library(tmp_path):
     assert np.allclose(table[0, :3], 0.8) and (table[0, 4:7] == 0).all()
 
 
+def test_material_names_and_edits(tmp_path):
+    """Material names as "material select" finds them: "default", then the
+    library's in definition order ("m<id>" for usemtl m<id> files); albedo and
+    emission can be set per material."""
+    (tmp_path / "lib.mtl").write_text("newmtl red\nKd 0.8 0.1 0.1\nnewmtl lamp\nKd 0.5 0.5 0.5\n")
+    (tmp_path / "m.obj").write_text("mtllib lib.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl lamp\nf 1 2 3\n")
+    sc = hiprt.HostScene.load_obj(tmp_path / "m.obj")
+    assert sc.material_names() == ["default", "red", "lamp"]
+    sc.set_material(2, emissive=[5, 4, 3])
+    sc.set_material(1, albedo=[0.25, 0.5, 0.75])
+    t = sc.materials()
+    assert t[2, 4:7].tolist() == [5, 4, 3] and t[1, :3].tolist() == [0.25, 0.5, 0.75]
+    assert np.allclose(t[2, :3], 0.5 ** 2.2)
+    with pytest.raises(hiprt.HiprtError, match="material id"):
+        sc.set_material(3, albedo=[1, 1, 1])
+    (tmp_path / "l.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl m1\nf 1 2 3\n")
+    assert hiprt.HostScene.load_obj(tmp_path / "l.obj").material_names() == ["m0", "m1"]
+    assert hiprt.HostScene.generate(1).material_names() == [""]
+
+
 def test_obj_winding_follows_shading_normals(tmp_path):
     """A triangle whose geometric normal points away from the mean of its
     corner normals has b and c swapped (libgi/scene.cpp:271-277); one whose
