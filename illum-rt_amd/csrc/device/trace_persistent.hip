@@ -263,7 +263,9 @@ void k_trace_persistent(scene_desc sc, ray_soa rays, hit_soa hits, uint8_t *occl
                 if (++dry >= n_chunks) exhausted = true;
             }
             if (!alive) {
-                uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                // idle lanes below this one (mbcnt: no 64-bit lane mask held in registers)
+                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 uint64_t idx64 = first + rank;
                 uint32_t idx = (uint32_t)idx64;
                 if (idx64 < hi) {
