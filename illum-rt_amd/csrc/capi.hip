@@ -187,6 +187,23 @@ int hiprt_scene_load_obj(const char *path, hiprt_host_scene *out) {
     return ok();
 }
 
+int hiprt_scene_add_obj(hiprt_host_scene *scene, const char *path, const char *name, const float *trafo) {
+    if (!scene || !path) return fail(HIPRT_ERR_ARG, "null argument");
+    rt::scene loaded;
+    std::string err;
+    if (!rt::read_obj(path, loaded, err, trafo)) return fail(HIPRT_ERR_IO, err);
+    const std::string prefix = name ? name : "";
+    if (*scene) {
+        rt::append_scene((*scene)->sc, loaded, prefix);
+        return ok();
+    }
+    auto *h = new (std::nothrow) hiprt_host_scene_s;
+    if (!h) return fail(HIPRT_ERR_ARG, "out of host memory");
+    rt::append_scene(h->sc, loaded, prefix);
+    *scene = h;
+    return ok();
+}
+
 int hiprt_scene_write_obj(hiprt_host_scene s, const char *path) {
     if (!s || !path) return fail(HIPRT_ERR_ARG, "null argument");
     if (!rt::write_obj(s->sc, path)) return fail(HIPRT_ERR_IO, std::string("cannot write ") + path);

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <array>
 #include <map>
 #include <tuple>
 #include <sstream>
@@ -254,62 +255,230 @@ namespace {
 
 float luma(vec3f c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }
 
-// Material of an MTL "newmtl" block as the reference's scene::add derives it
-// (libgi/scene.cpp:317-332): albedo = Kd if its luma exceeds 1e-4, else Ks,
-// raised to the power 2.2; emission = Ke.
+// An MTL material as assimp's OBJ importer holds it (diffuse defaults to 0.6
+// grey, specular and emission to black), resolved as the reference's
+// scene::add derives its material (libgi/scene.cpp:317-332): albedo = Kd if
+// its luma exceeds 1e-4, else Ks, raised to the power 2.2; emission = Ke.
 struct mtl_entry {
-    vec3f kd{0, 0, 0}, ks{0, 0, 0}, ke{0, 0, 0};
+    vec3f kd{0.6f, 0.6f, 0.6f}, ks{0, 0, 0}, ke{0, 0, 0};
     material resolve() const {
         vec3f a = luma(kd) > 1e-4f ? kd : ks;
         return material{vec3f{std::pow(a.x, 2.2f), std::pow(a.y, 2.2f), std::pow(a.z, 2.2f)}, ke};
     }
 };
 
-bool read_mtl(const std::string &path, std::vector<std::string> &names, std::vector<mtl_entry> &mats,
-              std::string &err) {
-    std::ifstream in(path);
-    if (!in) {
-        err = "cannot open material library " + path;
-        return false;
+// The importer's material list: "DefaultMaterial" first, then library
+// materials in definition order, then names "usemtl" uses without a
+// definition, in first-use order (each with the default values).
+struct mtl_list {
+    std::vector<std::string> names{"DefaultMaterial"};
+    std::vector<mtl_entry> mats{mtl_entry{}};
+    int32_t find(const std::string &n) const {
+        for (size_t k = 0; k < names.size(); ++k)
+            if (names[k] == n) return (int32_t)k;
+        return -1;
     }
+    int32_t find_or_add(const std::string &n) {
+        int32_t k = find(n);
+        if (k >= 0) return k;
+        names.push_back(n);
+        mats.emplace_back();
+        return (int32_t)names.size() - 1;
+    }
+};
+
+std::string trimmed(const std::string &s) {
+    size_t a = s.find_first_not_of(" \t\r\f\v"), b = s.find_last_not_of(" \t\r\f\v");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+// What follows the keyword on a line, trimmed.
+std::string rest_of(std::istringstream &ls) {
+    std::string r;
+    std::getline(ls, r);
+    return trimmed(r);
+}
+
+// "newmtl" (the rest of the line names the material; a bare "newmtl" is the
+// default material) opens a material, or re-opens one of that name; "Kd",
+// "Ks", "Ke" set its colours (one value sets red only, as assimp reads it).
+bool read_mtl(std::istream &in, mtl_list &lib) {
     std::string line;
+    int32_t cur = -1;
     while (std::getline(in, line)) {
         std::istringstream ls(line);
         std::string tag;
         if (!(ls >> tag)) continue;
         if (tag == "newmtl") {
-            std::string name;
-            ls >> name;
-            names.push_back(name);
-            mats.emplace_back();
-        } else if (!mats.empty() && (tag == "Kd" || tag == "Ks" || tag == "Ke")) {
+            std::string name = rest_of(ls);
+            cur = lib.find_or_add(name.empty() ? "DefaultMaterial" : name);
+        } else if (cur >= 0 && (tag == "Kd" || tag == "Ks" || tag == "Ke")) {
             vec3f c{0, 0, 0};
             ls >> c.x >> c.y >> c.z;
-            (tag == "Kd" ? mats.back().kd : tag == "Ks" ? mats.back().ks : mats.back().ke) = c;
+            mtl_entry &e = lib.mats[cur];
+            (tag == "Kd" ? e.kd : tag == "Ks" ? e.ks : e.ke) = c;
         }
     }
     return true;
 }
 
+// The object and mesh bookkeeping of assimp's OBJ parser, which decides the
+// triangle order and materials the reference's loader sees: it walks the
+// node tree, one node per object in creation order, each node's meshes in
+// creation order, each mesh's faces in file order (libgi/scene.cpp:222-289).
+//   o name   re-opens the object of that name (the current mesh stays
+//            current) or creates one;
+//   g name   a group name other than the active one creates an object;
+//   usemtl   switches material: a new mesh when the current one already has
+//            faces and a different material; a mesh without a material yet
+//            takes this one (faces already in it included);
+//   f        creates an object ("defaultobject") and mesh when none is open.
+// Every new object opens a mesh with the current material.
+struct obj_corner {
+    long v, t, n;   // 0-based indices; t, n = -1 for none
+};
+struct obj_face {
+    uint32_t first, n;   // corners [first, first + n)
+};
+struct obj_mesh {
+    int32_t mat = -1;   // -1: none yet (the importer then uses material 0)
+    std::vector<obj_face> faces;
+};
+struct obj_object {
+    std::string name;
+    std::vector<uint32_t> meshes;
+};
+struct obj_model {
+    std::vector<obj_object> objects;
+    std::vector<obj_mesh> meshes;
+    int32_t cur_object = -1, cur_mesh = -1, cur_mat = -1;
+    std::string active_group;
+    void create_mesh() {
+        meshes.emplace_back();
+        cur_mesh = (int32_t)meshes.size() - 1;
+        if (cur_object >= 0) objects[(size_t)cur_object].meshes.push_back((uint32_t)cur_mesh);
+    }
+    void create_object(const std::string &name) {
+        objects.push_back(obj_object{name, {}});
+        cur_object = (int32_t)objects.size() - 1;
+        create_mesh();
+        if (cur_mat >= 0) meshes[(size_t)cur_mesh].mat = cur_mat;
+    }
+    void use_material(const std::string &name, mtl_list &lib) {
+        if (name.empty() || (cur_mat >= 0 && lib.names[(size_t)cur_mat] == name)) return;
+        cur_mat = lib.find_or_add(name);
+        if (cur_mesh < 0) {
+            create_mesh();
+        } else {
+            const obj_mesh &m = meshes[(size_t)cur_mesh];
+            if (m.mat >= 0 && m.mat != cur_mat && !m.faces.empty()) create_mesh();
+        }
+        meshes[(size_t)cur_mesh].mat = cur_mat;
+    }
+    void open_object(const std::string &name) {
+        for (size_t k = 0; k < objects.size(); ++k)
+            if (objects[k].name == name) {
+                cur_object = (int32_t)k;
+                return;
+            }
+        create_object(name);
+    }
+    void open_group(const std::string &name) {
+        if (name == active_group) return;
+        create_object(name);
+        active_group = name;
+    }
+    void add_face(obj_face f) {
+        if (cur_object < 0) create_object("defaultobject");
+        if (cur_mesh < 0) create_mesh();
+        meshes[(size_t)cur_mesh].faces.push_back(f);
+    }
+};
+
+// float arithmetic of a 4x4 column-major matrix (m[4 * column + row]) and
+// 3-vectors, summed in the order the reference's matrix library sums.
+vec3f xform_point(const float *m, vec3f p) {
+    float r[3];
+    for (int i = 0; i < 3; ++i) r[i] = (m[i] * p.x + m[4 + i] * p.y) + (m[8 + i] * p.z + m[12 + i] * 1.0f);
+    return vec3f{r[0], r[1], r[2]};
+}
+// transpose(inverse(upper 3x3 of m)), as nm[3 * column + row]: the adjugate
+// over the determinant.
+void normal_matrix(const float *m, float *nm) {
+    auto a = [m](int c, int r) { return m[4 * c + r]; };
+    const float c00 = a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2);
+    const float c01 = a(0, 1) * a(2, 2) - a(2, 1) * a(0, 2);
+    const float c02 = a(0, 1) * a(1, 2) - a(1, 1) * a(0, 2);
+    const float inv_det = 1.0f / (a(0, 0) * c00 - a(1, 0) * c01 + a(2, 0) * c02);
+    float inv[9];   // inverse, inv[3 * column + row]
+    inv[0] = c00 * inv_det;
+    inv[3] = -(a(1, 0) * a(2, 2) - a(2, 0) * a(1, 2)) * inv_det;
+    inv[6] = (a(1, 0) * a(2, 1) - a(2, 0) * a(1, 1)) * inv_det;
+    inv[1] = -c01 * inv_det;
+    inv[4] = (a(0, 0) * a(2, 2) - a(2, 0) * a(0, 2)) * inv_det;
+    inv[7] = -(a(0, 0) * a(2, 1) - a(2, 0) * a(0, 1)) * inv_det;
+    inv[2] = c02 * inv_det;
+    inv[5] = -(a(0, 0) * a(1, 2) - a(1, 0) * a(0, 2)) * inv_det;
+    inv[8] = (a(0, 0) * a(1, 1) - a(1, 0) * a(0, 1)) * inv_det;
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) nm[3 * c + r] = inv[3 * r + c];
+}
+vec3f xform_normal(const float *nm, vec3f n) {
+    vec3f t{nm[0] * n.x + nm[3] * n.y + nm[6] * n.z, nm[1] * n.x + nm[4] * n.y + nm[7] * n.z,
+            nm[2] * n.x + nm[5] * n.y + nm[8] * n.z};
+    const float d = (t.x * t.x + t.y * t.y) + t.z * t.z;
+    const float s = 1.0f / std::sqrt(d);   // a zero vector gives NaN, as the reference's normalize does
+    return vec3f{t.x * s, t.y * s, t.z * s};
+}
+vec3f cross_ref(vec3f x, vec3f y) {
+    return vec3f{x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
+}
+float dot_ref(vec3f a, vec3f b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+
+// The flat normal assimp generates for a face without normals: the cross
+// product of its first two edges, normalized unless it is zero.
+vec3f face_normal(vec3f p0, vec3f p1, vec3f p2) {
+    vec3f e1 = p1 - p0, e2 = p2 - p0;
+    vec3f c{e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+    const float len = std::sqrt(c.x * c.x + c.y * c.y + c.z * c.z);
+    if (len > 0.0f) c = vec3f{c.x / len, c.y / len, c.z / len};
+    return c;
+}
+
+// assimp's split of a quad: fanned from its concave corner (the corner whose
+// two angles to the opposite diagonal sum past pi), else from corner 0.
+int quad_start(const vec3f *q) {
+    auto unit = [](vec3f v) {
+        const float len = std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        if (len == 0.0f) return v;
+        const float inv = 1.0f / len;
+        return vec3f{v.x * inv, v.y * inv, v.z * inv};
+    };
+    auto dot3 = [](vec3f a, vec3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+    for (int i = 0; i < 4; ++i) {
+        const vec3f v = q[i];
+        const vec3f left = unit(q[(i + 3) % 4] - v), diag = unit(q[(i + 2) % 4] - v),
+                    right = unit(q[(i + 1) % 4] - v);
+        const float angle = std::acos(dot3(left, diag)) + std::acos(dot3(right, diag));
+        if (angle > 3.14159265358979323846f) return i;
+    }
+    return 0;
+}
+
 } // namespace
 
-bool read_obj(const std::string &path, scene &out, std::string &err) {
+bool read_obj(const std::string &path, scene &out, std::string &err, const float *trafo) {
     std::ifstream in(path);
     if (!in) {
         err = "cannot open " + path;
         return false;
     }
     out = scene{};
-    std::vector<vec3f> normals;
-    std::vector<float> uvs;                 // "vt" entries, (u, v) pairs
-    std::vector<long> corner_vt;            // per triangle corner: vt index, -1 none
-    std::vector<long> corner_vn;            // per triangle corner: vn index, -1 none
-    std::vector<std::string> mtl_names;
-    std::vector<mtl_entry> mtl;
-    bool legacy = false;        // "usemtl m<id>" without a material library (write_obj's files)
-    uint32_t cur_mat = 0;       // index into out.mats once resolved; see below
-    int32_t cur_named = -1;     // current MTL material, -1 = none
-    std::vector<int32_t> tri_named;
+    std::vector<vec3f> pos, normals;
+    std::vector<float> uvs;   // "vt" entries, (u, v) pairs
+    std::vector<obj_corner> corners;
+    mtl_list lib;
+    obj_model model;
     std::string dir;
     if (size_t slash = path.find_last_of('/'); slash != std::string::npos) dir = path.substr(0, slash + 1);
     std::string line;
@@ -326,7 +495,7 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
                 err = where() + "malformed " + tag;
                 return false;
             }
-            (tag == "v" ? out.pos : normals).push_back(vec3f{x, y, z});
+            (tag == "v" ? pos : normals).push_back(vec3f{x, y, z});
         } else if (tag == "vt") {
             float u, v = 0.0f;
             if (!(ls >> u)) {
@@ -337,7 +506,7 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
             uvs.push_back(u);
             uvs.push_back(v);
         } else if (tag == "f") {
-            std::vector<long> ids, nids, tids;
+            const uint32_t first = (uint32_t)corners.size();
             std::string tok;
             while (ls >> tok) {
                 // "a", "a/b", "a//c", "a/b/c" (1-based, negative = relative)
@@ -348,7 +517,7 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
                     if (s2 != std::string::npos && s2 + 1 < tok.size()) n = std::strtol(tok.c_str() + s2 + 1, nullptr, 10);
                     if (s1 + 1 < tok.size() && tok[s1 + 1] != '/') tx = std::strtol(tok.c_str() + s1 + 1, nullptr, 10);
                 }
-                if (v < 0) v = (long)out.pos.size() + v + 1;
+                if (v < 0) v = (long)pos.size() + v + 1;
                 if (n < 0) n = (long)normals.size() + n + 1;
                 if (uvs.empty()) tx = 0;   // a file without "vt" lines: corner coordinate indices are ignored
                 if (tx < 0) tx = (long)(uvs.size() / 2) + tx + 1;
@@ -356,8 +525,7 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
                     err = where() + "texture coordinate index out of range";
                     return false;
                 }
-                tids.push_back(tx - 1);   // -1: no texture coordinate
-                if (v <= 0 || (size_t)v > out.pos.size()) {
+                if (v <= 0 || (size_t)v > pos.size()) {
                     err = where() + "face index out of range";
                     return false;
                 }
@@ -365,92 +533,112 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
                     err = where() + "normal index out of range";
                     return false;
                 }
-                ids.push_back(v - 1);
-                nids.push_back(n - 1);   // -1: no normal
+                corners.push_back(obj_corner{v - 1, tx - 1, n - 1});
             }
-            // fan triangulation; winding follows the shading normals when
-            // every corner has one (libgi/scene.cpp:264-277)
-            for (size_t k = 2; k < ids.size(); ++k) {
-                uint32_t a = (uint32_t)ids[0], b = (uint32_t)ids[k - 1], c = (uint32_t)ids[k];
-                long ta = tids[0], tb = tids[k - 1], tcn = tids[k];
-                long na = nids[0], nb = nids[k - 1], nc = nids[k];
-                if (na >= 0 && nb >= 0 && nc >= 0) {
-                    vec3f pa = out.pos[a], pb = out.pos[b], pc = out.pos[c];
-                    vec3f g = cross(pb - pa, pc - pa);
-                    vec3f m = (normals[na] + normals[nb] + normals[nc]) * 0.333f;
-                    if (!(dot(g, m) > 0)) {
-                        std::swap(b, c);
-                        std::swap(tb, tcn);
-                        std::swap(nb, nc);
-                    }
-                }
-                out.idx.push_back(a);
-                out.idx.push_back(b);
-                out.idx.push_back(c);
-                corner_vt.push_back(ta);
-                corner_vt.push_back(tb);
-                corner_vt.push_back(tcn);
-                corner_vn.push_back(na);
-                corner_vn.push_back(nb);
-                corner_vn.push_back(nc);
-                out.mat.push_back(cur_mat);
-                tri_named.push_back(cur_named);
-            }
+            if (corners.size() > first) model.add_face(obj_face{first, (uint32_t)corners.size() - first});
         } else if (tag == "mtllib") {
-            std::string name;
-            while (ls >> name)
-                if (!read_mtl(dir + name, mtl_names, mtl, err)) return false;
+            // the rest of the line names one library, beside the OBJ file; a
+            // missing one falls back to the OBJ's name with "mtl" for "obj",
+            // and without that the file loads with no library
+            std::string name = rest_of(ls);
+            if (name.empty()) continue;
+            std::ifstream lf(dir + name);
+            if (!lf) lf = std::ifstream(path.substr(0, path.size() >= 3 ? path.size() - 3 : 0) + "mtl");
+            if (lf) read_mtl(lf, lib);
         } else if (tag == "usemtl") {
+            model.use_material(rest_of(ls), lib);
+        } else if (tag == "o") {
             std::string name;
-            ls >> name;
-            cur_named = -1;
-            for (size_t k = 0; k < mtl_names.size(); ++k)
-                if (mtl_names[k] == name) cur_named = (int32_t)k;
-            if (cur_named < 0 && name.size() > 1 && name[0] == 'm' &&
-                name.find_first_not_of("0123456789", 1) == std::string::npos) {
-                legacy = true;
-                cur_mat = (uint32_t)std::strtoul(name.c_str() + 1, nullptr, 10);
-            } else {
-                cur_mat = 0;
-            }
+            if (ls >> name) model.open_object(name);
+        } else if (tag == "g") {
+            model.open_group(rest_of(ls));
         }
     }
+
+    float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const float *m = trafo ? trafo : ident;
+    float nm[9];
+    normal_matrix(m, nm);
+    std::vector<vec3f> tpos(pos.size()), tnorm(normals.size());
+    for (size_t i = 0; i < pos.size(); ++i) tpos[i] = xform_point(m, pos[i]);
+    for (size_t i = 0; i < normals.size(); ++i) tnorm[i] = xform_normal(nm, normals[i]);
     bool with_vn = false;
-    for (long n : corner_vn) with_vn = with_vn || n >= 0;
+    for (const obj_corner &c : corners) with_vn = with_vn || c.n >= 0;
+
+    // Triangles in node order; per triangle corner: (v, vt, normal key), the
+    // key a "vn" index or -2 - t for triangle t's own (face) normal.
+    std::vector<obj_corner> tri_corners;
+    std::vector<vec3f> own_normal;   // per triangle, its transformed face normal
+    for (const obj_object &o : model.objects)
+        for (uint32_t mid : o.meshes) {
+            const obj_mesh &mesh = model.meshes[mid];
+            const uint32_t mat = mesh.mat < 0 ? 0u : (uint32_t)mesh.mat;
+            for (const obj_face &f : mesh.faces) {
+                const obj_corner *c = &corners[f.first];
+                // quads as assimp splits them, larger polygons as a fan
+                // (assimp clips ears there: same triangles for convex
+                // polygons only up to their order)
+                std::vector<std::array<uint32_t, 3>> tris;
+                if (f.n == 4) {
+                    const vec3f q[4] = {pos[(size_t)c[0].v], pos[(size_t)c[1].v], pos[(size_t)c[2].v],
+                                        pos[(size_t)c[3].v]};
+                    const uint32_t s = (uint32_t)quad_start(q);
+                    tris.push_back({s, (s + 1) % 4, (s + 2) % 4});
+                    tris.push_back({s, (s + 2) % 4, (s + 3) % 4});
+                } else {
+                    for (uint32_t k = 2; k < f.n; ++k) tris.push_back({0, k - 1, k});
+                }
+                auto emit = [&](uint32_t i0, uint32_t i1, uint32_t i2) {
+                    obj_corner a = c[i0], b = c[i1], cc = c[i2];
+                    const vec3f fn = face_normal(pos[(size_t)a.v], pos[(size_t)b.v], pos[(size_t)cc.v]);
+                    const vec3f own = xform_normal(nm, fn);
+                    auto nrm = [&](const obj_corner &k) { return k.n >= 0 ? tnorm[(size_t)k.n] : own; };
+                    const vec3f mean = (nrm(a) + nrm(b) + nrm(cc)) * 0.333f;
+                    const vec3f pa = tpos[(size_t)a.v];
+                    const vec3f g = cross_ref(tpos[(size_t)b.v] - pa, tpos[(size_t)cc.v] - pa);
+                    if (!(dot_ref(g, mean) > 0.0f)) std::swap(b, cc);   // libgi/scene.cpp:271-277
+                    const long t = (long)own_normal.size();
+                    for (obj_corner k : {a, b, cc}) {
+                        if (k.n < 0) k.n = -2 - t;
+                        tri_corners.push_back(k);
+                    }
+                    own_normal.push_back(own);
+                    out.mat.push_back(mat);
+                };
+                for (const auto &t3 : tris) emit(t3[0], t3[1], t3[2]);
+            }
+        }
+
+    out.pos = tpos;
+    out.idx.resize(tri_corners.size());
+    for (size_t k = 0; k < tri_corners.size(); ++k) out.idx[k] = (uint32_t)tri_corners[k].v;
     if (!uvs.empty() || with_vn) {
         // Texture coordinates and normals belong to face corners (assimp
         // gives every corner its own vertex, libgi/scene.cpp:251-262): each
-        // distinct (position, vt, vn) corner becomes a vertex of its own
+        // distinct (position, vt, normal) corner becomes a vertex of its own
         // after the "v" vertices, which keep (0, 0) and no normal.  With
-        // normals in the file, a corner without one gets its face's normal,
-        // on a vertex of its own.
+        // normals in the file, a corner without one gets its triangle's flat
+        // normal, on a vertex of its own.
         const size_t nv = out.pos.size();
         if (!uvs.empty()) out.tc.assign(2 * nv, 0.0f);
         if (with_vn) out.norm.assign(3 * nv, 0.0f);
-        std::map<std::tuple<uint32_t, long, long>, uint32_t> split;
-        for (size_t k = 0; k < out.idx.size(); ++k) {
-            long t = uvs.empty() ? -1 : corner_vt[k];
-            long n = with_vn ? corner_vn[k] : -1;
+        std::map<std::tuple<long, long, long>, uint32_t> split;
+        for (size_t k = 0; k < tri_corners.size(); ++k) {
+            const obj_corner &c = tri_corners[k];
+            const long t = uvs.empty() ? -1 : c.t;
+            const long n = with_vn ? c.n : -1;
             if (t < 0 && !with_vn) continue;
-            if (with_vn && n < 0) n = -2 - (long)(k / 3);   // this face's own normal
-            auto key = std::make_tuple(out.idx[k], t, n);
+            auto key = std::make_tuple(c.v, t, n);
             auto it = split.find(key);
             if (it == split.end()) {
-                uint32_t id = (uint32_t)out.pos.size();
-                out.pos.push_back(out.pos[out.idx[k]]);
+                const uint32_t id = (uint32_t)out.pos.size();
+                out.pos.push_back(tpos[(size_t)c.v]);
                 if (!uvs.empty()) {
-                    out.tc.push_back(t >= 0 ? uvs[2 * t] : 0.0f);
-                    out.tc.push_back(t >= 0 ? uvs[2 * t + 1] : 0.0f);
+                    out.tc.push_back(t >= 0 ? uvs[2 * (size_t)t] : 0.0f);
+                    out.tc.push_back(t >= 0 ? uvs[2 * (size_t)t + 1] : 0.0f);
                 }
                 if (with_vn) {
-                    vec3f nn;
-                    if (n >= 0) {
-                        nn = normals[n];
-                    } else {
-                        const size_t f = 3 * (k / 3);
-                        vec3f pa = out.pos[out.idx[f]], pb = out.pos[out.idx[f + 1]], pc = out.pos[out.idx[f + 2]];
-                        nn = normalize(cross(pb - pa, pc - pa));
-                    }
+                    const vec3f nn = n >= 0 ? tnorm[(size_t)n] : own_normal[(size_t)(-2 - n)];
                     out.norm.push_back(nn.x);
                     out.norm.push_back(nn.y);
                     out.norm.push_back(nn.z);
@@ -460,21 +648,81 @@ bool read_obj(const std::string &path, scene &out, std::string &err) {
             out.idx[k] = it->second;
         }
     }
-    if (!mtl.empty()) {
-        // material 0: faces without a known material (default albedo 0.8);
-        // 1 + k: the k-th material of the libraries, in definition order
-        out.mats.assign(1, material{v3(0.8f), v3(0.0f)});
-        for (const mtl_entry &e : mtl) out.mats.push_back(e.resolve());
-        out.mat_names.assign(1, "default");
-        out.mat_names.insert(out.mat_names.end(), mtl_names.begin(), mtl_names.end());
-        for (size_t t = 0; t < out.mat.size(); ++t) out.mat[t] = tri_named[t] >= 0 ? (uint32_t)tri_named[t] + 1 : 0u;
-        return true;
+    out.mats.clear();
+    for (const mtl_entry &e : lib.mats) out.mats.push_back(e.resolve());
+    out.mat_names = lib.names;
+    return true;
+}
+
+namespace {
+
+// Gives every triangle corner a vertex of its own carrying the triangle's
+// flat normal (and the corner's texture coordinate), for a scene without
+// shading normals that is merged with one that has them.
+void give_flat_normals(scene &s) {
+    const uint32_t nt = s.n_tris();
+    std::vector<vec3f> pos(3 * (size_t)nt);
+    std::vector<float> tc(s.tc.empty() ? 0 : 6 * (size_t)nt), norm(9 * (size_t)nt);
+    std::vector<uint32_t> idx(3 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        const vec3f n = face_normal(s.a(t), s.b(t), s.c(t));
+        for (int k = 0; k < 3; ++k) {
+            const size_t j = 3 * (size_t)t + k, v = s.idx[j];
+            pos[j] = s.pos[v];
+            if (!tc.empty()) {
+                tc[2 * j] = s.tc[2 * v];
+                tc[2 * j + 1] = s.tc[2 * v + 1];
+            }
+            norm[3 * j] = n.x;
+            norm[3 * j + 1] = n.y;
+            norm[3 * j + 2] = n.z;
+            idx[j] = (uint32_t)j;
+        }
     }
-    uint32_t max_mat = 0;
-    for (uint32_t m : out.mat) max_mat = m > max_mat ? m : max_mat;
-    out.mats.assign(max_mat + 1, material{v3(0.8f), v3(0.0f)});
-    out.mat_names.clear();
-    for (uint32_t m = 0; m <= max_mat; ++m) out.mat_names.push_back(legacy ? "m" + std::to_string(m) : "default");
+    s.pos = std::move(pos);
+    s.tc = std::move(tc);
+    s.norm = std::move(norm);
+    s.idx = std::move(idx);
+}
+
+} // namespace
+
+void append_scene(scene &dst, const scene &src, const std::string &prefix) {
+    scene add = src;
+    if (dst.norm.empty() != add.norm.empty()) {
+        // the side without shading normals gets the flat ones the
+        // reference's loader generates (zeros for vertices of no triangle)
+        scene &lacking = dst.norm.empty() ? dst : add;
+        if (lacking.n_tris() > 0) give_flat_normals(lacking);
+        else lacking.norm.assign(3 * lacking.pos.size(), 0.0f);
+    }
+    const uint32_t base_v = (uint32_t)dst.pos.size(), base_m = (uint32_t)dst.mats.size();
+    const int32_t base_tex = (int32_t)dst.texs.size();
+    if (!add.tc.empty() && dst.tc.empty()) dst.tc.assign(2 * dst.pos.size(), 0.0f);
+    if (add.tc.empty() && !dst.tc.empty()) add.tc.assign(2 * add.pos.size(), 0.0f);
+    dst.pos.insert(dst.pos.end(), add.pos.begin(), add.pos.end());
+    dst.tc.insert(dst.tc.end(), add.tc.begin(), add.tc.end());
+    dst.norm.insert(dst.norm.end(), add.norm.begin(), add.norm.end());
+    for (uint32_t v : add.idx) dst.idx.push_back(v + base_v);
+    for (uint32_t t = 0; t < add.n_tris(); ++t) dst.mat.push_back((add.mat.empty() ? 0u : add.mat[t]) + base_m);
+    const bool named = !dst.mat_names.empty() || !add.mat_names.empty();
+    if (named) dst.mat_names.resize(dst.mats.size());
+    for (size_t k = 0; k < add.mats.size(); ++k) {
+        material m = add.mats[k];
+        if (m.tex >= 0) m.tex += base_tex;
+        dst.mats.push_back(m);
+        if (named) {
+            std::string n = k < add.mat_names.size() ? add.mat_names[k] : std::string();
+            dst.mat_names.push_back(prefix.empty() ? n : prefix + "/" + n);
+        }
+    }
+    dst.texs.insert(dst.texs.end(), add.texs.begin(), add.texs.end());
+}
+
+bool add_obj(scene &dst, const std::string &path, const std::string &name, const float *trafo, std::string &err) {
+    scene s;
+    if (!read_obj(path, s, err, trafo)) return false;
+    append_scene(dst, s, name);
     return true;
 }
 

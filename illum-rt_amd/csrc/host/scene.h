@@ -85,20 +85,39 @@ uint32_t hall_debris_for_config3();
 
 // OBJ: write_obj writes "v x y z" with %.9g (round-trips every float) and
 // "f a b c" (1-based), one object, no normals, materials as "usemtl m<id>"
-// groups in triangle order.  read_obj keeps face order (the triangle order of
-// RTGI's assimp-based loader for files whose meshes follow the file), reads
-// v, vn and f ("a", "a/b", "a//c", "a/b/c", negative = relative), fan-
-// triangulates polygons, swaps b and c of a triangle whose geometric normal
-// disagrees with the mean of its corners' normals (libgi/scene.cpp:271-277),
-// and takes materials from "mtllib" libraries (newmtl/Kd/Ks/Ke, resolved as
-// libgi/scene.cpp:317-332 does: material 0 is the default, 1 + k the k-th
-// library material) or, without a library, from "usemtl m<id>" names.
-// Texture coordinates and shading normals belong to face corners, as assimp
-// gives every corner its own vertex: each distinct (v, vt, vn) corner becomes
-// a vertex of its own after the file's "v" vertices.  In a file with normals,
-// a corner without one gets its face's normal (the flat normal assimp
-// generates for meshes without normals).
+// groups in triangle order.
+//
+// read_obj loads a file as the reference's scene::add does through assimp's
+// OBJ importer (libgi/scene.cpp:292-356, 222-289):
+//  * triangle order and materials follow the importer's objects and meshes
+//    ("o" re-opens an object, "g" opens one, "usemtl" splits meshes), which
+//    the reference walks object by object, mesh by mesh;
+//  * materials: 0 "DefaultMaterial", then each "mtllib" library's in
+//    definition order, then names "usemtl" uses without a definition in
+//    first-use order; Kd defaults to 0.6 grey, and albedo = Kd^2.2 (Ks^2.2
+//    when Kd's luma is ~0), emission = Ke (libgi/scene.cpp:317-332);
+//  * v, vt, vn, f ("a", "a/b", "a//c", "a/b/c", negative = relative); quads
+//    split from their concave corner, larger polygons fanned;
+//  * `trafo` (a column-major 4x4 model matrix, null for identity) moves the
+//    positions, and its inverse transpose the normals (normalized), before a
+//    triangle whose geometric normal disagrees with the mean of its corners'
+//    normals has b and c swapped (libgi/scene.cpp:264-277);
+//  * texture coordinates and shading normals belong to face corners, as
+//    assimp gives every corner its own vertex: each distinct (v, vt, vn)
+//    corner becomes a vertex of its own after the file's "v" vertices.  In a
+//    file with normals, a corner without one gets its triangle's flat normal
+//    (the normal assimp generates for meshes without normals).
 bool write_obj(const scene &s, const std::string &path);
-bool read_obj(const std::string &path, scene &out, std::string &err);
+bool read_obj(const std::string &path, scene &out, std::string &err, const float *trafo = nullptr);
+
+// Appends src to dst as scene::add appends a model: vertices, triangles and
+// materials after dst's (material ids and texture indices offset), material
+// names prefixed "<prefix>/" when prefix is non-empty.  When only one side
+// has shading normals, the other gets flat ones (every corner a vertex of
+// its own); texture coordinates missing on one side are (0, 0).
+void append_scene(scene &dst, const scene &src, const std::string &prefix);
+
+// read_obj(path, trafo) appended to dst with name as the prefix.
+bool add_obj(scene &dst, const std::string &path, const std::string &name, const float *trafo, std::string &err);
 
 } // namespace rt

@@ -14,10 +14,12 @@
 // "command (<name>): <message>" and the script continues, as in the
 // reference; "run" checks the same preconditions.
 //
-// Commands: at, look, up, fovy, resolution, sppx, load, generate, pointlight,
-// material, default-brdf, platform, raytracer, tracer-option, bvh, triangles,
-// add-scene-step, commit, algo, run, outfile, stats, echo, disable, with / },
-// quit, and the algorithm's own (direct-wf: is uniform|cosine|light).
+// Commands: at, look, up, fovy, resolution, sppx, load (appends a model under
+// the model matrix), modelmatrix, modeltrafo, modelpath, generate,
+// pointlight, material, default-brdf, platform, raytracer, tracer-option,
+// bvh, triangles, add-scene-step, commit, algo, run, outfile, stats, echo,
+// disable, with / }, quit, and the algorithm's own (direct-wf: is
+// uniform|cosine|light).
 // Materials are Lambertian: "material roughness|ior|brdf" and "default-brdf"
 // are accepted and kept for "material show" but do not change the shading.
 // "with gl|cuda|optix {" blocks are skipped (those platforms are not built
@@ -28,6 +30,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <map>
@@ -59,6 +62,53 @@ struct brdf_params {
 };
 map<int, brdf_params> mat_params;
 string default_brdf = "lambert";
+// model matrix for "load" (column-major, m[4 * column + row]) and the model
+// search paths ("" first: the working directory), driver/interaction.cpp:203
+// and libgi/scene.h:164-170
+float modelmatrix[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+vector<string> modelpaths{""};
+
+// Column arithmetic of the reference's matrix library for "modeltrafo":
+// translate puts m * (v, 1) into the last column, scale scales the first
+// three columns, rotate multiplies by the rotation of `angle` about the unit
+// axis (Rodrigues' formula; the angle is taken as radians, as the library
+// takes it).
+void mat_translate(float *m, const float *v) {
+    for (int r = 0; r < 4; ++r) m[12 + r] = ((m[r] * v[0] + m[4 + r] * v[1]) + m[8 + r] * v[2]) + m[12 + r];
+}
+void mat_scale(float *m, const float *v) {
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 4; ++r) m[4 * c + r] *= v[c];
+}
+void mat_rotate(float *m, float angle, const float *axis) {
+    const float c = cos(angle), s = sin(angle);
+    const float t[3] = {(1.0f - c) * axis[0], (1.0f - c) * axis[1], (1.0f - c) * axis[2]};
+    float R[3][3];   // R[column][row]
+    R[0][0] = c + t[0] * axis[0];
+    R[0][1] = t[0] * axis[1] + s * axis[2];
+    R[0][2] = t[0] * axis[2] - s * axis[1];
+    R[1][0] = t[1] * axis[0] - s * axis[2];
+    R[1][1] = c + t[1] * axis[1];
+    R[1][2] = t[1] * axis[2] + s * axis[0];
+    R[2][0] = t[2] * axis[0] + s * axis[1];
+    R[2][1] = t[2] * axis[1] - s * axis[0];
+    R[2][2] = c + t[2] * axis[2];
+    float out[12];
+    for (int k = 0; k < 3; ++k)
+        for (int r = 0; r < 4; ++r) out[4 * k + r] = (m[r] * R[k][0] + m[4 + r] * R[k][1]) + m[8 + r] * R[k][2];
+    for (int i = 0; i < 12; ++i) m[i] = out[i];
+}
+
+// scene::find_model (libgi/scene.cpp:66-73): a relative path is looked up
+// under each model path in order, else taken as given.
+string find_model(const string &path) {
+    if (!path.empty() && path[0] != '/')
+        for (const string &p : modelpaths) {
+            string cand = p.empty() ? path : (p.back() == '/' ? p + path : p + "/" + path);
+            if (ifstream(cand).good()) return cand;
+        }
+    return path;
+}
 
 string material_name(hiprt_host_scene s, uint32_t m) {
     char buf[256];
@@ -240,16 +290,62 @@ void eval(const string &line) {
             check_in_complete("Syntax error, requires exactly one positive integral value");
             rc->sppx = s;
         } else if (command == "load") {
-            string file;
+            // interaction.cpp:395-402: the model joins the scene under the
+            // current model matrix, its materials named "<name>/..." when a
+            // name is given
+            string file, name;
             in >> file;
-            if (in.fail()) error("Syntax error, requires a file name (no spaces, sorry) and (optionally) a name");
-            hiprt_host_scene s = nullptr;
-            if (hiprt_scene_load_obj(file.c_str(), &s) != HIPRT_OK) error(hiprt_last_error());
-            if (rc->scene.geometry) hiprt_scene_free(rc->scene.geometry);
-            rc->scene.geometry = s;
-            selected_mat = -1;
-            mat_params.clear();
+            if (!in.eof()) in >> name;
+            check_in_complete("Syntax error, requires a file name (no spaces, sorry) and (optionally) a name");
+            string path = find_model(file);
+            if (hiprt_scene_add_obj(&rc->scene.geometry, path.c_str(), name.empty() ? nullptr : name.c_str(),
+                                    modelmatrix) != HIPRT_OK)
+                error(hiprt_last_error());
             uc.scene_touched_at = uc.cmdid;
+        } else if (command == "modelmatrix") {
+            // interaction.cpp:357-363: 16 values, column by column
+            for (float &v : modelmatrix) in >> v;
+            check_in_complete("Syntax error in model matrix");
+        } else if (command == "modeltrafo") {
+            // interaction.cpp:364-393
+            string trafo;
+            in >> trafo;
+            if (trafo == "shift" || trafo == "translate") {
+                float v[3];
+                in >> v[0] >> v[1] >> v[2];
+                check_in_complete("Too many components in shift transformation");
+                mat_translate(modelmatrix, v);
+            } else if (trafo == "rotate") {
+                const float axes[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+                string a;
+                in >> a;
+                int aid = 0;
+                if (a == "x" || a == "X") aid = 0;
+                else if (a == "y" || a == "Y") aid = 1;
+                else if (a == "z" || a == "Z") aid = 2;
+                else error("Invalid axis of rotation; " << a);
+                float deg;
+                in >> deg;
+                check_in_complete("Rotation requires one angle (in degrees)");
+                mat_rotate(modelmatrix, deg, axes[aid]);
+            } else if (trafo == "scale") {
+                float v[3];
+                in >> v[0] >> v[1] >> v[2];
+                check_in_complete("Too many components in scale transformation");
+                mat_scale(modelmatrix, v);
+            } else {
+                error("unsupported trafo: " << trafo);
+            }
+        } else if (command == "modelpath") {
+            // interaction.cpp:404-409, scene::add_modelpath (libgi/scene.cpp:44-53)
+            string name;
+            in >> name;
+            check_in_complete("Syntax error, requires a path name (no spaces, sorry)");
+            if (name == "~" || name.rfind("~/", 0) == 0) {
+                const char *home = getenv("HOME");
+                name = string(home ? home : "") + name.substr(1);
+            }
+            modelpaths.push_back(name);
         } else if (command == "generate") {
             int cfg;
             in >> cfg;
@@ -393,7 +489,13 @@ int main(int argc, char **argv) {
                 cerr << "cannot open script " << argv[i] << endl;
                 return 1;
             }
+            // models are also looked up beside the script while it runs
+            // (interaction.cpp:129-137)
+            string script = argv[i];
+            size_t slash = script.find_last_of('/');
+            modelpaths.push_back(slash == string::npos ? string() : script.substr(0, slash));
             process(f);
+            modelpaths.pop_back();
         }
     }
     algo.reset();

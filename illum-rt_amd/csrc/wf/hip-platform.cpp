@@ -609,9 +609,7 @@ public:
             throw std::runtime_error("pt-wf: sppx " + std::to_string(rc->sppx) +
                                      " is not a multiple of tracer-option samples " + std::to_string(samples));
         hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, o.stack_config, o.work_chunks, o.refill_threshold, o.blocks,
-                          pf->ray_order, samples};
-        p.park_leaves = o.park_leaves;
-        p.regen_pool = pf->regen_pool;
+                          pf->ray_order, samples, 0, o.park_leaves, 0, pf->regen_pool};
         check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
         frames = 0;
     }

@@ -93,7 +93,7 @@ public:
     hiprt_bvh_params bvh_params{2, 4, 16, 1};
     int tri_format = HIPRT_TRI_EXACT;
     // options of the persistent tracers, set by "tracer-option <name> <value>"
-    hiprt_tracer_opts opts{40, 0, nullptr, 0, 8, 0};
+    hiprt_tracer_opts opts{40, 0, nullptr, 0, 8, 0, 0};
     int ray_order = 0;   // path tracer bounce queues: 0 sorted, 1 shading order
     int samples = 1;     // path tracer: samples per pixel traced together (divides sppx)
     int regen_pool = 0;  // path tracer: path slots per pixel of path regeneration (0: off)

@@ -98,6 +98,7 @@ def lib():
             "hiprt_scene_from_arrays": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, c_void_p,
                                                 POINTER(c_void_p)]),
             "hiprt_scene_load_obj": (c_int, [c_char_p, POINTER(c_void_p)]),
+            "hiprt_scene_add_obj": (c_int, [POINTER(c_void_p), c_char_p, c_char_p, POINTER(c_float)]),
             "hiprt_scene_write_obj": (c_int, [c_void_p, c_char_p]),
             "hiprt_scene_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
             "hiprt_scene_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -232,6 +233,20 @@ class HostScene:
         h = c_void_p()
         check(lib().hiprt_scene_load_obj(str(path).encode(), ctypes.byref(h)))
         return cls(h)
+
+    @classmethod
+    def add_obj(cls, path, name="", trafo=None, scene=None):
+        """hiprt_scene_add_obj: the OBJ file appended to `scene` (a new scene
+        when None) under the 4x4 model matrix `trafo` (numpy layout, row
+        major: trafo[row, column]; None = identity), its material names
+        prefixed "<name>/" when a name is given.  Returns the scene."""
+        h = scene.h if scene is not None else c_void_p()
+        m = None
+        if trafo is not None:
+            m = np.ascontiguousarray(np.asarray(trafo, dtype=np.float32).reshape(4, 4).T)
+        check(lib().hiprt_scene_add_obj(ctypes.byref(h), str(path).encode(), name.encode() if name else None,
+                                        m.ctypes.data_as(POINTER(c_float)) if m is not None else None))
+        return scene if scene is not None else cls(h)
 
     def write_obj(self, path):
         check(lib().hiprt_scene_write_obj(self.h, str(path).encode()))

@@ -64,9 +64,25 @@ int hiprt_scene_random(uint32_t n_tris, float sigma, uint64_t seed, hiprt_host_s
 /* Copy of caller arrays: pos[3*n_vert], idx[3*n_tris], mat[n_tris] (mat may be NULL). */
 int hiprt_scene_from_arrays(const float *pos, uint32_t n_vert, const uint32_t *idx, uint32_t n_tris,
                             const uint32_t *mat, hiprt_host_scene *out);
-/* Minimal OBJ reader/writer ("load" of the reference, libgi/scene.cpp:291-364). */
+/* OBJ reader: the file as the reference's scene::add loads it through
+ * assimp's OBJ importer (libgi/scene.cpp:292-356): triangles in the
+ * importer's object/mesh order, materials "DefaultMaterial" then the
+ * libraries' in definition order then undefined "usemtl" names in first-use
+ * order (Kd 0.6 grey by default, albedo Kd^2.2), quads split from a concave
+ * corner, winding swapped where it disagrees with the corners' normals.
+ * Writer: positions, triangles and "usemtl m<id>" switches. */
 int hiprt_scene_load_obj(const char *path, hiprt_host_scene *out);
 int hiprt_scene_write_obj(hiprt_host_scene s, const char *path);
+/* scene::add(path, name, trafo) (libgi/scene.cpp:292-356), the script
+ * command "load <file> [name]" (driver/interaction.cpp:395-402): loads an OBJ
+ * file as hiprt_scene_load_obj does and appends it to *scene (a new scene
+ * when *scene is NULL).  name (NULL or "" for none) prefixes its material
+ * names as "<name>/<material>"; trafo is a column-major 4x4 model matrix
+ * (16 floats; NULL = identity) applied to positions, with its inverse
+ * transpose applied to normals, before the winding check
+ * (mesh_load_process_node, libgi/scene.cpp:222-289).  Material ids of the
+ * new triangles follow the scene's existing materials. */
+int hiprt_scene_add_obj(hiprt_host_scene *scene, const char *path, const char *name, const float *trafo);
 int hiprt_scene_info(hiprt_host_scene s, uint32_t *n_vert, uint32_t *n_tris);
 /* Any output pointer may be NULL. */
 int hiprt_scene_copy(hiprt_host_scene s, float *pos, uint32_t *idx, uint32_t *mat);
@@ -76,8 +92,8 @@ int hiprt_scene_materials(hiprt_host_scene s, float *table, uint32_t *n_material
 /* Name of material `material` (material::name, which "material select"
  * looks up, driver/interaction.cpp:624-635), NUL-terminated into name[size];
  * "" for a scene whose materials have no names.  An OBJ file's materials are
- * "default" (faces without a known material) and then the library's
- * materials in definition order, or "m<id>" for "usemtl m<id>" files. */
+ * "DefaultMaterial", then the libraries' materials in definition order, then
+ * names "usemtl" uses without a definition. */
 int hiprt_scene_material_name(hiprt_host_scene s, uint32_t material, char *name, uint32_t size);
 /* Sets material `material`'s albedo and/or emission (3 floats each; NULL
  * keeps the value), as "material albedo" / "material emissive" do

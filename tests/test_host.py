@@ -94,34 +94,44 @@ def test_obj_reader_fans_polygons(tmp_path):
 
 def test_obj_materials_from_a_library(tmp_path):
     """mtllib materials resolved as RTGI's loader does: albedo = Kd^2.2, or
-    Ks^2.2 when Kd's luma is ~0; emission = Ke; material 0 for faces with no
-    (or an unknown) material."""
+    Ks^2.2 when Kd's luma is ~0; emission = Ke.  As assimp's OBJ importer
+    lists them: "DefaultMaterial" first, the library's in definition order,
+    then names "usemtl" uses without a definition (Kd 0.6 grey, as assimp's
+    material defaults)."""
     (tmp_path / "lib.mtl").write_text("newmtl red\nKd 0.8 0.1 0.1\n\nnewmtl lamp\nKd 0 0 0\nKs 0.5 0.25 0.5\n"
-                                      "Ke 4 3 2\n")
+                                      "Ke 4 3 2\nnewmtl plain\n")
     (tmp_path / "m.obj").write_text(
         "mtllib lib.mtl\n" + "".join(f"v {i} {i % 2} {i % 3}\n" for i in range(7)) +
         "usemtl red\nf 1 2 3\nusemtl lamp\nf 4 5 6\nusemtl nothing\nf 1 3 5\nf 2 4 6 7\n")
     sc = hiprt.HostScene.load_obj(tmp_path / "m.obj")
     _, idx, mat = sc.arrays()
     table = sc.materials()
-    assert mat.tolist() == [1, 2, 0, 0, 0]
-    assert table.shape == (3, 8)
+    assert sc.material_names() == ["DefaultMaterial", "red", "lamp", "plain", "nothing"]
+    assert mat.tolist() == [1, 2, 4, 4, 4]
+    assert table.shape == (5, 8)
     f32 = np.float32
     want_red = np.array([0.8, 0.1, 0.1], f32) ** f32(2.2)
     want_lamp = np.array([0.5, 0.25, 0.5], f32) ** f32(2.2)
+    grey = f32(0.6) ** f32(2.2)
     assert np.allclose(table[1, :3], want_red, rtol=1e-6) and (table[1, 4:7] == 0).all()
     assert np.allclose(table[2, :3], want_lamp, rtol=1e-6) and table[2, 4:7].tolist() == [4, 3, 2]
-    assert np.allclose(table[0, :3], 0.8) and (table[0, 4:7] == 0).all()
+    for k in (0, 3, 4):   # the default, a definition without Kd, an undefined name
+        assert np.allclose(table[k, :3], grey, rtol=1e-6) and (table[k, 4:7] == 0).all()
+    # without any library: one default material
+    (tmp_path / "p.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    sc = hiprt.HostScene.load_obj(tmp_path / "p.obj")
+    assert sc.material_names() == ["DefaultMaterial"] and sc.arrays()[2].tolist() == [0]
+    assert np.allclose(sc.materials()[0, :3], grey, rtol=1e-6)
 
 
 def test_material_names_and_edits(tmp_path):
-    """Material names as "material select" finds them: "default", then the
-    library's in definition order ("m<id>" for usemtl m<id> files); albedo and
-    emission can be set per material."""
+    """Material names as "material select" finds them: "DefaultMaterial",
+    then the library's in definition order, then undefined "usemtl" names;
+    albedo and emission can be set per material."""
     (tmp_path / "lib.mtl").write_text("newmtl red\nKd 0.8 0.1 0.1\nnewmtl lamp\nKd 0.5 0.5 0.5\n")
     (tmp_path / "m.obj").write_text("mtllib lib.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl lamp\nf 1 2 3\n")
     sc = hiprt.HostScene.load_obj(tmp_path / "m.obj")
-    assert sc.material_names() == ["default", "red", "lamp"]
+    assert sc.material_names() == ["DefaultMaterial", "red", "lamp"]
     sc.set_material(2, emissive=[5, 4, 3])
     sc.set_material(1, albedo=[0.25, 0.5, 0.75])
     t = sc.materials()
@@ -130,8 +140,118 @@ def test_material_names_and_edits(tmp_path):
     with pytest.raises(hiprt.HiprtError, match="material id"):
         sc.set_material(3, albedo=[1, 1, 1])
     (tmp_path / "l.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl m1\nf 1 2 3\n")
-    assert hiprt.HostScene.load_obj(tmp_path / "l.obj").material_names() == ["m0", "m1"]
+    assert hiprt.HostScene.load_obj(tmp_path / "l.obj").material_names() == ["DefaultMaterial", "m1"]
     assert hiprt.HostScene.generate(1).material_names() == [""]
+
+
+def test_obj_missing_library_falls_back_to_the_obj_name(tmp_path):
+    """A "mtllib" that cannot be opened falls back to the OBJ file's name with
+    "mtl" for "obj" (assimp), and loads without a library when that is
+    missing too."""
+    body = "mtllib gone.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl warm\nf 1 2 3\n"
+    (tmp_path / "x.obj").write_text(body)
+    (tmp_path / "x.mtl").write_text("newmtl warm\nKd 1 0.5 0.25\n")
+    sc = hiprt.HostScene.load_obj(tmp_path / "x.obj")
+    assert sc.material_names() == ["DefaultMaterial", "warm"]
+    assert np.allclose(sc.materials()[1, :3], np.array([1, 0.5, 0.25], np.float32) ** np.float32(2.2))
+    (tmp_path / "y.obj").write_text(body)
+    sc = hiprt.HostScene.load_obj(tmp_path / "y.obj")
+    assert sc.material_names() == ["DefaultMaterial", "warm"]   # undefined: default values
+    assert np.allclose(sc.materials()[1, :3], np.float32(0.6) ** np.float32(2.2))
+
+
+def _tri_lines(n):
+    """Vertices of n separate triangles, triangle k at x = k."""
+    return "".join(f"v {k} 0 0\nv {k + 0.5} 0 0\nv {k} 0.5 0\n" for k in range(n))
+
+
+def _tri_order(sc):
+    pos, idx, mat = sc.arrays()
+    return [int(round(float(pos[i[0], 0]))) for i in idx], mat.tolist()
+
+
+def test_obj_triangle_order_follows_assimp_objects_and_meshes(tmp_path):
+    """Triangle order and materials as the reference sees them through
+    assimp's OBJ importer: objects in creation order, each object's meshes in
+    creation order.  "o" re-opens an object of the same name; "usemtl" with
+    another material starts a mesh in the current object once the current
+    mesh has faces; faces before an object's first "usemtl" take that
+    material; "g" opens a new object whenever the group name changes."""
+    (tmp_path / "lib.mtl").write_text("newmtl a\nKd 1 0 0\nnewmtl b\nKd 0 1 0\n")
+
+    def f(k):
+        return f"f {3 * k + 1} {3 * k + 2} {3 * k + 3}\n"
+
+    # o A (a) t0 | o B (b) t1 | o A, usemtl a: a new mesh in A -> t2, t3
+    (tmp_path / "o.obj").write_text("mtllib lib.mtl\n" + _tri_lines(4) + "o A\nusemtl a\n" + f(0) +
+                                    "o B\nusemtl b\n" + f(1) + "o A\nusemtl a\n" + f(2) + f(3))
+    assert _tri_order(hiprt.HostScene.load_obj(tmp_path / "o.obj")) == ([0, 2, 3, 1], [1, 1, 1, 2])
+    # re-opening without a material switch: faces stay in the current mesh (B's)
+    (tmp_path / "r.obj").write_text("mtllib lib.mtl\n" + _tri_lines(3) + "o A\nusemtl a\n" + f(0) +
+                                    "o B\nusemtl b\n" + f(1) + "o A\n" + f(2))
+    assert _tri_order(hiprt.HostScene.load_obj(tmp_path / "r.obj")) == ([0, 1, 2], [1, 2, 2])
+    # a face before the first usemtl of its mesh takes that material
+    (tmp_path / "u.obj").write_text("mtllib lib.mtl\n" + _tri_lines(3) + "o A\n" + f(0) + "usemtl b\n" + f(1) +
+                                    "usemtl a\n" + f(2))
+    assert _tri_order(hiprt.HostScene.load_obj(tmp_path / "u.obj")) == ([0, 1, 2], [2, 2, 1])
+    # groups: a new object per group change, even for a name seen before
+    (tmp_path / "g.obj").write_text("mtllib lib.mtl\n" + _tri_lines(3) + "g A\nusemtl a\n" + f(0) +
+                                    "g B\nusemtl b\n" + f(1) + "g A\nusemtl a\n" + f(2))
+    assert _tri_order(hiprt.HostScene.load_obj(tmp_path / "g.obj")) == ([0, 1, 2], [1, 2, 1])
+    # usemtl before any object; faces with no object open "defaultobject"
+    (tmp_path / "d.obj").write_text("mtllib lib.mtl\n" + _tri_lines(2) + "usemtl b\n" + f(0) + f(1))
+    assert _tri_order(hiprt.HostScene.load_obj(tmp_path / "d.obj")) == ([0, 1], [2, 2])
+
+
+def test_obj_concave_quad_splits_from_its_concave_corner(tmp_path):
+    """A quad is split along the diagonal through its concave corner (assimp's
+    quad rule): the two triangles then tile the quad, with the quad's
+    orientation and its area.  A fan from corner 0 would not: one of its
+    triangles would lie outside the dart."""
+    (tmp_path / "q.obj").write_text("v 0 0 0\nv 2 1 0\nv 0 2 0\nv 0.5 1 0\nf 1 2 3 4\n"
+                                    "v 10 0 0\nv 12 0 0\nv 12 2 0\nv 10 2 0\nf 5 6 7 8\n")
+    pos, idx, _ = hiprt.HostScene.load_obj(tmp_path / "q.obj").arrays()
+    p = pos.astype(np.float64)
+    signed = [0.5 * np.cross(p[b] - p[a], p[c] - p[a])[2] for a, b, c in idx]
+    assert len(idx) == 4 and all(s > 0 for s in signed)
+    assert abs(signed[0] + signed[1] - 1.5) < 1e-6   # the dart's area (shoelace)
+    assert abs(signed[2] + signed[3] - 4.0) < 1e-6
+    assert idx[0].tolist() == [3, 0, 1] and idx[1].tolist() == [3, 1, 2]
+    assert idx[2].tolist() == [4, 5, 6] and idx[3].tolist() == [4, 6, 7]   # convex: from corner 0
+
+
+def test_add_obj_appends_under_a_model_matrix(tmp_path):
+    """hiprt_scene_add_obj (scene::add): models accumulate; material ids of a
+    later model follow the scene's materials and its names carry the
+    "<name>/" prefix; positions move by the model matrix and normals by its
+    inverse transpose (normalized), before the winding check -- so under a
+    mirroring matrix b and c swap and geometric and shading normals still
+    agree."""
+    (tmp_path / "lib.mtl").write_text("newmtl red\nKd 1 0 0\n")
+    (tmp_path / "n.obj").write_text("mtllib lib.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0.6 0 0.8\nusemtl red\n"
+                                    "f 1//1 2//1 3//1\n")
+    (tmp_path / "p.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    sc = hiprt.HostScene.add_obj(tmp_path / "p.obj")
+    assert sc.normals() is None
+    M = np.array([[-2, 0, 0, 5], [0, 1, 0, 1], [0, 0, 1, 0], [0, 0, 0, 1]], np.float32)   # mirror x, scale 2, move
+    hiprt.HostScene.add_obj(tmp_path / "n.obj", name="lamp", trafo=M, scene=sc)
+    pos, idx, mat = sc.arrays()
+    assert sc.material_names() == ["DefaultMaterial", "lamp/DefaultMaterial", "lamp/red"]
+    assert mat.tolist() == [0, 2]
+    nrm = sc.normals()
+    assert nrm is not None and len(nrm) == len(pos)
+    # the first model gained flat normals: its face normal, +z
+    assert np.allclose(nrm[idx[0]], [[0, 0, 1]] * 3)
+    # the second: corners moved, b and c swapped by the mirror
+    want = (M[:3, :3] @ np.array([[0, 0, 0], [0, 1, 0], [1, 0, 0]], np.float32).T).T + M[:3, 3]
+    assert np.allclose(pos[idx[1]], want)
+    n = np.linalg.inv(M[:3, :3].astype(np.float64)).T @ np.array([0.6, 0, 0.8])
+    assert np.allclose(nrm[idx[1]], [n / np.linalg.norm(n)] * 3, atol=1e-6)
+    a, b, c = pos[idx[1]].astype(np.float64)
+    assert np.dot(np.cross(b - a, c - a), nrm[idx[1][0]]) > 0
+    with pytest.raises(hiprt.HiprtError, match="cannot open"):
+        hiprt.HostScene.add_obj(tmp_path / "none.obj", scene=sc)
+    assert sc.n_tris == 2
 
 
 def test_obj_winding_follows_shading_normals(tmp_path):
