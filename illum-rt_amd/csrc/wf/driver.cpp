@@ -11,15 +11,21 @@
 //     run
 //
 // runs as it would on RTGI's cuda platform.  Errors print
-// "command (<name>): <message>" and the script continues, as in the
+// "command <n> (<name>): <message>" (n counts the script's lines; the
+// platform's own messages have no number, as the reference's platform prints
+// them) and the script continues, as in the
 // reference; "run" checks the same preconditions.
 //
-// Commands: at, look, up, fovy, resolution, sppx, load (appends a model under
-// the model matrix), modelmatrix, modeltrafo, modelpath, generate,
-// pointlight, material, default-brdf, platform, raytracer, tracer-option,
-// bvh, triangles, add-scene-step, commit, algo, run, outfile, stats, echo,
-// disable, with / }, quit, and the algorithm's own (direct-wf: is
-// uniform|cosine|light).
+// Commands: at, look, up, camup, fovy, resolution, sppx, load (appends a
+// model under the model matrix), modelmatrix, modeltrafo, modelpath,
+// generate, pointlight, material, default-brdf, platform, raytracer,
+// tracer-option, bvh, triangles, add-scene-step, commit, algo, run, outfile,
+// stats, echo, disable, with / }, quit, and the algorithm's own (direct-wf:
+// is uniform|cosine|light; pt-wf: path len|rr-start|bounce).  skylight,
+// preview-offset, omp and denoise are checked as the reference checks them
+// and change nothing here: the reference's wavefront steps do not sample a
+// sky either, there is no preview window, no host threading and no
+// OpenImageDenoise.
 // Materials are Lambertian: "material roughness|ior|brdf" and "default-brdf"
 // are accepted and kept for "material show" but do not change the shading.
 // "with gl|cuda|optix {" blocks are skipped (those platforms are not built
@@ -62,6 +68,23 @@ struct brdf_params {
 };
 map<int, brdf_params> mat_params;
 string default_brdf = "lambert";
+bool brdfs_applied = false;   // a model was loaded: scene::add applied the default BRDF to its materials
+// camera/scene up state of interaction.cpp:229-233
+bool cam_has_up = false, scene_up_set = false;
+float scene_up[3] = {0, 1, 0};
+// "skylight": the reference's wavefront steps take triangle lights only
+// (rt/cuda/preprocessing.cpp:20-28), so the sky is checked and kept, not sampled
+bool have_sky = false;
+float sky_scale = 1.0f;
+
+// The BRDF names of new_brdf (libgi/material.cpp:326-370).
+bool known_brdf(const string &n) {
+    static const char *names[] = {"lambert", "mirror", "specular", "dielectric-specular", "thin-dielectric-specular",
+                                  "phong", "layered-phong", "gtr2", "layered-gtr2"};
+    for (const char *k : names)
+        if (n == k) return true;
+    return false;
+}
 // model matrix for "load" (column-major, m[4 * column + row]) and the model
 // search paths ("" first: the working directory), driver/interaction.cpp:203
 // and libgi/scene.h:164-170
@@ -124,7 +147,7 @@ string rest_of_line(istringstream &in) {
 
 #define error(x)                                                              \
     {                                                                         \
-        cerr << "command (" << command << "): " << x << endl;                  \
+        cerr << "command " << uc.cmdid << " (" << command << "): " << x << endl; \
         return;                                                               \
     }
 #define check_in_complete(x)                                                  \
@@ -215,6 +238,7 @@ void eval(const string &line) {
                 return;
             }
             if (in.fail()) error("Syntax error, requires material name, command and subsequent arguments");
+            command = sub;   // later errors name the subcommand (interaction.cpp:632)
             if (sub == "select") {
                 string name = rest_of_line(in);
                 int found = -1;
@@ -243,6 +267,7 @@ void eval(const string &line) {
                 string name;
                 in >> name;
                 check_in_complete("Expected a single (no whitespace) string value");
+                if (!known_brdf(name)) error("No such brdf defined: " << name);
                 mat_params[selected_mat].brdf = name;
             } else if (sub == "texture") {
                 error("Texture files are not decoded here; attach RGBA texels with hiprt_scene_set_texture");
@@ -261,15 +286,80 @@ void eval(const string &line) {
                 error("Unknown subcommand");
             }
         } else if (command == "default-brdf") {
+            // interaction.cpp:709-722
             string name;
             in >> name;
             check_in_complete("Expected a single (no whitespace) string value");
+            if (brdfs_applied) cout << "Replacing default brdfs that has already been applied to some materials." << endl;
+            if (!known_brdf(name)) error("No such brdf defined: " << name);
             default_brdf = name;
-        } else if (command == "at" || command == "look" || command == "up") {
+        } else if (command == "at" || command == "look" || command == "camup" || command == "up") {
+            // interaction.cpp:301-350: without arguments they print the
+            // vector; "up" sets the scene's up vector once, and the camera's
+            // too unless "camup" set that
+            float *cur = command == "at" ? rc->scene.cam_pos
+                         : command == "look" ? rc->scene.cam_dir
+                         : command == "camup" ? rc->scene.cam_up : scene_up;
+            in.get();
+            if (in.eof() && (command != "up" || scene_up_set)) {
+                cout << cur[0] << " " << cur[1] << " " << cur[2] << endl;
+                return;
+            }
+            if (command == "up" && scene_up_set) error("Cannot set scene up vector twice, did you mean camup?");
             float v[3];
-            if (!read3(in, v)) error("Syntax error, requires 3 numerical components");
-            float *dst = command == "at" ? rc->scene.cam_pos : (command == "look" ? rc->scene.cam_dir : rc->scene.cam_up);
-            for (int k = 0; k < 3; ++k) dst[k] = v[k];
+            in >> v[0] >> v[1] >> v[2];
+            check_in_complete("Syntax error, requires 3 numerical components");
+            for (int k = 0; k < 3; ++k) cur[k] = v[k];
+            if (command == "camup") cam_has_up = true;
+            if (command == "up") {
+                scene_up_set = true;
+                if (!cam_has_up) {
+                    for (int k = 0; k < 3; ++k) rc->scene.cam_up[k] = v[k];
+                    cam_has_up = true;
+                }
+            }
+        } else if (command == "skylight") {
+            // interaction.cpp:754-777
+            string sub;
+            in >> sub;
+            if (sub == "load") {
+                float scale;
+                string file;
+                in >> file >> scale;
+                check_in_complete("Syntax error: skylight load filename intensity-scale (note: filename is not allowed "
+                                  "to contain spaces)");
+                string path = find_model(file);
+                if (!ifstream(path, ios::binary).good())
+                    error("Cannot open file '" << path << "' for hdr floats texture.");
+                have_sky = true;
+                sky_scale = scale;
+            } else if (sub == "scale") {
+                float scale;
+                in >> scale;
+                check_in_complete("Syntax error: skylight scale new-scale-value");
+                if (!have_sky) error("There is no skylight to scale");
+                if (scale <= 0) error("Intensity scale has to be a value > 0");
+                sky_scale = scale;
+            } else {
+                error("No such skylight subcommand");
+            }
+        } else if (command == "preview-offset") {
+            // interaction.cpp:598-605 (no preview window here)
+            int offset;
+            in >> offset;
+            check_in_complete("Syntax error, requires exactly one positive integral value");
+            if (offset <= 0) error("Thge preview offset needs to be larger than zero");
+        } else if (command == "omp") {
+            // interaction.cpp:819-828: host threads; the tracing runs on the GPU
+            string sub;
+            in >> sub;
+            if (sub != "on" && sub != "off") error("Syntax error: expected 'on' or 'off'");
+        } else if (command == "denoise") {
+            // interaction.cpp:573-590 on a build without OpenImageDenoise
+            string option;
+            in >> option;
+            if (option == "on") cout << "Denoising not available: OpenImageDenoise library not found" << endl;
+            else if (option != "off") error("Invalid option for denoise, valid are on/off");
         } else if (command == "fovy") {
             float f;
             in >> f;
@@ -301,6 +391,7 @@ void eval(const string &line) {
             if (hiprt_scene_add_obj(&rc->scene.geometry, path.c_str(), name.empty() ? nullptr : name.c_str(),
                                     modelmatrix) != HIPRT_OK)
                 error(hiprt_last_error());
+            brdfs_applied = true;
             uc.scene_touched_at = uc.cmdid;
         } else if (command == "modelmatrix") {
             // interaction.cpp:357-363: 16 values, column by column

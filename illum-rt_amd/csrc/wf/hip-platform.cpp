@@ -586,10 +586,44 @@ public:
 // k path slots per pixel instead of all k samples in flight.
 class pt_wf : public simple_algorithm {
     hiprt_pt pt = nullptr;
-    int frames = 0;    // render calls so far
-    int samples = 1;   // samples per render call
+    int frames = 0;      // render calls so far
+    int samples = 1;     // samples per render call
+    int max_depth = 4;   // last trace level (levels 0..max_depth)
+    int rr_start = 3;    // first level whose ray is subject to Russian roulette
 
 public:
+    // "path len|bounce|rr-start" of simple_pt (gi/pt.cpp:121-154), with its
+    // messages.  The reference's loop traces rays i = 0 .. len-1 and, shading
+    // the hit of ray i > rr-start, plays roulette on ray i + 1: here "len n"
+    // traces levels 0 .. n-1 and "rr-start n" subjects the rays of levels
+    // n + 2 and later.  Bounces sample the Lambertian BRDF, whose importance
+    // sampling is the cosine distribution: "bounce brdf" and "bounce cosine"
+    // both select it.
+    bool interprete(const std::string &command, std::istringstream &in) override {
+        if (command != "path") return false;
+        std::string sub, val;
+        in >> sub;
+        if (sub == "len") {
+            int i = 0;
+            in >> i;
+            if (i <= 0) std::cerr << "error in path len: expected a positive integer, got " << i << std::endl;
+            else if (i > 65) std::cerr << "error in path len: pt-wf traces at most 65 levels, got " << i << std::endl;
+            else max_depth = i - 1;
+        } else if (sub == "bounce") {
+            in >> val;
+            if (val == "uniform") std::cerr << "error: pt-wf samples bounces by the BRDF (cosine) only" << std::endl;
+            else if (val != "cosine" && val != "brdf")
+                std::cerr << "error: invalid kind of path bounce: '" << val << "'" << std::endl;
+        } else if (sub == "rr-start") {
+            int i = 0;
+            in >> i;
+            if (i <= 0) std::cerr << "error in russian roulette offset: expected a positive integer > 0, got " << i << std::endl;
+            else rr_start = i + 2;
+        } else {
+            std::cerr << "unknown subcommand to path: '" << sub << "'" << std::endl;
+        }
+        return true;
+    }
     ~pt_wf() override {
         if (pt) hiprt_pt_destroy(pt);
     }
@@ -608,8 +642,8 @@ public:
         if (rc->sppx % samples != 0)
             throw std::runtime_error("pt-wf: sppx " + std::to_string(rc->sppx) +
                                      " is not a multiple of tracer-option samples " + std::to_string(samples));
-        hiprt_pt_params p{4, 3, 0.8f, 1, tracer, 1, o.stack_config, o.work_chunks, o.refill_threshold, o.blocks,
-                          pf->ray_order, samples, 0, o.park_leaves, 0, pf->regen_pool};
+        hiprt_pt_params p{max_depth, rr_start, 0.8f, 1, tracer, 1, o.stack_config, o.work_chunks, o.refill_threshold,
+                          o.blocks, pf->ray_order, samples, 0, o.park_leaves, 0, pf->regen_pool};
         check(hiprt_pt_create(pf->ds, cam, &p, &pt), "pt-wf");
         frames = 0;
     }

@@ -36,13 +36,14 @@ def test_driver_errors_follow_the_reference(tmp_path):
                    "generate 9\nbvh type sah\ntracer-option stack 3\n# comment\n// comment\n", tmp_path)
     assert r.returncode == 0
     err = r.stderr.splitlines()
-    assert "command (platform): There is no platform called 'cuda'" in err
-    assert any(e.startswith("command (raytracer): No platform selected") for e in err)
-    assert "command (run): Invalid platform" in err
-    assert "command (resolution): Resolution must be positive" in err
-    assert "command (sppx): Syntax error, requires exactly one positive integral value" in err
-    assert "command (bogus): Unknown command" in err
-    assert any(e.startswith("command (tracer-option): No platform selected") for e in err)
+    # "command <n> (<name>): ..." with n counting every script line (interaction.cpp:96, 226)
+    assert "command 2 (platform): There is no platform called 'cuda'" in err
+    assert any(e.startswith("command 3 (raytracer): No platform selected") for e in err)
+    assert "command 4 (run): Invalid platform" in err
+    assert "command 5 (resolution): Resolution must be positive" in err
+    assert "command 6 (sppx): Syntax error, requires exactly one positive integral value" in err
+    assert "command 7 (bogus): Unknown command" in err
+    assert any(e.startswith("command 10 (tracer-option): No platform selected") for e in err)
     assert r.stdout.strip() == "hi"
     assert len(err) == 9
 
@@ -68,10 +69,12 @@ def test_driver_blocks_and_material_commands(tmp_path):
     assert out[5:8] == ["DefaultMaterial", "floor", "lamp"]
     assert "emissive   5 4 3" in out and "roughness  0.3" in out
     err = r.stderr.splitlines()
-    assert err == ["command (with): Syntax error: expected one of gl, cuda, optix, hip",
-                   "command (material): No material selected",
-                   "command (material): No material called 'nope'",
-                   "command (material): Unknown subcommand"], err
+    # errors inside "material" name the subcommand (interaction.cpp:632); skipped lines count too
+    assert err == ["command 13 (with): Syntax error: expected one of gl, cuda, optix, hip",
+                   "command 16 (albedo): No material selected",
+                   "command 17 (select): No material called 'nope'",
+                   "command 22 (frobnicate): Unknown subcommand"], err
+    assert out[-1] == "Replacing default brdfs that has already been applied to some materials."
 
 
 SCRIPT_HEAD = """platform hip
@@ -114,6 +117,7 @@ def test_tracer_options_keep_the_image(tmp_path, torch_cuda):
                     "tracer-option stack 9\ntracer-option speed 1\nalgo primary-wf\noutfile b.pfm\nrun\n", tmp_path)
     assert r0.returncode == 0 and not r0.stderr.strip(), r0.stderr
     err = r1.stderr.splitlines()
+    # the platform's own messages have no command number, as rt/cuda/platform.cpp:18 prints them
     assert err == ["command (tracer-option): Parameter error, stack configuration must be 0..4",
                    "command (tracer-option): Unknown tracer option speed (refill, blocks, stack, chunks, ray-order, samples, "
                    "park, regen)"], err
@@ -187,6 +191,43 @@ def test_pt_wf_matches_library_path_tracer(tmp_path, torch_cuda, options, sample
 
 
 @pytest.mark.gpu
+def test_pt_wf_path_commands(tmp_path, torch_cuda):
+    """simple_pt's "path len n" / "path rr-start n" (gi/pt.cpp:121-154) on
+    pt-wf: the reference's loop traces rays 0 .. n-1 and, shading the hit of
+    ray i > rr-start, plays roulette on ray i + 1, so "len 7" traces levels
+    0..6 and "rr-start 1" subjects levels 3 on; the image equals the
+    library's path tracer with max_depth 6 and rr_start 3.  Bad values print
+    the reference's messages and change nothing."""
+    print("case: rtgi-hip pt-wf, hall 128x72, 2 spp, path len 7 / rr-start 1 vs hiprt.PathTracer", flush=True)
+    w, h = 128, 72
+    r = run_script(SCRIPT_HEAD.format(cfg=3, w=w, h=h, tracer="persistent-while-while") +
+                   "sppx 2\nalgo pt-wf\npath len 7\npath rr-start 1\npath bounce brdf\npath len 0\n"
+                   "path rr-start -2\npath bounce uniform\npath mis on\noutfile pt.pfm\nrun\n", tmp_path, timeout=300)
+    assert r.returncode == 0
+    assert r.stderr.splitlines() == ["error in path len: expected a positive integer, got 0",
+                                     "error in russian roulette offset: expected a positive integer > 0, got -2",
+                                     "error: pt-wf samples bounces by the BRDF (cosine) only",
+                                     "unknown subcommand to path: 'mis'"], r.stderr
+    img = read_pfm(tmp_path / "pt.pfm")
+    sc = hiprt.HostScene.generate(3)
+    bvh = hiprt.Bvh(sc)
+    ctx = hiprt.Context(0)
+    ds = hiprt.DeviceScene(ctx, sc, bvh)
+    cam = sc.cam.copy()
+    cam[10], cam[11] = w, h
+    pt = hiprt.PathTracer(ds, cam, max_depth=6, rr_start=3, seed=1, batched=True, tracer="persistent-while-while")
+    for f in range(2):
+        pt.render(f)
+    want = pt.image() / np.float32(2)
+    levels = pt.ray_totals()
+    pt.close()
+    ds.close()
+    ctx.close()
+    assert len(levels) == 7 and levels[6] > 0
+    assert np.array_equal(img.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.gpu
 def test_pt_wf_samples_must_divide_sppx(tmp_path, torch_cuda):
     print("case: rtgi-hip pt-wf, sppx 3 with tracer-option samples 2", flush=True)
     r = run_script(SCRIPT_HEAD.format(cfg=1, w=32, h=24, tracer="dynamic-while-while") +
@@ -194,7 +235,7 @@ def test_pt_wf_samples_must_divide_sppx(tmp_path, torch_cuda):
                    tmp_path)
     err = r.stderr.splitlines()
     assert "command (tracer-option): Parameter error, samples must be 1..64" in err
-    assert "command (run): pt-wf: sppx 3 is not a multiple of tracer-option samples 2" in err
+    assert "command 12 (run): pt-wf: sppx 3 is not a multiple of tracer-option samples 2" in err
 
 
 @pytest.mark.gpu
@@ -202,7 +243,7 @@ def test_unknown_tracer_and_stale_bvh(tmp_path, torch_cuda):
     r = run_script("platform hip\nraytracer if-else\ngenerate 1\nalgo primary-wf\nrun\n", tmp_path)
     err = r.stderr.splitlines()
     assert "command (raytracer): There is no such hip ray tracer variant" in err
-    assert any(e.startswith("command (run): We have to have a scene loaded") for e in err)
+    assert any(e.startswith("command 5 (run): We have to have a scene loaded") for e in err)
 
 
 CUBE_OBJ = ("mtllib lib.mtl\n" + "".join(f"v {x} {y} {z}\n" for x in (0, 1) for y in (0, 1) for z in (0, 1)) +
@@ -229,11 +270,38 @@ def test_load_accumulates_and_finds_models(tmp_path):
     assert r.stdout.splitlines() == ["DefaultMaterial", "floor", "red", "box/DefaultMaterial", "box/floor",
                                      "box/red", "DefaultMaterial"]
     err = r.stderr.splitlines()
-    assert err[:4] == ["command (modeltrafo): unsupported trafo: spin",
-                       "command (modeltrafo): Invalid axis of rotation; w",
-                       "command (modeltrafo): Too many components in scale transformation",
-                       "command (modelmatrix): Syntax error in model matrix"], err
-    assert len(err) == 5 and err[4].startswith("command (load): cannot open missing.obj")
+    assert err[:4] == ["command 6 (modeltrafo): unsupported trafo: spin",
+                       "command 7 (modeltrafo): Invalid axis of rotation; w",
+                       "command 8 (modeltrafo): Too many components in scale transformation",
+                       "command 9 (modelmatrix): Syntax error in model matrix"], err
+    assert len(err) == 5 and err[4].startswith("command 10 (load): cannot open missing.obj")
+
+
+def test_camera_sky_and_host_commands(tmp_path):
+    """"up" sets the scene's up vector once and the camera's unless "camup"
+    did; at/look/up/camup without arguments print the vector
+    (interaction.cpp:301-350).  skylight, preview-offset, omp and denoise are
+    checked with the reference's messages (interaction.cpp:573-605, 754-777,
+    819-828); the BRDF names are new_brdf's (libgi/material.cpp:326-370)."""
+    (tmp_path / "sky.floats").write_bytes(b"\0" * 16)
+    r = run_script("up 0 0 1\nup 0 1 0\ncamup\ncamup 1 0 0\ncamup\nup\nat 1 2 3 4\nat\nskylight scale 2\n"
+                   "skylight load sky.floats 30\nskylight scale -1\nskylight load none.floats 3\nskylight fly\n"
+                   "preview-offset 0\nomp maybe\nomp on\ndenoise on\ndenoise soon\ndefault-brdf phong\n"
+                   "default-brdf fancy\nskylight scale 4\npreview-offset 8\ndenoise off\n", tmp_path)
+    assert r.returncode == 0
+    assert r.stdout.splitlines() == ["0 0 1", "1 0 0", "0 0 1", "0 0 0",
+                                     "Denoising not available: OpenImageDenoise library not found"]
+    assert r.stderr.splitlines() == [
+        "command 2 (up): Cannot set scene up vector twice, did you mean camup?",
+        "command 7 (at): Syntax error, requires 3 numerical components",
+        "command 9 (skylight): There is no skylight to scale",
+        "command 11 (skylight): Intensity scale has to be a value > 0",
+        "command 12 (skylight): Cannot open file 'none.floats' for hdr floats texture.",
+        "command 13 (skylight): No such skylight subcommand",
+        "command 14 (preview-offset): Thge preview offset needs to be larger than zero",
+        "command 15 (omp): Syntax error: expected 'on' or 'off'",
+        "command 18 (denoise): Invalid option for denoise, valid are on/off",
+        "command 20 (default-brdf): No such brdf defined: fancy"]
 
 
 @pytest.mark.gpu
