@@ -7,8 +7,8 @@
 #include <cstdlib>
 #include <fstream>
 #include <array>
-#include <map>
 #include <tuple>
+#include <unordered_map>
 #include <sstream>
 
 namespace rt {
@@ -569,6 +569,7 @@ bool read_obj(const std::string &path, scene &out, std::string &err, const float
     // key a "vn" index or -2 - t for triangle t's own (face) normal.
     std::vector<obj_corner> tri_corners;
     std::vector<vec3f> own_normal;   // per triangle, its transformed face normal
+    std::vector<std::array<uint32_t, 3>> tris;   // one face's triangles (corner positions in the face)
     for (const obj_object &o : model.objects)
         for (uint32_t mid : o.meshes) {
             const obj_mesh &mesh = model.meshes[mid];
@@ -578,7 +579,7 @@ bool read_obj(const std::string &path, scene &out, std::string &err, const float
                 // quads as assimp splits them, larger polygons as a fan
                 // (assimp clips ears there: same triangles for convex
                 // polygons only up to their order)
-                std::vector<std::array<uint32_t, 3>> tris;
+                tris.clear();
                 if (f.n == 4) {
                     const vec3f q[4] = {pos[(size_t)c[0].v], pos[(size_t)c[1].v], pos[(size_t)c[2].v],
                                         pos[(size_t)c[3].v]};
@@ -622,7 +623,16 @@ bool read_obj(const std::string &path, scene &out, std::string &err, const float
         const size_t nv = out.pos.size();
         if (!uvs.empty()) out.tc.assign(2 * nv, 0.0f);
         if (with_vn) out.norm.assign(3 * nv, 0.0f);
-        std::map<std::tuple<long, long, long>, uint32_t> split;
+        struct key_hash {
+            size_t operator()(const std::tuple<long, long, long> &k) const {
+                uint64_t h = (uint64_t)std::get<0>(k) * 0x9E3779B97F4A7C15ull;
+                h ^= (uint64_t)std::get<1>(k) * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+                h ^= (uint64_t)std::get<2>(k) * 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+                return (size_t)h;
+            }
+        };
+        std::unordered_map<std::tuple<long, long, long>, uint32_t, key_hash> split;
+        split.reserve(tri_corners.size());
         for (size_t k = 0; k < tri_corners.size(); ++k) {
             const obj_corner &c = tri_corners[k];
             const long t = uvs.empty() ? -1 : c.t;
