@@ -264,6 +264,13 @@ int hiprt_scene_get_texcoords(hiprt_host_scene s, float *tc, uint32_t *n_vert) {
     return ok();
 }
 
+int hiprt_scene_set_positions(hiprt_host_scene s, const float *pos, uint32_t n_vert) {
+    if (!s || (n_vert && !pos)) return fail(HIPRT_ERR_ARG, "null argument");
+    if (n_vert != s->sc.pos.size()) return fail(HIPRT_ERR_ARG, "position count differs from the vertex count");
+    for (uint32_t i = 0; i < n_vert; ++i) s->sc.pos[i] = rt::vec3f{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    return ok();
+}
+
 int hiprt_scene_set_normals(hiprt_host_scene s, const float *normals, uint32_t n_vert) {
     if (!s || (n_vert && !normals)) return fail(HIPRT_ERR_ARG, "null argument");
     if (n_vert && n_vert != s->sc.pos.size()) return fail(HIPRT_ERR_ARG, "normal count differs from the vertex count");

@@ -2,6 +2,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cmath>
 #include <cstring>
 #include <iostream>
 #include <map>
@@ -310,6 +311,97 @@ struct build_accel_struct : public wf::build_accel_struct {
     }
 };
 
+// The reference's example scene steps (rt/cuda/preprocessing.cpp:41-72,
+// preprocessing.h:29-52).  "rotate scene" turns the vertex positions by
+// Rz(pi/4) Ry(pi/4) Rx(pi/4) about the origin, for the steps that follow it
+// at commit (normally "build accel struct"); normals stay as they are, as
+// the reference's kernel is given no normal buffer.  "rotate scene keeping
+// original data" keeps the positions it replaces, and "drop scene view" puts
+// the last kept ones back for the next commit (a device scene built in
+// between keeps the turned ones).
+struct rotate_scene : public wf::step {
+    static constexpr char id[] = "rotate scene";
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        pf->rotate_scene(false);
+    }
+};
+struct rotate_scene_keep_org : public wf::step {
+    static constexpr char id[] = "rotate scene keeping original data";
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        pf->rotate_scene(true);
+    }
+};
+struct drop_scene_view : public wf::step {
+    static constexpr char id[] = "drop scene view";
+    std::string id_str = id;
+    void run() override {
+        time_this_wf_step;
+        pf->drop_scene_view();
+    }
+};
+
+// The rotation of the example steps, rot = m_z * m_y * m_x with
+// m_a = rotate(pi/4, a) (Rodrigues' formula about the unit axis a), in float
+// and in the reference's matrix library's summation order; R[column][row].
+void rotation_xyz_quarter(float R[3][3]) {
+    const float angle = 3.14159265358979323846f / 4;
+    const float c = std::cos(angle), s = std::sin(angle);
+    float M[3][3][3];   // M[axis][column][row]
+    for (int a = 0; a < 3; ++a) {
+        float ax[3] = {0, 0, 0};
+        ax[a] = 1;
+        const float t[3] = {(1.0f - c) * ax[0], (1.0f - c) * ax[1], (1.0f - c) * ax[2]};
+        M[a][0][0] = c + t[0] * ax[0];
+        M[a][0][1] = t[0] * ax[1] + s * ax[2];
+        M[a][0][2] = t[0] * ax[2] - s * ax[1];
+        M[a][1][0] = t[1] * ax[0] - s * ax[2];
+        M[a][1][1] = c + t[1] * ax[1];
+        M[a][1][2] = t[1] * ax[2] + s * ax[0];
+        M[a][2][0] = t[2] * ax[0] + s * ax[1];
+        M[a][2][1] = t[2] * ax[1] - s * ax[0];
+        M[a][2][2] = c + t[2] * ax[2];
+    }
+    auto mul = [](const float A[3][3], const float B[3][3], float out[3][3]) {
+        for (int col = 0; col < 3; ++col)
+            for (int r = 0; r < 3; ++r)
+                out[col][r] = (A[0][r] * B[col][0] + A[1][r] * B[col][1]) + A[2][r] * B[col][2];
+    };
+    float zy[3][3];
+    mul(M[2], M[1], zy);
+    mul(zy, M[0], R);
+}
+
+void platform::rotate_scene(bool keep) {
+    if (!sd || !sd->geometry) throw std::logic_error("There is no scene data to work with");
+    uint32_t nv = 0, nt = 0;
+    check(hiprt_scene_info(sd->geometry, &nv, &nt), "rotate scene");
+    std::vector<float> pos(3ull * nv);
+    check(hiprt_scene_copy(sd->geometry, pos.data(), nullptr, nullptr), "rotate scene");
+    if (keep) views.push_back(pos);
+    float R[3][3];
+    rotation_xyz_quarter(R);
+    for (size_t i = 0; i < nv; ++i) {
+        const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        for (int r = 0; r < 3; ++r) pos[3 * i + r] = (x * R[0][r] + y * R[1][r]) + z * R[2][r];
+    }
+    check(hiprt_scene_set_positions(sd->geometry, pos.data(), nv), "rotate scene");
+}
+
+void platform::drop_scene_view() {
+    if (views.empty()) throw std::logic_error("drop scene view: there is no scene view to drop");
+    if (!sd || !sd->geometry) throw std::logic_error("There is no scene data to work with");
+    uint32_t nv = 0;
+    check(hiprt_scene_info(sd->geometry, &nv, nullptr), "drop scene view");
+    if (3ull * nv != views.back().size())
+        throw std::logic_error("drop scene view: the scene's vertex count changed since its view was made");
+    check(hiprt_scene_set_positions(sd->geometry, views.back().data(), nv), "drop scene view");
+    views.pop_back();
+}
+
 // ---------------------------------------------------------------- platform
 
 platform::platform(const std::vector<std::string> &args) : wf::platform("hip") {
@@ -348,6 +440,9 @@ platform::platform(const std::vector<std::string> &args) : wf::platform("hip") {
     register_wf_step_by_id(, integrate_light_sample);
     register_wf_step_by_id(, download_framebuffer);
     register_wf_step_by_id(, hip::build_accel_struct);
+    register_wf_step_by_id(, hip::rotate_scene);
+    register_wf_step_by_id(, hip::rotate_scene_keep_org);
+    register_wf_step_by_id(, hip::drop_scene_view);
     timer = new event_timer;
 }
 

@@ -109,6 +109,10 @@ public:
     void commit_scene(wf::scene *s) override;
     bool interprete(const std::string &command, std::istringstream &in) override;
     void build();                      // host BVH + flatten + upload
+    // scene steps "rotate scene [keeping original data]" / "drop scene view"
+    void rotate_scene(bool keep);
+    void drop_scene_view();
+    std::vector<std::vector<float>> views;   // positions kept by "rotate scene keeping original data"
 };
 
 // Algorithms (the gi/ step graphs): primary-wf (gi/primary-hit.cpp:85-117),

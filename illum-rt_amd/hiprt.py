@@ -106,6 +106,7 @@ def lib():
             "hiprt_scene_set_texcoords": (c_int, [c_void_p, c_void_p, c_uint32]),
             "hiprt_scene_get_texcoords": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
             "hiprt_scene_set_normals": (c_int, [c_void_p, c_void_p, c_uint32]),
+            "hiprt_scene_set_positions": (c_int, [c_void_p, c_void_p, c_uint32]),
             "hiprt_scene_material_name": (c_int, [c_void_p, c_uint32, c_char_p, c_uint32]),
             "hiprt_scene_set_material": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
             "hiprt_scene_get_normals": (c_int, [c_void_p, c_void_p, POINTER(c_uint32)]),
@@ -289,6 +290,11 @@ class HostScene:
         e = None if emissive is None else np.ascontiguousarray(emissive, dtype=np.float32).reshape(3)
         check(lib().hiprt_scene_set_material(self.h, material, _ptr(a) if a is not None else None,
                                              _ptr(e) if e is not None else None))
+
+    def set_positions(self, pos):
+        """Vertex positions (n_vert, 3) float32, one per existing vertex."""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        check(lib().hiprt_scene_set_positions(self.h, _ptr(p), len(p)))
 
     def set_normals(self, normals):
         """Per-vertex shading normals (n_vert, 3) float32; None removes them."""

@@ -109,6 +109,12 @@ int hiprt_scene_set_texcoords(hiprt_host_scene s, const float *tc, uint32_t n_ve
  * "vt" coordinates arrive here: every distinct (position, vt) corner pair is
  * a vertex of its own, after the file's "v" vertices. */
 int hiprt_scene_get_texcoords(hiprt_host_scene s, float *tc, uint32_t *n_vert);
+/* Replaces the vertex positions (vertex::pos of libgi/rt.h), n_vert (x, y,
+ * z) triples; n_vert must be the scene's vertex count.  Read at the next
+ * build or commit; a committed scene moves only through hiprt_scene_refit.
+ * The platform's scene steps "rotate scene" / "drop scene view"
+ * (rt/cuda/preprocessing.cpp:41-72) edit positions this way. */
+int hiprt_scene_set_positions(hiprt_host_scene s, const float *pos, uint32_t n_vert);
 /* Per-vertex shading normals, n_vert (x, y, z) triples: vertex::norm of
  * libgi/rt.h, interpolated at a hit into diff_geom::ns (libgi/rt.cpp:7-15).
  * The wavefront steps flip normals to the ray's side by it

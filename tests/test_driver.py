@@ -358,6 +358,70 @@ def test_load_under_modeltrafo_matches_oracle(tmp_path, torch_cuda):
         assert same >= bar, (k, same)
 
 
+def _quarter_turns():
+    """Rz(pi/4) Ry(pi/4) Rx(pi/4) in float32, R[column][row]."""
+    f = np.float32
+    a = f(np.pi) / f(4)
+    c, s = f(np.cos(a)), f(np.sin(a))
+    Rx = np.array([[1, 0, 0], [0, c, s], [0, -s, c]], f)      # [column][row]
+    Ry = np.array([[c, 0, -s], [0, 1, 0], [s, 0, c]], f)
+    Rz = np.array([[c, s, 0], [-s, c, 0], [0, 0, 1]], f)
+
+    def mul(A, B):   # (A B)[col][r] = sum_k A[k][r] B[col][k]
+        return np.array([[(A[0][r] * B[col][0] + A[1][r] * B[col][1]) + A[2][r] * B[col][2] for r in range(3)]
+                         for col in range(3)], f)
+
+    return mul(mul(Rz, Ry), Rx)
+
+
+@pytest.mark.gpu
+def test_rotate_scene_steps_match_oracle(tmp_path, torch_cuda):
+    """The reference's example scene steps (rt/cuda/preprocessing.cpp:41-72):
+    "rotate scene" turns the vertices by Rz Ry Rx (pi/4 each) before "build
+    accel struct"; "rotate scene keeping original data" then "drop scene
+    view" traces the turned scene and restores the original for the next
+    commit, which turns it again.  primary-wf's image equals the oracle's
+    hits on the turned cube (up to silhouette pixels, where numpy's float
+    cosine may differ from the host library's in the last bit); dropping a
+    view that was never made is an error."""
+    print("case: rtgi-hip primary-wf, cube turned by the rotate scene steps, 96x96", flush=True)
+    (tmp_path / "cube.obj").write_text(
+        "".join(f"v {x} {y} {z}\n" for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)) +
+        "f 1 2 4 3\nf 5 7 8 6\nf 1 5 6 2\nf 3 4 8 7\nf 1 3 7 5\nf 2 6 8 4\n")
+    w = h = 96
+    cam = np.array([0, 0, -3, 0, 0, 1, 0, 1, 0, 40, w, h], np.float32)
+    head = (f"platform hip\nload cube.obj\nat 0 0 -3\nlook 0 0 1\nup 0 1 0\nfovy 40\nresolution {w} {h}\n"
+            "raytracer while-while\n")
+    tail = "algo primary-wf\noutfile {}\nrun\n"
+    r = run_script(head + "add-scene-step rotate scene\nadd-scene-step build accel struct\ncommit\n" +
+                   tail.format("a.pfm"), tmp_path)
+    assert r.returncode == 0 and not r.stderr.strip(), r.stderr
+    r = run_script(head + "add-scene-step rotate scene keeping original data\nadd-scene-step build accel struct\n"
+                   "add-scene-step drop scene view\ncommit\ncommit\n" + tail.format("b.pfm"), tmp_path)
+    assert r.returncode == 0 and not r.stderr.strip(), r.stderr
+    r = run_script(head + "add-scene-step drop scene view\ncommit\n", tmp_path)
+    assert "command 10 (commit): drop scene view: there is no scene view to drop" in r.stderr.splitlines(), r.stderr
+    sc = hiprt.HostScene.load_obj(tmp_path / "cube.obj")
+    pos, idx, mat = sc.arrays()
+    R = _quarter_turns()
+    turned = np.stack([(pos[:, 0] * R[0][r] + pos[:, 1] * R[1][r]) + pos[:, 2] * R[2][r] for r in range(3)], 1)
+    sc.set_positions(turned.astype(np.float32))
+    bvh = hiprt.Bvh(sc)
+    t, _, _, tid = orc.trace_closest(turned.astype(np.float32), idx, bvh.nodes, bvh.root, bvh.index,
+                                     hiprt.camera_rays(cam))
+    hit = t != np.float32(3.4028234663852886e38)
+    want = np.zeros((w * h, 3), np.float32)
+    want[hit] = sc.materials()[mat[tid[hit]], :3]
+    bvh0 = hiprt.Bvh(hiprt.HostScene.load_obj(tmp_path / "cube.obj"))   # the unturned cube
+    t0, *_ = orc.trace_closest(pos, idx, bvh0.nodes, bvh0.root, bvh0.index, hiprt.camera_rays(cam))
+    plain = t0 != np.float32(3.4028234663852886e38)
+    assert (hit != plain).sum() > 100   # the turn changes the silhouette
+    for name in ("a.pfm", "b.pfm"):
+        img = read_pfm(tmp_path / name).reshape(-1, 3)
+        same = (img == want).all(axis=1).mean()
+        assert same >= 0.995, (name, same)
+
+
 @pytest.mark.gpu
 def test_material_commands_light_an_obj_scene(tmp_path, torch_cuda):
     """An OBJ scene whose lamp material has no emission in its library gets

@@ -278,6 +278,19 @@ def test_obj_winding_follows_shading_normals(tmp_path):
     assert hiprt.HostScene.load_obj(tmp_path / "n.obj").normals() is not None
 
 
+def test_scene_positions_setter():
+    """hiprt_scene_set_positions replaces every vertex position and refuses a
+    count that differs from the scene's."""
+    sc = hiprt.HostScene.random(10, 0.1, 3)
+    pos, idx, _ = sc.arrays()
+    moved = pos * np.float32(2) + np.float32(1)
+    sc.set_positions(moved)
+    p2, i2, _ = sc.arrays()
+    assert np.array_equal(p2, moved) and np.array_equal(i2, idx)
+    with pytest.raises(hiprt.HiprtError, match="position count"):
+        sc.set_positions(moved[:-1])
+
+
 def test_scene_normals_round_trip_and_counts(tmp_path):
     """hiprt_scene_set_normals takes one normal per vertex (or none); a scene
     without normals reports none, and an OBJ without "vn" has none."""
