@@ -10,6 +10,7 @@
 #include <tuple>
 #include <unordered_map>
 #include <sstream>
+#include <string_view>
 
 namespace rt {
 
@@ -287,6 +288,32 @@ struct mtl_list {
     }
 };
 
+// Whitespace-separated fields of one line, read in place.
+struct field_cursor {
+    const char *p;
+    void skip() {
+        while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\f' || *p == '\v') ++p;
+    }
+    bool token(const char *&b, const char *&e) {
+        skip();
+        if (!*p) return false;
+        b = p;
+        while (*p && *p != ' ' && *p != '\t' && *p != '\r' && *p != '\f' && *p != '\v') ++p;
+        e = p;
+        return true;
+    }
+    bool number(float &x) {
+        skip();
+        if (!*p) return false;
+        char *end = nullptr;
+        const float v = std::strtof(p, &end);
+        if (end == p) return false;
+        x = v;
+        p = end;
+        return true;
+    }
+};
+
 std::string trimmed(const std::string &s) {
     size_t a = s.find_first_not_of(" \t\r\f\v"), b = s.find_last_not_of(" \t\r\f\v");
     return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
@@ -486,36 +513,47 @@ bool read_obj(const std::string &path, scene &out, std::string &err, const float
     auto where = [&]() { return path + ":" + std::to_string(lineno) + ": "; };
     while (std::getline(in, line)) {
         ++lineno;
-        std::istringstream ls(line);
-        std::string tag;
-        if (!(ls >> tag)) continue;
+        // geometry lines are parsed in place (they are nearly all of a file);
+        // the rest through a string stream
+        field_cursor fc{line.c_str()};
+        const char *tb, *te;
+        if (!fc.token(tb, te)) continue;
+        const std::string_view tag(tb, (size_t)(te - tb));
         if (tag == "v" || tag == "vn") {
             float x, y, z;
-            if (!(ls >> x >> y >> z)) {
-                err = where() + "malformed " + tag;
+            if (!fc.number(x) || !fc.number(y) || !fc.number(z)) {
+                err = where() + "malformed " + std::string(tag);
                 return false;
             }
             (tag == "v" ? pos : normals).push_back(vec3f{x, y, z});
-        } else if (tag == "vt") {
+            continue;
+        }
+        if (tag == "vt") {
             float u, v = 0.0f;
-            if (!(ls >> u)) {
+            if (!fc.number(u)) {
                 err = where() + "malformed vt";
                 return false;
             }
-            ls >> v;   // a 1-component coordinate has v = 0
+            fc.number(v);   // a 1-component coordinate has v = 0
             uvs.push_back(u);
             uvs.push_back(v);
-        } else if (tag == "f") {
+            continue;
+        }
+        if (tag == "f") {
             const uint32_t first = (uint32_t)corners.size();
-            std::string tok;
-            while (ls >> tok) {
+            const char *b, *e;
+            while (fc.token(b, e)) {
                 // "a", "a/b", "a//c", "a/b/c" (1-based, negative = relative)
-                long v = std::strtol(tok.c_str(), nullptr, 10), n = 0, tx = 0;
-                size_t s1 = tok.find('/');
-                if (s1 != std::string::npos) {
-                    size_t s2 = tok.find('/', s1 + 1);
-                    if (s2 != std::string::npos && s2 + 1 < tok.size()) n = std::strtol(tok.c_str() + s2 + 1, nullptr, 10);
-                    if (s1 + 1 < tok.size() && tok[s1 + 1] != '/') tx = std::strtol(tok.c_str() + s1 + 1, nullptr, 10);
+                char *q = nullptr;
+                long v = std::strtol(b, &q, 10), n = 0, tx = 0;
+                if (q < e && *q == '/') {
+                    if (q + 1 < e && q[1] == '/') {
+                        if (q + 2 < e) n = std::strtol(q + 2, nullptr, 10);
+                    } else if (q + 1 < e) {
+                        char *q2 = nullptr;
+                        tx = std::strtol(q + 1, &q2, 10);
+                        if (q2 < e && *q2 == '/' && q2 + 1 < e) n = std::strtol(q2 + 1, nullptr, 10);
+                    }
                 }
                 if (v < 0) v = (long)pos.size() + v + 1;
                 if (n < 0) n = (long)normals.size() + n + 1;
@@ -536,7 +574,10 @@ bool read_obj(const std::string &path, scene &out, std::string &err, const float
                 corners.push_back(obj_corner{v - 1, tx - 1, n - 1});
             }
             if (corners.size() > first) model.add_face(obj_face{first, (uint32_t)corners.size() - first});
-        } else if (tag == "mtllib") {
+            continue;
+        }
+        std::istringstream ls(fc.p);
+        if (tag == "mtllib") {
             // the rest of the line names one library, beside the OBJ file; a
             // missing one falls back to the OBJ's name with "mtl" for "obj",
             // and without that the file loads with no library
