@@ -470,6 +470,50 @@ __global__ void k_set_u64(uint64_t *p, uint64_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// Incoherent benchmark rays: setup_ray_incoherent of the reference's camera
+// step under "incoherence r1 r2" (rt/cuda/kernels.cu:75-156).  Ray i starts
+// at sphere 1's centre moved by a uniform point of the ball of radius
+// r1 * r_max and runs to sphere 2's centre moved by a uniform point of the
+// ball of radius r2 * r_max (a zero radius keeps the centre); the direction
+// is that difference, unnormalized as the reference leaves it, over
+// [0, FLT_MAX].  A ball point takes three numbers: phi = 2 pi xi1,
+// cos theta = 2 xi2 - 1, r = R cbrt(xi3).  The numbers come from ray i's
+// PCG32 stream (the reference draws from cuRAND's MTGP32), the origin's
+// three first.
+__device__ __forceinline__ vec3f ball_point(pcg32 &g, float radius) {
+    const float pi = 3.14159265358979323846f;
+    const float phi = g.next_float() * 2 * pi;
+    const float costheta = g.next_float() * 2 - 1;
+    const float u = g.next_float();
+    const float theta = acosf(costheta);
+    const float r = radius * cbrtf(u);
+    return vec3f{r * sinf(theta) * cosf(phi), r * sinf(theta) * sinf(phi), r * cosf(theta)};
+}
+
+__global__ __launch_bounds__(256) void k_incoherent_rays(vec3f s1, vec3f s2, float r1, float r2, float r_max,
+                                                        uint64_t seed, ray_out out, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    pcg32 g(pcg_stream_id(seed, i, 0));
+    vec3f o = s1, t = s2;
+    if (r1 > 0) {
+        const vec3f b = ball_point(g, r1 * r_max);
+        o = vec3f{s1.x + b.x, s1.y + b.y, s1.z + b.z};
+    }
+    if (r2 > 0) {
+        const vec3f b = ball_point(g, r2 * r_max);
+        t = vec3f{s2.x + b.x, s2.y + b.y, s2.z + b.z};
+    }
+    write_ray(out, i, o, 0.0f, vec3f{t.x - o.x, t.y - o.y, t.z - o.z}, FLT_MAX);
+}
+
+hipError_t launch_incoherent_rays(vec3f s1, vec3f s2, float r1, float r2, float r_max, uint64_t seed,
+                                  const ray_out &out, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_incoherent_rays, dim3((n + 255) / 256), dim3(256), 0, s, s1, s2, r1, r2, r_max, seed, out, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
                               bool clear_radiance, hipStream_t s, uint32_t n_paths, float *rad_paths,
                               const uint64_t *jitter_seeds) {

@@ -136,6 +136,9 @@ hipError_t launch_shade_regen(const pt_scene &ps, const pt_params &pp, const reg
 hipError_t launch_set_u64(uint64_t *p, uint64_t v, hipStream_t s);
 // totals[i] += counts[i] for i < levels.
 hipError_t launch_add_counts(unsigned long long *totals, const uint32_t *counts, int levels, hipStream_t s);
+// Incoherent benchmark rays between two balls (device/pt.hip, k_incoherent_rays).
+hipError_t launch_incoherent_rays(vec3f s1, vec3f s2, float r1, float r2, float r_max, uint64_t seed,
+                                  const ray_out &out, uint32_t n, hipStream_t s);
 hipError_t launch_cosine_bounce(const pt_scene &ps, const ray_soa &in, const hit_soa &hits, const ray_out &out,
                                 uint64_t seed, int level, uint32_t n, hipStream_t s);
 

@@ -315,6 +315,22 @@ int hiprt_camera_rays(const float *cam, const hiprt_ray_buffers *out, void *stre
     return ok();
 }
 
+int hiprt_incoherent_rays(const float *sphere1, const float *sphere2, float r1, float r2, float r_max, uint64_t seed,
+                          const hiprt_ray_buffers *out, uint32_t n, void *stream) {
+    if (!sphere1 || !sphere2 || !out) return fail(HIPRT_ERR_ARG, "null argument");
+    if (!(r1 >= 0) || !(r2 >= 0) || !(r_max >= 0) || !std::isfinite(r1) || !std::isfinite(r2) || !std::isfinite(r_max))
+        return fail(HIPRT_ERR_ARG, "incoherence radii must be finite and non-negative");
+    if (n == 0) return ok();
+    if (!out->d_ox || !out->d_oy || !out->d_oz || !out->d_tmin || !out->d_dx || !out->d_dy || !out->d_dz ||
+        !out->d_tmax)
+        return fail(HIPRT_ERR_ARG, "null ray plane");
+    rt::dev::ray_out o{out->d_ox, out->d_oy, out->d_oz, out->d_tmin, out->d_dx, out->d_dy, out->d_dz, out->d_tmax};
+    HIP_TRY(rt::dev::launch_incoherent_rays(rt::vec3f{sphere1[0], sphere1[1], sphere1[2]},
+                                            rt::vec3f{sphere2[0], sphere2[1], sphere2[2]}, r1, r2, r_max, seed, o, n,
+                                            static_cast<hipStream_t>(stream)));
+    return ok();
+}
+
 int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, uint64_t seed,
                         int32_t level, const hiprt_ray_buffers *out, void *stream) {
     if (!sc || !rays || !hits || !out) return fail(HIPRT_ERR_ARG, "null argument");

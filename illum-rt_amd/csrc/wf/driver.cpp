@@ -501,7 +501,7 @@ void eval(const string &line) {
             }
             uc.tracer_touched_at = uc.cmdid;
         } else if (command == "raytracer" || command == "bvh" || command == "triangles" ||
-                   command == "tracer-option") {
+                   command == "tracer-option" || command == "incoherence") {
             if (!rc->platform) error("No platform selected, this command needs the hip platform");
             if (!rc->platform->interprete(command, in)) error("unknown command");
             if (command == "raytracer") uc.tracer_touched_at = uc.cmdid;

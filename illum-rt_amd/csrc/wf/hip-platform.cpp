@@ -2,10 +2,12 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <iostream>
 #include <map>
+#include <vector>
 
 namespace wf {
 
@@ -161,10 +163,41 @@ struct sample_camera_rays : public rd_step {
     std::string id_str = id;
     void run() override {
         time_this_wf_step;
+        hiprt_ray_buffers o = rd->out();
+        if (pf->rt && pf->rt->use_incoherence) {
+            // rt/cuda/rni.cpp:36-89: two spheres along the scene's largest
+            // extent, a quarter in from either end; the generator restarts
+            // from one seed (1234) for every sample, as the reference's does
+            float s1[3], s2[3], r_max;
+            incoherence_spheres(s1, s2, r_max);
+            check(hiprt_incoherent_rays(s1, s2, pf->rt->incoherence_r1, pf->rt->incoherence_r2, r_max, 1234, &o,
+                                        rd->n(), nullptr),
+                  id);
+            return;
+        }
         float cam[12];
         rc->camera(cam);
-        hiprt_ray_buffers o = rd->out();
         check(hiprt_camera_rays(cam, &o, nullptr), id);
+    }
+    // Sphere centres and r_max from the extent of the scene's vertices.
+    static void incoherence_spheres(float *s1, float *s2, float &r_max) {
+        uint32_t nv = 0;
+        check(hiprt_scene_info(rc->scene.geometry, &nv, nullptr), id);
+        std::vector<float> pos(3ull * nv);
+        check(hiprt_scene_copy(rc->scene.geometry, pos.data(), nullptr, nullptr), id);
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (size_t i = 0; i < nv; ++i)
+            for (int k = 0; k < 3; ++k) {
+                if (pos[3 * i + k] < lo[k]) lo[k] = pos[3 * i + k];
+                if (pos[3 * i + k] > hi[k]) hi[k] = pos[3 * i + k];
+            }
+        float size[3];
+        for (int k = 0; k < 3; ++k) size[k] = std::fabs(hi[k] - lo[k]);
+        const int a = (size[0] >= size[1] && size[0] >= size[2]) ? 0 : (size[1] >= size[2] ? 1 : 2);
+        for (int k = 0; k < 3; ++k) s1[k] = s2[k] = lo[k] + size[k] / 2;
+        s1[a] = lo[a] + size[a] / 4;
+        s2[a] = hi[a] - size[a] / 4;
+        r_max = size[a] / 4;
     }
 };
 
@@ -493,6 +526,22 @@ bool platform::interprete(const std::string &command, std::istringstream &in) {
         check_in_complete("Syntax error, requires a hip ray tracer variant name");
         if (tracers.count(variant) == 0) error("There is no such hip ray tracer variant");
         rt = dynamic_cast<batch_rt *>(select(variant));
+        return true;
+    }
+    if (command == "incoherence") {
+        // batch_rt::interprete (rt/cuda/base.cpp:120-133): a setting of the
+        // current tracer; "0 0" leaves an earlier one in place
+        float r1 = 0, r2 = 0;
+        in >> r1;
+        in >> r2;
+        check_in_complete("Syntax error, \"incoherence\" requires exactly two positive float values");
+        if (r1 < 0 || r2 < 0) error("Parameter error, \"incoherence\" requires exactly two positive float values");
+        if (!rt) rt = dynamic_cast<batch_rt *>(select("default"));
+        if (r1 != 0 || r2 != 0) {
+            rt->use_incoherence = true;
+            rt->incoherence_r1 = r1;
+            rt->incoherence_r2 = r2;
+        }
         return true;
     }
     if (command == "bvh") {

@@ -81,6 +81,10 @@ struct raydata : public wf::raydata {
 struct batch_rt : public wf::batch_ray_tracer {
     int tracer;
     raydata *rd = nullptr;
+    // "incoherence r1 r2" (rt/cuda/base.cpp:120-133, base.h:405-407): the
+    // camera step then traces rays between two balls instead
+    bool use_incoherence = false;
+    float incoherence_r1 = 0, incoherence_r2 = 0;
     explicit batch_rt(int id) : tracer(id) {}
     void compute_closest_hit() override;
     void compute_any_hit() override;

@@ -141,6 +141,8 @@ def lib():
             "hiprt_trace_closest_noalloc": (c_int, [c_void_p, POINTER(_Rays), c_uint32, POINTER(_Hits), c_int32,
                                                     POINTER(_TracerOpts), c_void_p]),
             "hiprt_camera_rays": (c_int, [POINTER(c_float), POINTER(_Rays), c_void_p]),
+            "hiprt_incoherent_rays": (c_int, [POINTER(c_float), POINTER(c_float), c_float, c_float, c_float, c_uint64,
+                                              POINTER(_Rays), c_uint32, c_void_p]),
             "hiprt_bounce_cosine": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), c_uint32, c_uint64, c_int32,
                                             POINTER(_Rays), c_void_p]),
             "hiprt_pt_create": (c_int, [c_void_p, POINTER(c_float), POINTER(_PtParams), POINTER(c_void_p)]),
@@ -538,6 +540,32 @@ def camera_rays_device(cam, out, stream=None):
     c = np.ascontiguousarray(cam[:12], dtype=np.float32)
     r = _rays_struct(out)
     check(lib().hiprt_camera_rays(c.ctypes.data_as(POINTER(c_float)), ctypes.byref(r), _stream(stream)))
+
+
+def incoherence_spheres(pos):
+    """The two spheres of the camera step's "incoherence" rays for a scene
+    with vertex positions `pos` (rt/cuda/rni.cpp:36-72): along the largest
+    extent, centred a quarter in from each end, the other axes centred; r_max
+    is a quarter of that extent.  float32 arithmetic."""
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    lo, hi = pos.min(0), pos.max(0)
+    size = np.abs(hi - lo)
+    f = np.float32
+    a = 0 if (size[0] >= size[1] and size[0] >= size[2]) else (1 if size[1] >= size[2] else 2)
+    s1 = lo + size / f(2)
+    s2 = lo + size / f(2)
+    s1[a] = lo[a] + size[a] / f(4)
+    s2[a] = hi[a] - size[a] / f(4)
+    return s1.astype(np.float32), s2.astype(np.float32), f(size[a] / f(4))
+
+
+def incoherent_rays_device(sphere1, sphere2, r1, r2, r_max, seed, out, stream=None):
+    """hiprt_incoherent_rays into `out`, a (8, n) float32 CUDA tensor."""
+    s1 = np.ascontiguousarray(sphere1, dtype=np.float32)
+    s2 = np.ascontiguousarray(sphere2, dtype=np.float32)
+    r = _rays_struct(out)
+    check(lib().hiprt_incoherent_rays(s1.ctypes.data_as(POINTER(c_float)), s2.ctypes.data_as(POINTER(c_float)), r1, r2,
+                                      r_max, seed, ctypes.byref(r), out.shape[1], _stream(stream)))
 
 
 def bounce_cosine(dscene, rays, hits, out, seed, level, stream=None):

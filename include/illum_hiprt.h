@@ -381,6 +381,15 @@ typedef struct {
 
 /* w*h camera rays (cam as for hiprt_camera_rays_host); bit-identical to it. */
 int hiprt_camera_rays(const float *cam, const hiprt_ray_buffers *out, void *stream);
+/* The camera step's incoherent benchmark rays under "incoherence r1 r2"
+ * (launch_setup_ray_incoherent, rt/cuda/kernels.cu:75-165): n rays from
+ * sphere1 (3 floats) moved by a uniform point of the ball of radius
+ * r1 * r_max, towards sphere2 moved by a uniform point of the ball of radius
+ * r2 * r_max; the direction is the unnormalized difference, t in
+ * [0, FLT_MAX].  Ray i draws from the PCG32 stream of (seed, i) (the
+ * reference uses cuRAND's MTGP32).  r1, r2, r_max >= 0. */
+int hiprt_incoherent_rays(const float *sphere1, const float *sphere2, float r1, float r2, float r_max, uint64_t seed,
+                          const hiprt_ray_buffers *out, uint32_t n, void *stream);
 /* out[i] = cosine-weighted bounce ray off hit i of ray i (disabled ray
  * o = 0, d = (1,1,1), t_max = -FLT_MAX for misses).  PCG32 stream per
  * (seed, i, level).  Needs a scene committed from a host scene. */

@@ -1149,3 +1149,42 @@ double orc_time_closest(const float *pos, const uint32_t *idx, uint32_t n_tris, 
 }
 
 } // extern "C"
+
+namespace {
+
+// Uniform point of the ball of radius R about the origin from three PCG32
+// numbers, as setup_ray_incoherent draws it (rt/cuda/kernels.cu:88-103):
+// phi = 2 pi xi1, cos theta = 2 xi2 - 1, r = R cbrt(xi3).
+v3 ball_offset(pcg32 &g, float R) {
+    const float pi = float(M_PI);
+    float xi1 = g.next_float(), xi2 = g.next_float(), xi3 = g.next_float();
+    float phi = xi1 * 2 * pi;
+    float theta = acosf(xi2 * 2 - 1);
+    float r = R * cbrtf(xi3);
+    return v3{r * sinf(theta) * cosf(phi), r * sinf(theta) * sinf(phi), r * cosf(theta)};
+}
+
+} // namespace
+
+extern "C" {
+
+// The incoherent benchmark rays of the cuda platform's camera step
+// (setup_ray_incoherent, rt/cuda/kernels.cu:75-156): origin = s1 + a ball
+// point of radius r1 * r_max (s1 itself for r1 = 0), direction = (s2 + a ball
+// point of radius r2 * r_max) - origin, not normalized (kernels.cu:144 drops
+// normalize's result), t in [0, FLT_MAX].  Ray i draws from the stream
+// stream_of(seed, i, 0), origin first; the reference's cuRAND MTGP32 numbers
+// are not reproducible here, so the geometry is what is pinned.
+void orc_incoherent_rays(const float *s1, const float *s2, float r1, float r2, float r_max, uint64_t seed, uint32_t n,
+                         float *const *out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        pcg32 g(stream_of(seed, i, 0));
+        v3 o{s1[0], s1[1], s1[2]}, t{s2[0], s2[1], s2[2]};
+        if (r1 > 0) o = add(o, ball_offset(g, r1 * r_max));
+        if (r2 > 0) t = add(t, ball_offset(g, r2 * r_max));
+        const float row[8] = {o.x, o.y, o.z, 0.0f, t.x - o.x, t.y - o.y, t.z - o.z, FLT_MAX};
+        for (int k = 0; k < 8; ++k) out[k][i] = row[k];
+    }
+}
+
+} // extern "C"

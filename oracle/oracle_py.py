@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_double, c_float, c_int, c_uint32, c_void_p
+from ctypes import POINTER, c_double, c_float, c_int, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
@@ -192,6 +192,17 @@ def camera_rays(pos, dir, up, fovy, w, h, ox=None, oy=None):
     ox = None if ox is None else np.ascontiguousarray(ox, np.float32)
     oy = None if oy is None else np.ascontiguousarray(oy, np.float32)
     lib().orc_camera_rays(_p(P), _p(D), _p(U), c_float(fovy), c_int(w), c_int(h), _p(ox), _p(oy), arr)
+    return out
+
+
+def incoherent_rays(sphere1, sphere2, r1, r2, r_max, seed, n):
+    """orc_incoherent_rays: (8, n) rays between two balls."""
+    out = np.empty((8, n), np.float32)
+    arr = (c_void_p * 8)(*[out[k].ctypes.data for k in range(8)])
+    s1 = np.asarray(sphere1, np.float32)
+    s2 = np.asarray(sphere2, np.float32)
+    lib().orc_incoherent_rays(_p(s1), _p(s2), c_float(r1), c_float(r2), c_float(r_max), c_uint64(seed), c_uint32(n),
+                              arr)
     return out
 
 

@@ -13,6 +13,7 @@ import oracle_py as orc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "illum-rt_amd", "rtgi-hip")
+FLT_MAX32 = np.float32(3.4028234663852886e38)
 
 
 def run_script(text, tmp_path, timeout=120):
@@ -225,6 +226,38 @@ def test_pt_wf_path_commands(tmp_path, torch_cuda):
     ctx.close()
     assert len(levels) == 7 and levels[6] > 0
     assert np.array_equal(img.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_incoherence_rays_in_the_camera_step(tmp_path, torch_cuda):
+    """"incoherence r1 r2" (rt/cuda/base.cpp:120-133): the camera step of
+    primary-wf then traces rays between two balls placed along the scene's
+    largest extent (rt/cuda/rni.cpp:36-89), the same rays every sample
+    (seed 1234); the image is the albedo at the oracle's hits on the oracle's
+    rays, up to the rays the device's libm ulps move across an edge.  The
+    platform's own messages follow base.cpp; "0 0" keeps the setting."""
+    print("case: rtgi-hip primary-wf, hall 160x90, incoherence 0.5 0.5", flush=True)
+    w, h = 160, 90
+    r = run_script(SCRIPT_HEAD.format(cfg=2, w=w, h=h, tracer="while-while") +
+                   "incoherence 1\nincoherence -1 2\nincoherence 0.5 0.5\nincoherence 0 0\nsppx 2\n"
+                   "algo primary-wf\noutfile i.pfm\nrun\n", tmp_path)
+    assert r.returncode == 0
+    assert r.stderr.splitlines() == [
+        'command (incoherence): Syntax error, "incoherence" requires exactly two positive float values',
+        'command (incoherence): Parameter error, "incoherence" requires exactly two positive float values'], r.stderr
+    img = read_pfm(tmp_path / "i.pfm").reshape(-1, 3)
+    sc = hiprt.HostScene.generate(2)
+    pos, idx, mat = sc.arrays()
+    s1, s2, r_max = hiprt.incoherence_spheres(pos)
+    rays = orc.incoherent_rays(s1, s2, 0.5, 0.5, r_max, 1234, w * h)
+    bvh = hiprt.Bvh(sc)
+    t, _, _, tid = orc.trace_closest(pos, idx, bvh.nodes, bvh.root, bvh.index, rays)
+    hit = t != FLT_MAX32
+    want = np.zeros((w * h, 3), np.float32)
+    want[hit] = sc.materials()[mat[tid[hit]], :3]
+    assert hit.mean() > 0.9 and len(np.unique(mat[tid[hit]])) > 2
+    same = (img == want).all(axis=1).mean()
+    assert same >= 0.995, same
 
 
 @pytest.mark.gpu

@@ -49,6 +49,42 @@ def test_device_camera_rays_equal_host(torch_cuda, hall):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32))
 
 
+def test_incoherent_rays_match_oracle_and_trace(torch_cuda, hall):
+    """hiprt_incoherent_rays (the camera step's "incoherence" rays,
+    rt/cuda/kernels.cu:75-156) on the hall's spheres: the same PCG32 streams
+    as the oracle, so points agree to the ulps of the device's sinf / cosf /
+    acosf / cbrtf (5e-6 of the scene's extent), zero radii keep the centres
+    exactly, t spans [0, FLT_MAX]; the rays trace to the oracle's hits
+    except where those ulps move a ray across an edge."""
+    print("case: hall, 1M incoherent rays r1 = 0.5 r2 = 0.3, device vs oracle", flush=True)
+    torch = torch_cuda
+    s1, s2, r_max = hiprt.incoherence_spheres(hall["pos"])
+    n = 1 << 20
+    out = torch.empty((8, n), dtype=torch.float32, device="cuda")
+    hiprt.incoherent_rays_device(s1, s2, 0.5, 0.3, r_max, 1234, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = orc.incoherent_rays(s1, s2, 0.5, 0.3, r_max, 1234, n)
+    scale = float(r_max) * 4
+    assert np.abs(got[0:3] - want[0:3]).max() <= 5e-6 * scale
+    assert np.abs(got[4:7] - want[4:7]).max() <= 1e-5 * scale
+    assert (got[3] == 0).all() and (got[7] == FLT_MAX).all()
+    hits = hiprt.new_hits(n)
+    hiprt.trace_closest(hall["ds"], out, hits, tracer="while-while")
+    torch.cuda.synchronize()
+    t, _, _, ids = _hits_np(hits)
+    bvh = hall["bvh"]
+    tw, _, _, iw = orc.trace_closest(hall["pos"], hall["idx"], bvh.nodes, bvh.root, bvh.index, got[:, :65536])
+    assert np.array_equal(ids[:65536], iw) and np.array_equal(t[:65536].view(np.uint32), tw.view(np.uint32))
+    assert (t < FLT_MAX).mean() > 0.9
+    zero = torch.empty((8, 256), dtype=torch.float32, device="cuda")
+    hiprt.incoherent_rays_device(s1, s2, 0.0, 0.0, r_max, 1, zero)
+    z = zero.cpu().numpy()
+    assert (z[0:3].T == s1).all() and (z[4:7].T == s2 - s1).all()
+    with pytest.raises(hiprt.HiprtError, match="non-negative"):
+        hiprt.incoherent_rays_device(s1, s2, -0.5, 0.0, r_max, 1, zero)
+
+
 def test_bounce_rays_match_oracle_and_trace_exactly(torch_cuda, hall):
     print("case: hall 480x270 primary hits -> cosine bounce -> diffuse trace", flush=True)
     torch = torch_cuda
