@@ -22,7 +22,7 @@ reads for a 1 GiB device copy in the same child -- and divided by the
 kernel's average launch time from HIP events on its stream.  A third pass
 reads the VALU counters (instructions, active lanes), the roof that binds,
 and a kernel-trace pass times the same dispatches as rocprofv3 sees them.
-The CSVs stay under gpurun_out/bench_counters/.
+The CSVs stay under COUNTER_DIR (one folder per pass).
 """
 import argparse
 import csv
