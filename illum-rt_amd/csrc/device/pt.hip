@@ -134,24 +134,36 @@ __device__ __forceinline__ uint32_t block_append(bool live, uint32_t *count, uin
 
 } // namespace
 
-// Camera rays for n paths; path i looks through pixel i % n_px.  Also resets
-// throughput (when thr is non-null), the per-pixel accumulator (when
-// `clear_radiance`), the per-path radiance (when rad_paths is non-null) and
-// the per-path id table (when path_ids is non-null).
+// Camera rays for n paths; path p = sample * n_px + pixel looks through its
+// pixel.  Queue slot i holds path i, or, with `per_px` > 1 samples per pixel
+// in pixel-major order, path (i % per_px) * n_px + i / per_px: the samples of
+// one pixel then share a wavefront and traverse the tree almost as one ray.
+// Also resets the path's throughput (when thr is non-null), its pixel's
+// accumulator (when `clear_radiance`, once per pixel), the path's radiance
+// (when rad_paths is non-null) and the slot's path id (when path_ids is
+// non-null).
 __global__ __launch_bounds__(256) void k_camera_rays(cam_frame f, ray_out out, uint32_t *path_ids, float *thr,
                                                     float *rad, int clear_radiance, uint32_t n, uint32_t n_px,
-                                                    float *rad_paths, const uint64_t *jitter_seeds) {
+                                                    float *rad_paths, const uint64_t *jitter_seeds, uint32_t per_px) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    uint32_t px = i % n_px;
+    uint32_t px, sample;
+    if (per_px > 1) {
+        px = i / per_px;
+        sample = i - px * per_px;
+    } else {
+        sample = i / n_px;
+        px = i - sample * n_px;
+    }
+    const uint32_t path = sample * n_px + px;
     int x = (int)(px % (uint32_t)f.w), y = (int)(px / (uint32_t)f.w);
     float ox = 0.0f, oy = 0.0f;
-    if (jitter_seeds) pixel_jitter(jitter_seeds[i / n_px], px, ox, oy);
+    if (jitter_seeds) pixel_jitter(jitter_seeds[sample], px, ox, oy);
     write_ray(out, i, f.pos, kRayEps, cam_dir(f, x, y, ox, oy), FLT_MAX);
-    if (path_ids) path_ids[i] = i;
-    if (thr) thr[3ull * i] = thr[3ull * i + 1] = thr[3ull * i + 2] = 1.0f;
-    if (rad_paths) rad_paths[3ull * i] = rad_paths[3ull * i + 1] = rad_paths[3ull * i + 2] = 0.0f;
-    if (clear_radiance && i < n_px) rad[3ull * i] = rad[3ull * i + 1] = rad[3ull * i + 2] = 0.0f;
+    if (path_ids) path_ids[i] = path;
+    if (thr) thr[3ull * path] = thr[3ull * path + 1] = thr[3ull * path + 2] = 1.0f;
+    if (rad_paths) rad_paths[3ull * path] = rad_paths[3ull * path + 1] = rad_paths[3ull * path + 2] = 0.0f;
+    if (clear_radiance && sample == 0) rad[3ull * px] = rad[3ull * px + 1] = rad[3ull * px + 2] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_add_samples(float *image, const float *rad_paths, uint32_t n_px,
@@ -516,11 +528,13 @@ hipError_t launch_incoherent_rays(vec3f s1, vec3f s2, float r1, float r2, float 
 
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
                               bool clear_radiance, hipStream_t s, uint32_t n_paths, float *rad_paths,
-                              const uint64_t *jitter_seeds) {
+                              const uint64_t *jitter_seeds, uint32_t pixel_major_samples) {
     uint32_t n_px = (uint32_t)f.w * (uint32_t)f.h;
     uint32_t n = n_paths ? n_paths : n_px;
+    if (pixel_major_samples > 1 && (!path_ids || (uint64_t)pixel_major_samples * n_px != n))
+        return hipErrorInvalidValue;   // a permuted queue needs its path ids, and whole pixels
     hipLaunchKernelGGL(k_camera_rays, dim3((n + 255) / 256), dim3(256), 0, s, f, out, path_ids, thr, rad,
-                       clear_radiance ? 1 : 0, n, n_px, rad_paths, jitter_seeds);
+                       clear_radiance ? 1 : 0, n, n_px, rad_paths, jitter_seeds, pixel_major_samples);
     return hipGetLastError();
 }
 

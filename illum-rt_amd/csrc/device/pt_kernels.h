@@ -71,9 +71,11 @@ RT_HD void pixel_jitter(uint64_t frame_seed, uint32_t pixel, float &ox, float &o
 // resets path ids and throughput (when non-null), clears the image
 // accumulator `rad` (per pixel, when clear_radiance) and the per-path
 // radiance `rad_paths` (when non-null).
+// pixel_major_samples k > 1: slot i holds path (i % k) * n_px + i / k (the k
+// samples of a pixel adjacent; path_ids required, n_paths = k * n_px).
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
                               bool clear_radiance, hipStream_t s, uint32_t n_paths = 0, float *rad_paths = nullptr,
-                              const uint64_t *jitter_seeds = nullptr);
+                              const uint64_t *jitter_seeds = nullptr, uint32_t pixel_major_samples = 0);
 // image[pixel] += rad_paths[k * n_px + pixel] for k = 0 .. samples-1 in order.
 hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s);
 // p[k] = base + k for k < n (the frame seeds of consecutive samples).

@@ -226,9 +226,10 @@ struct hiprt_pt_s {
         if (e != hipSuccess) return e;
         float *prad = samples > 1 ? rad_paths.as<float>() : rad.as<float>();   // radiance per path
         const uint64_t *fs = frame_seed.as<const uint64_t>();
+        // batched: the samples of a pixel share wavefronts at the camera level
         e = launch_camera_rays(f, rays[0].out, batched ? paths[0].as<uint32_t>() : nullptr, thr.as<float>(),
                                rad.as<float>(), clear_radiance, s, n, samples > 1 ? prad : nullptr,
-                               jitter ? fs : nullptr);
+                               jitter ? fs : nullptr, batched && samples > 1 ? (uint32_t)samples : 0u);
         if (e != hipSuccess) return e;
         for (int L = 0; L < levels; ++L) {
             if (ev) {
