@@ -89,11 +89,13 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     return so;
 }
 
-// shade_one for path `path` = sample * n_px + pixel, radiance in rad[3 path ..].
+// shade_one for path `path` (numbered as pt_params::decode says), radiance in
+// rad[3 path ..].
 __device__ __forceinline__ shade_out shade_path(const pt_scene &ps, const pt_params &pp, uint32_t path, int level,
                                                 const uint64_t *frame_seed, vec3f in_dir, float t, float beta,
                                                 float gamma, uint32_t tri, vec3f &T, float *rad) {
-    const uint32_t sample = path / pp.n_px, pixel = path - sample * pp.n_px;
+    uint32_t sample, pixel;
+    pp.decode(path, sample, pixel);
     return shade_one(ps, pp, sample, pixel, level, frame_seed, in_dir, t, beta, gamma, tri, T, rad + 3ull * path);
 }
 
@@ -134,45 +136,45 @@ __device__ __forceinline__ uint32_t block_append(bool live, uint32_t *count, uin
 
 } // namespace
 
-// Camera rays for n paths; path p = sample * n_px + pixel looks through its
-// pixel.  Queue slot i holds path i, or, with `per_px` > 1 samples per pixel
-// in pixel-major order, path (i % per_px) * n_px + i / per_px: the samples of
-// one pixel then share a wavefront and traverse the tree almost as one ray.
-// Also resets the path's throughput (when thr is non-null), its pixel's
-// accumulator (when `clear_radiance`, once per pixel), the path's radiance
-// (when rad_paths is non-null) and the slot's path id (when path_ids is
+// Camera rays for n paths, ray i for path i: with `px_samples` k > 1 path i
+// is sample i % k of pixel i / k (pixel-major: the samples of a pixel share a
+// wavefront and traverse the tree almost as one ray), else sample i / n_px of
+// pixel i % n_px.  Also resets the path's throughput (when thr is non-null),
+// its pixel's accumulator (when `clear_radiance`, once per pixel), the path's
+// radiance (when rad_paths is non-null) and its id (when path_ids is
 // non-null).
 __global__ __launch_bounds__(256) void k_camera_rays(cam_frame f, ray_out out, uint32_t *path_ids, float *thr,
                                                     float *rad, int clear_radiance, uint32_t n, uint32_t n_px,
-                                                    float *rad_paths, const uint64_t *jitter_seeds, uint32_t per_px) {
+                                                    float *rad_paths, const uint64_t *jitter_seeds,
+                                                    uint32_t px_samples) {
     uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     uint32_t px, sample;
-    if (per_px > 1) {
-        px = i / per_px;
-        sample = i - px * per_px;
+    if (px_samples > 1) {
+        px = i / px_samples;
+        sample = i - px * px_samples;
     } else {
         sample = i / n_px;
         px = i - sample * n_px;
     }
-    const uint32_t path = sample * n_px + px;
     int x = (int)(px % (uint32_t)f.w), y = (int)(px / (uint32_t)f.w);
     float ox = 0.0f, oy = 0.0f;
     if (jitter_seeds) pixel_jitter(jitter_seeds[sample], px, ox, oy);
     write_ray(out, i, f.pos, kRayEps, cam_dir(f, x, y, ox, oy), FLT_MAX);
-    if (path_ids) path_ids[i] = path;
-    if (thr) thr[3ull * path] = thr[3ull * path + 1] = thr[3ull * path + 2] = 1.0f;
-    if (rad_paths) rad_paths[3ull * path] = rad_paths[3ull * path + 1] = rad_paths[3ull * path + 2] = 0.0f;
+    if (path_ids) path_ids[i] = i;
+    if (thr) thr[3ull * i] = thr[3ull * i + 1] = thr[3ull * i + 2] = 1.0f;
+    if (rad_paths) rad_paths[3ull * i] = rad_paths[3ull * i + 1] = rad_paths[3ull * i + 2] = 0.0f;
     if (clear_radiance && sample == 0) rad[3ull * px] = rad[3ull * px + 1] = rad[3ull * px + 2] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_add_samples(float *image, const float *rad_paths, uint32_t n_px,
-                                                    uint32_t samples) {
+                                                    uint32_t samples, int pixel_major) {
     uint32_t px = blockIdx.x * 256u + threadIdx.x;
     if (px >= n_px) return;
     float r = image[3ull * px], g = image[3ull * px + 1], b = image[3ull * px + 2];
     for (uint32_t k = 0; k < samples; ++k) {
-        const float *q = rad_paths + 3ull * ((uint64_t)k * n_px + px);
+        const uint64_t path = pixel_major ? (uint64_t)px * samples + k : (uint64_t)k * n_px + px;
+        const float *q = rad_paths + 3ull * path;
         r += q[0];
         g += q[1];
         b += q[2];
@@ -528,18 +530,19 @@ hipError_t launch_incoherent_rays(vec3f s1, vec3f s2, float r1, float r2, float 
 
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
                               bool clear_radiance, hipStream_t s, uint32_t n_paths, float *rad_paths,
-                              const uint64_t *jitter_seeds, uint32_t pixel_major_samples) {
+                              const uint64_t *jitter_seeds, uint32_t px_samples) {
     uint32_t n_px = (uint32_t)f.w * (uint32_t)f.h;
     uint32_t n = n_paths ? n_paths : n_px;
-    if (pixel_major_samples > 1 && (!path_ids || (uint64_t)pixel_major_samples * n_px != n))
-        return hipErrorInvalidValue;   // a permuted queue needs its path ids, and whole pixels
+    if (px_samples > 1 && (uint64_t)px_samples * n_px != n) return hipErrorInvalidValue;   // whole pixels only
     hipLaunchKernelGGL(k_camera_rays, dim3((n + 255) / 256), dim3(256), 0, s, f, out, path_ids, thr, rad,
-                       clear_radiance ? 1 : 0, n, n_px, rad_paths, jitter_seeds, pixel_major_samples);
+                       clear_radiance ? 1 : 0, n, n_px, rad_paths, jitter_seeds, px_samples);
     return hipGetLastError();
 }
 
-hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s) {
-    hipLaunchKernelGGL(k_add_samples, dim3((n_px + 255) / 256), dim3(256), 0, s, image, rad_paths, n_px, samples);
+hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s,
+                              bool pixel_major) {
+    hipLaunchKernelGGL(k_add_samples, dim3((n_px + 255) / 256), dim3(256), 0, s, image, rad_paths, n_px, samples,
+                       pixel_major ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -53,6 +53,19 @@ struct pt_params {
     // simple_pt::path does (gi/pt.cpp:86-89); 1: it adds the emission and
     // bounces on like any other surface.
     int emitters_reflect;
+    // > 1: paths are numbered pixel-major instead -- path p is sample
+    // p % px_samples of pixel p / px_samples -- so the samples of a pixel sit
+    // in adjacent queue slots and share wavefronts (samples in flight).
+    uint32_t px_samples = 0;
+    __host__ __device__ __forceinline__ void decode(uint32_t path, uint32_t &sample, uint32_t &pixel) const {
+        if (px_samples > 1) {
+            pixel = path / px_samples;
+            sample = path - pixel * px_samples;
+        } else {
+            sample = path / n_px;
+            pixel = path - sample * n_px;
+        }
+    }
 };
 
 // Sub-pixel offset (ox, oy) in [-0.5, 0.5)^2 of path `path`'s camera ray:
@@ -71,13 +84,16 @@ RT_HD void pixel_jitter(uint64_t frame_seed, uint32_t pixel, float &ox, float &o
 // resets path ids and throughput (when non-null), clears the image
 // accumulator `rad` (per pixel, when clear_radiance) and the per-path
 // radiance `rad_paths` (when non-null).
-// pixel_major_samples k > 1: slot i holds path (i % k) * n_px + i / k (the k
-// samples of a pixel adjacent; path_ids required, n_paths = k * n_px).
+// px_samples k > 1: paths numbered pixel-major, as pt_params::px_samples
+// (n_paths = k * n_px).
 hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *path_ids, float *thr, float *rad,
                               bool clear_radiance, hipStream_t s, uint32_t n_paths = 0, float *rad_paths = nullptr,
-                              const uint64_t *jitter_seeds = nullptr, uint32_t pixel_major_samples = 0);
-// image[pixel] += rad_paths[k * n_px + pixel] for k = 0 .. samples-1 in order.
-hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s);
+                              const uint64_t *jitter_seeds = nullptr, uint32_t px_samples = 0);
+// image[pixel] += the pixel's per-path radiance for samples k = 0 .. samples-1
+// in order: rad_paths[k * n_px + pixel], or rad_paths[pixel * samples + k]
+// when `pixel_major`.
+hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s,
+                              bool pixel_major = false);
 // p[k] = base + k for k < n (the frame seeds of consecutive samples).
 hipError_t launch_set_seeds(uint64_t *p, uint64_t base, uint32_t n, hipStream_t s);
 hipError_t launch_shade_plain(const pt_scene &ps, const pt_params &pp, int level, const uint64_t *frame_seed,

@@ -226,10 +226,9 @@ struct hiprt_pt_s {
         if (e != hipSuccess) return e;
         float *prad = samples > 1 ? rad_paths.as<float>() : rad.as<float>();   // radiance per path
         const uint64_t *fs = frame_seed.as<const uint64_t>();
-        // batched: the samples of a pixel share wavefronts at the camera level
         e = launch_camera_rays(f, rays[0].out, batched ? paths[0].as<uint32_t>() : nullptr, thr.as<float>(),
                                rad.as<float>(), clear_radiance, s, n, samples > 1 ? prad : nullptr,
-                               jitter ? fs : nullptr, batched && samples > 1 ? (uint32_t)samples : 0u);
+                               jitter ? fs : nullptr, pp.px_samples);
         if (e != hipSuccess) return e;
         for (int L = 0; L < levels; ++L) {
             if (ev) {
@@ -295,7 +294,8 @@ struct hiprt_pt_s {
             if (e != hipSuccess) return e;
         }
         if (samples > 1) {
-            e = launch_add_samples(rad.as<float>(), rad_paths.as<const float>(), n_px, samples, s);
+            e = launch_add_samples(rad.as<float>(), rad_paths.as<const float>(), n_px, samples, s,
+                                   pp.px_samples > 1);
             if (e != hipSuccess) return e;
         }
         return launch_add_counts(totals.as<unsigned long long>(), cnt, levels, s);
@@ -591,6 +591,9 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->n = (uint32_t)npix * (p->lanes ? p->lanes : samples);
     p->pp = rt::dev::pt_params{params->max_depth, params->rr_start, params->rr_survive, (uint32_t)npix,
                                params->emitters_reflect ? 1 : 0};
+    // samples in flight: paths numbered pixel-major, so a pixel's samples
+    // share wavefronts from the camera level on
+    if (!p->lanes && samples > 1) p->pp.px_samples = samples;
     p->seed = params->seed;
     p->batched = params->batched != 0;
     p->jitter = params->pixel_centre == 0;
