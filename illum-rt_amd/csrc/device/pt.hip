@@ -247,9 +247,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
 // neighbouring slots start anywhere in the scene and point anywhere, so the
 // rays a wavefront traces together share few nodes.  In the sorting form the
 // shade step writes each next-level ray as a 32-byte record, its (path id,
-// throughput) as a 16-byte record and a 15-bit key (the octant of the
-// direction, then the Morton code of the origin's cell in a 16 x 16 x 16
-// grid over the scene box), all in append order; a stable radix sort of the
+// throughput) as a 16-byte record and a 16-bit key (the direction's cell of
+// 16 on its octahedral projection, then the Morton code of the origin's cell
+// in a 16 x 16 x 16 grid over the scene box), all in append order; the
+// stable radix sort of the
 // keys (device/ray_sort.hip) gives the permutation the trace kernel reads
 // the ray records through, and the next shade step reads directions and
 // path records through the same permutation.  A ray and its path share one
@@ -276,11 +277,34 @@ __device__ __forceinline__ uint32_t cell_of(float x, float lo, float scale) {
     return (uint32_t)q;
 }
 
+__device__ __forceinline__ uint32_t origin_cell(const sort_grid &g, vec3f o) {
+    return (spread3(cell_of(o.x, g.lo[0], g.scale[0])) << 2) | (spread3(cell_of(o.y, g.lo[1], g.scale[1])) << 1) |
+           spread3(cell_of(o.z, g.lo[2], g.scale[2]));
+}
+
+// Key of a bounce ray in the samples-in-flight queues: the direction's cell in
+// a 4 x 4 grid over its octahedral projection (|x| + |y| + |z| = 1, the lower
+// half folded over the diagonals; 2 bits per axis, Morton order), then the
+// origin's cell.  NaN coordinates land in cell 0.
 __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d) {
-    uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
-    uint32_t m = (spread3(cell_of(o.x, g.lo[0], g.scale[0])) << 2) |
-                 (spread3(cell_of(o.y, g.lo[1], g.scale[1])) << 1) | spread3(cell_of(o.z, g.lo[2], g.scale[2]));
-    return (oct << (3 * kSortCellBits)) | m;
+    const float inv = __builtin_amdgcn_rcpf(fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
+    float u = d.x * inv, v = d.y * inv;
+    if (d.z < 0.0f) {
+        const float uu = copysignf(1.0f - fabsf(v), u), vv = copysignf(1.0f - fabsf(u), v);
+        u = uu;
+        v = vv;
+    }
+    const uint32_t qu = (uint32_t)fminf(fmaxf((u * 0.5f + 0.5f) * 4.0f, 0.0f), 3.0f);
+    const uint32_t qv = (uint32_t)fminf(fmaxf((v * 0.5f + 0.5f) * 4.0f, 0.0f), 3.0f);
+    const uint32_t dir = ((qu & 1u) << 1) | (qv & 1u) | ((qu & 2u) << 2) | ((qv & 2u) << 1);
+    return (dir << (3 * kSortCellBits)) | origin_cell(g, o);
+}
+
+// Key of a bounce ray in the regeneration queues: direction octant, then the
+// origin's cell (kOctantKeyBits).
+__device__ __forceinline__ uint32_t octant_key(const sort_grid &g, vec3f o, vec3f d) {
+    const uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
+    return (oct << (3 * kSortCellBits)) | origin_cell(g, o);
 }
 
 // Shade step of the sorting form.  Queue slot i < counts[level] holds the
@@ -348,7 +372,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
 // slot accumulates its own samples in `acc` and the lanes are added at the
 // end of the render call.
 
-// Key of a camera ray: above every bounce ray's key (bit kRayKeyBits), so
+// Key of a camera ray: above every bounce ray's key (bit kOctantKeyBits), so
 // that camera rays and bounce rays do not share wavefronts; then its
 // direction octant and the Morton code of its pixel's cell in a 64 x 64
 // grid over the image (camera rays share one origin, so the origin cell
@@ -359,7 +383,7 @@ __device__ __forceinline__ uint32_t camera_key(const cam_frame &f, int x, int y,
     uint32_t m = 0;
 #pragma unroll
     for (int b = 0; b < 6; ++b) m |= (((cx >> b) & 1u) << (2 * b + 1)) | (((cy >> b) & 1u) << (2 * b));
-    return (1u << kRayKeyBits) | (oct << 12) | m;
+    return (1u << kOctantKeyBits) | (oct << 12) | m;
 }
 
 // Camera-ray record of sample `sample` of pixel `px` for slot `slot`, at
@@ -434,7 +458,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_regen(pt_scene ps, pt_par
         out_rec[3ull * at] = make_float4(so.org.x, so.org.y, so.org.z, 0.0f);
         out_rec[3ull * at + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, __uint_as_float(next_word));
         out_rec[3ull * at + 2] = make_float4(__uint_as_float(slot), T.x, T.y, T.z);
-        if (keys) keys[at] = ray_key(grid, so.org, so.dir);
+        if (keys) keys[at] = octant_key(grid, so.org, so.dir);
     } else {
         write_camera_record(rp, slot, px, sample + rp.lanes, out_rec, keys, at);
     }

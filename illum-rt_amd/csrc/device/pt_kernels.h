@@ -103,11 +103,16 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 const ray_out &rays_in, const hit_soa &hits, const uint32_t *path_in,
                                 const ray_out &rays_out, uint32_t *path_out, float *thr, float *rad,
                                 uint32_t *counts, uint32_t n_max, hipStream_t s);
-// Sorted bounce queues: ray keys are the direction octant (3 bits) and the
-// Morton code of the origin's cell in a grid of 2^kSortCellBits cells per
-// axis over the scene box.
+// Sorted bounce queues: ray keys are the direction's cell in a 4 x 4 grid
+// over its octahedral projection (4 bits, 16 cells of similar solid angle),
+// then the Morton code of the origin's cell in a grid of 2^kSortCellBits
+// cells per axis over the scene box: 16 bits, two radix passes.
 constexpr int kSortCellBits = 4;
-constexpr int kRayKeyBits = 3 + 3 * kSortCellBits;
+constexpr int kRayKeyBits = 4 + 3 * kSortCellBits;
+// Path regeneration sorts camera rays above bounce rays with one more key
+// bit; its bounce keys use the direction octant (3 bits) instead, so that
+// key and camera bit still fit 16 bits.
+constexpr int kOctantKeyBits = 3 + 3 * kSortCellBits;
 struct sort_grid {
     float lo[3];      // scene box minimum
     float scale[3];   // cells per axis / extent (0 for a flat axis)
@@ -129,7 +134,7 @@ hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level
 // Path regeneration (a fixed pool of lanes * n_px path slots, each slot
 // rendering its pixel's samples one after another; device/pt.hip).
 constexpr int kMaxPathLevels = 65;   // max_depth <= 64
-constexpr int kRegenKeyBits = kRayKeyBits + 1;   // bounce-ray keys, then camera-ray keys
+constexpr int kRegenKeyBits = kOctantKeyBits + 1;   // bounce-ray keys, then camera-ray keys
 struct regen_params {
     cam_frame f;
     const uint64_t *seeds;     // frame seed of each sample of the render call
