@@ -167,7 +167,7 @@ hipError_t launch_adjoint_closest(const adjoint_args &a_in, uint32_t n, void *so
     if (e != hipSuccess || !vertices) return e;
     int bits = 1;
     while (bits < 32 && (a.n_tris >> bits) != 0u) ++bits;   // keys 0 .. n_tris
-    e = launch_radix_sort(rb, n, a.count, bits, s);
+    e = launch_radix_sort(rb, n, a.count, bits, s, true);   // the vertex pass reads the sorted keys
     if (e != hipSuccess) return e;
     // the last of the sort's passes leaves the keys in keys_tmp after an odd
     // number of passes, in keys_in after an even one

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *
             uint32_t pos = base[d] + run[d] + rank;
             for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][d];
             if (pos < n) {   // always true when the counts match the keys
-                keys_out[pos] = key;
+                if (keys_out) keys_out[pos] = key;   // the last pass writes only the permutation
                 vals_out[pos] = val;
             }
         }
@@ -143,17 +143,18 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *
 uint32_t radix_tiles(uint32_t n_max) { return (uint32_t)(((uint64_t)n_max + kSortTile - 1) / kSortTile); }
 
 hipError_t launch_radix_sort(const radix_sort_buffers &b, uint32_t n_max, const uint32_t *count_ptr, int bits,
-                             hipStream_t s) {
+                             hipStream_t s, bool sorted_keys) {
     const uint32_t tiles = radix_tiles(n_max);
     if (tiles == 0) return hipSuccess;
     const int passes = (bits + 7) / 8;
     if (passes < 1 || passes > 4) return hipErrorInvalidValue;
     // pass p reads (src keys, src values) and writes (dst keys, dst values);
-    // the last pass writes b.vals_out; the first reads the slot index itself
+    // the last pass writes b.vals_out (and the keys only when the caller
+    // reads them); the first reads the slot index itself
     uint32_t *kbuf[2] = {b.keys_tmp, b.keys_in};
     for (int p = 0; p < passes; ++p) {
         const uint32_t *kin = p == 0 ? b.keys_in : kbuf[(p - 1) & 1];
-        uint32_t *kout = kbuf[p & 1];
+        uint32_t *kout = (p + 1 < passes || sorted_keys) ? kbuf[p & 1] : nullptr;
         const uint32_t *vin = p == 0 ? nullptr : ((passes - p) & 1 ? b.vals_tmp : b.vals_out);
         uint32_t *vout = ((passes - 1 - p) & 1) ? b.vals_tmp : b.vals_out;
         const uint32_t shift = 8u * (uint32_t)p;
