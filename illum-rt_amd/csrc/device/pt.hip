@@ -563,8 +563,43 @@ hipError_t launch_camera_rays(const cam_frame &f, const ray_out &out, uint32_t *
     return hipGetLastError();
 }
 
+// Pixel-major per-path radiance: the samples of pixel px are the contiguous
+// paths px * samples + k.  A block stages the radiance of `per_block` pixels
+// through LDS with coalesced loads (each pixel's 3 * samples floats padded by
+// one word, so the summing threads below fall in different banks), then one
+// thread per (pixel, channel) adds that channel's samples to the image in
+// sample order, as k_add_samples does.
+constexpr uint32_t kAddLdsFloats = 12288;   // 48 KiB
+__global__ __launch_bounds__(256) void k_add_samples_px(float *image, const float *rad_paths, uint32_t n_px,
+                                                       uint32_t samples, uint32_t per_block) {
+    extern __shared__ float lds[];
+    const uint32_t px0 = blockIdx.x * per_block;
+    if (px0 >= n_px) return;
+    const uint32_t np = min(per_block, n_px - px0);
+    const uint32_t row = 3 * samples, stride = row + 1;
+    const float *src = rad_paths + (uint64_t)px0 * row;
+    for (uint32_t j = threadIdx.x; j < np * row; j += 256) {
+        const uint32_t p = j / row;
+        lds[p * stride + (j - p * row)] = src[j];
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < np * 3; c += 256) {
+        const uint32_t p = c / 3, ch = c - p * 3;
+        float acc = image[3ull * (px0 + p) + ch];
+        for (uint32_t k = 0; k < samples; ++k) acc += lds[p * stride + 3 * k + ch];
+        image[3ull * (px0 + p) + ch] = acc;
+    }
+}
+
 hipError_t launch_add_samples(float *image, const float *rad_paths, uint32_t n_px, uint32_t samples, hipStream_t s,
                               bool pixel_major) {
+    const uint32_t per_block = pixel_major ? kAddLdsFloats / (3 * samples + 1) : 0u;
+    if (per_block > 0) {
+        const uint32_t blocks = (n_px + per_block - 1) / per_block;
+        hipLaunchKernelGGL(k_add_samples_px, dim3(blocks), dim3(256), (size_t)per_block * (3 * samples + 1) * 4, s,
+                           image, rad_paths, n_px, samples, per_block);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_add_samples, dim3((n_px + 255) / 256), dim3(256), 0, s, image, rad_paths, n_px, samples,
                        pixel_major ? 1 : 0);
     return hipGetLastError();
