@@ -300,8 +300,8 @@ __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d
     return (dir << (3 * kSortCellBits)) | origin_cell(g, o);
 }
 
-// Key of a bounce ray in the regeneration queues: direction octant, then the
-// origin's cell (kOctantKeyBits).
+// Key of a bounce ray in the regeneration queues and in one-sample queues:
+// direction octant, then the origin's cell (kOctantKeyBits).
 __device__ __forceinline__ uint32_t octant_key(const sort_grid &g, vec3f o, vec3f d) {
     const uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
     return (oct << (3 * kSortCellBits)) | origin_cell(g, o);
@@ -353,7 +353,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
         out_rec[3ull * slot] = make_float4(so.org.x, so.org.y, so.org.z, kRayEps);
         out_rec[3ull * slot + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, FLT_MAX);
         out_rec[3ull * slot + 2] = make_float4(__uint_as_float(path), T.x, T.y, T.z);
-        keys[slot] = ray_key(grid, so.org, so.dir);
+        // 16 direction cells when a pixel's samples share the queue (their rays
+        // leave one point in many directions), the octant for one sample per
+        // pixel: each traced its queue faster in that setting
+        keys[slot] = pp.px_samples > 1 ? ray_key(grid, so.org, so.dir) : octant_key(grid, so.org, so.dir);
     }
 }
 

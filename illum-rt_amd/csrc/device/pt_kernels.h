@@ -109,9 +109,9 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
 // cells per axis over the scene box: 16 bits, two radix passes.
 constexpr int kSortCellBits = 4;
 constexpr int kRayKeyBits = 4 + 3 * kSortCellBits;
-// Path regeneration sorts camera rays above bounce rays with one more key
-// bit; its bounce keys use the direction octant (3 bits) instead, so that
-// key and camera bit still fit 16 bits.
+// One sample per pixel in flight sorts by the direction octant (3 bits)
+// instead, and so does path regeneration, which puts camera rays above bounce
+// rays with one more key bit: key and camera bit still fit 16 bits.
 constexpr int kOctantKeyBits = 3 + 3 * kSortCellBits;
 struct sort_grid {
     float lo[3];      // scene box minimum
