@@ -332,6 +332,39 @@ int hiprt_incoherent_rays(const float *sphere1, const float *sphere2, float r1, 
     return ok();
 }
 
+size_t hiprt_sort_keys_workspace(uint32_t n) {
+    // keys_tmp, vals_tmp (n words each), the tile counts, the digit totals
+    // and the item count, each 256-byte aligned
+    const size_t words[4] = {n, n, 256ull * rt::dev::radix_tiles(n), 256ull + 64ull};
+    size_t bytes = 0;
+    for (size_t w : words) bytes += (4 * w + 255) & ~(size_t)255;
+    return bytes;
+}
+
+int hiprt_sort_keys(uint32_t *d_keys, uint32_t *d_perm, uint32_t n, int bits, void *d_workspace, void *stream) {
+    if (bits < 1 || bits > 32) return fail(HIPRT_ERR_ARG, "bits must be 1..32");
+    if (n == 0) return ok();
+    if (!d_keys || !d_perm || !d_workspace) return fail(HIPRT_ERR_ARG, "null argument");
+    char *w = static_cast<char *>(d_workspace);
+    auto take = [&w](size_t words) {
+        uint32_t *p = reinterpret_cast<uint32_t *>(w);
+        w += (4 * words + 255) & ~(size_t)255;
+        return p;
+    };
+    rt::dev::radix_sort_buffers rb{};
+    rb.keys_in = d_keys;
+    rb.vals_out = d_perm;
+    rb.keys_tmp = take(n);
+    rb.vals_tmp = take(n);
+    rb.counts = take(256ull * rt::dev::radix_tiles(n));
+    rb.totals = take(256ull + 64ull);
+    uint32_t *count = rb.totals + 256;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(rt::dev::launch_fill_u32(count, n, 1u, s));
+    HIP_TRY(rt::dev::launch_radix_sort(rb, n, count, bits, s));
+    return ok();
+}
+
 int hiprt_bounce_cosine(hiprt_scene sc, const hiprt_rays *rays, const hiprt_hits *hits, uint32_t n, uint64_t seed,
                         int32_t level, const hiprt_ray_buffers *out, void *stream) {
     if (!sc || !rays || !hits || !out) return fail(HIPRT_ERR_ARG, "null argument");

@@ -143,6 +143,8 @@ def lib():
             "hiprt_camera_rays": (c_int, [POINTER(c_float), POINTER(_Rays), c_void_p]),
             "hiprt_incoherent_rays": (c_int, [POINTER(c_float), POINTER(c_float), c_float, c_float, c_float, c_uint64,
                                               POINTER(_Rays), c_uint32, c_void_p]),
+            "hiprt_sort_keys_workspace": (ctypes.c_size_t, [c_uint32]),
+            "hiprt_sort_keys": (c_int, [c_void_p, c_void_p, c_uint32, c_int, c_void_p, c_void_p]),
             "hiprt_bounce_cosine": (c_int, [c_void_p, POINTER(_Rays), POINTER(_Hits), c_uint32, c_uint64, c_int32,
                                             POINTER(_Rays), c_void_p]),
             "hiprt_pt_create": (c_int, [c_void_p, POINTER(c_float), POINTER(_PtParams), POINTER(c_void_p)]),
@@ -566,6 +568,20 @@ def incoherent_rays_device(sphere1, sphere2, r1, r2, r_max, seed, out, stream=No
     r = _rays_struct(out)
     check(lib().hiprt_incoherent_rays(s1.ctypes.data_as(POINTER(c_float)), s2.ctypes.data_as(POINTER(c_float)), r1, r2,
                                       r_max, seed, ctypes.byref(r), out.shape[1], _stream(stream)))
+
+
+def sort_keys(keys, bits, stream=None):
+    """hiprt_sort_keys: the stable order of the uint32 keys (a 1-D CUDA
+    tensor, int32 or uint32 storage; left unchanged) by their low `bits`
+    bits, as an int32 CUDA tensor of indices."""
+    import torch
+    n = keys.numel()
+    work = keys.contiguous().clone().view(torch.int32)
+    perm = torch.empty(n, dtype=torch.int32, device=keys.device)
+    ws = torch.empty(max(int(lib().hiprt_sort_keys_workspace(n)), 1), dtype=torch.uint8, device=keys.device)
+    check(lib().hiprt_sort_keys(c_void_p(work.data_ptr()), c_void_p(perm.data_ptr()), n, bits,
+                                c_void_p(ws.data_ptr()), _stream(stream)))
+    return perm
 
 
 def bounce_cosine(dscene, rays, hits, out, seed, level, stream=None):

@@ -390,6 +390,18 @@ int hiprt_camera_rays(const float *cam, const hiprt_ray_buffers *out, void *stre
  * reference uses cuRAND's MTGP32).  r1, r2, r_max >= 0. */
 int hiprt_incoherent_rays(const float *sphere1, const float *sphere2, float r1, float r2, float r_max, uint64_t seed,
                           const hiprt_ray_buffers *out, uint32_t n, void *stream);
+
+/* Stable radix sort of ray keys: the sort that orders the path tracer's
+ * bounce queues (8 key bits per pass).  No counterpart in the reference; it
+ * is for callers that order their own ray batches before tracing them (rays
+ * keyed by direction and origin cell trace faster in key order) and for
+ * tests of the queue order.  Sorts the n keys in d_keys (device memory; used
+ * as scratch, contents undefined afterwards) by their low `bits` bits
+ * (1..32): d_perm[j] receives the index of the key at sorted position j,
+ * equal keys in index order.  d_workspace: hiprt_sort_keys_workspace(n)
+ * bytes of device memory, 256-byte aligned.  Asynchronous on `stream`. */
+size_t hiprt_sort_keys_workspace(uint32_t n);
+int hiprt_sort_keys(uint32_t *d_keys, uint32_t *d_perm, uint32_t n, int bits, void *d_workspace, void *stream);
 /* out[i] = cosine-weighted bounce ray off hit i of ray i (disabled ray
  * o = 0, d = (1,1,1), t_max = -FLT_MAX for misses).  PCG32 stream per
  * (seed, i, level).  Needs a scene committed from a host scene. */
