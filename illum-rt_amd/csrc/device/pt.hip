@@ -247,17 +247,16 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
 // neighbouring slots start anywhere in the scene and point anywhere, so the
 // rays a wavefront traces together share few nodes.  In the sorting form the
 // shade step writes each next-level ray as a 32-byte record, its (path id,
-// throughput) as a 16-byte record and a 16-bit key (the direction's cell of
-// 16 on its octahedral projection, then the Morton code of the origin's cell
-// in a 16 x 16 x 16 grid over the scene box), all in append order; the
-// stable radix sort of the
-// keys (device/ray_sort.hip) gives the permutation the trace kernel reads
+// throughput) as a 16-byte record and a key (the direction's cell of 256 on
+// its octahedral projection, then the Morton code of the origin's cell in a
+// 32 x 32 x 32 grid over the scene box: 23 bits), all in append order; the
+// stable radix sort of the keys (device/ray_sort.hip) gives the permutation the trace kernel reads
 // the ray records through, and the next shade step reads directions and
 // path records through the same permutation.  A ray and its path share one
 // 48-byte record (origin, t_min | direction, t_max | path id, throughput):
 // the trace kernel reads its first 32 bytes, the shade step its last 32.  Sorted rays fill the
 // persistent kernel's work chunks key range by key range, so each XCD
-// traces rays of one direction octant and one part of the scene.  The
+// traces rays of a few direction cells and one part of the scene.  The
 // throughput travels with the ray because a sorted queue visits pixels in
 // no useful order: a per-pixel throughput array would be read and written
 // at random addresses (that alone cost 110 us per level at 2M rays).
@@ -283,9 +282,9 @@ __device__ __forceinline__ uint32_t origin_cell(const sort_grid &g, vec3f o) {
 }
 
 // Key of a bounce ray in the samples-in-flight queues: the direction's cell in
-// a 4 x 4 grid over its octahedral projection (|x| + |y| + |z| = 1, the lower
-// half folded over the diagonals; 2 bits per axis, Morton order), then the
-// origin's cell.  NaN coordinates land in cell 0.
+// a 16 x 16 grid over its octahedral projection (|x| + |y| + |z| = 1, the
+// lower half folded over the diagonals; kDirHalfBits bits per axis, Morton
+// order), then the origin's cell.  NaN coordinates land in cell 0.
 __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d) {
     const float inv = __builtin_amdgcn_rcpf(fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
     float u = d.x * inv, v = d.y * inv;
@@ -294,17 +293,21 @@ __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d
         u = uu;
         v = vv;
     }
-    const uint32_t qu = (uint32_t)fminf(fmaxf((u * 0.5f + 0.5f) * 4.0f, 0.0f), 3.0f);
-    const uint32_t qv = (uint32_t)fminf(fmaxf((v * 0.5f + 0.5f) * 4.0f, 0.0f), 3.0f);
-    const uint32_t dir = ((qu & 1u) << 1) | (qv & 1u) | ((qu & 2u) << 2) | ((qv & 2u) << 1);
+    constexpr float kCells = (float)(1 << kDirHalfBits);
+    const uint32_t qu = (uint32_t)fminf(fmaxf((u * 0.5f + 0.5f) * kCells, 0.0f), kCells - 1.0f);
+    const uint32_t qv = (uint32_t)fminf(fmaxf((v * 0.5f + 0.5f) * kCells, 0.0f), kCells - 1.0f);
+    uint32_t dir = 0;
+#pragma unroll
+    for (int b = 0; b < kDirHalfBits; ++b) dir |= (((qu >> b) & 1u) << (2 * b + 1)) | (((qv >> b) & 1u) << (2 * b));
     return (dir << (3 * kSortCellBits)) | origin_cell(g, o);
 }
 
 // Key of a bounce ray in the regeneration queues and in one-sample queues:
-// direction octant, then the origin's cell (kOctantKeyBits).
+// direction octant, then the origin's cell at half the grid resolution (the
+// Morton code without each axis's lowest bit; kOctantKeyBits).
 __device__ __forceinline__ uint32_t octant_key(const sort_grid &g, vec3f o, vec3f d) {
     const uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
-    return (oct << (3 * kSortCellBits)) | origin_cell(g, o);
+    return (oct << (3 * kOctantCellBits)) | (origin_cell(g, o) >> 3);
 }
 
 // Shade step of the sorting form.  Queue slot i < counts[level] holds the
@@ -353,9 +356,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
         out_rec[3ull * slot] = make_float4(so.org.x, so.org.y, so.org.z, kRayEps);
         out_rec[3ull * slot + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, FLT_MAX);
         out_rec[3ull * slot + 2] = make_float4(__uint_as_float(path), T.x, T.y, T.z);
-        // 16 direction cells when a pixel's samples share the queue (their rays
-        // leave one point in many directions), the octant for one sample per
-        // pixel: each traced its queue faster in that setting
+        // 256 direction cells over a 32^3 grid when a pixel's samples share the
+        // queue (their rays leave one point in many directions), the octant
+        // over a 16^3 grid for one sample per pixel: each traced its queue
+        // fastest in that setting
         keys[slot] = pp.px_samples > 1 ? ray_key(grid, so.org, so.dir) : octant_key(grid, so.org, so.dir);
     }
 }

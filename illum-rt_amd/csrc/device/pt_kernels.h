@@ -103,16 +103,20 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 const ray_out &rays_in, const hit_soa &hits, const uint32_t *path_in,
                                 const ray_out &rays_out, uint32_t *path_out, float *thr, float *rad,
                                 uint32_t *counts, uint32_t n_max, hipStream_t s);
-// Sorted bounce queues: ray keys are the direction's cell in a 4 x 4 grid
-// over its octahedral projection (4 bits, 16 cells of similar solid angle),
-// then the Morton code of the origin's cell in a grid of 2^kSortCellBits
-// cells per axis over the scene box: 16 bits, two radix passes.
-constexpr int kSortCellBits = 4;
-constexpr int kRayKeyBits = 4 + 3 * kSortCellBits;
-// One sample per pixel in flight sorts by the direction octant (3 bits)
-// instead, and so does path regeneration, which puts camera rays above bounce
-// rays with one more key bit: key and camera bit still fit 16 bits.
-constexpr int kOctantKeyBits = 3 + 3 * kSortCellBits;
+// Sorted bounce queues with a pixel's samples in flight: ray keys are the
+// direction's cell in a 2^kDirHalfBits-square grid over its octahedral
+// projection (256 cells of similar solid angle, Morton order), then the
+// Morton code of the origin's cell in a grid of 2^kSortCellBits cells per
+// axis over the scene box: 23 bits, three radix passes.
+constexpr int kDirHalfBits = 4;
+constexpr int kSortCellBits = 5;
+constexpr int kRayKeyBits = 2 * kDirHalfBits + 3 * kSortCellBits;
+// One sample per pixel in flight sorts by the direction octant (3 bits) and
+// the origin's cell in a grid of half that resolution instead, and so does
+// path regeneration, which puts camera rays above bounce rays with one more
+// key bit: key and camera bit fit 16 bits, two passes.
+constexpr int kOctantCellBits = kSortCellBits - 1;
+constexpr int kOctantKeyBits = 3 + 3 * kOctantCellBits;
 struct sort_grid {
     float lo[3];      // scene box minimum
     float scale[3];   // cells per axis / extent (0 for a flat axis)

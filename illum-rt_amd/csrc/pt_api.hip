@@ -287,7 +287,7 @@ struct hiprt_pt_s {
                         radix_sort_buffers rb{keys.as<uint32_t>(),      keys_tmp.as<uint32_t>(),
                                               vals_tmp.as<uint32_t>(),  perm.as<uint32_t>(),
                                               sort_counts.as<uint32_t>(), sort_totals.as<uint32_t>()};
-                        e = launch_radix_sort(rb, n, cnt + L + 1, kRayKeyBits, s);
+                        e = launch_radix_sort(rb, n, cnt + L + 1, pp.px_samples > 1 ? kRayKeyBits : kOctantKeyBits, s);
                     }
                 }
             }
