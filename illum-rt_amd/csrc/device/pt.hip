@@ -248,8 +248,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
 // rays a wavefront traces together share few nodes.  In the sorting form the
 // shade step writes each next-level ray as a 32-byte record, its (path id,
 // throughput) as a 16-byte record and a key (the direction's cell of 256 on
-// its octahedral projection, then the Morton code of the origin's cell in a
-// 32 x 32 x 32 grid over the scene box: 23 bits), all in append order; the
+// its octahedral projection and the Morton code of the origin's cell in a
+// 32 x 32 x 32 grid over the scene box, coarse bits of both first: 23 bits),
+// all in append order; the
 // stable radix sort of the keys (device/ray_sort.hip) gives the permutation the trace kernel reads
 // the ray records through, and the next shade step reads directions and
 // path records through the same permutation.  A ray and its path share one
@@ -281,10 +282,12 @@ __device__ __forceinline__ uint32_t origin_cell(const sort_grid &g, vec3f o) {
            spread3(cell_of(o.z, g.lo[2], g.scale[2]));
 }
 
-// Key of a bounce ray in the samples-in-flight queues: the direction's cell in
-// a 16 x 16 grid over its octahedral projection (|x| + |y| + |z| = 1, the
-// lower half folded over the diagonals; kDirHalfBits bits per axis, Morton
-// order), then the origin's cell.  NaN coordinates land in cell 0.
+// Key of a bounce ray in the samples-in-flight queues, from the direction's
+// cell in a 16 x 16 grid over its octahedral projection (|x| + |y| + |z| = 1,
+// the lower half folded over the diagonals; kDirHalfBits bits per axis,
+// Morton order) and the origin's cell, in two levels: the direction's cell of
+// the coarse 4 x 4 grid, the origin's cell of the coarse 16^3 grid, then the
+// fine bits of each.  NaN coordinates land in cell 0.
 __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d) {
     const float inv = __builtin_amdgcn_rcpf(fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
     float u = d.x * inv, v = d.y * inv;
@@ -299,7 +302,11 @@ __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d
     uint32_t dir = 0;
 #pragma unroll
     for (int b = 0; b < kDirHalfBits; ++b) dir |= (((qu >> b) & 1u) << (2 * b + 1)) | (((qv >> b) & 1u) << (2 * b));
-    return (dir << (3 * kSortCellBits)) | origin_cell(g, o);
+    constexpr int kFineDir = 2 * kDirHalfBits - 4, kFineOrg = 3 * kSortCellBits - 12;
+    static_assert(kFineDir >= 0 && kFineOrg >= 0, "the coarse level is 4 x 4 directions by 16^3 origins");
+    const uint32_t org = origin_cell(g, o);
+    return ((dir >> kFineDir) << (12 + kFineDir + kFineOrg)) | ((org >> kFineOrg) << (kFineDir + kFineOrg)) |
+           ((dir & ((1u << kFineDir) - 1u)) << kFineOrg) | (org & ((1u << kFineOrg) - 1u));
 }
 
 // Key of a bounce ray in the regeneration queues and in one-sample queues:

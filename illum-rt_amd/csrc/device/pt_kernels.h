@@ -103,11 +103,12 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 const ray_out &rays_in, const hit_soa &hits, const uint32_t *path_in,
                                 const ray_out &rays_out, uint32_t *path_out, float *thr, float *rad,
                                 uint32_t *counts, uint32_t n_max, hipStream_t s);
-// Sorted bounce queues with a pixel's samples in flight: ray keys are the
+// Sorted bounce queues with a pixel's samples in flight: ray keys combine the
 // direction's cell in a 2^kDirHalfBits-square grid over its octahedral
-// projection (256 cells of similar solid angle, Morton order), then the
-// Morton code of the origin's cell in a grid of 2^kSortCellBits cells per
-// axis over the scene box: 23 bits, three radix passes.
+// projection (256 cells of similar solid angle, Morton order) and the Morton
+// code of the origin's cell in a grid of 2^kSortCellBits cells per axis over
+// the scene box, coarse levels of both above the fine ones: 23 bits, three
+// radix passes.
 constexpr int kDirHalfBits = 4;
 constexpr int kSortCellBits = 5;
 constexpr int kRayKeyBits = 2 * kDirHalfBits + 3 * kSortCellBits;
