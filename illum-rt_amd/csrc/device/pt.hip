@@ -247,9 +247,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_compact(pt_scene ps, pt_p
 // neighbouring slots start anywhere in the scene and point anywhere, so the
 // rays a wavefront traces together share few nodes.  In the sorting form the
 // shade step writes each next-level ray as a 32-byte record, its (path id,
-// throughput) as a 16-byte record and a key (the direction's cell of 256 on
+// throughput) as a 16-byte record and a key (the direction's cell of 1024 on
 // its octahedral projection and the Morton code of the origin's cell in a
-// 32 x 32 x 32 grid over the scene box, coarse bits of both first: 23 bits),
+// 128^3 grid over the scene box, in three levels, coarse first: 31 bits),
 // all in append order; the
 // stable radix sort of the keys (device/ray_sort.hip) gives the permutation the trace kernel reads
 // the ray records through, and the next shade step reads directions and
@@ -283,11 +283,12 @@ __device__ __forceinline__ uint32_t origin_cell(const sort_grid &g, vec3f o) {
 }
 
 // Key of a bounce ray in the samples-in-flight queues, from the direction's
-// cell in a 16 x 16 grid over its octahedral projection (|x| + |y| + |z| = 1,
+// cell in a 32 x 32 grid over its octahedral projection (|x| + |y| + |z| = 1,
 // the lower half folded over the diagonals; kDirHalfBits bits per axis,
-// Morton order) and the origin's cell, in two levels: the direction's cell of
-// the coarse 4 x 4 grid, the origin's cell of the coarse 16^3 grid, then the
-// fine bits of each.  NaN coordinates land in cell 0.
+// Morton order) and the origin's cell in a 128^3 grid, in three levels: the
+// direction's cell of a 4 x 4 grid and the origin's of a 16^3 grid, then
+// those of 16 x 16 and 64^3, then the last direction and origin bits.  NaN
+// coordinates land in cell 0.
 __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d) {
     const float inv = __builtin_amdgcn_rcpf(fabsf(d.x) + fabsf(d.y) + fabsf(d.z));
     float u = d.x * inv, v = d.y * inv;
@@ -302,19 +303,19 @@ __device__ __forceinline__ uint32_t ray_key(const sort_grid &g, vec3f o, vec3f d
     uint32_t dir = 0;
 #pragma unroll
     for (int b = 0; b < kDirHalfBits; ++b) dir |= (((qu >> b) & 1u) << (2 * b + 1)) | (((qv >> b) & 1u) << (2 * b));
-    constexpr int kFineDir = 2 * kDirHalfBits - 4, kFineOrg = 3 * kSortCellBits - 12;
-    static_assert(kFineDir >= 0 && kFineOrg >= 0, "the coarse level is 4 x 4 directions by 16^3 origins");
+    static_assert(kDirHalfBits == 5 && kSortCellBits == 7, "levels: 4 + 4 + 2 direction bits, 12 + 6 + 3 origin bits");
     const uint32_t org = origin_cell(g, o);
-    return ((dir >> kFineDir) << (12 + kFineDir + kFineOrg)) | ((org >> kFineOrg) << (kFineDir + kFineOrg)) |
-           ((dir & ((1u << kFineDir) - 1u)) << kFineOrg) | (org & ((1u << kFineOrg) - 1u));
+    return ((dir >> 6) << 27) | ((org >> 9) << 15) | (((dir >> 2) & 15u) << 11) | (((org >> 3) & 63u) << 5) |
+           ((dir & 3u) << 3) | (org & 7u);
 }
 
 // Key of a bounce ray in the regeneration queues and in one-sample queues:
-// direction octant, then the origin's cell at half the grid resolution (the
-// Morton code without each axis's lowest bit; kOctantKeyBits).
+// direction octant, then the origin's cell in a 16^3 grid (the Morton code
+// without each axis's low kSortCellBits - kOctantCellBits bits;
+// kOctantKeyBits).
 __device__ __forceinline__ uint32_t octant_key(const sort_grid &g, vec3f o, vec3f d) {
     const uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
-    return (oct << (3 * kOctantCellBits)) | (origin_cell(g, o) >> 3);
+    return (oct << (3 * kOctantCellBits)) | (origin_cell(g, o) >> (3 * (kSortCellBits - kOctantCellBits)));
 }
 
 // Shade step of the sorting form.  Queue slot i < counts[level] holds the
@@ -363,10 +364,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
         out_rec[3ull * slot] = make_float4(so.org.x, so.org.y, so.org.z, kRayEps);
         out_rec[3ull * slot + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, FLT_MAX);
         out_rec[3ull * slot + 2] = make_float4(__uint_as_float(path), T.x, T.y, T.z);
-        // 256 direction cells over a 32^3 grid when a pixel's samples share the
-        // queue (their rays leave one point in many directions), the octant
-        // over a 16^3 grid for one sample per pixel: each traced its queue
-        // fastest in that setting
+        // 1024 direction cells over a 128^3 grid when a pixel's samples share
+        // the queue (their rays leave one point in many directions), the
+        // octant over a 16^3 grid for one sample per pixel: each traced its
+        // queue fastest in that setting
         keys[slot] = pp.px_samples > 1 ? ray_key(grid, so.org, so.dir) : octant_key(grid, so.org, so.dir);
     }
 }

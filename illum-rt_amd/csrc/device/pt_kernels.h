@@ -105,18 +105,18 @@ hipError_t launch_shade_compact(const pt_scene &ps, const pt_params &pp, int lev
                                 uint32_t *counts, uint32_t n_max, hipStream_t s);
 // Sorted bounce queues with a pixel's samples in flight: ray keys combine the
 // direction's cell in a 2^kDirHalfBits-square grid over its octahedral
-// projection (256 cells of similar solid angle, Morton order) and the Morton
+// projection (1024 cells of similar solid angle, Morton order) and the Morton
 // code of the origin's cell in a grid of 2^kSortCellBits cells per axis over
-// the scene box, coarse levels of both above the fine ones: 23 bits, three
+// the scene box (128^3), in three levels, coarse above fine: 31 bits, four
 // radix passes.
-constexpr int kDirHalfBits = 4;
-constexpr int kSortCellBits = 5;
+constexpr int kDirHalfBits = 5;
+constexpr int kSortCellBits = 7;
 constexpr int kRayKeyBits = 2 * kDirHalfBits + 3 * kSortCellBits;
 // One sample per pixel in flight sorts by the direction octant (3 bits) and
-// the origin's cell in a grid of half that resolution instead, and so does
-// path regeneration, which puts camera rays above bounce rays with one more
-// key bit: key and camera bit fit 16 bits, two passes.
-constexpr int kOctantCellBits = kSortCellBits - 1;
+// the origin's cell in a 16^3 grid instead, and so does path regeneration,
+// which puts camera rays above bounce rays with one more key bit: key and
+// camera bit fit 16 bits, two passes.
+constexpr int kOctantCellBits = 4;
 constexpr int kOctantKeyBits = 3 + 3 * kOctantCellBits;
 struct sort_grid {
     float lo[3];      // scene box minimum
