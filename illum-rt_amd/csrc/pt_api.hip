@@ -100,6 +100,11 @@ struct hiprt_pt_s {
     bool batched = false;
     bool jitter = true;   // camera rays through a random point of the pixel (else its centre)
     rt::dev::persistent_cfg pcfg{};
+    // refill threshold of levels 0 and 1 with a pixel's samples in flight:
+    // queued pixel-major, a wavefront holds one pixel's camera rays (and,
+    // sorted, rays leaving nearby points), and refilling it early mixes in
+    // other pixels' rays
+    int refill_coherent = 16;
     ray_planes rays[2];
     dbuf hits, paths[2], thr, rad, counts, totals, frame_seed, work, spill;
     // samples in flight: n = samples * n_px paths; with more than one the
@@ -249,6 +254,7 @@ struct hiprt_pt_s {
                 // persistent tracers read the queue length from counts[L]
                 persistent_cfg cfg = pcfg;
                 cfg.stats = lstats;
+                if (pp.px_samples > 1 && L <= 1) cfg.refill_thr = refill_coherent;
                 if (!sorted) {
                     const ray_planes &in = rays[L & 1], &outq = rays[(L + 1) & 1];
                     e = launch_trace_persistent(hiprt_detail::desc_for(sc, alpha), in.out.as_in(), h, nullptr, n,
@@ -680,6 +686,7 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->pcfg.speculative = p->tracer >= HIPRT_TRACER_PERSISTENT_SPECULATIVE_WHILE_WHILE;
     p->pcfg.refill = p->tracer == HIPRT_TRACER_DYNAMIC_WHILE_WHILE;
     p->pcfg.refill_thr = params->refill_threshold > 0 ? std::min(params->refill_threshold, 64) : 40;
+    p->refill_coherent = params->refill_coherent > 0 ? std::min(params->refill_coherent, 64) : 16;
     p->pcfg.blocks = 0;
     p->pcfg.stack = stack;
     p->pcfg.chunks = params->work_chunks == 1 ? 1 : 8;

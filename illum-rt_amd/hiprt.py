@@ -76,7 +76,7 @@ class _PtParams(ctypes.Structure):
                 ("tracer", c_int32), ("batched", c_int32), ("stack_config", c_int32), ("work_chunks", c_int32),
                 ("refill_threshold", c_int32), ("blocks", c_int32), ("ray_order", c_int32), ("samples", c_int32),
                 ("emitters_reflect", c_int32), ("park_leaves", c_int32), ("pixel_centre", c_int32),
-                ("regen_pool", c_int32)]
+                ("regen_pool", c_int32), ("refill_coherent", c_int32)]
 
 
 _lib = None
@@ -657,9 +657,11 @@ class PathTracer:
 
     def __init__(self, dscene, cam, max_depth=4, rr_start=3, rr_survive=0.8, seed=1, batched=True, tracer=None,
                  stack_config=0, work_chunks=0, refill_threshold=0, blocks=0, sort_rays=True, samples=1,
-                 emitters_reflect=False, jitter=True, park_leaves=0, regen_pool=0):
+                 emitters_reflect=False, jitter=True, park_leaves=0, regen_pool=0, refill_coherent=0):
         """regen_pool k > 0: path regeneration over k path slots per pixel
-        (hiprt_pt_params.regen_pool) instead of all `samples` in flight."""
+        (hiprt_pt_params.regen_pool) instead of all `samples` in flight.
+        refill_coherent: the dynamic tracer's refill threshold on the camera
+        level and the first bounce with samples in flight (0 = 16)."""
         self.dscene = dscene
         c = np.ascontiguousarray(cam[:12], dtype=np.float32)
         self.w, self.h = int(c[10]), int(c[11])
@@ -667,7 +669,7 @@ class PathTracer:
         tid = -1 if tracer is None else tracer_id(tracer)
         p = _PtParams(max_depth, rr_start, rr_survive, seed, tid, 1 if batched else 0, stack_config, work_chunks,
                       refill_threshold, blocks, 0 if sort_rays else 1, samples, 1 if emitters_reflect else 0,
-                      park_leaves, 0 if jitter else 1, regen_pool)
+                      park_leaves, 0 if jitter else 1, regen_pool, refill_coherent)
         self.h_ = c_void_p()
         check(lib().hiprt_pt_create(dscene.h, c.ctypes.data_as(POINTER(c_float)), ctypes.byref(p),
                                     ctypes.byref(self.h_)))

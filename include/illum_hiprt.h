@@ -530,6 +530,10 @@ typedef struct {
                                end, which reassociates the per-pixel sum.  render() then
                                waits on the device: it reads the pool's queue length every
                                few iterations to know when the frame is done. */
+    int32_t refill_coherent; /* dynamic tracer with samples in flight (samples > 1, regen_pool 0):
+                                refill threshold of the camera level and the first bounce, whose
+                                rays of one pixel share wavefronts (0 = 16); refill_threshold
+                                applies to the later levels */
 } hiprt_pt_params;
 
 typedef struct hiprt_pt_s *hiprt_pt;
