@@ -243,7 +243,9 @@ def measure_roofline(child_cfg, kname, levels, avg_launch_ms):
                        "active_lanes": round(sum(thr) / max(sum(act), 1.0), 2),
                        "active_lane_frac": round(sum(thr) / max(sum(act), 1.0) / 64.0, 4),
                        "basis": "SQ_INSTS_VALU x 4 cycles per wave64 instruction over 1024 SIMDs x launch time x "
-                                "2.4 GHz; active lanes = SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU"}
+                                "2.4 GHz (an issue-rate estimate: it passes 1 when some counted instructions "
+                                "take fewer than 4 cycles of the SIMD or issue beside it, as transcendental "
+                                "ones do); active lanes = SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU"}
     durs = trace_pass(child_cfg, kname, levels)
     out["kernel_trace"] = {"avg_launch_ms": round(sum(durs) / len(durs), 4), "dispatches": len(durs),
                            "basis": "rocprofv3 --kernel-trace over the counter child's frames, warm-up frame "
