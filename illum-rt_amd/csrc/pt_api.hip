@@ -31,7 +31,7 @@ using hiprt_detail::ok;
 namespace {
 
 // Sort grid over the scene box: the union of the root's child boxes (a
-// one-leaf or empty tree sorts by direction octant only).
+// one-leaf or empty tree sorts by the direction's cell or octant only).
 rt::dev::sort_grid grid_of(const rt::flat_scene &h) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     auto grow = [&](float xl, float xh, float yl, float yh, float zl, float zh) {
