@@ -61,7 +61,11 @@ __device__ __forceinline__ shade_out shade_one(const pt_scene &ps, const pt_para
     shade_out so{false, v3(0.0f), v3(0.0f)};
     if (!ps.valid(t, tri)) return so;
     const float4 *g = ps.tri_geo + 3 * (size_t)tri;
-    float4 g0 = g[0], g1 = g[1], g2 = g[2];
+    float4 g0 = g[0], g1{}, g2{};
+    if (level < pp.max_depth) {   // uniform; the last level only adds emission (material in g0.w)
+        g1 = g[1];
+        g2 = g[2];
+    }
     vec3f A = f3(g0), B = f3(g1), C = f3(g2);
     uint32_t m = __float_as_uint(g0.w);
     float4 alb = ps.mats[2 * m], emi = ps.mats[2 * m + 1];
@@ -321,8 +325,10 @@ __device__ __forceinline__ uint32_t octant_key(const sort_grid &g, vec3f o, vec3
 // Shade step of the sorting form.  Queue slot i < counts[level] holds the
 // ray of record src = perm[i] (src = i without a permutation).  At the camera
 // level (in_rec null) the direction comes from plane i of rays_in and the
-// path and throughput from path_in[i] and thr; later, direction and (path id
-// bits, throughput) from record src of in_rec.  The hit is slot i.  Live
+// path and throughput from path_in[i] and thr (path i and throughput 1 when
+// they are null); later, direction and (path id bits, throughput) from record
+// src of in_rec.  The last level reads no direction: it only adds emission.
+// The hit is slot i.  Live
 // next-level rays are appended as records to out_rec with their keys: the
 // throughput travels with the ray, so no step reads or writes per-pixel
 // throughput at a random address.
@@ -341,18 +347,21 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
     uint32_t path = 0;
     vec3f T{};
     if (i < n) {
-        vec3f in_dir;
+        vec3f in_dir{};
+        const bool bounce = level < pp.max_depth;   // uniform
         if (in_rec) {
             uint32_t src = perm ? perm[i] : i;
-            float4 d = in_rec[3ull * src + 1];
             float4 pr = in_rec[3ull * src + 2];
-            in_dir = vec3f{d.x, d.y, d.z};
+            if (bounce) {
+                float4 d = in_rec[3ull * src + 1];
+                in_dir = vec3f{d.x, d.y, d.z};
+            }
             path = __float_as_uint(pr.x);
             T = vec3f{pr.y, pr.z, pr.w};
         } else {
-            path = path_in[i];
-            in_dir = vec3f{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
-            T = load3(thr, path);
+            path = path_in ? path_in[i] : i;
+            if (bounce) in_dir = vec3f{rays_in.dx[i], rays_in.dy[i], rays_in.dz[i]};
+            T = thr ? load3(thr, path) : vec3f{1.0f, 1.0f, 1.0f};
         }
         so = shade_path(ps, pp, path, level, frame_seed, in_dir, hits.t[i], hits.u[i], hits.v[i], hits.id[i], T,
                        rad);

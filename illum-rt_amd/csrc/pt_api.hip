@@ -231,7 +231,10 @@ struct hiprt_pt_s {
         if (e != hipSuccess) return e;
         float *prad = samples > 1 ? rad_paths.as<float>() : rad.as<float>();   // radiance per path
         const uint64_t *fs = frame_seed.as<const uint64_t>();
-        e = launch_camera_rays(f, rays[0].out, batched ? paths[0].as<uint32_t>() : nullptr, thr.as<float>(),
+        // the sorting form's camera level is path i with throughput 1 in slot i:
+        // no path ids or throughputs to write
+        e = launch_camera_rays(f, rays[0].out, batched && !sorted ? paths[0].as<uint32_t>() : nullptr,
+                               sorted ? nullptr : thr.as<float>(),
                                rad.as<float>(), clear_radiance, s, n, samples > 1 ? prad : nullptr,
                                jitter ? fs : nullptr, pp.px_samples);
         if (e != hipSuccess) return e;
@@ -286,8 +289,8 @@ struct hiprt_pt_s {
                         e = hipEventRecord(ev[2 * L + 1], s);
                         if (e != hipSuccess) return e;
                     }
-                    e = launch_shade_stage(ps(), pp, L, fs, rays[0].out, paths[0].as<const uint32_t>(),
-                                           thr.as<const float>(), in_rec, pm, h, rec[L & 1].as<float4>(),
+                    e = launch_shade_stage(ps(), pp, L, fs, rays[0].out, nullptr, nullptr, in_rec, pm, h,
+                                           rec[L & 1].as<float4>(),
                                            keys.as<uint32_t>(), prad, cnt, grid, n, s);
                     if (e == hipSuccess && L < pp.max_depth) {
                         radix_sort_buffers rb{keys.as<uint32_t>(),      keys_tmp.as<uint32_t>(),
@@ -655,14 +658,12 @@ int hiprt_pt_create(hiprt_scene sc, const float *cam, const hiprt_pt_params *par
     p->sorted = p->batched && params->ray_order == 0 && p->pp.max_depth > 0;
     if (!regen) {
         HIP_TRY(p->rays[0].alloc(n));
-        if (p->batched) {
+        if (p->batched && !p->sorted) {   // the sorting form carries path ids and throughputs in its records
             HIP_TRY(p->paths[0].alloc(4ull * n));
-            if (!p->sorted) {
-                HIP_TRY(p->rays[1].alloc(n));
-                HIP_TRY(p->paths[1].alloc(4ull * n));
-            }
+            HIP_TRY(p->rays[1].alloc(n));
+            HIP_TRY(p->paths[1].alloc(4ull * n));
         }
-        HIP_TRY(p->thr.alloc(12ull * n));
+        if (!p->sorted) HIP_TRY(p->thr.alloc(12ull * n));
         if (samples > 1) HIP_TRY(p->rad_paths.alloc(12ull * n));
     } else {
         // records in both parities, queue lengths, per-slot radiance
