@@ -55,6 +55,8 @@ CASES = [
      dict(batched=True, tracer="persistent-while-while", sort_rays=False, work_chunks=1), (333, 97), 2, 2),
     ("batched, dynamic-while-while, sorted queues, stack 4, 64 samples in flight",
      dict(batched=True, stack_config=4, samples=64), (80, 45), 64, 1),
+    ("batched, dynamic-while-while, 16 samples in flight, camera and first-bounce refill below 60 lanes",
+     dict(batched=True, stack_config=4, samples=16, refill_coherent=60), (160, 90), 16, 1),
     ("batched, speculative, 4-wide tree, 8 samples in flight",
      dict(batched=True, tracer="persistent-speculative-while-while", samples=8, bvh4=True), (160, 90), 8, 1),
     ("plain, emitters reflect", dict(batched=False, emitters_reflect=True), (320, 180), 1, 1),
