@@ -396,18 +396,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_stage(pt_scene ps, pt_par
 // slot accumulates its own samples in `acc` and the lanes are added at the
 // end of the render call.
 
-// Key of a camera ray: above every bounce ray's key (bit kOctantKeyBits), so
-// that camera rays and bounce rays do not share wavefronts; then its
+// Key of a camera ray: above every bounce ray's key (the top bit of
+// regen_key_bits), so that camera rays and bounce rays do not share
+// wavefronts; then its
 // direction octant and the Morton code of its pixel's cell in a 64 x 64
 // grid over the image (camera rays share one origin, so the origin cell
 // would not tell them apart).
-__device__ __forceinline__ uint32_t camera_key(const cam_frame &f, int x, int y, vec3f d) {
+__device__ __forceinline__ uint32_t camera_key(const cam_frame &f, int x, int y, vec3f d, uint32_t lanes) {
     uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
     uint32_t cx = (uint32_t)((uint64_t)x * 64u / (uint32_t)f.w), cy = (uint32_t)((uint64_t)y * 64u / (uint32_t)f.h);
     uint32_t m = 0;
 #pragma unroll
     for (int b = 0; b < 6; ++b) m |= (((cx >> b) & 1u) << (2 * b + 1)) | (((cy >> b) & 1u) << (2 * b));
-    return (1u << kOctantKeyBits) | (oct << 12) | m;
+    return (1u << (regen_key_bits(lanes) - 1)) | (oct << 12) | m;
 }
 
 // Camera-ray record of sample `sample` of pixel `px` for slot `slot`, at
@@ -421,7 +422,7 @@ __device__ __forceinline__ void write_camera_record(const regen_params &rp, uint
     out_rec[3ull * at] = make_float4(rp.f.pos.x, rp.f.pos.y, rp.f.pos.z, 0.0f);
     out_rec[3ull * at + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(sample << 8));
     out_rec[3ull * at + 2] = make_float4(__uint_as_float(slot), 1.0f, 1.0f, 1.0f);
-    if (keys) keys[at] = camera_key(rp.f, x, y, d);
+    if (keys) keys[at] = camera_key(rp.f, x, y, d, rp.lanes);
 }
 
 // The pool's first paths: slot j < lanes * n_px gets sample j / n_px of its
@@ -482,7 +483,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade_regen(pt_scene ps, pt_par
         out_rec[3ull * at] = make_float4(so.org.x, so.org.y, so.org.z, 0.0f);
         out_rec[3ull * at + 1] = make_float4(so.dir.x, so.dir.y, so.dir.z, __uint_as_float(next_word));
         out_rec[3ull * at + 2] = make_float4(__uint_as_float(slot), T.x, T.y, T.z);
-        if (keys) keys[at] = octant_key(grid, so.org, so.dir);
+        if (keys) keys[at] = rp.lanes > 1 ? ray_key(grid, so.org, so.dir) : octant_key(grid, so.org, so.dir);
     } else {
         write_camera_record(rp, slot, px, sample + rp.lanes, out_rec, keys, at);
     }

@@ -139,7 +139,12 @@ hipError_t launch_shade_stage(const pt_scene &ps, const pt_params &pp, int level
 // Path regeneration (a fixed pool of lanes * n_px path slots, each slot
 // rendering its pixel's samples one after another; device/pt.hip).
 constexpr int kMaxPathLevels = 65;   // max_depth <= 64
-constexpr int kRegenKeyBits = kOctantKeyBits + 1;   // bounce-ray keys, then camera-ray keys
+// Regeneration keys: bounce-ray keys as with samples in flight (the 31-bit
+// keys with more than one slot per pixel, the 15-bit octant keys with one),
+// camera-ray keys above them (one more bit).
+constexpr int kRegenKeyBits = kOctantKeyBits + 1;
+constexpr int kRegenFineKeyBits = kRayKeyBits + 1;
+__host__ __device__ inline int regen_key_bits(uint32_t lanes) { return lanes > 1 ? kRegenFineKeyBits : kRegenKeyBits; }
 struct regen_params {
     cam_frame f;
     const uint64_t *seeds;     // frame seed of each sample of the render call

@@ -171,7 +171,7 @@ struct hiprt_pt_s {
             if (e == hipSuccess)
                 e = launch_shade_regen(ps(), pp, rp, rec[a].as<const float4>(), g.perm, h, rec[b].as<float4>(),
                                        sorted ? keys.as<uint32_t>() : nullptr, q + a, q + b, grid, n, s);
-            if (e == hipSuccess && sorted) e = launch_radix_sort(sort_buffers(), n, q + b, kRegenKeyBits, s);
+            if (e == hipSuccess && sorted) e = launch_radix_sort(sort_buffers(), n, q + b, regen_key_bits(lanes), s);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -187,7 +187,7 @@ struct hiprt_pt_s {
         HIP_TRY(launch_fill_u32(q, n, 1u, s));
         HIP_TRY(launch_fill_u32(q + 1, 0u, 1u, s));
         HIP_TRY(launch_regen_init(regen(), rec[0].as<float4>(), sorted ? keys.as<uint32_t>() : nullptr, s));
-        if (sorted) HIP_TRY(launch_radix_sort(sort_buffers(), n, q, kRegenKeyBits, s));
+        if (sorted) HIP_TRY(launch_radix_sort(sort_buffers(), n, q, regen_key_bits(lanes), s));
         // every slot ends within ceil(samples / lanes) paths of at most
         // max_depth + 1 iterations each
         const uint64_t bound = (uint64_t)((samples + lanes - 1) / lanes) * (uint64_t)levels;
