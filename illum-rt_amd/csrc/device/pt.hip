@@ -603,9 +603,26 @@ __global__ __launch_bounds__(256) void k_add_samples_px(float *image, const floa
     const uint32_t np = min(per_block, n_px - px0);
     const uint32_t row = 3 * samples, stride = row + 1;
     const float *src = rad_paths + (uint64_t)px0 * row;
-    for (uint32_t j = threadIdx.x; j < np * row; j += 256) {
-        const uint32_t p = j / row;
-        lds[p * stride + (j - p * row)] = src[j];
+    // element j of the block's rows is word e of row p; a step of 256 moves
+    // (p, e) by (256 / row, 256 % row) and at most one carry
+    // and eight loads are in flight before their LDS stores
+    constexpr uint32_t kBatch = 8;
+    const uint32_t dq = 256u / row, dr = 256u - dq * row, total = np * row;
+    uint32_t p = threadIdx.x / row, e = threadIdx.x - p * row;
+    for (uint32_t j0 = threadIdx.x; j0 < total; j0 += 256 * kBatch) {
+        float v[kBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) v[u] = j0 + 256 * u < total ? src[j0 + 256 * u] : 0.0f;
+#pragma unroll
+        for (uint32_t u = 0; u < kBatch; ++u) {
+            if (j0 + 256 * u < total) lds[p * stride + e] = v[u];
+            p += dq;
+            e += dr;
+            if (e >= row) {
+                e -= row;
+                ++p;
+            }
+        }
     }
     __syncthreads();
     for (uint32_t c = threadIdx.x; c < np * 3; c += 256) {
