@@ -6,8 +6,8 @@
 //   k_radix_count   digit histogram of each tile (LDS), written digit-major
 //                   (counts[digit * n_tiles + tile]);
 //   k_radix_rows    one block per digit: exclusive scan of its row over the
-//                   tiles, in place (wave scans, one barrier per 1024
-//                   tiles), and the row total;
+//                   tiles, in place (four consecutive tiles per thread, wave
+//                   scans, one barrier per 4096 tiles), and the row total;
 //   k_radix_scatter each wave of a tile ranks its own contiguous quarter of
 //                   the tile: 64 items at a time, lanes of the same digit
 //                   found with eight ballots, each digit's running count in
@@ -96,15 +96,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *ke
 
 __global__ __launch_bounds__(kRowThreads) void k_radix_rows(uint32_t *counts, uint32_t n_tiles, uint32_t *totals) {
     constexpr int kRowWaves = kRowThreads / 64;
+    constexpr uint32_t kPer = 4;                 // consecutive tiles per thread: four loads in flight
+    constexpr uint32_t kChunk = kRowThreads * kPer;
     __shared__ uint32_t wsum[2][kRowWaves];   // wave totals, double-buffered over chunks
     const uint32_t d = blockIdx.x, t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     uint32_t *row = counts + (size_t)d * n_tiles;
     uint32_t carry = 0;
     int buf = 0;
-    for (uint32_t base = 0; base < n_tiles; base += kRowThreads) {
-        const uint32_t j = base + t;
-        const uint32_t v = j < n_tiles ? row[j] : 0u;
-        const uint32_t incl = wave_scan_inclusive(v, lane);
+    for (uint32_t base = 0; base < n_tiles; base += kChunk) {
+        const uint32_t j0 = base + kPer * t;
+        uint32_t v[kPer], mine = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) v[k] = j0 + k < n_tiles ? row[j0 + k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) mine += v[k];
+        const uint32_t incl = wave_scan_inclusive(mine, lane);
         if (lane == 63) wsum[buf][wave] = incl;
         __syncthreads();
         uint32_t before = 0, chunk = 0;
@@ -114,7 +120,12 @@ __global__ __launch_bounds__(kRowThreads) void k_radix_rows(uint32_t *counts, ui
             before += (uint32_t)w < wave ? s : 0u;
             chunk += s;
         }
-        if (j < n_tiles) row[j] = carry + before + incl - v;
+        uint32_t run = carry + before + incl - mine;   // exclusive prefix of tile j0
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            if (j0 + k < n_tiles) row[j0 + k] = run;
+            run += v[k];
+        }
         carry += chunk;
         // the next chunk writes the other buffer; the one after it writes this
         // buffer only after the next chunk's barrier, which every wave passes
