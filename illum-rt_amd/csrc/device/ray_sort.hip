@@ -3,7 +3,8 @@
 // keys.
 //
 // Per pass, over tiles of kSortTile consecutive items:
-//   k_radix_count   digit histogram of each tile (LDS), written digit-major
+//   k_radix_count   digit histogram of each tile (LDS; one atomic for a wave
+//                   whose items share a digit), written digit-major
 //                   (counts[digit * n_tiles + tile]);
 //   k_radix_rows    one block per digit: exclusive scan of its row over the
 //                   tiles, in place (four consecutive tiles per thread, wave
@@ -86,9 +87,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_count(const uint32_t *ke
             const uint64_t i = first + (uint64_t)s * kSortThreads + t;
             k[s] = i < n ? keys[i] : 0u;
         }
+        // a wave whose valid lanes share one digit (common in the later
+        // passes, once lower digits have grouped the keys) adds once instead
+        // of 64 times to the same LDS word
 #pragma unroll
-        for (int s = 0; s < kSortRounds; ++s)
-            if (first + (uint64_t)s * kSortThreads + t < n) atomicAdd(&hist[(k[s] >> shift) & dmask], 1u);
+        for (int s = 0; s < kSortRounds; ++s) {
+            const bool valid = first + (uint64_t)s * kSortThreads + t < n;
+            const uint32_t d = (k[s] >> shift) & dmask;
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            const uint64_t vm = __ballot(valid), same = __ballot(valid && d == d0);
+            if (same == vm) {
+                if ((t & 63u) == 0u && vm) atomicAdd(&hist[d0], (uint32_t)__popcll(vm));
+            } else if (valid) {
+                atomicAdd(&hist[d], 1u);
+            }
+        }
     }
     __syncthreads();
     counts[(size_t)t * n_tiles + tile] = hist[t];
