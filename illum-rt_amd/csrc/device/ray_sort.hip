@@ -185,7 +185,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *
     for (int r = 0; r < kSortRounds; ++r) {
         const bool valid = quarter + (uint64_t)r * 64u + lane < n;
         const uint32_t d = (key[r] >> shift) & dmask;
-        const uint64_t same = same_digit_lanes(d, valid);
+        // one ballot settles a round whose valid lanes share a digit
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint64_t vm = __ballot(valid), one = __ballot(valid && d == d0);
+        const uint64_t same = one == vm ? vm : same_digit_lanes(d, valid);
         const uint32_t rank = (uint32_t)__popcll(same & below);
         const uint32_t prior = valid ? wrun[wave][d] : 0u;
         loc[r] = prior + rank;
