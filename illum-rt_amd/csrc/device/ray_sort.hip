@@ -11,8 +11,9 @@
 //                   scans, one barrier per 4096 tiles), and the row total;
 //   k_radix_scatter each wave of a tile ranks its own contiguous quarter of
 //                   the tile: 64 items at a time, lanes of the same digit
-//                   found with eight ballots, each digit's running count in
-//                   the wave's LDS row (no block barrier between rounds).
+//                   found with eight ballots (one when all share a digit),
+//                   each digit's running count in the wave's LDS row (no
+//                   block barrier between rounds).
 //                   The waves' counts then give every item its position in
 //                   the tile sorted by digit (digit, then input order); the
 //                   tile is staged there in LDS and written out in that
@@ -21,7 +22,7 @@
 // Input order decides ties (waves' quarters in order, rounds in order, lanes
 // in order), so every pass is stable and the passes compose into a sort by
 // all the bits.  Blocks are dealt to the 8 XCDs round robin; block b sorts
-// tile (b mod 8) * n_tiles / 8 + b / 8 (sort_tile), so one XCD holds a
+// tile about (b mod 8) * n_tiles / 8 + b / 8 (sort_tile), so one XCD holds a
 // contiguous range of tiles and writes the adjacent slots of each digit's
 // run through its own L2.  The item count is read from device memory, so a
 // captured graph sorts queues whose length an earlier kernel produced; tiles
